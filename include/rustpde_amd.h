@@ -1,0 +1,139 @@
+/*
+ * rustpde_amd: MI355X (gfx950) implementation of rustpde's 2-D Rayleigh-Benard time step.
+ *
+ * C ABI of librustpde_amd.so.  Plain pointers and sizes only.  Every entry point returns
+ * 0 on success and a negative status on failure (the message is kept per thread and read
+ * with rbc_last_error()); nothing throws or aborts across the ABI.  One handle owns its
+ * device buffers and one HIP stream; a handle is not thread-safe.  All arithmetic is FP64.
+ *
+ * Array layouts are the reference's (ndarray, C order): a field of shape [n0, n1] is
+ * row-major with axis 1 (y, Chebyshev) contiguous.  Spectral arrays of periodic
+ * (Fourier-x) fields are complex, stored interleaved (re, im) exactly like
+ * num_complex::Complex<f64> (#[repr(C)]), i.e. 2*n0*n1 doubles.
+ *
+ * Citations (file:line) are relative to the reference repository (preiter93/rustpde).
+ */
+#ifndef RUSTPDE_AMD_H
+#define RUSTPDE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status / errors */
+#define RBC_OK 0
+#define RBC_ERR_ARG (-1)     /* bad argument (shape, index, size)            */
+#define RBC_ERR_DEVICE (-2)  /* HIP runtime error                              */
+#define RBC_ERR_NUMERIC (-3) /* set-up failure (e.g. complex eigenvalues)      */
+
+/* Last error message of the calling thread ("" if none). */
+const char* rbc_last_error(void);
+/* Library version string. */
+const char* rbc_version(void);
+
+/* ------------------------------------------------------------------- Navier2D */
+typedef struct rbc_navier rbc_navier;
+
+/* Field indices (members of struct Navier2D, src/navier/navier.rs:153-195). */
+#define RBC_TEMP 0    /* temp                         */
+#define RBC_UX 1      /* ux                           */
+#define RBC_UY 2      /* uy                           */
+#define RBC_PRES 3    /* pres[0]  (pressure)          */
+#define RBC_PSEU 4    /* pres[1]  (pseudo pressure)   */
+#define RBC_FIELDBC 6 /* fieldbc  (temperature BC)    */
+/* Spaces */
+#define RBC_PHYS 0 /* Field2::v     (physical, real)        */
+#define RBC_SPEC 1 /* Field2::vhat  (spectral, real/complex) */
+
+/* Navier2D::new(nx, ny, ra, pr, dt, aspect, adiabatic)  navier.rs:219-307.
+ * Fields start at zero (the reference's unseeded random_disturbance, navier.rs:304,
+ * is left to the caller: set temp/ux/uy with rbc_set_field + rbc_forward). */
+int rbc_create_confined(int nx, int ny, double ra, double pr, double dt, double aspect, int adiabatic,
+                        rbc_navier** out);
+/* Navier2D::new_periodic(nx, ny, ra, pr, dt, aspect)  navier.rs:384-467 (nx even). */
+int rbc_create_periodic(int nx, int ny, double ra, double pr, double dt, double aspect, rbc_navier** out);
+/* drop(Navier2D) */
+int rbc_destroy(rbc_navier* h);
+
+/* Shape of field `which` in `space`; *is_complex = 1 for complex spectral arrays. */
+int rbc_field_shape(rbc_navier* h, int which, int space, int* n0, int* n1, int* is_complex);
+/* Read / write Field2::v or Field2::vhat (host buffers, reference layout). */
+int rbc_get_field(rbc_navier* h, int which, int space, double* host_out);
+int rbc_set_field(rbc_navier* h, int which, int space, const double* host_in);
+/* Field2::forward (v -> vhat) / Field2::backward (vhat -> v)  src/field.rs:103-110 */
+int rbc_forward(rbc_navier* h, int which);
+int rbc_backward(rbc_navier* h, int which);
+
+/* Navier2D::set_temp_bc (navier.rs:517-519): replaces fieldbc.vhat (shape of RBC_FIELDBC,
+ * RBC_SPEC) and every quantity derived from it. */
+int rbc_set_temp_bc(rbc_navier* h, const double* host_vhat);
+
+/* Navier2D::set_velocity(amp, m, n) / set_temperature(amp, m, n)  navier.rs:927-937 */
+int rbc_set_velocity(rbc_navier* h, double amp, double m, double n);
+int rbc_set_temperature(rbc_navier* h, double amp, double m, double n);
+
+/* Integrate::update  navier.rs:737-765, `nsteps` times (queued on the handle's stream). */
+int rbc_update(rbc_navier* h, int nsteps);
+/* Wait until queued work of the handle has finished. */
+int rbc_synchronize(rbc_navier* h);
+/* Integrate::get_time / get_dt  navier.rs:767-773; time is settable (Navier2D::time, pub). */
+double rbc_get_time(rbc_navier* h);
+double rbc_get_dt(rbc_navier* h);
+int rbc_set_time(rbc_navier* h, double time);
+
+/* Diagnostics, FP64 on the device:
+ *   RBC_NU     eval_nu     navier.rs:890-893 -> functions.rs:12-36
+ *   RBC_NUVOL  eval_nuvol  navier.rs:899-909 -> functions.rs:42-75
+ *   RBC_RE     eval_re     navier.rs:912-921 -> functions.rs:82-101
+ *   RBC_DIV    |divergence|_2 as printed by callback / tested by exit (navier.rs:816-861) */
+#define RBC_NU 0
+#define RBC_NUVOL 1
+#define RBC_RE 2
+#define RBC_DIV 3
+int rbc_eval(rbc_navier* h, int what, double* out);
+/* Integrate::exit  navier.rs:855-862: *out = 1 if |div|_2 is NaN. */
+int rbc_exit(rbc_navier* h, int* out);
+
+/* Tensor solver set-up (FdmaTensor pub fields lam/fwd/bwd, src/solver/fdma_tensor.rs:73-91)
+ * of solver s (0 ux, 1 uy, 2 temp, 3 pres; navier.rs:165).  *has_fb = 0 when axis 0 is
+ * diagonal (Fourier); n is the number of eigenvalues, fwd/bwd are n x n. */
+int rbc_tensor_info(rbc_navier* h, int s, int* n, int* has_fb);
+int rbc_get_tensor(rbc_navier* h, int s, double* lam, double* fwd, double* bwd);
+/* Overwrite lam (and fwd/bwd if has_fb) of solver s, as assigning the pub fields. */
+int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, const double* bwd);
+
+/* ------------------------------------------------------------- Space2 (funspace) */
+typedef struct fs_space2 fs_space2;
+/* Base kinds: funspace constructors chebyshev, cheb_dirichlet, cheb_neumann,
+ * cheb_dirichlet_bc, fourier_r2c (funspace/src/lib.rs:230-345). */
+#define FS_CHEBYSHEV 0
+#define FS_CHEB_DIRICHLET 1
+#define FS_CHEB_NEUMANN 2
+#define FS_CHEB_DIRICHLET_BC 3
+#define FS_FOURIER_R2C 4
+
+/* Space2::new(base0(n0), base1(n1))  funspace/src/space2.rs:35-83 */
+int fs_space2_create(int kind0, int n0, int kind1, int n1, fs_space2** out);
+int fs_space2_destroy(fs_space2* sp);
+/* shapes: physical [p0,p1], spectral [s0,s1], orthogonal [o0,o1]; complex if kind0 is Fourier */
+int fs_space2_shape(fs_space2* sp, int* p0, int* p1, int* s0, int* s1, int* o0, int* o1, int* is_complex);
+/* BaseSpace ops (funspace/src/space_traits.rs:71-311) on DEVICE buffers, batched over
+ * `batch` contiguous fields (reference layouts, complex interleaved).  Synchronous. */
+int fs_forward(fs_space2* sp, const double* d_phys, double* d_spec, int batch);     /* space2.rs:323-333 */
+int fs_backward(fs_space2* sp, const double* d_spec, double* d_phys, int batch);    /* space2.rs:346-356 */
+int fs_to_ortho(fs_space2* sp, const double* d_spec, double* d_ortho, int batch);   /* space2.rs:181-201 */
+int fs_from_ortho(fs_space2* sp, const double* d_ortho, double* d_spec, int batch); /* space2.rs:203-223 */
+int fs_gradient(fs_space2* sp, const double* d_spec, double* d_ortho, int d0, int d1, double scale0,
+                double scale1, int batch); /* space2.rs:247-264 */
+
+/* ------------------------------------------------------------ device memory helpers
+ * For callers without their own HIP runtime binding (e.g. ctypes). */
+int rbc_dev_malloc(void** out, unsigned long long bytes);
+int rbc_dev_free(void* p);
+int rbc_memcpy_h2d(void* dst, const void* src, unsigned long long bytes);
+int rbc_memcpy_d2h(void* dst, const void* src, unsigned long long bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RUSTPDE_AMD_H */

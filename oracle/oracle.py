@@ -50,6 +50,7 @@ def lib():
         L.orc_navier_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_navier_tensor_info.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.orc_navier_tensor_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_navier_tensor_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.orc_space_apply.argtypes = [ctypes.c_int] * 7 + [_dp, _dp, _dp]
         L.orc_solver_apply.argtypes = [ctypes.c_int] * 5 + [_dp, ctypes.c_int, _dp, _dp]
         L.orc_eig.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -237,3 +238,9 @@ class Navier2D:
         bwd = np.zeros((m, m))
         _check(lib().orc_navier_tensor_get(self._h, s, _p(lam), _p(fwd), _p(bwd)))
         return (lam, fwd, bwd) if fb.value else (lam, None, None)
+
+    def set_tensor(self, s, lam, fwd=None, bwd=None):
+        lam = np.ascontiguousarray(lam, dtype=float)
+        f = None if fwd is None else _p(np.ascontiguousarray(fwd, dtype=float))
+        b = None if bwd is None else _p(np.ascontiguousarray(bwd, dtype=float))
+        _check(lib().orc_navier_tensor_set(self._h, s, _p(lam), f, b))

@@ -1324,6 +1324,20 @@ int orc_navier_tensor_get(void* p, int s, double* lam, double* fwd, double* bwd)
   })
 }
 
+int orc_navier_tensor_set(void* p, int s, const double* lam, const double* fwd, const double* bwd) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    Solver2& sv = h->periodic ? *h->c->solver[s] : *h->r->solver[s];
+    if (sv.kind == S_HHOLTZ_ADI) throw std::runtime_error("solver is ADI");
+    std::memcpy(sv.t.lam.data(), lam, sv.t.lam.size() * 8);
+    if (sv.t.has_fb) {
+      std::memcpy(sv.t.fwd.d.data(), fwd, sv.t.fwd.d.size() * 8);
+      std::memcpy(sv.t.bwd.d.data(), bwd, sv.t.bwd.d.size() * 8);
+    }
+    sv.t.build_rows();
+  })
+}
+
 // ---- funspace / solver entry points for the unit tests -------------------------
 // Space2 of (kind0, n0) x (kind1, n1); complex spectral when kind0 is Fourier.
 // op: 0 forward (phys -> spec), 1 backward (spec -> phys), 2 to_ortho, 3 from_ortho,

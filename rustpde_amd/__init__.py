@@ -1,0 +1,11 @@
+"""rustpde_amd: MI355X (gfx950) implementation of rustpde's 2-D Rayleigh-Benard step.
+
+Host mirror of the reference's API over librustpde_amd.so (HIP kernels + C ABI):
+Navier2D.new / Navier2D.new_periodic, the Integrate methods, integrate(), and the
+funspace Space2 transforms.
+"""
+from ._lib import RbcError, load  # noqa: F401
+from .funspace import (Space2, cheb_dirichlet, cheb_dirichlet_bc, cheb_neumann, chebyshev,  # noqa: F401
+                       fourier_r2c)
+from .integrate import integrate  # noqa: F401
+from .navier import Navier2D, get_ka, get_nu  # noqa: F401

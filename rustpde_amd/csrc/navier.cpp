@@ -1,0 +1,1090 @@
+// Navier2D on one MI355X: device state, operators and the time step, plus the C ABI.
+//
+// The step follows Integrate::update (src/navier/navier.rs:737-765) operation by
+// operation.  Each 1-D operator along x (axis 0, strided) is applied as a left GEMM
+// X[out, in] * A[in, cols]; each operator along y (axis 1, contiguous) as a right GEMM
+// A[rows, in] * Y^T[in, out].  Operators are pre-composed on the host (setup.cpp), so a
+// 2-D transform is two GEMMs and the convective product u * d(q) is the epilogue of the
+// second one.  Complex spectral arrays are split planes [2m, cols].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rustpde_amd.h"
+#include "ops.hpp"
+#include "setup.hpp"
+
+namespace rbc {
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct NumericError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                       \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) throw rbc::DeviceError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    HIPCHECK(hipMalloc(&p, count * sizeof(double)));
+    n = count;
+    HIPCHECK(hipMemset(p, 0, count * sizeof(double)));
+  }
+  void upload(const std::vector<double>& v) {
+    if (v.size() > n) alloc(v.size());
+    HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  std::vector<double> download(size_t count) const {
+    std::vector<double> v(count);
+    HIPCHECK(hipMemcpy(v.data(), p, count * sizeof(double), hipMemcpyDeviceToHost));
+    return v;
+  }
+};
+
+// Device matrix: x-operators are stored as [out, in], y-operators transposed [in, out].
+struct DOp {
+  DBuf b;
+  int rows = 0, cols = 0;
+};
+
+static ld ipow(ld x, int e) {
+  ld r = 1.0L;
+  for (int i = 0; i < e; ++i) r *= x;
+  return r;
+}
+
+// Dense 1-D operator `name` of basis b along an axis of scale `sc`.
+//   S    to_ortho                      (composite_stencil.rs:207-229)
+//   BS   backward                      (composite.rs:430-443 ; r2c.rs:211-226)
+//   BDS  d/dx then backward            (conv_term.rs:35-38)
+//   DS   d/dx (to ortho)               (space2.rs:247-264)
+//   DDS  d2/dx2 (to ortho)
+//   FO   from_ortho                    (composite_stencil.rs:250-276)
+//   FS   forward                       (composite.rs:383-396 ; r2c.rs:160-175)
+//   Fdl  forward then 2/3 dealias      (navier.rs:562-568, 1022-1032)
+static LMat build_axis_op(const Basis& b, const std::string& name, ld sc) {
+  const int n = b.n;
+  if (b.k == FO) {
+    if (name == "BS") return dft_c2r(n);
+    if (name == "FS") return dft_r2c(n);
+    if (name == "Fdl") {
+      LMat a = dft_r2c(n);
+      const int cut = b.m * 2 / 3;
+      for (int k = cut; k < b.m; ++k)
+        for (int j = 0; j < n; ++j) a(k, j) = a(b.m + k, j) = 0.0L;
+      return a;
+    }
+    throw std::runtime_error("no dense Fourier operator " + name);
+  }
+  LMat S = stencil(b);
+  if (name == "S") return S;
+  if (name == "BS") return cheb_backward(n) * S;
+  if (name == "BDS") return scaled(cheb_backward(n) * cheb_diff(n, 1) * S, 1.0L / sc);
+  if (name == "DS") return scaled(cheb_diff(n, 1) * S, 1.0L / sc);
+  if (name == "DDS") return scaled(cheb_diff(n, 2) * S, 1.0L / (sc * sc));
+  if (name == "FO") return from_ortho(b);
+  if (name == "FS") return from_ortho(b) * cheb_forward(n);
+  if (name == "Fdl") {
+    LMat f = cheb_forward(n);
+    const int cut = n * 2 / 3;
+    for (int k = cut; k < n; ++k)
+      for (int j = 0; j < n; ++j) f(k, j) = 0.0L;
+    return f;
+  }
+  throw std::runtime_error("unknown operator " + name);
+}
+
+// Host application of axis operators in long double (set-up of constant fields).
+static LMat host_fourier_diff(const LMat& in, int m, int times, ld sc) {
+  LMat out = in;
+  for (int k = 0; k < m; ++k)
+    for (int j = 0; j < in.c; ++j) {
+      ld a = in(k, j), b = in(m + k, j);
+      for (int t = 0; t < times; ++t) {
+        ld na = -b * (ld)k, nb = a * (ld)k;
+        a = na;
+        b = nb;
+      }
+      out(k, j) = a / ipow(sc, times);
+      out(m + k, j) = b / ipow(sc, times);
+    }
+  return out;
+}
+static LMat host_x(const Basis& b, const std::string& name, const LMat& in, ld sc) {
+  if (b.k == FO) {
+    if (name == "S" || name == "FO") return in;
+    if (name == "DS") return host_fourier_diff(in, b.m, 1, sc);
+    if (name == "DDS") return host_fourier_diff(in, b.m, 2, sc);
+    if (name == "BDS") return dft_c2r(b.n) * host_fourier_diff(in, b.m, 1, sc);
+  }
+  return build_axis_op(b, name, sc) * in;
+}
+static LMat host_y(const Basis& b, const std::string& name, const LMat& in) {
+  return in * build_axis_op(b, name, 1.0L).t();
+}
+
+class OpCache {
+ public:
+  OpCache(bool transposed, double scale) : transposed_(transposed), scale_(scale) {}
+  const DOp& get(const Basis& b, const std::string& name) {
+    std::string key = std::to_string((int)b.k) + ":" + std::to_string(b.n) + ":" + name;
+    auto it = ops_.find(key);
+    if (it != ops_.end()) return *it->second;
+    LMat m = build_axis_op(b, name, (ld)scale_);
+    if (transposed_) m = m.t();
+    auto op = std::make_unique<DOp>();
+    op->rows = m.r;
+    op->cols = m.c;
+    op->b.upload(m.to_double());
+    const DOp& ref = *op;
+    ops_[key] = std::move(op);
+    return ref;
+  }
+
+ private:
+  bool transposed_;
+  double scale_;
+  std::map<std::string, std::unique_ptr<DOp>> ops_;
+};
+
+static void upload_op(DOp& op, const LMat& m) {
+  op.rows = m.r;
+  op.cols = m.c;
+  op.b.upload(m.to_double());
+}
+
+// ------------------------------------------------------------------------- solvers
+enum SKind { ADI = 0, HHOLTZ = 1, POISSON = 2 };
+
+struct Solver {
+  SKind kind;
+  Basis bx, by;
+  int nx_in = 0;     // ortho rows of the rhs (confined) or 2m (periodic)
+  // ADI: u = (My^-1 Py) o (Mx^-1 Px) f  (hholtz_adi.rs:106-131)
+  DOp hx, hy;
+  // tensor (hholtz.rs:159-197, fdma_tensor.rs:195-234)
+  bool has_fb = false;
+  std::vector<double> lam, fwd, bwd;  // fwd/bwd m x m (row-major) when has_fb
+  std::vector<double> px;             // P_x (m x n) in double, for composing fwd * P_x
+  DOp pf, pb, py;
+  DBuf low, dia, up1, up2;
+  int nsets = 0, m = 0;
+  Band4 fa, fc;
+  double alpha = 0.0;
+
+  Solver(SKind k, const Basis& b0, const Basis& b1) : kind(k), bx(b0), by(b1) {}
+
+  void build_rows() {  // per-row systems (A_y + (lam_i + alpha) C_y), swept once
+    nsets = (int)lam.size();
+    m = fa.n;
+    std::vector<double> L((size_t)nsets * m, 0.0), D((size_t)nsets * m, 0.0), U1((size_t)nsets * m, 0.0),
+        U2((size_t)nsets * m, 0.0);
+    for (int i = 0; i < nsets; ++i) {
+      const double l = lam[i] + alpha;
+      Band4 f = fa.add_scaled(fc, l);
+      f.sweep();
+      for (int j = 0; j < m - 2; ++j) L[(size_t)i * m + j] = f.low[j];
+      for (int j = 0; j < m; ++j) D[(size_t)i * m + j] = f.dia[j];
+      for (int j = 0; j < m - 2; ++j) U1[(size_t)i * m + j] = f.up1[j];
+      for (int j = 0; j < m - 4; ++j) U2[(size_t)i * m + j] = f.up2[j];
+    }
+    low.upload(L);
+    dia.upload(D);
+    up1.upload(U1);
+    up2.upload(U2);
+    if (has_fb) {
+      const int mm = (int)lam.size();
+      LMat F(mm, mm), P((int)px.size() / bx.n, bx.n), Q(mm, mm);
+      for (size_t i = 0; i < F.a.size(); ++i) F.a[i] = fwd[i];
+      for (size_t i = 0; i < P.a.size(); ++i) P.a[i] = px[i];
+      for (size_t i = 0; i < Q.a.size(); ++i) Q.a[i] = bwd[i];
+      upload_op(pf, F * P);
+      upload_op(pb, Q);
+    }
+  }
+};
+
+static std::vector<double> to_vec(const LMat& m) { return m.to_double(); }
+
+// Build the solver of the field with bases (b0, b1) and constants c (navier.rs:250-265, 410-425)
+static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Basis& b1, const double c[2]) {
+  auto s = std::make_unique<Solver>(kind, b0, b1);
+  Ingredients g0 = ingredients(b0), g1 = ingredients(b1);
+  if (kind == ADI) {  // hholtz_adi.rs:44-63 : Fdma(mat_a - mat_b c) and MatVecFdma(precond)
+    for (int ax = 0; ax < 2; ++ax) {
+      const Ingredients& g = ax == 0 ? g0 : g1;
+      const int m = g.a.r;
+      LMat M(m, m);
+      for (size_t i = 0; i < M.a.size(); ++i) M.a[i] = (ld)((double)g.a.a[i] - (double)g.b.a[i] * c[ax]);
+      LMat H = inverse(M) * g.p;  // (m x n)
+      if (ax == 0) upload_op(s->hx, H);
+      else upload_op(s->hy, H.t());
+    }
+    return s;
+  }
+  // tensor solvers: A = laplacian (scaled mat_b), C = mass (mat_a)
+  auto lap = [&](const Ingredients& g, double ci) {
+    std::vector<double> a(g.b.a.size());
+    for (size_t i = 0; i < a.size(); ++i)
+      a[i] = kind == HHOLTZ ? (-1.0 * (double)g.b.a[i]) * ci : (double)g.b.a[i] * ci;
+    return a;
+  };
+  std::vector<double> A0 = lap(g0, c[0]), A1 = lap(g1, c[1]);
+  const int m0 = g0.a.r, m1 = g1.a.r;
+  s->alpha = kind == HHOLTZ ? 1.0 : 0.0;
+  if (g0.is_diag) {  // fdma_tensor.rs:117-120
+    s->has_fb = false;
+    s->lam.resize(m0);
+    for (int i = 0; i < m0; ++i) s->lam[i] = A0[(size_t)i * m0 + i];
+  } else {  // fdma_tensor.rs:121-126 : xmat = inv(C) A ; eig ; fwd = Q^-1 inv(C) ; bwd = Q
+    LMat C(m0, m0), Am(m0, m0);
+    for (size_t i = 0; i < C.a.size(); ++i) C.a[i] = g0.a.a[i];
+    for (size_t i = 0; i < Am.a.size(); ++i) Am.a[i] = A0[i];
+    LMat Ci = inverse(C);
+    std::vector<double> xmat = (Ci * Am).to_double();
+    std::vector<double> q, qi;
+    std::string msg;
+    if (!eig_real(xmat, m0, s->lam, q, qi, msg)) throw NumericError(msg);
+    LMat Qi(m0, m0);
+    for (size_t i = 0; i < Qi.a.size(); ++i) Qi.a[i] = qi[i];
+    s->fwd = (Qi * Ci).to_double();
+    s->bwd = q;
+    s->has_fb = true;
+  }
+  if (kind == POISSON && std::fabs(s->lam[0]) < 1e-10)  // poisson.rs:80-83
+    for (auto& v : s->lam) v -= 1e-10;
+  s->fa = Band4::from_matrix(A1, m1);
+  std::vector<double> C1(g1.a.a.size());
+  for (size_t i = 0; i < C1.size(); ++i) C1[i] = (double)g1.a.a[i];
+  s->fc = Band4::from_matrix(C1, m1);
+  if (g0.has_p) s->px = to_vec(g0.p);
+  upload_op(s->py, g1.p.t());  // MatVecFdma(precond) along y as a right GEMM
+  s->build_rows();
+  return s;
+}
+
+// ----------------------------------------------------------------------- Navier2D
+class Navier {
+ public:
+  bool periodic;
+  int nx, ny;
+  double ra, pr, dt, aspect, nu, ka, time = 0.0;
+  double scale[2];
+  hipStream_t st = nullptr;
+  Basis bx_u, bx_t, bx_p, bx_phi, bx_f, by_u, by_p, by_phi, by_bc;
+  OpCache X, Y;
+  std::unique_ptr<Solver> solver[4];
+  // state
+  DBuf T, UX, UY, P0, P1, BC;        // spectral (split planes when periodic)
+  DBuf Tv, UXv, UYv, P0v, P1v, BCv;  // physical
+  // constants derived from fieldbc
+  DBuf THAT_BC, DTBCX, DTBCY, GXX, GYY;
+  // workspaces
+  DBuf THAT, UXP, UYP, A, G, CONV, CV, RHS, DIV, W1, W2, W3, W4, W9, RED;
+  std::vector<double> wx_dx, wy_dx;  // cell widths of `field` (unscaled, field.rs:135-163)
+  double len0 = 0, len1 = 0;
+  DBuf DX0, DX1;
+
+  int mx() const { return nx / 2 + 1; }
+  int rows_spec(const Basis& b) const { return periodic ? 2 * b.m : b.m; }
+  int rows_ortho(const Basis& b) const { return periodic ? 2 * b.m : b.n; }
+
+  Navier(bool per, int nx_, int ny_, double ra_, double pr_, double dt_, double aspect_, bool adiabatic)
+      : periodic(per), nx(nx_), ny(ny_), ra(ra_), pr(pr_), dt(dt_), aspect(aspect_),
+        bx_u(per ? FO : CD, nx_), bx_t(per ? FO : (adiabatic ? CN : CD), nx_), bx_p(per ? FO : CHEB, nx_),
+        bx_phi(per ? FO : CN, nx_), bx_f(per ? FO : CHEB, nx_), by_u(CD, ny_), by_p(CHEB, ny_), by_phi(CN, ny_),
+        by_bc(CDBC, ny_), X(false, aspect_), Y(true, 1.0) {
+    if (nx < 8 || ny < 8) throw std::invalid_argument("nx and ny must be >= 8");
+    if (per && (nx % 2) != 0) throw std::invalid_argument("periodic nx must be even");
+    scale[0] = aspect;
+    scale[1] = 1.0;
+    // get_nu / get_ka with height 2 (navier.rs:46-55, 231-232)
+    nu = std::sqrt(pr / (ra / std::pow(scale[1] * 2.0, 3.0)));
+    ka = std::sqrt(1.0 / ((ra / std::pow(scale[1] * 2.0, 3.0)) * pr));
+    HIPCHECK(hipStreamCreate(&st));
+    const int R = std::max(nx, periodic ? 2 * mx() : nx);
+    const size_t full = (size_t)R * ny;
+    for (DBuf* b : {&T, &UX, &UY, &P0, &P1, &Tv, &UXv, &UYv, &P0v, &P1v, &THAT_BC, &DTBCX, &DTBCY, &GXX, &GYY, &THAT,
+                    &UXP, &UYP, &A, &G, &CONV, &CV, &RHS, &DIV, &W1, &W2, &W3, &W4, &W9})
+      b->alloc(full);
+    BC.alloc((size_t)R * 2);
+    BCv.alloc((size_t)nx * ny);
+    RED.alloc(std::max(ny, 8) + 8);
+    // solvers
+    const double cv[2] = {dt * nu / std::pow(scale[0], 2.0), dt * nu / std::pow(scale[1], 2.0)};
+    const double ck[2] = {dt * ka / std::pow(scale[0], 2.0), dt * ka / std::pow(scale[1], 2.0)};
+    const double cp[2] = {1.0 / std::pow(scale[0], 2.0), 1.0 / std::pow(scale[1], 2.0)};
+    const SKind hk = per ? HHOLTZ : ADI;
+    solver[0] = make_solver(hk, bx_u, by_u, cv);
+    solver[1] = make_solver(hk, bx_u, by_u, cv);
+    solver[2] = make_solver(hk, bx_t, by_u, ck);
+    solver[3] = make_solver(POISSON, bx_phi, by_phi, cp);
+    // averaging weights of `field` (its coordinates are not rescaled, navier.rs:500-515)
+    wx_dx = cell_widths(bx_f.coords(), bx_f.fourier());
+    wy_dx = cell_widths(by_p.coords(), false);
+    std::vector<double> x0 = bx_f.coords(), x1 = by_p.coords();
+    len0 = std::fabs(x0.back() - x0[0]);
+    len1 = std::fabs(x1.back() - x1[0]);
+    DX0.upload(wx_dx);
+    DX1.upload(wy_dx);
+    bc_rbc();
+  }
+  ~Navier() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  static std::vector<double> cell_widths(const std::vector<double>& x, bool periodic) {  // field.rs:135-163
+    const int n = (int)x.size();
+    std::vector<double> d(n);
+    if (periodic) {
+      for (int i = 0; i < n; ++i) d[i] = x[2] - x[1];
+      return d;
+    }
+    for (int i = 0; i < n; ++i) {
+      double l = i == 0 ? x[0] : (x[i] + x[i - 1]) / 2.0;
+      double r = i == n - 1 ? x[n - 1] : (x[i + 1] + x[i]) / 2.0;
+      d[i] = r - l;
+    }
+    return d;
+  }
+
+  // -------------------------------------------------------------- axis application
+  // x (axis 0): in [rows_in, cols] -> out [rows_out, cols]; identity returns `in`.
+  const double* xop(const Basis& b, const std::string& name, const double* in, int cols, double* out, double* tmp) {
+    if (b.k == FO) {
+      const int m = b.m;
+      if (name == "S" || name == "FO") return in;
+      if (name == "DS") { fourier_diff(st, in, out, m, cols, cols, 1, scale[0]); return out; }
+      if (name == "DDS") { fourier_diff(st, in, out, m, cols, cols, 2, scale[0] * scale[0]); return out; }
+      if (name == "BDS") {
+        fourier_diff(st, in, tmp, m, cols, cols, 1, scale[0]);
+        const DOp& op = X.get(b, "BS");
+        gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, tmp, cols, out, cols, 1.0, 0.0);
+        return out;
+      }
+    } else if (b.k == CHEB && (name == "S" || name == "FO")) {
+      return in;
+    }
+    const DOp& op = X.get(b, name);
+    gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 1.0, 0.0);
+    return out;
+  }
+  void xmat(const DOp& op, const double* in, int cols, double* out) {
+    gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 1.0, 0.0);
+  }
+  // y (axis 1): in [rows, cin] -> out [rows, cout]; out = (in Y^T) .* U + beta out
+  const double* yop(const Basis& b, const std::string& name, const double* in, int rows, double* out,
+                    const double* U = nullptr, double beta = 0.0) {
+    if (b.k == CHEB && (name == "S" || name == "FO") && !U && beta == 0.0) return in;
+    const DOp& op = Y.get(b, name);
+    gemm(st, rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, 1.0, beta, U, op.cols);
+    return out;
+  }
+  void ymat(const DOp& op, const double* in, int rows, double* out) {
+    gemm(st, rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, 1.0, 0.0);
+  }
+
+  // ------------------------------------------------------------------ set-up pieces
+  Basis xb_field() const { return bx_f; }
+
+  // bc_rbc (navier.rs:314-332, 474-492), computed in long double on the host
+  void bc_rbc() {
+    Basis xb = bx_f;
+    LMat bc(nx, 2);
+    for (int i = 0; i < nx; ++i) { bc(i, 0) = 0.5L; bc(i, 1) = -0.5L; }
+    LMat vh = periodic ? dft_r2c(nx) * bc : cheb_forward(nx) * bc;
+    // backward then forward of the Space2(x, cheb_dirichlet_bc(ny))
+    LMat v = host_y(by_bc, "BS", host_x(xb, "BS", vh, (ld)scale[0]));
+    LMat vh2 = host_x(xb, "FS", host_y(by_bc, "FS", v), (ld)scale[0]);
+    set_bc_spec(vh2.to_double());
+  }
+
+  // fieldbc.vhat (device layout [rows_ortho(x), 2]) and all derived constants
+  void set_bc_spec(const std::vector<double>& vh) {
+    const int R = rows_ortho(bx_f);
+    if ((int)vh.size() != R * 2) throw std::invalid_argument("fieldbc size mismatch");
+    BC.upload(vh);
+    LMat V(R, 2);
+    for (size_t i = 0; i < V.a.size(); ++i) V.a[i] = vh[i];
+    const ld sx = (ld)scale[0];
+    const Basis& xb = bx_f;
+    // to_ortho (that contribution, navier.rs:740-742)
+    LMat that = host_y(by_bc, "S", V);
+    // gradient [1,0] / [0,1] then field.backward (conv_term.rs:35-41)
+    LMat gx = host_y(by_bc, "S", host_x(xb, "DS", V, sx));
+    LMat gy = host_y(by_bc, "DS", V);
+    Basis by_cheb(CHEB, ny);
+    LMat dxp = host_y(by_cheb, "BS", host_x(xb, "BS", gx, sx));
+    LMat dyp = host_y(by_cheb, "BS", host_x(xb, "BS", gy, sx));
+    // gradient [2,0], [0,2] * dt * ka (navier.rs:665-668)
+    LMat gxx = host_y(by_bc, "S", host_x(xb, "DDS", V, sx));
+    LMat gyy = host_y(by_bc, "DDS", V);
+    std::vector<double> a = gxx.to_double(), b = gyy.to_double();
+    for (auto& v : a) v = (v * dt) * ka;
+    for (auto& v : b) v = (v * dt) * ka;
+    THAT_BC.upload(that.to_double());
+    DTBCX.upload(dxp.to_double());
+    DTBCY.upload(dyp.to_double());
+    GXX.upload(a);
+    GYY.upload(b);
+    // physical fieldbc (Field2::v after bc_rbc's backward)
+    LMat v = host_y(by_bc, "BS", host_x(xb, "BS", V, sx));
+    BCv.upload(v.to_double());
+  }
+
+  // ------------------------------------------------------------------------ step
+  void solve(int which, double* rhs, double* out) {
+    Solver& s = *solver[which];
+    const int my = s.by.m;
+    if (s.kind == ADI) {
+      xmat(s.hx, rhs, ny, W1.p);         // [mx, ny]
+      ymat(s.hy, W1.p, s.hx.rows, out);  // [mx, my]
+      return;
+    }
+    if (!s.has_fb) {  // Fourier x: P_y then the per-wavenumber row systems
+      ymat(s.py, rhs, 2 * s.bx.m, out);
+      fdma_rows(st, out, my, 2 * s.bx.m, my, s.nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+      return;
+    }
+    xmat(s.pf, rhs, ny, W1.p);               // fwd * P_x : [m0, ny]
+    ymat(s.py, W1.p, s.pf.rows, W2.p);       // P_y       : [m0, my]
+    fdma_rows(st, W2.p, my, s.pf.rows, my, s.nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+    xmat(s.pb, W2.p, my, out);               // bwd       : [m0, my]
+  }
+
+  // convective term of q with x-basis bxq (y basis cheb_dirichlet), into CONV (physical)
+  void conv_pair(const Basis& bxq, const double* q) {
+    const int my = by_u.m;
+    const double* a = xop(bxq, "BDS", q, my, W1.p, W9.p);  // d/dx, x-backward  [nx, my]
+    yop(by_u, "BS", a, nx, CONV.p, UXP.p, 0.0);             // y-backward .* ux
+    const double* b = xop(bxq, "BS", q, my, W1.p, W9.p);
+    yop(by_u, "BDS", b, nx, CONV.p, UYP.p, 1.0);            // d/dy .* uy, accumulate
+  }
+  void forward_dealias_conv() {  // CONV [nx, ny] -> CV [rows_ortho, ny]
+    yop(by_p, "Fdl", CONV.p, nx, W2.p);
+    xop(bx_f, "Fdl", W2.p, ny, CV.p, W9.p);
+  }
+  void to_ortho(const Basis& bx, const Basis& by, const double* q, double* out) {
+    const double* a = xop(bx, "S", q, by.m, W1.p, W9.p);
+    const double* r = yop(by, "S", a, rows_ortho(bx), out);
+    if (r != out) HIPCHECK(hipMemcpyAsync(out, r, sizeof(double) * rows_ortho(bx) * by.n, hipMemcpyDeviceToDevice, st));
+  }
+
+  void solve_vel(int which, double* U, const double* buoy) {  // navier.rs:622-654
+    const long no = (long)rows_ortho(bx_f) * ny;
+    to_ortho(bx_u, by_u, U, A.p);
+    if (which == 0) xop(bx_p, "DS", P0.p, ny, G.p, W9.p);   // d/dx pres
+    else yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p);      // d/dy pres
+    conv_pair(bx_u, U);
+    forward_dealias_conv();
+    ew_rhs_vel(st, RHS.p, A.p, G.p, buoy, CV.p, dt, no);
+    solve(which, RHS.p, U);
+  }
+  void divergence(double* out) {  // navier.rs:698-703
+    const int my = by_u.m, ro = rows_ortho(bx_u);
+    const double* a = xop(bx_u, "DS", UX.p, my, W1.p, W9.p);
+    yop(by_u, "S", a, ro, out);
+    const double* b = xop(bx_u, "S", UY.p, my, W1.p, W9.p);
+    yop(by_u, "DS", b, ro, out, nullptr, 1.0);
+  }
+  void project_velocity() {  // navier.rs:683-695 with c = 1
+    const int myp = by_phi.m, ro = rows_ortho(bx_phi);
+    const long ns = (long)rows_spec(bx_u) * by_u.m;
+    // ux
+    const double* a = xop(bx_phi, "DS", P1.p, myp, W1.p, W9.p);
+    const double* g = yop(by_phi, "S", a, ro, W2.p);
+    const double* f = xop(bx_u, "FO", g, ny, W3.p, W9.p);
+    yop(by_u, "FO", f, rows_spec(bx_u), W4.p);
+    ew_project(st, UX.p, W4.p, 1.0, ns);
+    // uy
+    a = xop(bx_phi, "S", P1.p, myp, W1.p, W9.p);
+    g = yop(by_phi, "DS", a, ro, W2.p);
+    f = xop(bx_u, "FO", g, ny, W3.p, W9.p);
+    yop(by_u, "FO", f, rows_spec(bx_u), W4.p);
+    ew_project(st, UY.p, W4.p, 1.0, ns);
+  }
+  void solve_pres() {  // navier.rs:710-715
+    solve(3, DIV.p, P1.p);
+    if (periodic) set_zero_entries(st, P1.p, 2, 0, (long)bx_phi.m * by_phi.m);
+    else set_zero_entries(st, P1.p, 1, 0, 0);
+  }
+  void update_pres() {  // navier.rs:717-721
+    const long no = (long)rows_ortho(bx_p) * ny;
+    to_ortho(bx_phi, by_phi, P1.p, W3.p);
+    ew_update_pres(st, P0.p, DIV.p, W3.p, nu, 1.0 / dt, no);
+  }
+  void solve_temp() {  // navier.rs:660-674
+    const long no = (long)rows_ortho(bx_f) * ny;
+    const long np = (long)nx * ny;
+    to_ortho(bx_t, by_u, T.p, A.p);
+    conv_pair(bx_t, T.p);
+    ew_acc_prod(st, CONV.p, UXP.p, DTBCX.p, np);
+    ew_acc_prod(st, CONV.p, UYP.p, DTBCY.p, np);
+    forward_dealias_conv();
+    ew_rhs_temp(st, RHS.p, A.p, GXX.p, GYY.p, CV.p, dt, no);
+    solve(2, RHS.p, T.p);
+  }
+  void backward_vel(const double* U, double* out) {
+    const double* a = xop(bx_u, "BS", U, by_u.m, W1.p, W9.p);
+    yop(by_u, "BS", a, nx, out);
+  }
+  void update() {  // navier.rs:737-765
+    const long no = (long)rows_ortho(bx_f) * ny;
+    to_ortho(bx_t, by_u, T.p, W3.p);
+    ew_add(st, THAT.p, W3.p, THAT_BC.p, no);
+    backward_vel(UX.p, UXP.p);
+    backward_vel(UY.p, UYP.p);
+    solve_vel(0, UX.p, nullptr);
+    solve_vel(1, UY.p, THAT.p);
+    divergence(DIV.p);
+    solve_pres();
+    project_velocity();
+    update_pres();
+    solve_temp();
+    time += dt;
+  }
+
+  // ----------------------------------------------------------------- field access
+  struct FieldRef {
+    DBuf* spec;
+    DBuf* phys;
+    Basis bx, by;
+  };
+  FieldRef field(int which) {
+    switch (which) {
+      case RBC_TEMP: return {&T, &Tv, bx_t, by_u};
+      case RBC_UX: return {&UX, &UXv, bx_u, by_u};
+      case RBC_UY: return {&UY, &UYv, bx_u, by_u};
+      case RBC_PRES: return {&P0, &P0v, bx_p, by_p};
+      case RBC_PSEU: return {&P1, &P1v, bx_phi, by_phi};
+      case RBC_FIELDBC: return {&BC, &BCv, bx_f, by_bc};
+    }
+    throw std::invalid_argument("unknown field index");
+  }
+  void shape(int which, int space, int& n0, int& n1, int& cplx) {
+    FieldRef f = field(which);
+    if (space == RBC_PHYS) { n0 = nx; n1 = ny; cplx = 0; return; }
+    n0 = f.bx.m;
+    n1 = f.by.m;
+    cplx = periodic ? 1 : 0;
+  }
+  void forward(int which) {  // Space2::forward: axis 1 then axis 0
+    FieldRef f = field(which);
+    const double* a = yop(f.by, "FS", f.phys->p, nx, W1.p);
+    const double* r = xop(f.bx, "FS", a, f.by.m, W2.p, W9.p);
+    HIPCHECK(hipMemcpyAsync(f.spec->p, r, sizeof(double) * rows_spec(f.bx) * f.by.m, hipMemcpyDeviceToDevice, st));
+  }
+  void backward(int which) {  // Space2::backward: axis 0 then axis 1
+    FieldRef f = field(which);
+    const double* a = xop(f.bx, "BS", f.spec->p, f.by.m, W1.p, W9.p);
+    const double* r = yop(f.by, "BS", a, nx, W2.p);
+    HIPCHECK(hipMemcpyAsync(f.phys->p, r, sizeof(double) * nx * ny, hipMemcpyDeviceToDevice, st));
+  }
+  void get(int which, int space, double* host) {
+    int n0, n1, c;
+    shape(which, space, n0, n1, c);
+    FieldRef f = field(which);
+    HIPCHECK(hipStreamSynchronize(st));
+    if (space == RBC_PHYS || !c) {
+      std::vector<double> v = (space == RBC_PHYS ? f.phys : f.spec)->download((size_t)n0 * n1);
+      std::memcpy(host, v.data(), v.size() * sizeof(double));
+      return;
+    }
+    std::vector<double> v = f.spec->download((size_t)2 * n0 * n1);
+    const size_t pl = (size_t)n0 * n1;
+    for (size_t i = 0; i < pl; ++i) { host[2 * i] = v[i]; host[2 * i + 1] = v[pl + i]; }
+  }
+  void set(int which, int space, const double* host) {
+    int n0, n1, c;
+    shape(which, space, n0, n1, c);
+    FieldRef f = field(which);
+    HIPCHECK(hipStreamSynchronize(st));
+    std::vector<double> v((size_t)(c ? 2 : 1) * n0 * n1);
+    if (space == RBC_PHYS || !c) {
+      std::memcpy(v.data(), host, v.size() * sizeof(double));
+    } else {
+      const size_t pl = (size_t)n0 * n1;
+      for (size_t i = 0; i < pl; ++i) { v[i] = host[2 * i]; v[pl + i] = host[2 * i + 1]; }
+    }
+    if (which == RBC_FIELDBC && space == RBC_SPEC) { set_bc_spec(v); return; }
+    HIPCHECK(hipMemcpy((space == RBC_PHYS ? f.phys : f.spec)->p, v.data(), v.size() * sizeof(double),
+                       hipMemcpyHostToDevice));
+  }
+
+  // set_velocity / set_temperature (navier.rs:927-937, 1035-1076): host trig, device forward
+  void apply_trig(int which, double amp, double m, double n, bool sin_cos) {
+    std::vector<double> x = field(which).bx.coords(), y = field(which).by.coords();
+    for (auto& v : x) v *= scale[0];
+    for (auto& v : y) v *= scale[1];
+    std::vector<double> xs(x.size()), ys(y.size());
+    for (size_t i = 0; i < x.size(); ++i) xs[i] = (x[i] - x[0]) / (x.back() - x[0]);
+    for (size_t j = 0; j < y.size(); ++j) ys[j] = (y[j] - y[0]) / (y.back() - y[0]);
+    const double ax = M_PI * m, ay = M_PI * n;
+    std::vector<double> v((size_t)nx * ny);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        v[(size_t)i * ny + j] = sin_cos ? amp * std::sin(ax * xs[i]) * std::cos(ay * ys[j])
+                                        : amp * std::cos(ax * xs[i]) * std::sin(ay * ys[j]);
+    set(which, RBC_PHYS, v.data());
+    forward(which);
+  }
+
+  // -------------------------------------------------------------- diagnostics
+  // field.backward of an ortho-space array (Space2(x field basis, chebyshev)) into W2
+  const double* field_backward(const double* ortho) {
+    const double* a = xop(bx_f, "BS", ortho, ny, W1.p, W9.p);
+    return yop(by_p, "BS", a, nx, W2.p);
+  }
+  void temp_total_ortho(double* out) {
+    const long no = (long)rows_ortho(bx_f) * ny;
+    to_ortho(bx_t, by_u, T.p, W3.p);
+    ew_add(st, out, W3.p, THAT_BC.p, no);
+  }
+  double eval_nu() {  // functions.rs:12-36
+    const long no = (long)rows_ortho(bx_f) * ny;
+    temp_total_ortho(A.p);
+    yop(by_p, "DS", A.p, rows_ortho(bx_f), G.p);
+    ew_scale(st, G.p, G.p, -1.0, no);
+    ew_scale(st, G.p, G.p, 1.0 / (scale[1] / 2.0), no);
+    const double* v = field_backward(G.p);
+    col_weighted_sum(st, v, nx, ny, DX0.p, RED.p);
+    std::vector<double> avg = RED.download(ny);
+    const double a0 = avg[0] / len0, a1 = avg[ny - 1] / len0;
+    return (a1 + a0) / 2.0;
+  }
+  double eval_nuvol() {  // functions.rs:42-75
+    const long no = (long)rows_ortho(bx_f) * ny, np = (long)nx * ny;
+    temp_total_ortho(A.p);
+    const double* tv = field_backward(A.p);
+    HIPCHECK(hipMemcpyAsync(CONV.p, tv, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    backward_vel(UY.p, UYv.p);
+    yop(by_p, "DS", A.p, rows_ortho(bx_f), G.p);
+    ew_scale(st, G.p, G.p, 1.0 / (scale[1] * -1.0), no);
+    const double* dv = field_backward(G.p);
+    ew_nuvol(st, CV.p, dv, CONV.p, UYv.p, ka, scale[1], np);
+    return weighted_average(CV.p);
+  }
+  double eval_re() {  // functions.rs:82-101
+    const long np = (long)nx * ny;
+    backward_vel(UX.p, UXv.p);
+    backward_vel(UY.p, UYv.p);
+    ew_speed(st, CV.p, UXv.p, UYv.p, 2.0 * scale[1] / nu, np);
+    return weighted_average(CV.p);
+  }
+  double weighted_average(const double* v) {
+    col_weighted_sum(st, v, nx, ny, DX0.p, RED.p);
+    std::vector<double> s = RED.download(ny);
+    double tot = 0.0;
+    for (int j = 0; j < ny; ++j) tot += (s[j] / len0) * wy_dx[j] / len1;
+    return tot;
+  }
+  double div_norm() {
+    divergence(G.p);
+    const long no = (long)rows_ortho(bx_u) * ny;
+    dot_and_norm(st, nullptr, nullptr, 0, G.p, no, RED.p);
+    std::vector<double> r = RED.download(2);
+    return std::sqrt(r[1]);
+  }
+};
+
+// ----------------------------------------------------------------- funspace Space2
+class Space {
+ public:
+  Basis b0, b1;
+  bool cplx;
+  hipStream_t st = nullptr;
+  OpCache X, Y;
+  DBuf W1, W2, W3, W9;
+  Space(int k0, int n0, int k1, int n1) : b0((Kind)k0, n0), b1((Kind)k1, n1), cplx(k0 == FO), X(false, 1.0), Y(true, 1.0) {
+    if (k1 == FO) throw std::invalid_argument("axis 1 must be a Chebyshev basis");
+    if (n0 < 6 || n1 < 6) throw std::invalid_argument("sizes must be >= 6");
+    HIPCHECK(hipStreamCreate(&st));
+  }
+  ~Space() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+  int r_phys() const { return b0.n; }
+  int r_spec() const { return cplx ? 2 * b0.m : b0.m; }
+  int r_ortho() const { return cplx ? 2 * b0.m : b0.n; }
+  void ensure(size_t n) {
+    if (W1.n < n) { W1.alloc(n); W2.alloc(n); W3.alloc(n); W9.alloc(n); }
+  }
+  // x-op with the scale used for gradients
+  const double* xop(const std::string& name, const double* in, int cols, double* out, double sc) {
+    if (b0.k == FO) {
+      if (name == "S" || name == "FO") return in;
+      if (name == "DS" || name == "DDS") {
+        fourier_diff(st, in, out, b0.m, cols, cols, name == "DS" ? 1 : 2, sc);
+        return out;
+      }
+    } else if (b0.k == CHEB && (name == "S" || name == "FO")) {
+      return in;
+    }
+    const DOp& op = X.get(b0, name);
+    gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 1.0, 0.0);
+    return out;
+  }
+  const double* yop(const std::string& name, const double* in, int rows, double* out) {
+    if (b1.k == CHEB && (name == "S" || name == "FO")) return in;
+    const DOp& op = Y.get(b1, name);
+    gemm(st, rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, 1.0, 0.0);
+    return out;
+  }
+  // generic 2-D application on device interleaved/real buffers
+  void run(int op, const double* in, double* out, int batch, int d0, int d1, double s0, double s1) {
+    const size_t big = (size_t)std::max(r_ortho(), b0.n) * b1.n + 16;
+    ensure(big);
+    const bool in_c = cplx && op != 0, out_c = cplx && op != 1;
+    int in_r, in_c1, out_r, out_c1;
+    switch (op) {
+      case 0: in_r = b0.n; in_c1 = b1.n; out_r = r_spec(); out_c1 = b1.m; break;
+      case 1: in_r = r_spec(); in_c1 = b1.m; out_r = b0.n; out_c1 = b1.n; break;
+      case 2: in_r = r_spec(); in_c1 = b1.m; out_r = r_ortho(); out_c1 = b1.n; break;
+      case 3: in_r = r_ortho(); in_c1 = b1.n; out_r = r_spec(); out_c1 = b1.m; break;
+      default: in_r = r_spec(); in_c1 = b1.m; out_r = r_ortho(); out_c1 = b1.n; break;
+    }
+    const size_t in_sz = (size_t)in_r * in_c1, out_sz = (size_t)out_r * out_c1;
+    for (int b = 0; b < batch; ++b) {
+      const double* src = in + b * in_sz;
+      double* dst = out + b * out_sz;
+      const double* x = src;
+      if (in_c) { interleaved_to_split(st, src, W3.p, in_r / 2, in_c1); x = W3.p; }
+      const double* r = nullptr;
+      // gradient scale divides by s0^d0 s1^d1 (space2.rs:256-262); folded into the x part
+      const double sc = std::pow(s0, d0) * std::pow(s1, d1);
+      if (op == 0) {  // forward: axis 1 then axis 0
+        const double* a = yop("FS", x, b0.n, W1.p);
+        r = xop("FS", a, b1.m, W2.p, 1.0);
+      } else if (op == 1) {  // backward: axis 0 then axis 1
+        const double* a = xop("BS", x, b1.m, W1.p, 1.0);
+        r = yop("BS", a, b0.n, W2.p);
+      } else if (op == 2) {
+        const double* a = xop("S", x, b1.m, W1.p, 1.0);
+        r = yop("S", a, r_ortho(), W2.p);
+      } else if (op == 3) {
+        const double* a = xop("FO", x, b1.n, W1.p, 1.0);
+        r = yop("FO", a, r_spec(), W2.p);
+      } else {
+        if (d0 > 2 || d1 > 2) throw std::invalid_argument("derivative order > 2");
+        const char* nx_ = d0 == 0 ? "S" : d0 == 1 ? "DS" : "DDS";
+        const char* ny_ = d1 == 0 ? "S" : d1 == 1 ? "DS" : "DDS";
+        const double* a = xop(nx_, x, b1.m, W1.p, 1.0);
+        r = yop(ny_, a, r_ortho(), W2.p);
+        if (sc != 1.0) {
+          if (r != W2.p) { HIPCHECK(hipMemcpyAsync(W2.p, r, sizeof(double) * r_ortho() * b1.n, hipMemcpyDeviceToDevice, st)); r = W2.p; }
+          if (b0.k == FO) {
+            // complex division by (sc, 0): (a sc) / sc^2 per component
+            fourier_diff(st, W2.p, W2.p, b0.m, b1.n, b1.n, 0, sc);
+          } else {
+            ew_scale(st, W2.p, W2.p, 1.0 / sc, (long)r_ortho() * b1.n);
+          }
+        }
+      }
+      if (out_c) split_to_interleaved(st, r, dst, out_r / 2, out_c1);
+      else HIPCHECK(hipMemcpyAsync(dst, r, sizeof(double) * out_sz, hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+  }
+};
+
+}  // namespace rbc
+
+// =================================================================== C ABI
+using rbc::Navier;
+using rbc::Space;
+
+struct rbc_navier {
+  std::unique_ptr<Navier> n;
+};
+struct fs_space2 {
+  std::unique_ptr<Space> s;
+};
+
+static thread_local std::string g_err;
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return RBC_OK;
+  } catch (const rbc::DeviceError& e) {
+    g_err = e.what();
+    return RBC_ERR_DEVICE;
+  } catch (const rbc::NumericError& e) {
+    g_err = e.what();
+    return RBC_ERR_NUMERIC;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return RBC_ERR_ARG;
+  } catch (...) {
+    g_err = "unknown error";
+    return RBC_ERR_ARG;
+  }
+}
+
+#define NEED(h)                                        \
+  do {                                                 \
+    if (!(h) || !(h)->n) throw std::invalid_argument("null handle"); \
+  } while (0)
+
+extern "C" {
+
+const char* rbc_last_error(void) { return g_err.c_str(); }
+const char* rbc_version(void) { return "rustpde_amd 0.1 (gfx950, fp64)"; }
+
+int rbc_create_confined(int nx, int ny, double ra, double pr, double dt, double aspect, int adiabatic,
+                        rbc_navier** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null out");
+    auto* h = new rbc_navier();
+    try {
+      h->n = std::make_unique<Navier>(false, nx, ny, ra, pr, dt, aspect, adiabatic != 0);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int rbc_create_periodic(int nx, int ny, double ra, double pr, double dt, double aspect, rbc_navier** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null out");
+    auto* h = new rbc_navier();
+    try {
+      h->n = std::make_unique<Navier>(true, nx, ny, ra, pr, dt, aspect, true);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int rbc_destroy(rbc_navier* h) {
+  return guarded([&] { delete h; });
+}
+int rbc_field_shape(rbc_navier* h, int which, int space, int* n0, int* n1, int* is_complex) {
+  return guarded([&] {
+    NEED(h);
+    int a, b, c;
+    h->n->shape(which, space, a, b, c);
+    if (n0) *n0 = a;
+    if (n1) *n1 = b;
+    if (is_complex) *is_complex = c;
+  });
+}
+int rbc_get_field(rbc_navier* h, int which, int space, double* out) {
+  return guarded([&] {
+    NEED(h);
+    if (!out) throw std::invalid_argument("null buffer");
+    h->n->get(which, space, out);
+  });
+}
+int rbc_set_field(rbc_navier* h, int which, int space, const double* in) {
+  return guarded([&] {
+    NEED(h);
+    if (!in) throw std::invalid_argument("null buffer");
+    h->n->set(which, space, in);
+  });
+}
+int rbc_forward(rbc_navier* h, int which) {
+  return guarded([&] {
+    NEED(h);
+    h->n->forward(which);
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+  });
+}
+int rbc_backward(rbc_navier* h, int which) {
+  return guarded([&] {
+    NEED(h);
+    h->n->backward(which);
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+  });
+}
+int rbc_set_temp_bc(rbc_navier* h, const double* vhat) {
+  return guarded([&] {
+    NEED(h);
+    if (!vhat) throw std::invalid_argument("null buffer");
+    h->n->set(RBC_FIELDBC, RBC_SPEC, vhat);
+  });
+}
+int rbc_set_velocity(rbc_navier* h, double amp, double m, double n) {
+  return guarded([&] {
+    NEED(h);
+    h->n->apply_trig(RBC_UX, amp, m, n, true);
+    h->n->apply_trig(RBC_UY, -amp, m, n, false);
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+  });
+}
+int rbc_set_temperature(rbc_navier* h, double amp, double m, double n) {
+  return guarded([&] {
+    NEED(h);
+    h->n->apply_trig(RBC_TEMP, -amp, m, n, false);
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+  });
+}
+int rbc_update(rbc_navier* h, int nsteps) {
+  return guarded([&] {
+    NEED(h);
+    if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
+    for (int i = 0; i < nsteps; ++i) h->n->update();
+  });
+}
+int rbc_synchronize(rbc_navier* h) {
+  return guarded([&] {
+    NEED(h);
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+  });
+}
+double rbc_get_time(rbc_navier* h) { return (h && h->n) ? h->n->time : NAN; }
+double rbc_get_dt(rbc_navier* h) { return (h && h->n) ? h->n->dt : NAN; }
+int rbc_set_time(rbc_navier* h, double t) {
+  return guarded([&] {
+    NEED(h);
+    h->n->time = t;
+  });
+}
+int rbc_eval(rbc_navier* h, int what, double* out) {
+  return guarded([&] {
+    NEED(h);
+    if (!out) throw std::invalid_argument("null out");
+    Navier& n = *h->n;
+    switch (what) {
+      case RBC_NU: *out = n.eval_nu(); break;
+      case RBC_NUVOL: *out = n.eval_nuvol(); break;
+      case RBC_RE: *out = n.eval_re(); break;
+      case RBC_DIV: *out = n.div_norm(); break;
+      default: throw std::invalid_argument("unknown diagnostic");
+    }
+  });
+}
+int rbc_exit(rbc_navier* h, int* out) {
+  return guarded([&] {
+    NEED(h);
+    if (!out) throw std::invalid_argument("null out");
+    *out = std::isnan(h->n->div_norm()) ? 1 : 0;
+  });
+}
+int rbc_tensor_info(rbc_navier* h, int s, int* n, int* has_fb) {
+  return guarded([&] {
+    NEED(h);
+    if (s < 0 || s > 3) throw std::invalid_argument("solver index");
+    const rbc::Solver& sv = *h->n->solver[s];
+    if (sv.kind == rbc::ADI) { *n = 0; *has_fb = 0; return; }
+    *n = (int)sv.lam.size();
+    *has_fb = sv.has_fb ? 1 : 0;
+  });
+}
+int rbc_get_tensor(rbc_navier* h, int s, double* lam, double* fwd, double* bwd) {
+  return guarded([&] {
+    NEED(h);
+    if (s < 0 || s > 3) throw std::invalid_argument("solver index");
+    const rbc::Solver& sv = *h->n->solver[s];
+    if (sv.kind == rbc::ADI) throw std::invalid_argument("solver is HholtzAdi (no tensor)");
+    std::memcpy(lam, sv.lam.data(), sv.lam.size() * sizeof(double));
+    if (sv.has_fb) {
+      std::memcpy(fwd, sv.fwd.data(), sv.fwd.size() * sizeof(double));
+      std::memcpy(bwd, sv.bwd.data(), sv.bwd.size() * sizeof(double));
+    }
+  });
+}
+int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, const double* bwd) {
+  return guarded([&] {
+    NEED(h);
+    if (s < 0 || s > 3) throw std::invalid_argument("solver index");
+    rbc::Solver& sv = *h->n->solver[s];
+    if (sv.kind == rbc::ADI) throw std::invalid_argument("solver is HholtzAdi (no tensor)");
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+    std::memcpy(sv.lam.data(), lam, sv.lam.size() * sizeof(double));
+    if (sv.has_fb) {
+      if (!fwd || !bwd) throw std::invalid_argument("fwd/bwd required");
+      std::memcpy(sv.fwd.data(), fwd, sv.fwd.size() * sizeof(double));
+      std::memcpy(sv.bwd.data(), bwd, sv.bwd.size() * sizeof(double));
+    }
+    sv.build_rows();
+  });
+}
+
+int fs_space2_create(int kind0, int n0, int kind1, int n1, fs_space2** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null out");
+    if (kind0 < 0 || kind0 > 4 || kind1 < 0 || kind1 > 4) throw std::invalid_argument("unknown base kind");
+    auto* s = new fs_space2();
+    try {
+      s->s = std::make_unique<Space>(kind0, n0, kind1, n1);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+int fs_space2_destroy(fs_space2* sp) {
+  return guarded([&] { delete sp; });
+}
+int fs_space2_shape(fs_space2* sp, int* p0, int* p1, int* s0, int* s1, int* o0, int* o1, int* is_complex) {
+  return guarded([&] {
+    if (!sp || !sp->s) throw std::invalid_argument("null space");
+    Space& s = *sp->s;
+    *p0 = s.b0.n; *p1 = s.b1.n;
+    *s0 = s.b0.m; *s1 = s.b1.m;
+    *o0 = s.b0.ortho(); *o1 = s.b1.ortho();
+    *is_complex = s.cplx ? 1 : 0;
+  });
+}
+#define FS_OP(name, code)                                                                    \
+  int name(fs_space2* sp, const double* in, double* out, int batch) {                        \
+    return guarded([&] {                                                                     \
+      if (!sp || !sp->s || !in || !out || batch < 1) throw std::invalid_argument("bad argument"); \
+      sp->s->run(code, in, out, batch, 0, 0, 1.0, 1.0);                                      \
+    });                                                                                      \
+  }
+FS_OP(fs_forward, 0)
+FS_OP(fs_backward, 1)
+FS_OP(fs_to_ortho, 2)
+FS_OP(fs_from_ortho, 3)
+int fs_gradient(fs_space2* sp, const double* in, double* out, int d0, int d1, double scale0, double scale1, int batch) {
+  return guarded([&] {
+    if (!sp || !sp->s || !in || !out || batch < 1 || d0 < 0 || d1 < 0) throw std::invalid_argument("bad argument");
+    sp->s->run(4, in, out, batch, d0, d1, scale0, scale1);
+  });
+}
+
+int rbc_dev_malloc(void** out, unsigned long long bytes) {
+  return guarded([&] { HIPCHECK(hipMalloc(out, bytes)); });
+}
+int rbc_dev_free(void* p) {
+  return guarded([&] { HIPCHECK(hipFree(p)); });
+}
+int rbc_memcpy_h2d(void* dst, const void* src, unsigned long long bytes) {
+  return guarded([&] { HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int rbc_memcpy_d2h(void* dst, const void* src, unsigned long long bytes) {
+  return guarded([&] { HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+// HIP kernels of the RBC step for gfx950 (CDNA4, wave64).  See ops.hpp.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "ops.hpp"
+
+namespace rbc {
+
+static void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------------------------- GEMM
+// Register-blocked FP64 GEMM: a BM x BN output tile per workgroup, each thread owns a
+// TM x TN strided sub-tile (rows ty + i*BM/TM, cols tx + j*BN/TN) so that stores of a
+// wavefront are contiguous along N.  A is staged transposed in LDS (As[k][m]).
+template <int BM, int BN, int BK, int TM, int TN>
+__global__ __launch_bounds__((BM / TM) * (BN / TN)) void gemm_kernel(
+    int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb,
+    double* __restrict__ C, int ldc, double alpha, double beta, const double* __restrict__ U, int ldu) {
+  constexpr int TX = BN / TN, TY = BM / TM, NT = TX * TY;
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN];
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  double acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = 0.0;
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    for (int e = tid; e < BM * BK; e += NT) {
+      const int mm = e / BK, kk = e % BK;
+      const int gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
+    }
+    for (int e = tid; e < BK * BN; e += NT) {
+      const int kk = e / BN, nn = e % BN;
+      const int gk = k0 + kk, gn = n0 + nn;
+      Bs[kk][nn] = (gk < K && gn < N) ? B[(size_t)gk * ldb + gn] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; ++kk) {
+      double a[TM], b[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) a[i] = As[kk][ty + i * TY];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) b[j] = Bs[kk][tx + j * TX];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int gm = m0 + ty + i * TY;
+    if (gm >= M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int gn = n0 + tx + j * TX;
+      if (gn >= N) continue;
+      double v = alpha * acc[i][j];
+      if (U) v = v * U[(size_t)gm * ldu + gn];
+      if (beta != 0.0) v = v + beta * C[(size_t)gm * ldc + gn];
+      C[(size_t)gm * ldc + gn] = v;
+    }
+  }
+}
+
+void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+          double alpha, double beta, const double* U, int ldu) {
+  if (M <= 0 || N <= 0) return;
+  if (K <= 0) throw std::runtime_error("gemm: K must be positive");
+  if ((long)M * N >= 256L * 64 * 64) {
+    dim3 grid(blocks_for(N, 64), blocks_for(M, 64));
+    hipLaunchKernelGGL((gemm_kernel<64, 64, 16, 4, 4>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
+                       beta, U, ldu);
+  } else {
+    dim3 grid(blocks_for(N, 32), blocks_for(M, 32));
+    hipLaunchKernelGGL((gemm_kernel<32, 32, 16, 2, 2>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
+                       beta, U, ldu);
+  }
+  check_launch("gemm");
+}
+
+// ------------------------------------------------------------------------ banded rows
+// One thread per (row, parity): the offsets -2, 0, 2, 4 couple only indices of equal
+// parity, so each parity chain repeats exactly the operations of fdma.rs:101-118.
+__global__ void fdma_rows_kernel(double* __restrict__ X, int ld, int nrows, int m, int nsets,
+                                 const double* __restrict__ low, const double* __restrict__ dia,
+                                 const double* __restrict__ up1, const double* __restrict__ up2) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * nrows) return;
+  const int r = t >> 1, p = t & 1;
+  const int set = r % nsets;
+  double* x = X + (size_t)r * ld;
+  const double* L = low + (size_t)set * m;
+  const double* D = dia + (size_t)set * m;
+  const double* U1 = up1 + (size_t)set * m;
+  const double* U2 = up2 + (size_t)set * m;
+  for (int i = 2 + p; i < m; i += 2) x[i] = x[i] - x[i - 2] * L[i - 2];
+  int i = m - 1;
+  if (((i - p) & 1) != 0) --i;
+  for (; i >= 0; i -= 2) {
+    double v = x[i];
+    if (i < m - 2) v = v - x[i + 2] * U1[i];
+    if (i < m - 4) v = v - x[i + 4] * U2[i];
+    x[i] = v / D[i];
+  }
+}
+
+void fdma_rows(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low, const double* dia,
+               const double* up1, const double* up2) {
+  if (m < 5) throw std::runtime_error("fdma_rows: system too small");
+  const long nt = 2L * nrows;
+  hipLaunchKernelGGL(fdma_rows_kernel, dim3(blocks_for(nt, 64)), dim3(64), 0, s, X, ld, nrows, m, nsets, low, dia, up1,
+                     up2);
+  check_launch("fdma_rows");
+}
+
+// ------------------------------------------------------------------ Fourier derivative
+__global__ void fourier_diff_kernel(const double* __restrict__ X, double* __restrict__ Y, int m, int cols, int ld,
+                                    int times, double sc) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)m * cols) return;
+  const int k = (int)(t / cols), j = (int)(t % cols);
+  double a = X[(size_t)k * ld + j], b = X[(size_t)(m + k) * ld + j];
+  const double kk = (double)k;
+  for (int q = 0; q < times; ++q) {  // (a + ib) * (0 + ik) as num-complex
+    const double na = a * 0.0 - b * kk;
+    const double nb = a * kk + b * 0.0;
+    a = na;
+    b = nb;
+  }
+  if (sc != 1.0) {  // Complex / Complex(sc, 0)
+    const double ns = sc * sc + 0.0 * 0.0;
+    const double na = (a * sc + b * 0.0) / ns;
+    const double nb = (b * sc - a * 0.0) / ns;
+    a = na;
+    b = nb;
+  }
+  Y[(size_t)k * ld + j] = a;
+  Y[(size_t)(m + k) * ld + j] = b;
+}
+
+void fourier_diff(hipStream_t s, const double* X, double* Y, int m, int cols, int ld, int times, double sc) {
+  const long n = (long)m * cols;
+  hipLaunchKernelGGL(fourier_diff_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, X, Y, m, cols, ld, times, sc);
+  check_launch("fourier_diff");
+}
+
+// ----------------------------------------------------------------------- element-wise
+#define EW_KERNEL(name, params, body)                                     \
+  __global__ void name##_kernel params {                                 \
+    _Pragma("clang fp contract(off)") const long i = (long)blockIdx.x * blockDim.x + threadIdx.x; \
+    if (i >= n) return;                                                   \
+    body;                                                                 \
+  }
+
+EW_KERNEL(ew_add, (double* out, const double* a, const double* b, long n), out[i] = a[i] + b[i])
+EW_KERNEL(ew_rhs_vel, (double* out, const double* a, const double* g, const double* buoy, const double* c, double dt, long n),
+          {
+            double r = a[i] - g[i] * dt;
+            if (buoy) r = r + buoy[i] * dt;
+            out[i] = r - c[i] * dt;
+          })
+EW_KERNEL(ew_rhs_temp, (double* out, const double* a, const double* bx, const double* by, const double* c, double dt, long n),
+          out[i] = ((a[i] + bx[i]) + by[i]) - c[i] * dt)
+EW_KERNEL(ew_acc_prod, (double* out, const double* a, const double* b, long n), out[i] = out[i] + a[i] * b[i])
+EW_KERNEL(ew_project, (double* u, const double* f, double c, long n), u[i] = f[i] * (-c) + u[i])
+EW_KERNEL(ew_update_pres, (double* p, const double* d, const double* phi, double nu, double inv_dt, long n),
+          {
+            double q = p[i] - d[i] * nu;
+            p[i] = q + phi[i] * inv_dt;
+          })
+EW_KERNEL(ew_scale, (double* out, const double* a, double f, long n), out[i] = a[i] * f)
+EW_KERNEL(ew_speed, (double* out, const double* a, const double* b, double f, long n),
+          out[i] = sqrt(a[i] * a[i] + b[i] * b[i]) * f)
+EW_KERNEL(ew_nuvol, (double* out, const double* d, const double* t, const double* uy, double ka, double sy, long n),
+          {
+            double uyt = t[i] * uy[i];
+            out[i] = ((d[i] + uyt / ka) * 2.0) * sy;
+          })
+
+#define EW_LAUNCH(name, n, ...)                                                                          \
+  do {                                                                                                  \
+    if ((n) > 0) {                                                                                      \
+      hipLaunchKernelGGL(name##_kernel, dim3(blocks_for((n), 256)), dim3(256), 0, s, __VA_ARGS__, (n)); \
+      check_launch(#name);                                                                              \
+    }                                                                                                   \
+  } while (0)
+
+void ew_add(hipStream_t s, double* out, const double* a, const double* b, long n) { EW_LAUNCH(ew_add, n, out, a, b); }
+void ew_rhs_vel(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
+                double dt, long n) {
+  EW_LAUNCH(ew_rhs_vel, n, out, a, g, buoy, c, dt);
+}
+void ew_rhs_temp(hipStream_t s, double* out, const double* a, const double* bx, const double* by, const double* c,
+                 double dt, long n) {
+  EW_LAUNCH(ew_rhs_temp, n, out, a, bx, by, c, dt);
+}
+void ew_acc_prod(hipStream_t s, double* out, const double* a, const double* b, long n) {
+  EW_LAUNCH(ew_acc_prod, n, out, a, b);
+}
+void ew_project(hipStream_t s, double* u, const double* f, double c, long n) { EW_LAUNCH(ew_project, n, u, f, c); }
+void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt, long n) {
+  EW_LAUNCH(ew_update_pres, n, p, div, phi, nu, inv_dt);
+}
+void ew_scale(hipStream_t s, double* out, const double* a, double f, long n) { EW_LAUNCH(ew_scale, n, out, a, f); }
+void ew_speed(hipStream_t s, double* out, const double* a, const double* b, double f, long n) {
+  EW_LAUNCH(ew_speed, n, out, a, b, f);
+}
+void ew_nuvol(hipStream_t s, double* out, const double* d, const double* t, const double* uy, double ka, double sy,
+              long n) {
+  EW_LAUNCH(ew_nuvol, n, out, d, t, uy, ka, sy);
+}
+
+__global__ void set_zero_kernel(double* x, int n_idx, long i0, long i1) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    x[i0] = 0.0;
+    if (n_idx > 1) x[i1] = 0.0;
+  }
+}
+void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1) {
+  hipLaunchKernelGGL(set_zero_kernel, dim3(1), dim3(64), 0, s, x, n_idx, i0, i1);
+  check_launch("set_zero_entries");
+}
+
+// ------------------------------------------------------------------------- reductions
+__global__ void col_weighted_sum_kernel(const double* __restrict__ v, int n0, int n1, const double* __restrict__ w,
+                                        double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n1) return;
+  double s = 0.0;
+  for (int i = 0; i < n0; ++i) s = s + v[(size_t)i * n1 + j] * w[i];
+  out[j] = s;
+}
+void col_weighted_sum(hipStream_t s, const double* v, int n0, int n1, const double* w, double* out) {
+  hipLaunchKernelGGL(col_weighted_sum_kernel, dim3(blocks_for(n1, 64)), dim3(64), 0, s, v, n0, n1, w, out);
+  check_launch("col_weighted_sum");
+}
+
+__global__ void dot_and_norm_kernel(const double* __restrict__ a, const double* __restrict__ w, int n,
+                                    const double* __restrict__ b, long nb, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  if (a)
+    for (int i = t; i < n; i += 256) s += a[i] * w[i];
+  red[t] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[0] = red[0];
+  __syncthreads();
+  double q = 0.0;
+  if (b)
+    for (long i = t; i < nb; i += 256) q += b[i] * b[i];
+  red[t] = q;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[1] = red[0];
+}
+void dot_and_norm(hipStream_t s, const double* a, const double* w, int n, const double* b, long nb, double* out) {
+  hipLaunchKernelGGL(dot_and_norm_kernel, dim3(1), dim3(256), 0, s, a, w, n, b, nb, out);
+  check_launch("dot_and_norm");
+}
+
+// --------------------------------------------------------------------- layout changes
+__global__ void i2s_kernel(const double* __restrict__ in, double* __restrict__ out, int m, int c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)m * c) return;
+  out[t] = in[2 * t];
+  out[(long)m * c + t] = in[2 * t + 1];
+}
+__global__ void s2i_kernel(const double* __restrict__ in, double* __restrict__ out, int m, int c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)m * c) return;
+  out[2 * t] = in[t];
+  out[2 * t + 1] = in[(long)m * c + t];
+}
+void interleaved_to_split(hipStream_t s, const double* in, double* out, int m, int c) {
+  hipLaunchKernelGGL(i2s_kernel, dim3(blocks_for((long)m * c, 256)), dim3(256), 0, s, in, out, m, c);
+  check_launch("interleaved_to_split");
+}
+void split_to_interleaved(hipStream_t s, const double* in, double* out, int m, int c) {
+  hipLaunchKernelGGL(s2i_kernel, dim3(blocks_for((long)m * c, 256)), dim3(256), 0, s, in, out, m, c);
+  check_launch("split_to_interleaved");
+}
+
+}  // namespace rbc

@@ -1,0 +1,49 @@
+// Launch wrappers of the HIP kernels of the RBC step (ops.hip).  All device arrays are
+// FP64 and row-major.  Complex spectral fields are stored as split planes: a complex
+// [m, c] array is the real array [2m, c] whose first m rows hold the real parts.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace rbc {
+
+// C[M,N] = alpha * (A[M,K] B[K,N]) (.* U if U) + beta * C
+void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
+          int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
+
+// Banded 4-diagonal solves (offsets -2, 0, 2, 4; pre-swept as fdma.rs:73-82) of every
+// row r of X[nrows, m] with diagonal set (r % nsets); diag arrays are [nsets, m].
+void fdma_rows(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low,
+               const double* dia, const double* up1, const double* up2);
+
+// Fourier derivative on split planes: X[2m, cols] -> Y, (i k)^times / sc^times
+// (r2c.rs:88-99; num-complex division by (sc, 0) as space2.rs:256-262).
+void fourier_diff(hipStream_t s, const double* X, double* Y, int m, int cols, int ld, int times, double sc);
+
+// Element-wise combinations of the step (n = number of doubles, all contiguous)
+void ew_add(hipStream_t s, double* out, const double* a, const double* b, long n);          // a + b
+void ew_rhs_vel(hipStream_t s, double* out, const double* a, const double* g, const double* buoy,
+                const double* c, double dt, long n);  // ((a - g dt) [+ buoy dt]) - c dt
+void ew_rhs_temp(hipStream_t s, double* out, const double* a, const double* bx, const double* by,
+                 const double* c, double dt, long n);  // ((a + bx) + by) - c dt
+void ew_acc_prod(hipStream_t s, double* out, const double* a, const double* b, long n);     // out + a b
+void ew_project(hipStream_t s, double* u, const double* f, double c, long n);              // f (-c) + u
+void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt,
+                    long n);  // (p - div nu) + phi inv_dt
+void ew_scale(hipStream_t s, double* out, const double* a, double f, long n);              // a f
+void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1);              // x[i0] = x[i1] = 0
+
+// Weighted column sums: out[j] = sum_i v[i, j] * w[i]  (i sequential, as average.rs:25-34)
+void col_weighted_sum(hipStream_t s, const double* v, int n0, int n1, const double* w, double* out);
+// out[0] = sum_j a[j] * w[j] (single block), out[1] = sum of squares of b[0..nb) if b
+void dot_and_norm(hipStream_t s, const double* a, const double* w, int n, const double* b, long nb, double* out);
+// v = sqrt(a^2 + b^2) * f
+void ew_speed(hipStream_t s, double* out, const double* a, const double* b, double f, long n);
+// v = ((d + uyt / ka) * 2) * sy   with uyt = t * uy
+void ew_nuvol(hipStream_t s, double* out, const double* d, const double* t, const double* uy, double ka, double sy,
+              long n);
+
+// complex layout conversion: interleaved (re, im) [m, c] <-> split planes [2m, c]
+void interleaved_to_split(hipStream_t s, const double* in, double* out, int m, int c);
+void split_to_interleaved(hipStream_t s, const double* in, double* out, int m, int c);
+
+}  // namespace rbc
