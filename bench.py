@@ -1,0 +1,164 @@
+"""Benchmark of Navier2D::update on one MI355X (one JSON line on stdout).
+
+Workload (default): the large periodic case, Navier2D::new_periodic(1024, 1025, Ra=1e9,
+Pr=1, dt=2e-4, aspect=2), temperature = conductive profile + seeded N(0, 1e-3^2) noise,
+velocity zero.  A step is one Integrate::update (navier.rs:737-765); only update() is
+timed, as benches/benchmark_navier.rs:34 does.  Inputs are resident on the GPU when the
+timed region starts.
+
+value      : steps/s of the FP64 GPU step over the timed region (K steps)
+roofline   : the kernel class with the largest summed time inside the timed region,
+             timed with HIP events on the handle's stream; achieved = algorithmic work
+             per launch / average launch duration
+cpu_baseline: the CPU oracle (FP64 restatement of the reference algorithm) on the host,
+             a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Navier2D steps/sec on 1 MI355X: mixed vs FP64 GPU at stated field/Nu error"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+CONFIGS = {
+    # name: (periodic, nx, ny, ra, pr, dt, aspect, noise kind, amplitude, velocity)
+    "c5": (True, 1024, 1025, 1e9, 1.0, 2e-4, 2.0, "gaussian", 1e-3, None),
+    "c3": (True, 256, 257, 1e7, 1.0, 2e-3, 2.0, "gaussian", 1e-3, None),
+    "c1": (False, 64, 64, 1e5, 1.0, 2e-2, 1.0, "uniform", 0.05, (0.2, 1.0, 1.0)),
+}
+WORKLOAD_NAMES = {
+    "c5": "Navier2D periodic RBC 1024x1025 Ra1e9 Pr1 aspect2 dt2e-4 (FP64)",
+    "c3": "Navier2D periodic RBC 256x257 Ra1e7 Pr1 aspect2 dt2e-3 (FP64)",
+    "c1": "Navier2D confined RBC 64x64 Ra1e5 Pr1 aspect1 dt2e-2 adiabatic (FP64)",
+}
+KERNEL_CLASSES = {0: "gemm_kernel", 1: "fdma_rows_kernel", 2: "fourier_diff_kernel", 3: "elementwise"}
+
+
+def make(cfg, lib_mod):
+    from rustpde_amd import Navier2D, seeded
+    periodic, nx, ny, ra, pr, dt, aspect, kind, amp, vel = CONFIGS[cfg]
+    if periodic:
+        n = lib_mod.Navier2D.new_periodic(nx, ny, ra, pr, dt, aspect)
+    else:
+        n = lib_mod.Navier2D.new(nx, ny, ra, pr, dt, aspect, True)
+    pert = seeded.interior_perturbation(nx, ny, amp, 5, kind=kind, periodic=periodic)
+    if vel is not None:
+        n.set_velocity(*vel)
+    if isinstance(n, Navier2D):
+        n.temp.v = pert
+        n.temp.forward()
+    else:
+        n.set_phys_forward("temp", pert)
+    return n
+
+
+def cpu_baseline(cfg, budget_s):
+    from oracle import oracle as O
+    if not os.path.exists(O._LIB_PATH):
+        O.build()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(threads)
+
+    class _M:
+        Navier2D = O.Navier2D
+
+    o = make(cfg, _M)
+    t0 = time.perf_counter()
+    o.update(1)
+    one = time.perf_counter() - t0
+    steps = max(2, min(200, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.update(steps)
+    el = time.perf_counter() - t0
+    return {"value": steps / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "%d steps of the same workload after 1 warm step (oracle/rbc_oracle.cpp, OpenMP over lanes)"
+                      % steps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import rustpde_amd
+    from rustpde_amd import _lib
+    L = _lib.load()
+    periodic, nx, ny = CONFIGS[args.config][:3]
+
+    nav = make(args.config, rustpde_amd)
+    nav.update(args.warmup)
+    nav.synchronize()
+
+    # timed region, kernel classes timed live with HIP events on the handle's stream
+    _lib.check(L.rbc_timing_enable(nav._h, 1))
+    nav.synchronize()
+    t0 = time.perf_counter()
+    nav.update(args.steps)
+    nav.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = {}
+    for k in KERNEL_CLASSES:
+        n, ms, w = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+        _lib.check(L.rbc_timing_read(nav._h, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(w)))
+        stats[k] = (n.value, ms.value, w.value)
+    _lib.check(L.rbc_timing_enable(nav._h, 0))
+
+    # same region without instrumentation, as a check of the event overhead
+    t0 = time.perf_counter()
+    nav.update(args.steps)
+    nav.synchronize()
+    elapsed_clean = time.perf_counter() - t0
+    if nav.exit():
+        print("bench.py: divergence is NaN after the timed steps", file=sys.stderr)
+        sys.exit(3)
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    dom = max(stats, key=lambda k: stats[k][1])
+    n, ms, w = stats[dom]
+    avg_ms = ms / max(n, 1)
+    if dom == 0:
+        achieved = (w / n) / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
+    else:
+        achieved = (w / n) / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None}
+    roof.update({"kernel": KERNEL_CLASSES[dom], "launches": n, "avg_launch_us": 1e3 * avg_ms,
+                 "work_per_launch": w / n, "share_of_step": ms / (1e3 * elapsed)})
+
+    value = args.steps / elapsed
+    out = {
+        "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": WORKLOAD_NAMES[args.config], "nx": nx, "ny": ny, "precision": "fp64",
+                   "parallelism": "single-gpu"},
+        "grid_point_updates_per_s": value * nx * ny,
+        "ms_per_step_uninstrumented": 1e3 * elapsed_clean / args.steps,
+        "kernel_classes": {KERNEL_CLASSES[k]: {"launches": v[0], "ms": v[1]} for k, v in stats.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -40,6 +40,9 @@ SIGNATURES = {
     "rbc_tensor_info": (_i, [_vp, _i, _ip, _ip]),
     "rbc_get_tensor": (_i, [_vp, _i, _dp, _dp, _dp]),
     "rbc_set_tensor": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "rbc_set_option": (_i, [_vp, _i, _i]),
+    "rbc_timing_enable": (_i, [_vp, _i]),
+    "rbc_timing_read": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong), _dp, _dp]),
     "fs_space2_create": (_i, [_i, _i, _i, _i, ctypes.POINTER(_vp)]),
     "fs_space2_destroy": (_i, [_vp]),
     "fs_space2_shape": (_i, [_vp, _ip, _ip, _ip, _ip, _ip, _ip, _ip]),

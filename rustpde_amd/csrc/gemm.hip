@@ -1,0 +1,109 @@
+// FP64 GEMM on the CDNA4 matrix cores: C = alpha (A B) [.* U] + beta C, row-major.
+//
+// v_mfma_f64_16x16x4f64 operand layout (wave64): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result register r of lane l holds D[4 r + l / 16][l % 16].
+// A workgroup of 4 wavefronts computes a BM x BN tile (2 x 2 wavefronts, each a
+// (BM/2) x (BN/2) sub-tile of 16 x 16 MFMA blocks).  K is staged through LDS in BK-deep
+// slabs; the next slab is fetched into registers while the current one is consumed.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "ops.hpp"
+
+namespace rbc {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, const double* __restrict__ A, int lda,
+                                                        const double* __restrict__ B, int ldb, double* __restrict__ C,
+                                                        int ldc, double alpha, double beta,
+                                                        const double* __restrict__ U, int ldu) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int NA = BM * BK / 256, NB = BK * BN / 256;
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int lr = lane & 15, lk = lane >> 4;
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  double ra[NA], rb[NB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+      const int e = tid + q * 256, mm = e / BK, kk = e % BK, gm = m0 + mm, gk = k0 + kk;
+      ra[q] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + q * 256, kk = e / BN, nn = e % BN, gk = k0 + kk, gn = n0 + nn;
+      rb[q] = (gk < K && gn < N) ? B[(size_t)gk * ldb + gn] : 0.0;
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+      const int e = tid + q * 256;
+      As[e % BK][e / BK] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + q * 256;
+      Bs[e / BN][e % BN] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + BK < K) gload(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = As[kk + lk][wm0 + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = Bs[kk + lk][wn0 + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm0 + i * 16 + 4 * r + lk, gn = n0 + wn0 + j * 16 + lr;
+        if (gm < M && gn < N) {
+          double v = alpha * acc[i][j][r];
+          if (U) v = v * U[(size_t)gm * ldu + gn];
+          if (beta != 0.0) v = v + beta * C[(size_t)gm * ldc + gn];
+          C[(size_t)gm * ldc + gn] = v;
+        }
+      }
+}
+
+}  // namespace
+
+void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
+               int ldc, double alpha, double beta, const double* U, int ldu) {
+  if (M <= 0 || N <= 0) return;
+  if (K <= 0) throw std::runtime_error("gemm_mfma: K must be positive");
+  dim3 grid((N + 63) / 64, (M + 63) / 64);
+  hipLaunchKernelGGL((gemm_mfma_kernel<64, 64, 16>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
+                     beta, U, ldu);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw std::runtime_error(std::string("gemm_mfma: ") + hipGetErrorString(e));
+}
+
+}  // namespace rbc

@@ -1,0 +1,205 @@
+// Lane kernels of the RBC step: banded row solves and banded / stencil operators.
+//
+// fdma_rows_wave: one wavefront per row of X[nrows, m]; the offsets -2, 0, 2, 4 of the
+// 4-diagonal systems (fdma.rs:101-118) couple only indices of equal parity, so a row is
+// two independent chains.  Every lane owns the same contiguous slice of both chains.  A
+// pass is a linear recurrence: each lane composes the affine map of its slice, the maps
+// are scanned across the wavefront with shuffles, and each lane then replays its slice
+// with the reference's arithmetic from its incoming state.
+//
+// band_apply: out_i = sum of up to four taps at offsets 0, -2, +2, +4 (in that order of
+// summation) along axis 1 (rows) or axis 0 (columns); covers the composite stencil
+// to_ortho (composite_stencil.rs:207-229) and the banded matvec MatVecFdma
+// (matvec.rs:172-193).
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "ops.hpp"
+
+namespace rbc {
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int ROWS_PER_BLOCK = 4;
+constexpr int SLICE = 16;  // max elements per lane per parity (rows up to 2048)
+
+void check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+__global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
+    double* __restrict__ X, int ld, int nrows, int m, int nsets, const double* __restrict__ low,
+    const double* __restrict__ dia, const double* __restrict__ up1, const double* __restrict__ up2) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int r = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE);
+  if (r >= nrows) return;  // uniform per wavefront
+  const int set = r % nsets;
+  double* x = X + (size_t)r * ld;
+  const double* Lw = low + (size_t)set * m;
+  const double* D = dia + (size_t)set * m;
+  const double* U1 = up1 + (size_t)set * m;
+  const double* U2 = up2 + (size_t)set * m;
+  const int len0 = (m + 1) / 2;            // chain length of parity 0 (>= parity 1)
+  const int per = (len0 + WAVE - 1) / WAVE;  // slice length, <= SLICE
+  const int t0 = lane * per;
+  double v[2][SLICE];
+  int len[2] = {len0, m / 2};
+  int cnt[2];
+  for (int p = 0; p < 2; ++p) {
+    cnt[p] = max(0, min(per, len[p] - t0));
+    for (int q = 0; q < SLICE; ++q) {
+      v[p][q] = 0.0;
+      if (q < cnt[p]) v[p][q] = x[p + 2 * (t0 + q)];
+    }
+  }
+  // ---- forward elimination: y_t = x_t - y_{t-1} low[i-2]  (i = p + 2t >= 2)
+  for (int p = 0; p < 2; ++p) {
+    double A = 1.0, B = 0.0;  // slice map y_out = A y_in + B
+    for (int q = 0; q < SLICE; ++q) {
+      if (q >= cnt[p]) break;
+      const int t = t0 + q, i = p + 2 * t;
+      if (t == 0) { A = 0.0; B = v[p][q]; continue; }
+      const double l = Lw[i - 2];
+      A = -A * l;
+      B = v[p][q] - B * l;
+    }
+    double pa = A, pb = B;  // inclusive scan, lower lanes first
+    for (int off = 1; off < WAVE; off <<= 1) {
+      const double qa = __shfl_up(pa, off), qb = __shfl_up(pb, off);
+      if (lane >= off) { pb = pa * qb + pb; pa = pa * qa; }
+    }
+    double y = __shfl_up(pb, 1);
+    if (lane == 0) y = 0.0;
+    for (int q = 0; q < SLICE; ++q) {
+      if (q >= cnt[p]) break;
+      const int t = t0 + q, i = p + 2 * t;
+      y = (t == 0) ? v[p][q] : v[p][q] - y * Lw[i - 2];
+      v[p][q] = y;
+    }
+  }
+  // ---- back substitution: x_t = (y_t - x_{t+1} up1[i] - x_{t+2} up2[i]) / dia[i]
+  for (int p = 0; p < 2; ++p) {
+    // slice map on the state s = (x_{t+1}, x_{t+2}):  s' = M s + w
+    double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0, w0 = 0.0, w1 = 0.0;
+    for (int q = SLICE - 1; q >= 0; --q) {
+      if (q >= cnt[p]) continue;
+      const int t = t0 + q, i = p + 2 * t;
+      const double d = D[i];
+      const double u1 = (t + 1 < len[p]) ? U1[i] : 0.0;
+      const double u2 = (t + 2 < len[p]) ? U2[i] : 0.0;
+      const double c0 = v[p][q] / d, c1 = -u1 / d, c2 = -u2 / d;
+      // row 0 of the new map: c1 * row0 + c2 * row1 ; row 1: old row 0
+      const double n00 = c1 * m00 + c2 * m10, n01 = c1 * m01 + c2 * m11, nw0 = c1 * w0 + c2 * w1 + c0;
+      m10 = m00; m11 = m01; w1 = w0;
+      m00 = n00; m01 = n01; w0 = nw0;
+    }
+    // suffix scan: higher lanes are applied first
+    double a00 = m00, a01 = m01, a10 = m10, a11 = m11, b0 = w0, b1 = w1;
+    for (int off = 1; off < WAVE; off <<= 1) {
+      const double q00 = __shfl_down(a00, off), q01 = __shfl_down(a01, off), q10 = __shfl_down(a10, off),
+                   q11 = __shfl_down(a11, off), r0 = __shfl_down(b0, off), r1 = __shfl_down(b1, off);
+      if (lane + off < WAVE) {  // composite = this o q
+        const double n00 = a00 * q00 + a01 * q10, n01 = a00 * q01 + a01 * q11;
+        const double n10 = a10 * q00 + a11 * q10, n11 = a10 * q01 + a11 * q11;
+        const double nb0 = a00 * r0 + a01 * r1 + b0, nb1 = a10 * r0 + a11 * r1 + b1;
+        a00 = n00; a01 = n01; a10 = n10; a11 = n11; b0 = nb0; b1 = nb1;
+      }
+    }
+    double s0 = __shfl_down(b0, 1), s1 = __shfl_down(b1, 1);
+    if (lane == WAVE - 1) { s0 = 0.0; s1 = 0.0; }
+    for (int q = SLICE - 1; q >= 0; --q) {
+      if (q >= cnt[p]) continue;
+      const int t = t0 + q, i = p + 2 * t;
+      double val = v[p][q];
+      if (t + 1 < len[p]) val = val - s0 * U1[i];
+      if (t + 2 < len[p]) val = val - s1 * U2[i];
+      val = val / D[i];
+      s1 = s0;
+      s0 = val;
+      v[p][q] = val;
+    }
+  }
+  for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < SLICE; ++q)
+      if (q < cnt[p]) x[p + 2 * (t0 + q)] = v[p][q];
+}
+
+// taps: coef[k * mo + i] for k = 0..3 at input offsets {0, -2, +2, +4}; mask bit k set if
+// tap k is present at output index i
+__global__ void band_y_kernel(const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows,
+                              int mo, const double* __restrict__ coef, const unsigned char* __restrict__ mask,
+                              double beta) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)rows * mo) return;
+  const int r = (int)(t / mo), i = (int)(t % mo);
+  const unsigned mk = mask[i];
+  const double* a = in + (size_t)r * ldi;
+  const int off[4] = {0, -2, 2, 4};
+  double acc = 0.0;
+  bool first = true;
+  for (int k = 0; k < 4; ++k) {
+    if (!(mk & (1u << k))) continue;
+    const double term = a[i + off[k]] * coef[k * mo + i];
+    acc = first ? term : acc + term;
+    first = false;
+  }
+  double* o = out + (size_t)r * ldo + i;
+  *o = (beta != 0.0) ? acc + beta * (*o) : acc;
+}
+
+__global__ void band_x_kernel(const double* __restrict__ in, double* __restrict__ out, int cols, int mo,
+                              const double* __restrict__ coef, const unsigned char* __restrict__ mask, double beta) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)cols * mo) return;
+  const int i = (int)(t / cols), j = (int)(t % cols);
+  const unsigned mk = mask[i];
+  const int off[4] = {0, -2, 2, 4};
+  double acc = 0.0;
+  bool first = true;
+  for (int k = 0; k < 4; ++k) {
+    if (!(mk & (1u << k))) continue;
+    const double term = in[(size_t)(i + off[k]) * cols + j] * coef[k * mo + i];
+    acc = first ? term : acc + term;
+    first = false;
+  }
+  double* o = out + (size_t)i * cols + j;
+  *o = (beta != 0.0) ? acc + beta * (*o) : acc;
+}
+
+}  // namespace
+
+bool fdma_rows_wave_supported(int m) { return m >= 5 && (m + 1) / 2 <= WAVE * SLICE; }
+
+void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low, const double* dia,
+                    const double* up1, const double* up2) {
+  if (!fdma_rows_wave_supported(m)) throw std::invalid_argument("fdma_rows_wave: unsupported row length");
+  const unsigned blocks = (unsigned)((nrows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  hipLaunchKernelGGL(fdma_rows_wave_kernel, dim3(blocks), dim3(WAVE * ROWS_PER_BLOCK), 0, s, X, ld, nrows, m, nsets,
+                     low, dia, up1, up2);
+  check("fdma_rows_wave");
+}
+
+void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int mo, const double* coef,
+            const unsigned char* mask, double beta) {
+  const long n = (long)rows * mo;
+  hipLaunchKernelGGL(band_y_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, ldi, out, ldo, rows, mo,
+                     coef, mask, beta);
+  check("band_y");
+}
+
+void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
+            const unsigned char* mask, double beta) {
+  const long n = (long)cols * mo;
+  hipLaunchKernelGGL(band_x_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, cols, mo, coef, mask,
+                     beta);
+  check("band_x");
+}
+
+}  // namespace rbc

@@ -72,52 +72,66 @@ struct DOp {
   int rows = 0, cols = 0;
 };
 
+struct DBytes {
+  unsigned char* p = nullptr;
+  DBytes() = default;
+  DBytes(const DBytes&) = delete;
+  DBytes& operator=(const DBytes&) = delete;
+  ~DBytes() { if (p) (void)hipFree(p); }
+  void upload(const std::vector<unsigned char>& v) {
+    if (p) (void)hipFree(p);
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(v.size(), 1)));
+    HIPCHECK(hipMemcpy(p, v.data(), v.size(), hipMemcpyHostToDevice));
+  }
+};
+
+// Banded operator with taps at offsets {0, -2, +2, +4} (see band_y / band_x)
+struct DBand {
+  DBuf coef;
+  DBytes mask;
+  int mo = 0, mi = 0;
+};
+
+// to_ortho stencil of a composite basis (composite_stencil.rs:207-229): n outputs from m inputs
+static void make_stencil_band(const Basis& b, DBand& band) {
+  const int n = b.n, m = b.m;
+  std::vector<double> c(4 * (size_t)n, 0.0);
+  std::vector<unsigned char> mk(n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (i < m) { c[i] = b.diag[i]; mk[i] |= 1; }
+    if (i >= 2 && i - 2 < m) { c[(size_t)n + i] = b.low2[i - 2]; mk[i] |= 2; }
+  }
+  band.mo = n;
+  band.mi = m;
+  band.coef.upload(c);
+  band.mask.upload(mk);
+}
+
+// MatVecFdma of an (m x n) matrix (matvec.rs:142-193): taps as the reference's conditions
+static void make_matvec_band(const LMat& a, DBand& band) {
+  const int m = a.r;
+  std::vector<double> c(4 * (size_t)m, 0.0);
+  std::vector<unsigned char> mk(m, 0);
+  for (int i = 0; i < m; ++i) {
+    c[i] = (double)a(i, i);
+    mk[i] |= 1;
+    if (i > 1) { c[(size_t)m + i] = (double)a(i, i - 2); mk[i] |= 2; }
+    if (i < m - 2) { c[2 * (size_t)m + i] = (double)a(i, i + 2); mk[i] |= 4; }
+    if (i < m - 4) { c[3 * (size_t)m + i] = (double)a(i, i + 4); mk[i] |= 8; }
+  }
+  band.mo = m;
+  band.mi = a.c;
+  band.coef.upload(c);
+  band.mask.upload(mk);
+}
+
 static ld ipow(ld x, int e) {
   ld r = 1.0L;
   for (int i = 0; i < e; ++i) r *= x;
   return r;
 }
 
-// Dense 1-D operator `name` of basis b along an axis of scale `sc`.
-//   S    to_ortho                      (composite_stencil.rs:207-229)
-//   BS   backward                      (composite.rs:430-443 ; r2c.rs:211-226)
-//   BDS  d/dx then backward            (conv_term.rs:35-38)
-//   DS   d/dx (to ortho)               (space2.rs:247-264)
-//   DDS  d2/dx2 (to ortho)
-//   FO   from_ortho                    (composite_stencil.rs:250-276)
-//   FS   forward                       (composite.rs:383-396 ; r2c.rs:160-175)
-//   Fdl  forward then 2/3 dealias      (navier.rs:562-568, 1022-1032)
-static LMat build_axis_op(const Basis& b, const std::string& name, ld sc) {
-  const int n = b.n;
-  if (b.k == FO) {
-    if (name == "BS") return dft_c2r(n);
-    if (name == "FS") return dft_r2c(n);
-    if (name == "Fdl") {
-      LMat a = dft_r2c(n);
-      const int cut = b.m * 2 / 3;
-      for (int k = cut; k < b.m; ++k)
-        for (int j = 0; j < n; ++j) a(k, j) = a(b.m + k, j) = 0.0L;
-      return a;
-    }
-    throw std::runtime_error("no dense Fourier operator " + name);
-  }
-  LMat S = stencil(b);
-  if (name == "S") return S;
-  if (name == "BS") return cheb_backward(n) * S;
-  if (name == "BDS") return scaled(cheb_backward(n) * cheb_diff(n, 1) * S, 1.0L / sc);
-  if (name == "DS") return scaled(cheb_diff(n, 1) * S, 1.0L / sc);
-  if (name == "DDS") return scaled(cheb_diff(n, 2) * S, 1.0L / (sc * sc));
-  if (name == "FO") return from_ortho(b);
-  if (name == "FS") return from_ortho(b) * cheb_forward(n);
-  if (name == "Fdl") {
-    LMat f = cheb_forward(n);
-    const int cut = n * 2 / 3;
-    for (int k = cut; k < n; ++k)
-      for (int j = 0; j < n; ++j) f(k, j) = 0.0L;
-    return f;
-  }
-  throw std::runtime_error("unknown operator " + name);
-}
+static LMat build_axis_op(const Basis& b, const std::string& name, ld sc) { return axis_operator(b, name, sc); }
 
 // Host application of axis operators in long double (set-up of constant fields).
 static LMat host_fourier_diff(const LMat& in, int m, int times, ld sc) {
@@ -192,6 +206,7 @@ struct Solver {
   std::vector<double> lam, fwd, bwd;  // fwd/bwd m x m (row-major) when has_fb
   std::vector<double> px;             // P_x (m x n) in double, for composing fwd * P_x
   DOp pf, pb, py;
+  DBand pyb;  // P_y as a banded matvec
   DBuf low, dia, up1, up2;
   int nsets = 0, m = 0;
   Band4 fa, fc;
@@ -284,9 +299,65 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
   s->fc = Band4::from_matrix(C1, m1);
   if (g0.has_p) s->px = to_vec(g0.p);
   upload_op(s->py, g1.p.t());  // MatVecFdma(precond) along y as a right GEMM
+  make_matvec_band(g1.p, s->pyb);
   s->build_rows();
   return s;
 }
+
+// ------------------------------------------------------------------ kernel timing
+// HIP events recorded on the handle's stream around each launch of a kernel class, with
+// the launch's algorithmic work (flops for GEMMs, bytes for memory-bound kernels).
+class KernelTimer {
+ public:
+  static constexpr int NCAT = 4;  // RBC_K_GEMM, RBC_K_ROWSOLVE, RBC_K_FFT, RBC_K_ELEMENTWISE
+  bool on = false;
+  struct Rec { int cat; double work; hipEvent_t a, b; };
+  ~KernelTimer() { clear(); for (auto e : pool_) (void)hipEventDestroy(e); }
+  void begin(hipStream_t st, int cat, double work) {
+    if (!on) return;
+    Rec r{cat, work, get(), get()};
+    HIPCHECK(hipEventRecord(r.a, st));
+    recs_.push_back(r);
+  }
+  void end(hipStream_t st) {
+    if (!on) return;
+    HIPCHECK(hipEventRecord(recs_.back().b, st));
+  }
+  // accumulate finished records; returns per category (launches, ms, work)
+  void collect() {
+    for (auto& r : recs_) {
+      float ms = 0.f;
+      HIPCHECK(hipEventSynchronize(r.b));
+      HIPCHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      n_[r.cat] += 1;
+      ms_[r.cat] += ms;
+      work_[r.cat] += r.work;
+      pool_.push_back(r.a);
+      pool_.push_back(r.b);
+    }
+    recs_.clear();
+  }
+  void read(int cat, long long& n, double& ms, double& work) {
+    collect();
+    n = n_[cat];
+    ms = ms_[cat];
+    work = work_[cat];
+  }
+  void reset() { collect(); for (int i = 0; i < NCAT; ++i) { n_[i] = 0; ms_[i] = 0; work_[i] = 0; } }
+  void clear() { recs_.clear(); }
+
+ private:
+  hipEvent_t get() {
+    if (!pool_.empty()) { hipEvent_t e = pool_.back(); pool_.pop_back(); return e; }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> pool_;
+  long long n_[NCAT] = {0, 0, 0, 0};
+  double ms_[NCAT] = {0, 0, 0, 0}, work_[NCAT] = {0, 0, 0, 0};
+};
 
 // ----------------------------------------------------------------------- Navier2D
 class Navier {
@@ -309,6 +380,91 @@ class Navier {
   std::vector<double> wx_dx, wy_dx;  // cell widths of `field` (unscaled, field.rs:135-163)
   double len0 = 0, len1 = 0;
   DBuf DX0, DX1;
+  KernelTimer timer;
+  bool use_fft = true;
+  bool use_mfma = true;   // FP64 GEMMs on the matrix cores
+  bool use_graph = true;  // replay update() as a captured HIP graph
+  hipGraphExec_t gexec = nullptr;
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
+  // Integrate::update nsteps times: one eager step, then a captured step replayed
+  void update_n(int nsteps) {
+    int done = 0;
+    if (use_graph && !timer.on && nsteps > 1 && !gexec) {
+      update();
+      ++done;
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const double t_saved = time;
+      try {
+        update();
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        time = t_saved;
+        throw;
+      }
+      time = t_saved;
+      HIPCHECK(hipStreamEndCapture(st, &g));
+      hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) { gexec = nullptr; throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    }
+    for (; done < nsteps; ++done) {
+      if (use_graph && !timer.on && gexec) {
+        HIPCHECK(hipGraphLaunch(gexec, st));
+        time += dt;
+      } else {
+        update();
+      }
+    }
+  }  // Fourier x transforms by the FFT kernels (dense DFT GEMMs otherwise)
+
+  // instrumented launches
+  void GEMM(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc, double beta = 0.0,
+         const double* U = nullptr, int ldu = 0) {
+    timer.begin(st, 0, 2.0 * M * N * K);
+    if (use_mfma) gemm_mfma(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
+    else gemm(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
+    timer.end(st);
+  }
+  void ROWS(double* X, int ld, int nrows, int m, int nsets, const Solver& s) {
+    // bytes: the rows read and written once, the 4 diagonals of every set read once
+    timer.begin(st, 1, 8.0 * (2.0 * nrows * m + 4.0 * (double)nsets * m));
+    if (fdma_rows_wave_supported(m)) fdma_rows_wave(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+    else fdma_rows(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+    timer.end(st);
+  }
+  void BANDY(const DBand& b, const double* in, int ldi, double* out, int ldo, int rows, double beta) {
+    timer.begin(st, 3, 8.0 * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
+    band_y(st, in, ldi, out, ldo, rows, b.mo, b.coef.p, b.mask.p, beta);
+    timer.end(st);
+  }
+  void BANDX(const DBand& b, const double* in, double* out, int cols, double beta) {
+    timer.begin(st, 3, 8.0 * (double)cols * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
+    band_x(st, in, out, cols, b.mo, b.coef.p, b.mask.p, beta);
+    timer.end(st);
+  }
+  // cached stencil bands per basis
+  std::map<std::string, std::unique_ptr<DBand>> bands;
+  const DBand& stencil_band(const Basis& b) {
+    std::string key = std::to_string((int)b.k) + ":" + std::to_string(b.n);
+    auto it = bands.find(key);
+    if (it != bands.end()) return *it->second;
+    auto band = std::make_unique<DBand>();
+    make_stencil_band(b, *band);
+    const DBand& ref = *band;
+    bands[key] = std::move(band);
+    return ref;
+  }
+  bool use_bands = true;
+  void FDIFF(const double* X, double* Y, int m, int cols, int ld, int times, double sc) {
+    timer.begin(st, 2, 8.0 * 4.0 * m * cols);
+    fourier_diff(st, X, Y, m, cols, ld, times, sc);
+    timer.end(st);
+  }
 
   int mx() const { return nx / 2 + 1; }
   int rows_spec(const Basis& b) const { return periodic ? 2 * b.m : b.m; }
@@ -355,6 +511,7 @@ class Navier {
     bc_rbc();
   }
   ~Navier() {
+    drop_graph();
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -376,37 +533,59 @@ class Navier {
   // -------------------------------------------------------------- axis application
   // x (axis 0): in [rows_in, cols] -> out [rows_out, cols]; identity returns `in`.
   const double* xop(const Basis& b, const std::string& name, const double* in, int cols, double* out, double* tmp) {
+    if (b.k == FO && use_fft && fft_supported(b.n)) {  // batched real FFT kernels
+      if (name == "S" || name == "FO") return in;
+      if (name == "DS") { FDIFF(in, out, b.m, cols, cols, 1, scale[0]); return out; }
+      if (name == "DDS") { FDIFF(in, out, b.m, cols, cols, 2, scale[0] * scale[0]); return out; }
+      const double work = 8.0 * cols * ((double)b.n + 2.0 * b.m);
+      timer.begin(st, 2, work);
+      if (name == "BS") irfft_cols(st, in, out, b.n, cols, 0, 1.0);
+      else if (name == "BDS") irfft_cols(st, in, out, b.n, cols, 1, scale[0]);
+      else if (name == "FS") rfft_cols(st, in, out, b.n, cols, b.m);
+      else if (name == "Fdl") rfft_cols(st, in, out, b.n, cols, b.m * 2 / 3);
+      else throw std::runtime_error("no Fourier operator " + name);
+      timer.end(st);
+      return out;
+    }
     if (b.k == FO) {
       const int m = b.m;
       if (name == "S" || name == "FO") return in;
-      if (name == "DS") { fourier_diff(st, in, out, m, cols, cols, 1, scale[0]); return out; }
-      if (name == "DDS") { fourier_diff(st, in, out, m, cols, cols, 2, scale[0] * scale[0]); return out; }
+      if (name == "DS") { FDIFF(in, out, m, cols, cols, 1, scale[0]); return out; }
+      if (name == "DDS") { FDIFF(in, out, m, cols, cols, 2, scale[0] * scale[0]); return out; }
       if (name == "BDS") {
-        fourier_diff(st, in, tmp, m, cols, cols, 1, scale[0]);
+        FDIFF(in, tmp, m, cols, cols, 1, scale[0]);
         const DOp& op = X.get(b, "BS");
-        gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, tmp, cols, out, cols, 1.0, 0.0);
+        GEMM(op.rows, cols, op.cols, op.b.p, op.cols, tmp, cols, out, cols);
         return out;
       }
     } else if (b.k == CHEB && (name == "S" || name == "FO")) {
       return in;
     }
+    if (use_bands && name == "S" && (b.k == CD || b.k == CN)) {
+      BANDX(stencil_band(b), in, out, cols, 0.0);
+      return out;
+    }
     const DOp& op = X.get(b, name);
-    gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 1.0, 0.0);
+    GEMM(op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols);
     return out;
   }
   void xmat(const DOp& op, const double* in, int cols, double* out) {
-    gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 1.0, 0.0);
+    GEMM(op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols);
   }
   // y (axis 1): in [rows, cin] -> out [rows, cout]; out = (in Y^T) .* U + beta out
   const double* yop(const Basis& b, const std::string& name, const double* in, int rows, double* out,
                     const double* U = nullptr, double beta = 0.0) {
     if (b.k == CHEB && (name == "S" || name == "FO") && !U && beta == 0.0) return in;
+    if (use_bands && name == "S" && !U && (b.k == CD || b.k == CN)) {
+      BANDY(stencil_band(b), in, b.m, out, b.n, rows, beta);
+      return out;
+    }
     const DOp& op = Y.get(b, name);
-    gemm(st, rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, 1.0, beta, U, op.cols);
+    GEMM(rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, beta, U, op.cols);
     return out;
   }
   void ymat(const DOp& op, const double* in, int rows, double* out) {
-    gemm(st, rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, 1.0, 0.0);
+    GEMM(rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols);
   }
 
   // ------------------------------------------------------------------ set-up pieces
@@ -426,6 +605,7 @@ class Navier {
 
   // fieldbc.vhat (device layout [rows_ortho(x), 2]) and all derived constants
   void set_bc_spec(const std::vector<double>& vh) {
+    drop_graph();
     const int R = rows_ortho(bx_f);
     if ((int)vh.size() != R * 2) throw std::invalid_argument("fieldbc size mismatch");
     BC.upload(vh);
@@ -467,13 +647,13 @@ class Navier {
       return;
     }
     if (!s.has_fb) {  // Fourier x: P_y then the per-wavenumber row systems
-      ymat(s.py, rhs, 2 * s.bx.m, out);
-      fdma_rows(st, out, my, 2 * s.bx.m, my, s.nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+      BANDY(s.pyb, rhs, ny, out, my, 2 * s.bx.m, 0.0);
+      ROWS(out, my, 2 * s.bx.m, my, s.nsets, s);
       return;
     }
     xmat(s.pf, rhs, ny, W1.p);               // fwd * P_x : [m0, ny]
-    ymat(s.py, W1.p, s.pf.rows, W2.p);       // P_y       : [m0, my]
-    fdma_rows(st, W2.p, my, s.pf.rows, my, s.nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+    BANDY(s.pyb, W1.p, ny, W2.p, my, s.pf.rows, 0.0);  // P_y : [m0, my]
+    ROWS(W2.p, my, s.pf.rows, my, s.nsets, s);
     xmat(s.pb, W2.p, my, out);               // bwd       : [m0, my]
   }
 
@@ -951,7 +1131,7 @@ int rbc_update(rbc_navier* h, int nsteps) {
   return guarded([&] {
     NEED(h);
     if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
-    for (int i = 0; i < nsteps; ++i) h->n->update();
+    h->n->update_n(nsteps);
   });
 }
 int rbc_synchronize(rbc_navier* h) {
@@ -1019,6 +1199,7 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
     rbc::Solver& sv = *h->n->solver[s];
     if (sv.kind == rbc::ADI) throw std::invalid_argument("solver is HholtzAdi (no tensor)");
     HIPCHECK(hipStreamSynchronize(h->n->st));
+    h->n->drop_graph();
     std::memcpy(sv.lam.data(), lam, sv.lam.size() * sizeof(double));
     if (sv.has_fb) {
       if (!fwd || !bwd) throw std::invalid_argument("fwd/bwd required");
@@ -1026,6 +1207,45 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
       std::memcpy(sv.bwd.data(), bwd, sv.bwd.size() * sizeof(double));
     }
     sv.build_rows();
+  });
+}
+
+int rbc_set_option(rbc_navier* h, int option, int value) {
+  return guarded([&] {
+    NEED(h);
+    Navier& n = *h->n;
+    HIPCHECK(hipStreamSynchronize(n.st));
+    n.drop_graph();
+    switch (option) {
+      case RBC_OPT_GRAPH: n.use_graph = value != 0; break;
+      case RBC_OPT_MFMA: n.use_mfma = value != 0; break;
+      case RBC_OPT_FFT: n.use_fft = value != 0; break;
+      case RBC_OPT_BANDS: n.use_bands = value != 0; break;
+      default: throw std::invalid_argument("unknown option");
+    }
+  });
+}
+
+int rbc_timing_enable(rbc_navier* h, int on) {
+  return guarded([&] {
+    NEED(h);
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+    h->n->drop_graph();
+    h->n->timer.reset();
+    h->n->timer.on = on != 0;
+  });
+}
+int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work) {
+  return guarded([&] {
+    NEED(h);
+    if (kernel_class < 0 || kernel_class >= rbc::KernelTimer::NCAT) throw std::invalid_argument("kernel class");
+    HIPCHECK(hipStreamSynchronize(h->n->st));
+    long long n;
+    double ms, w;
+    h->n->timer.read(kernel_class, n, ms, w);
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    if (work) *work = w;
   });
 }
 

@@ -10,10 +10,28 @@ namespace rbc {
 void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
           int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
 
+// Same product on the FP64 matrix cores (v_mfma_f64_16x16x4f64), 64 x 64 tiles.
+void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
+               int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
+
 // Banded 4-diagonal solves (offsets -2, 0, 2, 4; pre-swept as fdma.rs:73-82) of every
 // row r of X[nrows, m] with diagonal set (r % nsets); diag arrays are [nsets, m].
 void fdma_rows(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low,
                const double* dia, const double* up1, const double* up2);
+
+// Same solves, one wavefront per row with a scan over slice maps (rows up to 2048).
+bool fdma_rows_wave_supported(int m);
+void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low,
+                    const double* dia, const double* up1, const double* up2);
+
+// Banded operator with taps at input offsets {0, -2, +2, +4} (summed in that order),
+// coefficients coef[k * mo + i] and presence bits mask[i]; out = op(in) [+ beta out].
+// band_y acts along axis 1 of in[rows, *] (leading dims ldi/ldo), band_x along axis 0
+// of in[*, cols].
+void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int mo, const double* coef,
+            const unsigned char* mask, double beta);
+void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
+            const unsigned char* mask, double beta);
 
 // Fourier derivative on split planes: X[2m, cols] -> Y, (i k)^times / sc^times
 // (r2c.rs:88-99; num-complex division by (sc, 0) as space2.rs:256-262).
@@ -41,6 +59,13 @@ void ew_speed(hipStream_t s, double* out, const double* a, const double* b, doub
 // v = ((d + uyt / ka) * 2) * sy   with uyt = t * uy
 void ew_nuvol(hipStream_t s, double* out, const double* d, const double* t, const double* uy, double ka, double sy,
               long n);
+
+// Batched real FFTs of length n (power of two) along axis 0 of a [n, cols] array:
+// rfft_cols: real -> split planes [2(n/2+1), cols], rows k >= cut set to zero;
+// irfft_cols: split planes -> real, with X_k first multiplied by (i k)^times / sc.
+bool fft_supported(int n);
+void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut);
+void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc);
 
 // complex layout conversion: interleaved (re, im) [m, c] <-> split planes [2m, c]
 void interleaved_to_split(hipStream_t s, const double* in, double* out, int m, int c);

@@ -10,12 +10,17 @@ namespace rbc {
 
 LMat operator*(const LMat& x, const LMat& y) {
   if (x.c != y.r) throw std::runtime_error("LMat product: shape mismatch");
+  // nonzero pattern of y's rows, so banded/sparse factors cost O(nnz) not O(n^3)
+  std::vector<std::vector<int>> nz(y.r);
+  for (int k = 0; k < y.r; ++k)
+    for (int j = 0; j < y.c; ++j)
+      if (y(k, j) != 0.0L) nz[k].push_back(j);
   LMat z(x.r, y.c);
   for (int i = 0; i < x.r; ++i)
     for (int k = 0; k < x.c; ++k) {
-      ld a = x(i, k);
+      const ld a = x(i, k);
       if (a == 0.0L) continue;
-      for (int j = 0; j < y.c; ++j) z(i, j) += a * y(k, j);
+      for (int j : nz[k]) z(i, j) += a * y(k, j);
     }
   return z;
 }
@@ -108,24 +113,83 @@ LMat cheb_forward(int n) {
   return f;
 }
 
-// ortho.rs:107-125 differentiate_lane, applied to unit vectors (exact integers)
-LMat cheb_diff(int n, int times) {
-  LMat d = LMat::eye(n);
-  for (int t = 0; t < times; ++t) {
-    LMat one(n, n);
-    for (int col = 0; col < n; ++col) {
-      std::vector<ld> c(n), dd(n);
-      for (int i = 0; i < n; ++i) c[i] = (i == col) ? 1.0L : 0.0L;
-      dd[0] = c[1];
-      for (int i = 1; i < n - 1; ++i) dd[i] = 2.0L * (ld)(i + 1) * c[i + 1];
-      dd[n - 1] = 0.0L;
-      for (int i = n - 3; i >= 1; --i) dd[i] = dd[i] + dd[i + 2];
-      dd[0] = dd[0] + dd[2] / 2.0L;
-      for (int i = 0; i < n; ++i) one(i, col) = dd[i];
-    }
-    d = one * d;
+// ortho.rs:107-125 differentiate_lane (one lane, long double)
+static void diff_lane(std::vector<ld>& c) {
+  const int n = (int)c.size();
+  std::vector<ld> d(n);
+  d[0] = c[1];
+  for (int i = 1; i < n - 1; ++i) d[i] = 2.0L * (ld)(i + 1) * c[i + 1];
+  d[n - 1] = 0.0L;
+  for (int i = n - 3; i >= 1; --i) d[i] = d[i] + d[i + 2];
+  d[0] = d[0] + d[2] / 2.0L;
+  c.swap(d);
+}
+
+// D^t applied column by column to `m` (n rows)
+static LMat diff_cols(const LMat& m, int times) {
+  LMat out = m;
+  std::vector<ld> col(m.r);
+  for (int j = 0; j < m.c; ++j) {
+    for (int i = 0; i < m.r; ++i) col[i] = m(i, j);
+    for (int t = 0; t < times; ++t) diff_lane(col);
+    for (int i = 0; i < m.r; ++i) out(i, j) = col[i];
   }
-  return d;
+  return out;
+}
+
+LMat cheb_diff(int n, int times) { return diff_cols(LMat::eye(n), times); }
+
+// B D: (B D)[j, k] = 2k sum_{i < k, i + k odd} B[j, i] / c_i  (c_0 = 2, else 1), O(n^2)
+static LMat cheb_backward_diff(int n) {
+  LMat b = cheb_backward(n), bd(n, n);
+  for (int j = 0; j < n; ++j) {
+    ld s[2] = {0.0L, 0.0L};  // running sums over even / odd i
+    for (int k = 0; k < n; ++k) {
+      bd(j, k) = 2.0L * (ld)k * s[(k + 1) & 1];
+      s[k & 1] += b(j, k) / (k == 0 ? 2.0L : 1.0L);
+    }
+  }
+  return bd;
+}
+
+// composite_stencil.rs:250-276 + linalg.rs:14-57 on one lane, long double
+static void from_ortho_lane(const Basis& b, const ld* p, ld* c) {
+  if (b.k == CDBC) {  // composite_stencil.rs:366-388
+    const ld t0[2] = {0.5L, 0.5L}, t1[2] = {-0.5L, 0.5L};
+    ld c0 = p[0] * t0[0] + p[1] * t1[0], c1 = p[0] * t0[1] + p[1] * t1[1];
+    ld a = t0[0] * t0[0] + t1[0] * t1[0], bb = t0[0] * t0[1] + t1[0] * t1[1];
+    ld cc = t0[1] * t0[0] + t1[1] * t1[0], d = t0[1] * t0[1] + t1[1] * t1[1];
+    ld det = 1.0L / (a * d - bb * cc);
+    c[0] = (c0 * d - c1 * bb) * det;
+    c[1] = (c1 * a - c0 * cc) * det;
+    return;
+  }
+  const int m = b.m;
+  std::vector<ld> mainv(m), off(m - 2), d(m), w(m - 2), g(m);
+  for (int i = 0; i < m; ++i) mainv[i] = (ld)b.diag[i] * b.diag[i] + (ld)b.low2[i] * b.low2[i];
+  for (int i = 0; i < m - 2; ++i) off[i] = (ld)b.diag[i + 2] * b.low2[i];
+  for (int i = 0; i < m; ++i) d[i] = p[i] * (ld)b.diag[i] + p[i + 2] * (ld)b.low2[i];
+  w[0] = off[0] / mainv[0];
+  g[0] = d[0] / mainv[0];
+  if (m - 2 > 1) w[1] = off[1] / mainv[1];
+  g[1] = d[1] / mainv[1];
+  for (int i = 2; i < m - 2; ++i) w[i] = off[i] / (mainv[i] - off[i - 2] * w[i - 2]);
+  for (int i = 2; i < m; ++i) g[i] = (d[i] - g[i - 2] * off[i - 2]) / (mainv[i] - off[i - 2] * w[i - 2]);
+  c[m - 1] = g[m - 1];
+  c[m - 2] = g[m - 2];
+  for (int i = m - 2; i >= 1; --i) c[i - 1] = g[i - 1] - c[i + 1] * w[i - 1];
+}
+
+// from_ortho applied to every column of `m` (ortho rows -> composite rows)
+static LMat from_ortho_cols(const Basis& b, const LMat& m) {
+  LMat out(b.m, m.c);
+  std::vector<ld> p(m.r), c(b.m);
+  for (int j = 0; j < m.c; ++j) {
+    for (int i = 0; i < m.r; ++i) p[i] = m(i, j);
+    from_ortho_lane(b, p.data(), c.data());
+    for (int i = 0; i < b.m; ++i) out(i, j) = c[i];
+  }
+  return out;
 }
 
 LMat inverse(const LMat& m0) {
@@ -154,9 +218,7 @@ LMat inverse(const LMat& m0) {
 // composite_stencil.rs:250-276: c = (S^T S)^-1 S^T p (the tridiagonal normal equations)
 LMat from_ortho(const Basis& b) {
   if (b.k == CHEB || b.k == FO) return LMat::eye(b.ortho());
-  LMat s = stencil(b);
-  LMat st = s.t();
-  return inverse(st * s) * st;
+  return from_ortho_cols(b, LMat::eye(b.n));
 }
 
 // r2c.rs:160-175 (realfft, unnormalised), split planes: rows [0,m) = Re, [m,2m) = Im
@@ -264,6 +326,47 @@ void Band4::sweep() {  // fdma.rs:73-82
     dia[i] -= low[i - 2] * up1[i - 2];
     if (i < n - 2) up1[i] -= low[i - 2] * up2[i - 2];
   }
+}
+
+// Dense 1-D operator `name` of basis b (axis scale sc), every construction O(n^2):
+//   S    to_ortho                      (composite_stencil.rs:207-229)
+//   BS   backward                      (composite.rs:430-443 ; r2c.rs:211-226)
+//   BDS  d/dx then backward            (conv_term.rs:35-38)
+//   DS   d/dx (to ortho)               (space2.rs:247-264, composite.rs:523-536)
+//   DDS  d2/dx2 (to ortho)
+//   FO   from_ortho                    (composite_stencil.rs:250-276)
+//   FS   forward                       (composite.rs:383-396 ; r2c.rs:160-175)
+//   Fdl  forward then 2/3 dealias      (navier.rs:562-568, 1022-1032)
+LMat axis_operator(const Basis& b, const std::string& name, ld sc) {
+  const int n = b.n;
+  if (b.k == FO) {
+    if (name == "BS") return dft_c2r(n);
+    if (name == "FS") return dft_r2c(n);
+    if (name == "Fdl") {
+      LMat a = dft_r2c(n);
+      const int cut = b.m * 2 / 3;
+      for (int k = cut; k < b.m; ++k)
+        for (int j = 0; j < n; ++j) a(k, j) = a(b.m + k, j) = 0.0L;
+      return a;
+    }
+    throw std::runtime_error("no dense Fourier operator " + name);
+  }
+  LMat S = stencil(b);
+  if (name == "S") return S;
+  if (name == "BS") return cheb_backward(n) * S;
+  if (name == "BDS") return scaled(cheb_backward_diff(n) * S, 1.0L / sc);
+  if (name == "DS") return scaled(diff_cols(S, 1), 1.0L / sc);
+  if (name == "DDS") return scaled(diff_cols(S, 2), 1.0L / (sc * sc));
+  if (name == "FO") return from_ortho(b);
+  if (name == "FS") return b.k == CHEB ? cheb_forward(n) : from_ortho_cols(b, cheb_forward(n));
+  if (name == "Fdl") {
+    LMat f = cheb_forward(n);
+    const int cut = n * 2 / 3;
+    for (int k = cut; k < n; ++k)
+      for (int j = 0; j < n; ++j) f(k, j) = 0.0L;
+    return f;
+  }
+  throw std::runtime_error("unknown operator " + name);
 }
 
 // ---------------------------------------------------------------- eigen-solver

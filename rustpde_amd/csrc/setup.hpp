@@ -48,6 +48,8 @@ LMat cheb_backward(int n);            // B : physical <- ortho    (n x n)
 LMat cheb_forward(int n);             // F : ortho <- physical    (n x n)
 LMat cheb_diff(int n, int times);     // D^t on ortho coefficients (n x n)
 LMat from_ortho(const Basis& b);      // (S^T S)^-1 S^T, composite <- ortho (m x n)
+// named dense 1-D operator of basis b along an axis of scale sc (see setup.cpp)
+LMat axis_operator(const Basis& b, const std::string& name, ld sc);
 // ---- Fourier (r2c) operators on split planes [re; im] ----
 LMat dft_r2c(int n);                  // (2m x n)
 LMat dft_c2r(int n);                  // (n x 2m), 1/n normalised, Im of k=0, n/2 ignored

@@ -177,6 +177,12 @@ class Navier2D:
         """Navier2D::set_temp_bc: replace fieldbc.vhat (and every quantity derived from it)."""
         self.fieldbc.vhat = vhat
 
+    # ---- execution options (equivalent implementations; for A/B checks)
+    OPT_GRAPH, OPT_MFMA, OPT_FFT, OPT_BANDS = 0, 1, 2, 3
+
+    def set_option(self, option, value):
+        check(_lib.load().rbc_set_option(self._h, int(option), int(value)))
+
     # ---- tensor solver set-up (FdmaTensor pub fields, fdma_tensor.rs:73-91)
     def tensor_setup(self, s):
         n, fb = ctypes.c_int(), ctypes.c_int()
