@@ -100,7 +100,7 @@ void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, con
   if (M <= 0 || N <= 0) return;
   if (K <= 0) throw std::runtime_error("gemm_mfma: K must be positive");
   dim3 grid((N + 63) / 64, (M + 63) / 64);
-  hipLaunchKernelGGL((gemm_mfma_kernel<64, 64, 16>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
+  hipLaunchKernelGGL((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
                      beta, U, ldu);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("gemm_mfma: ") + hipGetErrorString(e));

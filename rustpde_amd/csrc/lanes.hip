@@ -48,25 +48,31 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
   const int per = (len0 + WAVE - 1) / WAVE;  // slice length, <= SLICE
   const int t0 = lane * per;
   double v[2][SLICE];
-  int len[2] = {len0, m / 2};
+  const int len[2] = {len0, m / 2};
   int cnt[2];
+#pragma unroll
   for (int p = 0; p < 2; ++p) {
     cnt[p] = max(0, min(per, len[p] - t0));
-    for (int q = 0; q < SLICE; ++q) {
-      v[p][q] = 0.0;
-      if (q < cnt[p]) v[p][q] = x[p + 2 * (t0 + q)];
-    }
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) v[p][q] = (q < cnt[p]) ? x[p + 2 * (t0 + q)] : 0.0;
   }
   // ---- forward elimination: y_t = x_t - y_{t-1} low[i-2]  (i = p + 2t >= 2)
+#pragma unroll
   for (int p = 0; p < 2; ++p) {
     double A = 1.0, B = 0.0;  // slice map y_out = A y_in + B
+#pragma unroll
     for (int q = 0; q < SLICE; ++q) {
-      if (q >= cnt[p]) break;
-      const int t = t0 + q, i = p + 2 * t;
-      if (t == 0) { A = 0.0; B = v[p][q]; continue; }
-      const double l = Lw[i - 2];
-      A = -A * l;
-      B = v[p][q] - B * l;
+      if (q < cnt[p]) {
+        const int t = t0 + q, i = p + 2 * t;
+        if (t == 0) {
+          A = 0.0;
+          B = v[p][q];
+        } else {
+          const double l = Lw[i - 2];
+          A = -A * l;
+          B = v[p][q] - B * l;
+        }
+      }
     }
     double pa = A, pb = B;  // inclusive scan, lower lanes first
     for (int off = 1; off < WAVE; off <<= 1) {
@@ -75,17 +81,21 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
     }
     double y = __shfl_up(pb, 1);
     if (lane == 0) y = 0.0;
+#pragma unroll
     for (int q = 0; q < SLICE; ++q) {
-      if (q >= cnt[p]) break;
-      const int t = t0 + q, i = p + 2 * t;
-      y = (t == 0) ? v[p][q] : v[p][q] - y * Lw[i - 2];
-      v[p][q] = y;
+      if (q < cnt[p]) {
+        const int t = t0 + q, i = p + 2 * t;
+        y = (t == 0) ? v[p][q] : v[p][q] - y * Lw[i - 2];
+        v[p][q] = y;
+      }
     }
   }
   // ---- back substitution: x_t = (y_t - x_{t+1} up1[i] - x_{t+2} up2[i]) / dia[i]
+#pragma unroll
   for (int p = 0; p < 2; ++p) {
     // slice map on the state s = (x_{t+1}, x_{t+2}):  s' = M s + w
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0, w0 = 0.0, w1 = 0.0;
+#pragma unroll
     for (int q = SLICE - 1; q >= 0; --q) {
       if (q >= cnt[p]) continue;
       const int t = t0 + q, i = p + 2 * t;
@@ -112,6 +122,7 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
     }
     double s0 = __shfl_down(b0, 1), s1 = __shfl_down(b1, 1);
     if (lane == WAVE - 1) { s0 = 0.0; s1 = 0.0; }
+#pragma unroll
     for (int q = SLICE - 1; q >= 0; --q) {
       if (q >= cnt[p]) continue;
       const int t = t0 + q, i = p + 2 * t;
@@ -124,7 +135,9 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
       v[p][q] = val;
     }
   }
+#pragma unroll
   for (int p = 0; p < 2; ++p)
+#pragma unroll
     for (int q = 0; q < SLICE; ++q)
       if (q < cnt[p]) x[p + 2 * (t0 + q)] = v[p][q];
 }
@@ -173,7 +186,212 @@ __global__ void band_x_kernel(const double* __restrict__ in, double* __restrict_
   *o = (beta != 0.0) ? acc + beta * (*o) : acc;
 }
 
+
+// ---------------------------------------------------------------------------------
+// cheb_diff_y: d/dy of every row (ortho.rs:107-125, once), the composite stencil
+// (to_ortho) fused on load.  d_i = w_i + d_{i+2} with w_i = 2 (i+1) p_{i+1} (i >= 1),
+// d_{n-1} = 0, d_0 = p_1 + d_2 / 2: a suffix sum along each parity chain.  Output is
+// scaled by 1/sc and optionally accumulated: out = d / sc [+ beta out].
+__global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void cheb_diff_y_kernel(
+    const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows, int n,
+    const double* __restrict__ coef, const unsigned char* __restrict__ mask, double inv_sc, double beta) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int r = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE);
+  if (r >= rows) return;
+  const double* a = in + (size_t)r * ldi;
+  double* o = out + (size_t)r * ldo;
+  const int per = ((n + 1) / 2 + WAVE - 1) / WAVE;  // slice length per parity
+  const int i0 = 2 * lane * per;                      // first index of this lane's slice
+  // p_j (ortho coefficients) for j in [i0, i0 + 2 per + 1]
+  auto pval = [&](int j) -> double {
+    if (j >= n) return 0.0;
+    if (!coef) return a[j];
+    const unsigned mk = mask[j];
+    double acc = 0.0;
+    bool first = true;
+    if (mk & 1u) { acc = a[j] * coef[j]; first = false; }
+    if (mk & 2u) { const double t = a[j - 2] * coef[n + j]; acc = first ? t : acc + t; }
+    return acc;
+  };
+  double w[2][SLICE];
+  double s[2] = {0.0, 0.0};  // slice sums, accumulated top-down as the reference does
+#pragma unroll
+  for (int qq = SLICE - 1; qq >= 0; --qq)
+#pragma unroll
+    for (int p = 1; p >= 0; --p) {
+      const int i = i0 + 2 * qq + p;
+      double wi = 0.0;
+      if (qq < per && i >= 1 && i <= n - 2) {
+        wi = pval(i + 1) * (2.0 * (double)(i + 1));
+        s[p] = wi + s[p];
+      }
+      w[p][qq] = wi;
+    }
+  // suffix sums of slice sums across lanes (higher lanes first)
+  double in0 = s[0], in1 = s[1];
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const double u0 = __shfl_down(in0, off), u1 = __shfl_down(in1, off);
+    if (lane + off < WAVE) { in0 = in0 + u0; in1 = in1 + u1; }
+  }
+  double d[2] = {__shfl_down(in0, 1), __shfl_down(in1, 1)};
+  if (lane == WAVE - 1) { d[0] = 0.0; d[1] = 0.0; }
+  double res[2][SLICE];
+#pragma unroll
+  for (int qq = SLICE - 1; qq >= 0; --qq)
+#pragma unroll
+    for (int p = 1; p >= 0; --p) {
+      const int i = i0 + 2 * qq + p;
+      double v = 0.0;
+      if (qq < per && i < n) {
+        if (i == n - 1) v = 0.0;
+        else if (i == n - 2) v = w[p][qq];
+        else if (i >= 1) v = w[p][qq] + d[p];
+        if (i >= 1) d[p] = v;
+      }
+      res[p][qq] = v;
+    }
+  // d_0 = p_1 + d_2 / 2, with d_2 in lane 0 (slices of length >= 2) or lane 1
+  const double d2 = (per >= 2) ? res[0][1] : __shfl(res[0][0], 1);
+  if (lane == 0) res[0][0] = pval(1) + d2 / 2.0;
+#pragma unroll
+  for (int qq = 0; qq < SLICE; ++qq)
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int i = i0 + 2 * qq + p;
+      if (qq < per && i < n) {
+        double v = res[p][qq];
+        if (inv_sc != 1.0) v = v * inv_sc;
+        o[i] = (beta != 0.0) ? v + beta * o[i] : v;
+      }
+    }
+}
+
+// tdma_y: from_ortho of every row (composite_stencil.rs:250-276 + linalg.rs:14-57):
+// d_i = p_i diag_i + p_{i+2} low2_i (i < m), then the tridiagonal (offsets -2, 0, 2)
+// normal equations with the host-computed w_i and den_i = main_i - off_{i-2} w_{i-2}:
+// g_i = (d_i - g_{i-2} off_{i-2}) / den_i, x_{i} = g_{i} - x_{i+2} w_{i}.
+__global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void tdma_y_kernel(
+    const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows, int m,
+    const double* __restrict__ diag, const double* __restrict__ low2, const double* __restrict__ off,
+    const double* __restrict__ wv, const double* __restrict__ den) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int r = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE);
+  if (r >= rows) return;
+  const double* a = in + (size_t)r * ldi;
+  double* o = out + (size_t)r * ldo;
+  const int per = ((m + 1) / 2 + WAVE - 1) / WAVE;
+  const int t0 = lane * per;
+  const int len[2] = {(m + 1) / 2, m / 2};
+  double v[2][SLICE];
+  int cnt[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    cnt[p] = max(0, min(per, len[p] - t0));
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      v[p][q] = 0.0;
+      if (q < cnt[p]) {
+        const int i = p + 2 * (t0 + q);
+        v[p][q] = a[i] * diag[i] + a[i + 2] * low2[i];
+      }
+    }
+  }
+  // forward: g_i = (d_i - g_{i-2} off_{i-2}) / den_i  (i >= 2), g_i = d_i / den_i (i < 2)
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    double A = 1.0, B = 0.0;
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      if (q < cnt[p]) {
+        const int t = t0 + q, i = p + 2 * t;
+        if (t == 0) {
+          A = 0.0;
+          B = v[p][q] / den[i];
+        } else {
+          A = (-off[i - 2] / den[i]) * A;
+          B = (v[p][q] - B * off[i - 2]) / den[i];
+        }
+      }
+    }
+    double pa = A, pb = B;
+    for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
+      const double qa = __shfl_up(pa, o2), qb = __shfl_up(pb, o2);
+      if (lane >= o2) { pb = pa * qb + pb; pa = pa * qa; }
+    }
+    double g = __shfl_up(pb, 1);
+    if (lane == 0) g = 0.0;
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      if (q < cnt[p]) {
+        const int t = t0 + q, i = p + 2 * t;
+        g = (t == 0) ? v[p][q] / den[i] : (v[p][q] - g * off[i - 2]) / den[i];
+        v[p][q] = g;
+      }
+    }
+  }
+  // back: x_{m-1} = g_{m-1}, x_{m-2} = g_{m-2}, x_i = g_i - x_{i+2} w_i
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    double A = 1.0, B = 0.0;  // x_out = A x_in + B over the slice, top-down
+#pragma unroll
+    for (int q = SLICE - 1; q >= 0; --q) {
+      if (q < cnt[p]) {
+        const int t = t0 + q, i = p + 2 * t;
+        if (i >= m - 2) {
+          A = 0.0;
+          B = v[p][q];
+        } else {
+          A = -wv[i] * A;
+          B = v[p][q] - B * wv[i];
+        }
+      }
+    }
+    double pa = A, pb = B;
+    for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
+      const double qa = __shfl_down(pa, o2), qb = __shfl_down(pb, o2);
+      if (lane + o2 < WAVE) { pb = pa * qb + pb; pa = pa * qa; }
+    }
+    double x = __shfl_down(pb, 1);
+    if (lane == WAVE - 1) x = 0.0;
+#pragma unroll
+    for (int q = SLICE - 1; q >= 0; --q) {
+      if (q < cnt[p]) {
+        const int t = t0 + q, i = p + 2 * t;
+        x = (i >= m - 2) ? v[p][q] : v[p][q] - x * wv[i];
+        v[p][q] = x;
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q)
+      if (q < cnt[p]) o[p + 2 * (t0 + q)] = v[p][q];
+}
 }  // namespace
+
+bool cheb_diff_y_supported(int n) { return n >= 6 && (n + 1) / 2 <= WAVE * SLICE; }
+bool tdma_y_supported(int m) { return m >= 4 && (m + 1) / 2 <= WAVE * SLICE; }
+
+void cheb_diff_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, const double* coef,
+                 const unsigned char* mask, double inv_sc, double beta) {
+  if (!cheb_diff_y_supported(n)) throw std::invalid_argument("cheb_diff_y: unsupported length");
+  const unsigned blocks = (unsigned)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  hipLaunchKernelGGL(cheb_diff_y_kernel, dim3(blocks), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo, rows, n,
+                     coef, mask, inv_sc, beta);
+  check("cheb_diff_y");
+}
+
+void tdma_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int m, const double* diag,
+            const double* low2, const double* off, const double* w, const double* den) {
+  if (!tdma_y_supported(m)) throw std::invalid_argument("tdma_y: unsupported length");
+  const unsigned blocks = (unsigned)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
+  hipLaunchKernelGGL(tdma_y_kernel, dim3(blocks), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo, rows, m, diag,
+                     low2, off, w, den);
+  check("tdma_y");
+}
 
 bool fdma_rows_wave_supported(int m) { return m >= 5 && (m + 1) / 2 <= WAVE * SLICE; }
 

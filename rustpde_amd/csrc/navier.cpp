@@ -125,6 +125,32 @@ static void make_matvec_band(const LMat& a, DBand& band) {
   band.mask.upload(mk);
 }
 
+// from_ortho tables of a composite basis: stencil and the tdma factors of linalg.rs:14-57
+struct DTdma {
+  DBuf diag, low2, off, w, den;
+  int m = 0;
+};
+static void make_tdma(const Basis& b, DTdma& t) {
+  const int m = b.m;
+  std::vector<double> mainv(m), off(std::max(m - 2, 1)), w(m, 0.0), den(m, 0.0);
+  for (int i = 0; i < m; ++i) mainv[i] = b.diag[i] * b.diag[i] + b.low2[i] * b.low2[i];
+  for (int i = 0; i < m - 2; ++i) off[i] = b.diag[i + 2] * b.low2[i];
+  w[0] = off[0] / mainv[0];
+  den[0] = mainv[0];
+  if (m - 2 > 1) w[1] = off[1] / mainv[1];
+  den[1] = mainv[1];
+  for (int i = 2; i < m; ++i) {
+    den[i] = mainv[i] - off[i - 2] * w[i - 2];
+    if (i < m - 2) w[i] = off[i] / den[i];
+  }
+  t.m = m;
+  t.diag.upload(b.diag);
+  t.low2.upload(b.low2);
+  t.off.upload(off);
+  t.w.upload(w);
+  t.den.upload(den);
+}
+
 static ld ipow(ld x, int e) {
   ld r = 1.0L;
   for (int i = 0; i < e; ++i) r *= x;
@@ -460,6 +486,17 @@ class Navier {
     return ref;
   }
   bool use_bands = true;
+  std::map<std::string, std::unique_ptr<DTdma>> tdmas;
+  const DTdma& tdma_tables(const Basis& b) {
+    std::string key = std::to_string((int)b.k) + ":" + std::to_string(b.n);
+    auto it = tdmas.find(key);
+    if (it != tdmas.end()) return *it->second;
+    auto t = std::make_unique<DTdma>();
+    make_tdma(b, *t);
+    const DTdma& ref = *t;
+    tdmas[key] = std::move(t);
+    return ref;
+  }
   void FDIFF(const double* X, double* Y, int m, int cols, int ld, int times, double sc) {
     timer.begin(st, 2, 8.0 * 4.0 * m * cols);
     fourier_diff(st, X, Y, m, cols, ld, times, sc);
@@ -490,7 +527,7 @@ class Navier {
       b->alloc(full);
     BC.alloc((size_t)R * 2);
     BCv.alloc((size_t)nx * ny);
-    RED.alloc(std::max(ny, 8) + 8);
+    RED.alloc(std::max(ny, 8) + 300);
     // solvers
     const double cv[2] = {dt * nu / std::pow(scale[0], 2.0), dt * nu / std::pow(scale[1], 2.0)};
     const double ck[2] = {dt * ka / std::pow(scale[0], 2.0), dt * ka / std::pow(scale[1], 2.0)};
@@ -578,6 +615,22 @@ class Navier {
     if (b.k == CHEB && (name == "S" || name == "FO") && !U && beta == 0.0) return in;
     if (use_bands && name == "S" && !U && (b.k == CD || b.k == CN)) {
       BANDY(stencil_band(b), in, b.m, out, b.n, rows, beta);
+      return out;
+    }
+    if (use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) && cheb_diff_y_supported(b.n)) {
+      const bool comp = b.k != CHEB;
+      const DBand* band = comp ? &stencil_band(b) : nullptr;
+      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
+      cheb_diff_y(st, in, b.m, out, b.n, rows, b.n, comp ? band->coef.p : nullptr, comp ? band->mask.p : nullptr, 1.0,
+                  beta);
+      timer.end(st);
+      return out;
+    }
+    if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_y_supported(b.m)) {
+      const DTdma& t = tdma_tables(b);
+      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.n + b.m));
+      tdma_y(st, in, b.n, out, b.m, rows, b.m, t.diag.p, t.low2.p, t.off.p, t.w.p, t.den.p);
+      timer.end(st);
       return out;
     }
     const DOp& op = Y.get(b, name);

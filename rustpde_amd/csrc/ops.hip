@@ -253,13 +253,13 @@ void col_weighted_sum(hipStream_t s, const double* v, int n0, int n1, const doub
   check_launch("col_weighted_sum");
 }
 
-__global__ void dot_and_norm_kernel(const double* __restrict__ a, const double* __restrict__ w, int n,
-                                    const double* __restrict__ b, long nb, double* __restrict__ out) {
+// out[0] = sum a w (one block; n is small), out[1] = sum b^2 (grid-stride partial sums
+// per block into part[], then reduced in block 0 by a second launch)
+__global__ void dot_kernel(const double* __restrict__ a, const double* __restrict__ w, int n, double* __restrict__ out) {
   __shared__ double red[256];
   const int t = threadIdx.x;
   double s = 0.0;
-  if (a)
-    for (int i = t; i < n; i += 256) s += a[i] * w[i];
+  for (int i = t; i < n; i += 256) s += a[i] * w[i];
   red[t] = s;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
@@ -267,21 +267,46 @@ __global__ void dot_and_norm_kernel(const double* __restrict__ a, const double* 
     __syncthreads();
   }
   if (t == 0) out[0] = red[0];
-  __syncthreads();
+}
+__global__ void sumsq_partial_kernel(const double* __restrict__ b, long nb, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
   double q = 0.0;
-  if (b)
-    for (long i = t; i < nb; i += 256) q += b[i] * b[i];
+  for (long i = (long)blockIdx.x * 256 + t; i < nb; i += (long)gridDim.x * 256) q += b[i] * b[i];
   red[t] = q;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
     if (t < h) red[t] += red[t + h];
     __syncthreads();
   }
-  if (t == 0) out[1] = red[0];
+  if (t == 0) part[blockIdx.x] = red[0];
+}
+__global__ void sum_kernel(const double* __restrict__ part, int np, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double q = 0.0;
+  for (int i = t; i < np; i += 256) q += part[i];
+  red[t] = q;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) *out = red[0];
 }
 void dot_and_norm(hipStream_t s, const double* a, const double* w, int n, const double* b, long nb, double* out) {
-  hipLaunchKernelGGL(dot_and_norm_kernel, dim3(1), dim3(256), 0, s, a, w, n, b, nb, out);
-  check_launch("dot_and_norm");
+  if (a) {
+    hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(256), 0, s, a, w, n, out);
+    check_launch("dot");
+  }
+  if (b) {
+    // partial sums land in out[2 .. 2 + 256); the caller's buffer holds >= 258 doubles
+    const int np = 256;
+    hipLaunchKernelGGL(sumsq_partial_kernel, dim3(np), dim3(256), 0, s, b, nb, out + 2);
+    check_launch("sumsq_partial");
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, out + 2, np, out + 1);
+    check_launch("sum");
+  }
 }
 
 // --------------------------------------------------------------------- layout changes

@@ -33,6 +33,18 @@ void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int 
 void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
             const unsigned char* mask, double beta);
 
+// Chebyshev d/dy of every row of in[rows, *] (ortho length n), with the composite
+// stencil of band coef/mask fused on load (coef == nullptr: input already ortho);
+// out = d / sc [+ beta out].
+bool cheb_diff_y_supported(int n);
+void cheb_diff_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, const double* coef,
+                 const unsigned char* mask, double inv_sc, double beta);
+// from_ortho of every row (composite length m): d = S^T p then the tridiagonal solve with
+// the precomputed w and den of linalg.rs:14-57.
+bool tdma_y_supported(int m);
+void tdma_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int m, const double* diag,
+            const double* low2, const double* off, const double* w, const double* den);
+
 // Fourier derivative on split planes: X[2m, cols] -> Y, (i k)^times / sc^times
 // (r2c.rs:88-99; num-complex division by (sc, 0) as space2.rs:256-262).
 void fourier_diff(hipStream_t s, const double* X, double* Y, int m, int cols, int ld, int times, double sc);
@@ -52,7 +64,8 @@ void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1);   
 
 // Weighted column sums: out[j] = sum_i v[i, j] * w[i]  (i sequential, as average.rs:25-34)
 void col_weighted_sum(hipStream_t s, const double* v, int n0, int n1, const double* w, double* out);
-// out[0] = sum_j a[j] * w[j] (single block), out[1] = sum of squares of b[0..nb) if b
+// out[0] = sum_j a[j] * w[j] if a, out[1] = sum of squares of b[0..nb) if b; out must
+// hold >= 258 doubles (out[2..258) receives per-block partial sums)
 void dot_and_norm(hipStream_t s, const double* a, const double* w, int n, const double* b, long nb, double* out);
 // v = sqrt(a^2 + b^2) * f
 void ew_speed(hipStream_t s, double* out, const double* a, const double* b, double f, long n);
