@@ -82,18 +82,69 @@ def cpu_baseline(cfg, budget_s):
                       % steps}
 
 
+def bench_hholtz_batch(args):
+    """Helmholtz batch: Hholtz on cheb_dirichlet(512)^2, c = 1e-3, 256 right-hand sides."""
+    import ctypes
+
+    import numpy as np
+
+    from rustpde_amd import HHOLTZ, Solver, _lib, seeded
+    from rustpde_amd.funspace import _Dev
+    n, nb = 512, 256
+    L = _lib.load()
+    g = Solver(HHOLTZ, (1, n), (1, n), (1e-3, 1e-3))
+    k = np.arange(n, dtype=float)
+    w = (1.0 + k[:, None] + k[None, :]) ** -2
+    f = np.stack([seeded.gaussian(1 + b, (n, n), 1.0) * w for b in range(nb)])
+    din, dout = _Dev(f.nbytes), _Dev(8 * nb * (n - 2) ** 2)
+    _lib.check(L.rbc_memcpy_h2d(din.p, f.ctypes.data, f.nbytes))
+    for _ in range(max(1, args.warmup // 5)):
+        g.solve_device(din.p, dout.p, nb)
+    reps = max(1, args.steps // 50)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.solve_device(din.p, dout.p, nb)  # synchronous
+    el = (time.perf_counter() - t0) / reps
+    flops = nb * 2 * (2.0 * (n - 2) * n * n + 2.0 * (n - 2) * (n - 2) * (n - 2))
+    out = {"metric": "Hholtz 512^2 right-hand sides solved per second (FP64)", "value": nb / el, "unit": "rhs/s",
+           "n_gpus": 1, "steps": reps, "warmup": max(1, args.warmup // 5), "ms_per_step": 1e3 * el,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "Hholtz cheb_dirichlet 512x512 c=1e-3, 256 rhs per step (FP64)", "batch": nb},
+           "roofline": {"bound": "mfma", "achieved": flops / el / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": flops / el / 1e12 / PEAK_FP64_TFLOPS, "traffic": None,
+                        "kernel": "whole solve (eigen-GEMMs + rows + banded), GEMM flops only"}}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        O.set_threads(threads)
+        o = O.Solver("hholtz", O.CD, n, O.CD, n, [1e-3, 1e-3])
+        o.solve(f[0])
+        cnt = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_budget and cnt < nb:
+            o.solve(f[cnt])
+            cnt += 1
+        out["cpu_baseline"] = {"value": cnt / (time.perf_counter() - t0), "unit": "rhs/s", "cores": threads,
+                               "kind": "port", "sample": "%d right-hand sides of the same batch" % cnt}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS) + ["c2"])
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "c2":
+        bench_hholtz_batch(args)
+        return
 
     import ctypes
 

@@ -150,6 +150,26 @@ int fs_from_ortho(fs_space2* sp, const double* d_ortho, double* d_spec, int batc
 int fs_gradient(fs_space2* sp, const double* d_spec, double* d_ortho, int d0, int d1, double scale0,
                 double scale1, int batch); /* space2.rs:247-264 */
 
+/* ------------------------------------------------------- SolverField (batched solves) */
+typedef struct rbc_solver rbc_solver;
+/* SolverField variants (src/solver.rs:88-95) */
+#define RBC_SOLVER_HHOLTZ_ADI 0 /* HholtzAdi::new  hholtz_adi.rs:44-63 */
+#define RBC_SOLVER_HHOLTZ 1     /* Hholtz::new     hholtz.rs:42-76     */
+#define RBC_SOLVER_POISSON 2    /* Poisson::new    poisson.rs:50-90    */
+/* Solver of `solver` kind built from the field Space2(base0(n0), base1(n1)) with
+ * constants c = [c0, c1] (axis 1 cheb_dirichlet or cheb_neumann). */
+int rbc_solver_create(int solver, int kind0, int n0, int kind1, int n1, double c0, double c1, rbc_solver** out);
+int rbc_solver_destroy(rbc_solver* h);
+/* input shape [o0, o1] (orthogonal space), output [s0, s1]; complex if base0 is Fourier */
+int rbc_solver_shape(rbc_solver* h, int* o0, int* o1, int* s0, int* s1, int* is_complex);
+/* Solve::solve (hholtz.rs:159-197, hholtz_adi.rs:106-131, poisson.rs:122-150) of `batch`
+ * contiguous right-hand sides on DEVICE buffers (reference layouts).  Synchronous. */
+int rbc_solver_solve(rbc_solver* h, const double* d_rhs, double* d_out, int batch);
+/* the pub FdmaTensor fields lam/fwd/bwd of a Hholtz/Poisson solver (fdma_tensor.rs:73-91) */
+int rbc_solver_tensor_info(rbc_solver* h, int* n, int* has_fb);
+int rbc_solver_get_tensor(rbc_solver* h, double* lam, double* fwd, double* bwd);
+int rbc_solver_set_tensor(rbc_solver* h, const double* lam, const double* fwd, const double* bwd);
+
 /* ------------------------------------------------------------ device memory helpers
  * For callers without their own HIP runtime binding (e.g. ctypes). */
 int rbc_dev_malloc(void** out, unsigned long long bytes);

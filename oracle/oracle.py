@@ -53,6 +53,12 @@ def lib():
         L.orc_navier_tensor_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.orc_space_apply.argtypes = [ctypes.c_int] * 7 + [_dp, _dp, _dp]
         L.orc_solver_apply.argtypes = [ctypes.c_int] * 5 + [_dp, ctypes.c_int, _dp, _dp]
+        L.orc_solver_create.restype = ctypes.c_void_p
+        L.orc_solver_create.argtypes = [ctypes.c_int] * 5 + [_dp]
+        L.orc_solver_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_solver_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_solver_tensor.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]
         L.orc_eig.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_fdma_solve.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.orc_dct1.argtypes = [ctypes.c_int, _dp, _dp]
@@ -120,6 +126,42 @@ def solver_apply(kind, k0, n0, k1, n1, c, rhs):
     cc = np.ascontiguousarray(c, dtype=float)
     _check(lib().orc_solver_apply(sk, k0, n0, k1, n1, _p(cc), int(cplx), _p(inp.view(float)), _p(out.view(float))))
     return out
+
+
+class Solver:
+    """Build-once Hholtz / HholtzAdi / Poisson of the field bases (k0, n0) x (k1, n1)."""
+
+    def __init__(self, kind, k0, n0, k1, n1, c):
+        sk = {"hholtz_adi": 0, "hholtz": 1, "poisson": 2}[kind]
+        cc = np.ascontiguousarray(c, dtype=float)
+        self._h = lib().orc_solver_create(sk, k0, n0, k1, n1, _p(cc))
+        if not self._h:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+        self.shape_in = (_ortho_len(k0, n0), _ortho_len(k1, n1))
+        self.shape_out = (_base_len(k0, n0, True), _base_len(k1, n1, True))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_solver_destroy(self._h)
+            self._h = None
+
+    def solve(self, rhs):
+        cplx = np.iscomplexobj(rhs)
+        a = np.ascontiguousarray(rhs, dtype=complex if cplx else float)
+        out = np.zeros(self.shape_out, dtype=a.dtype)
+        _check(lib().orc_solver_solve(self._h, a.shape[0], a.shape[1], int(cplx), _p(a.view(float)),
+                                      _p(out.view(float))))
+        return out
+
+    def tensor_setup(self):
+        n, fb = ctypes.c_int(), ctypes.c_int()
+        _check(lib().orc_solver_tensor(self._h, None, None, None, ctypes.byref(n), ctypes.byref(fb)))
+        if n.value == 0:
+            return None
+        m = n.value
+        lam, fwd, bwd = np.zeros(m), np.zeros((m, m)), np.zeros((m, m))
+        _check(lib().orc_solver_tensor(self._h, _p(lam), _p(fwd), _p(bwd), ctypes.byref(n), ctypes.byref(fb)))
+        return (lam, fwd, bwd) if fb.value else (lam, None, None)
 
 
 def eig(a):

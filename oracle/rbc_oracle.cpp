@@ -495,8 +495,9 @@ Arr<double> inv(const Arr<double>& a0) {
 
 // Eigenvalues of a real upper-Hessenberg block by the implicit double-shift (Francis)
 // QR iteration (Golub & Van Loan, Alg. 7.5.1-7.5.2), eigenvalues only.
-void hessenberg(Arr<double>& h) {  // Householder reduction in place
+void hessenberg(Arr<double>& h, Arr<double>* q = nullptr) {  // Householder reduction in place, H = Q^T A Q
   const int n = h.n0;
+  if (q) { *q = Arr<double>(n, n); for (int i = 0; i < n; ++i) (*q)(i, i) = 1.0; }
   for (int k = 0; k < n - 2; ++k) {
     double alpha = 0.0;
     for (int i = k + 1; i < n; ++i) alpha += h(i, k) * h(i, k);
@@ -521,8 +522,56 @@ void hessenberg(Arr<double>& h) {  // Householder reduction in place
       s = 2.0 * s / vn;
       for (int j = k + 1; j < n; ++j) h(i, j) -= s * v[j];
     }
+    if (q)
+      for (int i = 0; i < n; ++i) {  // Q = Q (I - 2vv'/v'v)
+        double s = 0.0;
+        for (int j = k + 1; j < n; ++j) s += (*q)(i, j) * v[j];
+        s = 2.0 * s / vn;
+        for (int j = k + 1; j < n; ++j) (*q)(i, j) -= s * v[j];
+      }
     for (int i = k + 2; i < n; ++i) h(i, k) = 0.0;
   }
+}
+
+// Eigenvector of an upper-Hessenberg H for eigenvalue lam by inverse iteration; the LU
+// with partial pivoting of (H - lam I) only ever swaps adjacent rows, O(n^2).
+vecd hessenberg_inverse_iteration(const Arr<double>& h, double lam, double anorm) {
+  const int n = h.n0;
+  Arr<double> u = h;
+  for (int i = 0; i < n; ++i) u(i, i) -= lam;
+  std::vector<int> swp(n, 0);
+  vecd mult(n, 0.0);
+  const double tiny = 2.220446049250313e-16 * std::max(anorm, 1e-300);
+  for (int k = 0; k < n; ++k) {
+    if (k + 1 < n && std::fabs(u(k + 1, k)) > std::fabs(u(k, k))) {
+      for (int j = k; j < n; ++j) std::swap(u(k, j), u(k + 1, j));
+      swp[k] = 1;
+    }
+    if (std::fabs(u(k, k)) < tiny) u(k, k) = (u(k, k) >= 0 ? tiny : -tiny);
+    if (k + 1 < n) {
+      const double f = u(k + 1, k) / u(k, k);
+      mult[k] = f;
+      u(k + 1, k) = 0.0;
+      for (int j = k + 1; j < n; ++j) u(k + 1, j) -= f * u(k, j);
+    }
+  }
+  vecd x(n);
+  for (int i = 0; i < n; ++i) x[i] = 1.0 + 0.01 * std::sin(1.0 + i);
+  for (int it = 0; it < 3; ++it) {
+    for (int k = 0; k + 1 < n; ++k) {  // apply L^-1 P
+      if (swp[k]) std::swap(x[k], x[k + 1]);
+      x[k + 1] -= mult[k] * x[k];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      for (int j = i + 1; j < n; ++j) x[i] -= u(i, j) * x[j];
+      x[i] /= u(i, i);
+    }
+    double nr = 0.0;
+    for (double v : x) nr += v * v;
+    nr = std::sqrt(nr);
+    for (double& v : x) v /= nr;
+  }
+  return x;
 }
 
 void apply_reflector3(Arr<double>& h, int k, int nr, const double* v, int lo, int hi, int rlo, int rhi) {
@@ -655,8 +704,8 @@ void eig_sorted(const Arr<double>& a, vecd& lam, Arr<double>& q, Arr<double>& qi
   const int n = a.n0;
   double anorm = 0.0;
   for (double v : a.d) anorm = std::max(anorm, std::fabs(v));
-  Arr<double> h = a;
-  hessenberg(h);
+  Arr<double> h = a, hq;
+  hessenberg(h, &hq);
   bool cplx = false;
   vecd w = hqr_eigenvalues(h, cplx);
   if (cplx) throw std::runtime_error("eig: complex eigenvalues");
@@ -683,7 +732,10 @@ void eig_sorted(const Arr<double>& a, vecd& lam, Arr<double>& q, Arr<double>& qi
   }
   for (int k = 0; k < n; ++k) {
     if (used[k]) continue;
-    vecd x = inverse_iteration(a, w[k], anorm);
+    vecd y = hessenberg_inverse_iteration(h, w[k], anorm);
+    vecd x(n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) x[i] += hq(i, j) * y[j];
     lam0[col] = w[k];
     for (int i = 0; i < n; ++i) qq(i, col) = x[i];
     ++col;
@@ -1373,6 +1425,37 @@ int orc_solver_apply(int skind, int k0, int n0, int k1, int n1, const double* c,
     int r0 = b0.ortho_len(), r1 = b1.ortho_len();
     if (cplx) { Arr<cd> a(r0, r1); copy_in(a, in); copy_out(s.solve<cd>(a), out); }
     else { Arr<double> a(r0, r1); copy_in(a, in); copy_out(s.solve<double>(a), out); }
+  })
+}
+
+// build-once solver handle (same solvers as orc_solver_apply)
+void* orc_solver_create(int skind, int k0, int n0, int k1, int n1, const double* c) {
+  try {
+    return new Solver2((SolverKind)skind, mkbase(k0, n0), mkbase(k1, n1), c);
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void orc_solver_destroy(void* h) { delete (Solver2*)h; }
+int orc_solver_solve(void* h, int n0in, int n1in, int cplx, const double* in, double* out) {
+  GUARD({
+    Solver2& s = *(Solver2*)h;
+    if (cplx) { Arr<cd> a(n0in, n1in); copy_in(a, in); copy_out(s.solve<cd>(a), out); }
+    else { Arr<double> a(n0in, n1in); copy_in(a, in); copy_out(s.solve<double>(a), out); }
+  })
+}
+int orc_solver_tensor(void* h, double* lam, double* fwd, double* bwd, int* n, int* has_fb) {
+  GUARD({
+    Solver2& s = *(Solver2*)h;
+    if (s.kind == S_HHOLTZ_ADI) { *n = 0; *has_fb = 0; return 0; }
+    *n = (int)s.t.lam.size();
+    *has_fb = s.t.has_fb ? 1 : 0;
+    if (lam) std::memcpy(lam, s.t.lam.data(), s.t.lam.size() * 8);
+    if (s.t.has_fb && fwd && bwd) {
+      std::memcpy(fwd, s.t.fwd.d.data(), s.t.fwd.d.size() * 8);
+      std::memcpy(bwd, s.t.bwd.d.data(), s.t.bwd.d.size() * 8);
+    }
   })
 }
 

@@ -9,3 +9,4 @@ from .funspace import (Space2, cheb_dirichlet, cheb_dirichlet_bc, cheb_neumann, 
                        fourier_r2c)
 from .integrate import integrate  # noqa: F401
 from .navier import Navier2D, get_ka, get_nu  # noqa: F401
+from .solver import HHOLTZ, HHOLTZ_ADI, POISSON, Solver  # noqa: F401

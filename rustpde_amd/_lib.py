@@ -51,6 +51,13 @@ SIGNATURES = {
     "fs_to_ortho": (_i, [_vp, _vp, _vp, _i]),
     "fs_from_ortho": (_i, [_vp, _vp, _vp, _i]),
     "fs_gradient": (_i, [_vp, _vp, _vp, _i, _i, _d, _d, _i]),
+    "rbc_solver_create": (_i, [_i, _i, _i, _i, _i, _d, _d, ctypes.POINTER(_vp)]),
+    "rbc_solver_destroy": (_i, [_vp]),
+    "rbc_solver_shape": (_i, [_vp, _ip, _ip, _ip, _ip, _ip]),
+    "rbc_solver_solve": (_i, [_vp, _vp, _vp, _i]),
+    "rbc_solver_tensor_info": (_i, [_vp, _ip, _ip]),
+    "rbc_solver_get_tensor": (_i, [_vp, _dp, _dp, _dp]),
+    "rbc_solver_set_tensor": (_i, [_vp, _dp, _dp, _dp]),
     "rbc_dev_malloc": (_i, [ctypes.POINTER(_vp), ctypes.c_ulonglong]),
     "rbc_dev_free": (_i, [_vp]),
     "rbc_memcpy_h2d": (_i, [_vp, _vp, ctypes.c_ulonglong]),

@@ -99,8 +99,10 @@ template <int C>
 __global__ __launch_bounds__(FFT_THREADS) void rfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
                                                                  int n, int cols, int cut,
                                                                  const double* __restrict__ tw,
-                                                                 const double* __restrict__ post) {
+                                                                 const double* __restrict__ post, long sx, long sy) {
 #pragma clang fp contract(off)
+  X += sx * blockIdx.y;
+  Y += sy * blockIdx.y;
   extern __shared__ double lds[];
   const int N = n / 2, m = N + 1, bits = __ffs(N) - 1;
   double* re = lds;
@@ -141,8 +143,10 @@ template <int C>
 __global__ __launch_bounds__(FFT_THREADS) void irfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
                                                                   int n, int cols, int times, double sc,
                                                                   const double* __restrict__ tw,
-                                                                  const double* __restrict__ post) {
+                                                                  const double* __restrict__ post, long sx, long sy) {
 #pragma clang fp contract(off)
+  X += sx * blockIdx.y;
+  Y += sy * blockIdx.y;
   extern __shared__ double lds[];
   const int N = n / 2, m = N + 1, bits = __ffs(N) - 1, S = m;  // column stride S = N + 1
   double* re = lds;
@@ -224,40 +228,44 @@ void check_n(int n) {
 bool fft_supported(int n) { return n >= 8 && n <= 4096 && (n & (n - 1)) == 0; }
 
 template <int C>
-static void launch_rfft(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, const Twiddles& tw) {
+static void launch_rfft(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, const Twiddles& tw,
+                        int batch, long sx, long sy) {
   const size_t lds = sizeof(double) * 2 * C * (n / 2);
   const unsigned blocks = (unsigned)((cols + C - 1) / C);
-  hipLaunchKernelGGL(rfft_cols_kernel<C>, dim3(blocks), dim3(FFT_THREADS), lds, s, X, Y, n, cols, cut, tw.stage,
-                     tw.post);
+  hipLaunchKernelGGL(rfft_cols_kernel<C>, dim3(blocks, batch), dim3(FFT_THREADS), lds, s, X, Y, n, cols, cut,
+                     tw.stage, tw.post, sx, sy);
 }
 template <int C>
 static void launch_irfft(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc,
-                         const Twiddles& tw) {
+                         const Twiddles& tw, int batch, long sx, long sy) {
   const size_t lds = sizeof(double) * 2 * C * (n / 2 + 1);
   const unsigned blocks = (unsigned)((cols + C - 1) / C);
-  hipLaunchKernelGGL(irfft_cols_kernel<C>, dim3(blocks), dim3(FFT_THREADS), lds, s, X, Y, n, cols, times, sc,
-                     tw.stage, tw.post);
+  hipLaunchKernelGGL(irfft_cols_kernel<C>, dim3(blocks, batch), dim3(FFT_THREADS), lds, s, X, Y, n, cols, times, sc,
+                     tw.stage, tw.post, sx, sy);
 }
 
-void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut) {
+void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch, long sx, long sy) {
   check_n(n);
+  if (batch < 1 || batch > 65535) throw std::invalid_argument("rfft_cols: bad batch");
   const Twiddles& tw = twiddles(n);
   cut = std::min(cut, n / 2 + 1);
   switch (fft_cols_per_block(n)) {
-    case 4: launch_rfft<4>(s, X, Y, n, cols, cut, tw); break;
-    case 2: launch_rfft<2>(s, X, Y, n, cols, cut, tw); break;
-    default: launch_rfft<1>(s, X, Y, n, cols, cut, tw); break;
+    case 4: launch_rfft<4>(s, X, Y, n, cols, cut, tw, batch, sx, sy); break;
+    case 2: launch_rfft<2>(s, X, Y, n, cols, cut, tw, batch, sx, sy); break;
+    default: launch_rfft<1>(s, X, Y, n, cols, cut, tw, batch, sx, sy); break;
   }
   check("rfft_cols");
 }
 
-void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc) {
+void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch, long sx,
+                long sy) {
   check_n(n);
+  if (batch < 1 || batch > 65535) throw std::invalid_argument("irfft_cols: bad batch");
   const Twiddles& tw = twiddles(n);
   switch (fft_cols_per_block(n)) {
-    case 4: launch_irfft<4>(s, X, Y, n, cols, times, sc, tw); break;
-    case 2: launch_irfft<2>(s, X, Y, n, cols, times, sc, tw); break;
-    default: launch_irfft<1>(s, X, Y, n, cols, times, sc, tw); break;
+    case 4: launch_irfft<4>(s, X, Y, n, cols, times, sc, tw, batch, sx, sy); break;
+    case 2: launch_irfft<2>(s, X, Y, n, cols, times, sc, tw, batch, sx, sy); break;
+    default: launch_irfft<1>(s, X, Y, n, cols, times, sc, tw, batch, sx, sy); break;
   }
   check("irfft_cols");
 }

@@ -20,9 +20,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 template <int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, const double* __restrict__ A, int lda,
-                                                        const double* __restrict__ B, int ldb, double* __restrict__ C,
-                                                        int ldc, double alpha, double beta,
-                                                        const double* __restrict__ U, int ldu) {
+                                                        long sA, const double* __restrict__ B, int ldb, long sB,
+                                                        double* __restrict__ C, int ldc, long sC, double alpha,
+                                                        double beta, const double* __restrict__ U, int ldu, long sU) {
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
   constexpr int NA = BM * BK / 256, NB = BK * BN / 256;
   __shared__ double As[BK][BM + 1];
@@ -31,6 +31,10 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
   const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int lr = lane & 15, lk = lane >> 4;
+  A += sA * blockIdx.z;  // batch
+  B += sB * blockIdx.z;
+  C += sC * blockIdx.z;
+  if (U) U += sU * blockIdx.z;
   d4 acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -95,15 +99,22 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
 
 }  // namespace
 
-void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
-               int ldc, double alpha, double beta, const double* U, int ldu) {
-  if (M <= 0 || N <= 0) return;
+void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
+                       long sB, double* C, int ldc, long sC, int batch, double alpha, double beta, const double* U,
+                       int ldu, long sU) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
   if (K <= 0) throw std::runtime_error("gemm_mfma: K must be positive");
-  dim3 grid((N + 63) / 64, (M + 63) / 64);
-  hipLaunchKernelGGL((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
-                     beta, U, ldu);
+  if (batch > 65535) throw std::runtime_error("gemm_mfma: batch exceeds the grid limit");
+  dim3 grid((N + 63) / 64, (M + 63) / 64, batch);
+  hipLaunchKernelGGL((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
+                     alpha, beta, U, ldu, sU);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("gemm_mfma: ") + hipGetErrorString(e));
+}
+
+void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
+               int ldc, double alpha, double beta, const double* U, int ldu) {
+  gemm_mfma_batched(s, M, N, K, A, lda, 0, B, ldb, 0, C, ldc, 0, 1, alpha, beta, U, ldu, 0);
 }
 
 }  // namespace rbc

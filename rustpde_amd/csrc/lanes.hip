@@ -33,11 +33,12 @@ void check(const char* what) {
 
 __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
     double* __restrict__ X, int ld, int nrows, int m, int nsets, const double* __restrict__ low,
-    const double* __restrict__ dia, const double* __restrict__ up1, const double* __restrict__ up2) {
+    const double* __restrict__ dia, const double* __restrict__ up1, const double* __restrict__ up2, long sx) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (WAVE - 1);
   const int r = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE);
   if (r >= nrows) return;  // uniform per wavefront
+  X += sx * blockIdx.y;
   const int set = r % nsets;
   double* x = X + (size_t)r * ld;
   const double* Lw = low + (size_t)set * m;
@@ -146,10 +147,12 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
 // tap k is present at output index i
 __global__ void band_y_kernel(const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows,
                               int mo, const double* __restrict__ coef, const unsigned char* __restrict__ mask,
-                              double beta) {
+                              double beta, long si, long so) {
 #pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)rows * mo) return;
+  in += si * blockIdx.y;
+  out += so * blockIdx.y;
   const int r = (int)(t / mo), i = (int)(t % mo);
   const unsigned mk = mask[i];
   const double* a = in + (size_t)r * ldi;
@@ -167,10 +170,13 @@ __global__ void band_y_kernel(const double* __restrict__ in, int ldi, double* __
 }
 
 __global__ void band_x_kernel(const double* __restrict__ in, double* __restrict__ out, int cols, int mo,
-                              const double* __restrict__ coef, const unsigned char* __restrict__ mask, double beta) {
+                              const double* __restrict__ coef, const unsigned char* __restrict__ mask, double beta,
+                              long si, long so) {
 #pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)cols * mo) return;
+  in += si * blockIdx.y;
+  out += so * blockIdx.y;
   const int i = (int)(t / cols), j = (int)(t % cols);
   const unsigned mk = mask[i];
   const int off[4] = {0, -2, 2, 4};
@@ -194,11 +200,14 @@ __global__ void band_x_kernel(const double* __restrict__ in, double* __restrict_
 // scaled by 1/sc and optionally accumulated: out = d / sc [+ beta out].
 __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void cheb_diff_y_kernel(
     const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows, int n,
-    const double* __restrict__ coef, const unsigned char* __restrict__ mask, double inv_sc, double beta) {
+    const double* __restrict__ coef, const unsigned char* __restrict__ mask, double inv_sc, double beta, long si,
+    long so) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (WAVE - 1);
   const int r = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE);
   if (r >= rows) return;
+  in += si * blockIdx.y;
+  out += so * blockIdx.y;
   const double* a = in + (size_t)r * ldi;
   double* o = out + (size_t)r * ldo;
   const int per = ((n + 1) / 2 + WAVE - 1) / WAVE;  // slice length per parity
@@ -274,11 +283,13 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void cheb_diff_y_kernel(
 __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void tdma_y_kernel(
     const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows, int m,
     const double* __restrict__ diag, const double* __restrict__ low2, const double* __restrict__ off,
-    const double* __restrict__ wv, const double* __restrict__ den) {
+    const double* __restrict__ wv, const double* __restrict__ den, long si, long so) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (WAVE - 1);
   const int r = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE);
   if (r >= rows) return;
+  in += si * blockIdx.y;
+  out += so * blockIdx.y;
   const double* a = in + (size_t)r * ldi;
   double* o = out + (size_t)r * ldo;
   const int per = ((m + 1) / 2 + WAVE - 1) / WAVE;
@@ -376,47 +387,47 @@ bool cheb_diff_y_supported(int n) { return n >= 6 && (n + 1) / 2 <= WAVE * SLICE
 bool tdma_y_supported(int m) { return m >= 4 && (m + 1) / 2 <= WAVE * SLICE; }
 
 void cheb_diff_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, const double* coef,
-                 const unsigned char* mask, double inv_sc, double beta) {
+                 const unsigned char* mask, double inv_sc, double beta, int batch, long si, long so) {
   if (!cheb_diff_y_supported(n)) throw std::invalid_argument("cheb_diff_y: unsupported length");
   const unsigned blocks = (unsigned)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-  hipLaunchKernelGGL(cheb_diff_y_kernel, dim3(blocks), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo, rows, n,
-                     coef, mask, inv_sc, beta);
+  hipLaunchKernelGGL(cheb_diff_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo,
+                     rows, n, coef, mask, inv_sc, beta, si, so);
   check("cheb_diff_y");
 }
 
 void tdma_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int m, const double* diag,
-            const double* low2, const double* off, const double* w, const double* den) {
+            const double* low2, const double* off, const double* w, const double* den, int batch, long si, long so) {
   if (!tdma_y_supported(m)) throw std::invalid_argument("tdma_y: unsupported length");
   const unsigned blocks = (unsigned)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-  hipLaunchKernelGGL(tdma_y_kernel, dim3(blocks), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo, rows, m, diag,
-                     low2, off, w, den);
+  hipLaunchKernelGGL(tdma_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo, rows, m,
+                     diag, low2, off, w, den, si, so);
   check("tdma_y");
 }
 
 bool fdma_rows_wave_supported(int m) { return m >= 5 && (m + 1) / 2 <= WAVE * SLICE; }
 
 void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low, const double* dia,
-                    const double* up1, const double* up2) {
+                    const double* up1, const double* up2, int batch, long sx) {
   if (!fdma_rows_wave_supported(m)) throw std::invalid_argument("fdma_rows_wave: unsupported row length");
   const unsigned blocks = (unsigned)((nrows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-  hipLaunchKernelGGL(fdma_rows_wave_kernel, dim3(blocks), dim3(WAVE * ROWS_PER_BLOCK), 0, s, X, ld, nrows, m, nsets,
-                     low, dia, up1, up2);
+  hipLaunchKernelGGL(fdma_rows_wave_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, X, ld, nrows, m,
+                     nsets, low, dia, up1, up2, sx);
   check("fdma_rows_wave");
 }
 
 void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int mo, const double* coef,
-            const unsigned char* mask, double beta) {
+            const unsigned char* mask, double beta, int batch, long si, long so) {
   const long n = (long)rows * mo;
-  hipLaunchKernelGGL(band_y_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, ldi, out, ldo, rows, mo,
-                     coef, mask, beta);
+  hipLaunchKernelGGL(band_y_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, ldo, rows,
+                     mo, coef, mask, beta, si, so);
   check("band_y");
 }
 
 void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
-            const unsigned char* mask, double beta) {
+            const unsigned char* mask, double beta, int batch, long si, long so) {
   const long n = (long)cols * mo;
-  hipLaunchKernelGGL(band_x_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, cols, mo, coef, mask,
-                     beta);
+  hipLaunchKernelGGL(band_x_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, out, cols, mo, coef,
+                     mask, beta, si, so);
   check("band_x");
 }
 

@@ -1045,6 +1045,76 @@ class Space {
   }
 };
 
+// ------------------------------------------------------------ SolverField (batched)
+// Hholtz / HholtzAdi / Poisson built from the bases of a field (src/solver.rs:88-95,
+// hholtz.rs:42-76, hholtz_adi.rs:44-63, poisson.rs:50-90), solving `batch` right-hand
+// sides per call.  Input: ortho-space arrays [batch, o0, o1]; output [batch, m0, m1].
+class SolverDev {
+ public:
+  Basis b0, b1;
+  bool cplx;
+  std::unique_ptr<Solver> s;
+  hipStream_t st = nullptr;
+  DBuf W1, W2, IN, OUT;
+  SolverDev(int skind, int k0, int n0, int k1, int n1, double c0, double c1) : b0((Kind)k0, n0), b1((Kind)k1, n1), cplx(k0 == FO) {
+    if (k1 == FO || k1 == CHEB || k1 == CDBC) throw std::invalid_argument("axis 1 must be cheb_dirichlet or cheb_neumann");
+    if (k0 == CDBC) throw std::invalid_argument("axis 0 cannot be cheb_dirichlet_bc");
+    if (skind == ADI && k0 == FO) throw std::invalid_argument("HholtzAdi with a Fourier axis is not supported");
+    if (n0 < 8 || n1 < 8) throw std::invalid_argument("sizes must be >= 8");
+    const double c[2] = {c0, c1};
+    s = make_solver((SKind)skind, b0, b1, c);
+    HIPCHECK(hipStreamCreate(&st));
+  }
+  ~SolverDev() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+  int rows_in() const { return cplx ? 2 * b0.m : b0.ortho(); }
+  int rows_out() const { return cplx ? 2 * b0.m : b0.m; }
+  void ensure(DBuf& b, size_t n) {
+    if (b.n < n) b.alloc(n);
+  }
+  void solve(const double* rhs, double* out, int batch) {
+    const int o1 = b1.ortho(), m1 = b1.m, m0 = b0.m, ri = rows_in(), ro = rows_out();
+    const size_t in_sz = (size_t)ri * o1, out_sz = (size_t)ro * m1, big = (size_t)std::max(ri, ro) * o1;
+    ensure(W1, big * batch);
+    ensure(W2, big * batch);
+    const double* x = rhs;
+    if (cplx) {  // interleaved -> split planes, per right-hand side
+      ensure(IN, in_sz * batch);
+      for (int b = 0; b < batch; ++b) interleaved_to_split(st, rhs + 2 * (size_t)m0 * o1 * b, IN.p + in_sz * b, m0, o1);
+      x = IN.p;
+    }
+    double* y = out;
+    if (cplx) {
+      ensure(OUT, out_sz * batch);
+      y = OUT.p;
+    }
+    Solver& sv = *s;
+    if (sv.kind == ADI) {
+      gemm_mfma_batched(st, sv.hx.rows, o1, sv.hx.cols, sv.hx.b.p, sv.hx.cols, 0, x, o1, (long)in_sz, W1.p, o1,
+                        (long)sv.hx.rows * o1, batch, 1.0, 0.0);
+      gemm_mfma(st, batch * sv.hx.rows, m1, sv.hy.rows, W1.p, sv.hy.rows, sv.hy.b.p, sv.hy.cols, y, m1, 1.0, 0.0);
+    } else if (!sv.has_fb) {
+      band_y(st, x, o1, y, m1, batch * ro, sv.pyb.mo, sv.pyb.coef.p, sv.pyb.mask.p, 0.0);
+      fdma_rows_launch(y, m1, batch * ro, m1, sv);
+    } else {
+      gemm_mfma_batched(st, sv.pf.rows, o1, sv.pf.cols, sv.pf.b.p, sv.pf.cols, 0, x, o1, (long)in_sz, W1.p, o1,
+                        (long)sv.pf.rows * o1, batch, 1.0, 0.0);
+      band_y(st, W1.p, o1, W2.p, m1, batch * sv.pf.rows, sv.pyb.mo, sv.pyb.coef.p, sv.pyb.mask.p, 0.0);
+      fdma_rows_launch(W2.p, m1, batch * sv.pf.rows, m1, sv);
+      gemm_mfma_batched(st, sv.pb.rows, m1, sv.pb.cols, sv.pb.b.p, sv.pb.cols, 0, W2.p, m1, (long)sv.pf.rows * m1, y, m1,
+                        (long)out_sz, batch, 1.0, 0.0);
+    }
+    if (cplx)
+      for (int b = 0; b < batch; ++b) split_to_interleaved(st, y + out_sz * b, out + 2 * (size_t)m0 * m1 * b, m0, m1);
+    HIPCHECK(hipStreamSynchronize(st));
+  }
+  void fdma_rows_launch(double* X, int ld, int nrows, int m, const Solver& sv) {
+    if (fdma_rows_wave_supported(m)) fdma_rows_wave(st, X, ld, nrows, m, sv.nsets, sv.low.p, sv.dia.p, sv.up1.p, sv.up2.p);
+    else fdma_rows(st, X, ld, nrows, m, sv.nsets, sv.low.p, sv.dia.p, sv.up1.p, sv.up2.p);
+  }
+};
+
 }  // namespace rbc
 
 // =================================================================== C ABI
@@ -1056,6 +1126,9 @@ struct rbc_navier {
 };
 struct fs_space2 {
   std::unique_ptr<Space> s;
+};
+struct rbc_solver {
+  std::unique_ptr<rbc::SolverDev> s;
 };
 
 static thread_local std::string g_err;
@@ -1344,6 +1417,74 @@ int fs_gradient(fs_space2* sp, const double* in, double* out, int d0, int d1, do
   return guarded([&] {
     if (!sp || !sp->s || !in || !out || batch < 1 || d0 < 0 || d1 < 0) throw std::invalid_argument("bad argument");
     sp->s->run(4, in, out, batch, d0, d1, scale0, scale1);
+  });
+}
+
+
+int rbc_solver_create(int solver, int kind0, int n0, int kind1, int n1, double c0, double c1, rbc_solver** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null out");
+    if (solver < 0 || solver > 2) throw std::invalid_argument("unknown solver kind");
+    if (kind0 < 0 || kind0 > 4 || kind1 < 0 || kind1 > 4) throw std::invalid_argument("unknown base kind");
+    auto* h = new rbc_solver();
+    try {
+      h->s = std::make_unique<rbc::SolverDev>(solver, kind0, n0, kind1, n1, c0, c1);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int rbc_solver_destroy(rbc_solver* h) {
+  return guarded([&] { delete h; });
+}
+int rbc_solver_shape(rbc_solver* h, int* o0, int* o1, int* s0, int* s1, int* is_complex) {
+  return guarded([&] {
+    if (!h || !h->s) throw std::invalid_argument("null handle");
+    rbc::SolverDev& s = *h->s;
+    *o0 = s.b0.ortho(); *o1 = s.b1.ortho(); *s0 = s.b0.m; *s1 = s.b1.m; *is_complex = s.cplx ? 1 : 0;
+  });
+}
+int rbc_solver_solve(rbc_solver* h, const double* d_rhs, double* d_out, int batch) {
+  return guarded([&] {
+    if (!h || !h->s || !d_rhs || !d_out || batch < 1) throw std::invalid_argument("bad argument");
+    h->s->solve(d_rhs, d_out, batch);
+  });
+}
+int rbc_solver_tensor_info(rbc_solver* h, int* n, int* has_fb) {
+  return guarded([&] {
+    if (!h || !h->s) throw std::invalid_argument("null handle");
+    const rbc::Solver& sv = *h->s->s;
+    if (sv.kind == rbc::ADI) { *n = 0; *has_fb = 0; return; }
+    *n = (int)sv.lam.size();
+    *has_fb = sv.has_fb ? 1 : 0;
+  });
+}
+int rbc_solver_get_tensor(rbc_solver* h, double* lam, double* fwd, double* bwd) {
+  return guarded([&] {
+    if (!h || !h->s) throw std::invalid_argument("null handle");
+    const rbc::Solver& sv = *h->s->s;
+    if (sv.kind == rbc::ADI) throw std::invalid_argument("solver is HholtzAdi (no tensor)");
+    std::memcpy(lam, sv.lam.data(), sv.lam.size() * sizeof(double));
+    if (sv.has_fb) {
+      std::memcpy(fwd, sv.fwd.data(), sv.fwd.size() * sizeof(double));
+      std::memcpy(bwd, sv.bwd.data(), sv.bwd.size() * sizeof(double));
+    }
+  });
+}
+int rbc_solver_set_tensor(rbc_solver* h, const double* lam, const double* fwd, const double* bwd) {
+  return guarded([&] {
+    if (!h || !h->s) throw std::invalid_argument("null handle");
+    rbc::Solver& sv = *h->s->s;
+    if (sv.kind == rbc::ADI) throw std::invalid_argument("solver is HholtzAdi (no tensor)");
+    std::memcpy(sv.lam.data(), lam, sv.lam.size() * sizeof(double));
+    if (sv.has_fb) {
+      if (!fwd || !bwd) throw std::invalid_argument("fwd/bwd required");
+      std::memcpy(sv.fwd.data(), fwd, sv.fwd.size() * sizeof(double));
+      std::memcpy(sv.bwd.data(), bwd, sv.bwd.size() * sizeof(double));
+    }
+    sv.build_rows();
   });
 }
 

@@ -14,6 +14,11 @@ void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const do
 void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
                int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
 
+// Batched: operand b of batch index z starts at A + z sA, B + z sB, C + z sC, U + z sU.
+void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
+                       long sB, double* C, int ldc, long sC, int batch, double alpha, double beta,
+                       const double* U = nullptr, int ldu = 0, long sU = 0);
+
 // Banded 4-diagonal solves (offsets -2, 0, 2, 4; pre-swept as fdma.rs:73-82) of every
 // row r of X[nrows, m] with diagonal set (r % nsets); diag arrays are [nsets, m].
 void fdma_rows(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low,
@@ -22,28 +27,29 @@ void fdma_rows(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, co
 // Same solves, one wavefront per row with a scan over slice maps (rows up to 2048).
 bool fdma_rows_wave_supported(int m);
 void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, const double* low,
-                    const double* dia, const double* up1, const double* up2);
+                    const double* dia, const double* up1, const double* up2, int batch = 1, long sx = 0);
 
 // Banded operator with taps at input offsets {0, -2, +2, +4} (summed in that order),
 // coefficients coef[k * mo + i] and presence bits mask[i]; out = op(in) [+ beta out].
 // band_y acts along axis 1 of in[rows, *] (leading dims ldi/ldo), band_x along axis 0
 // of in[*, cols].
 void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int mo, const double* coef,
-            const unsigned char* mask, double beta);
+            const unsigned char* mask, double beta, int batch = 1, long si = 0, long so = 0);
 void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
-            const unsigned char* mask, double beta);
+            const unsigned char* mask, double beta, int batch = 1, long si = 0, long so = 0);
 
 // Chebyshev d/dy of every row of in[rows, *] (ortho length n), with the composite
 // stencil of band coef/mask fused on load (coef == nullptr: input already ortho);
 // out = d / sc [+ beta out].
 bool cheb_diff_y_supported(int n);
 void cheb_diff_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, const double* coef,
-                 const unsigned char* mask, double inv_sc, double beta);
+                 const unsigned char* mask, double inv_sc, double beta, int batch = 1, long si = 0, long so = 0);
 // from_ortho of every row (composite length m): d = S^T p then the tridiagonal solve with
 // the precomputed w and den of linalg.rs:14-57.
 bool tdma_y_supported(int m);
 void tdma_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int m, const double* diag,
-            const double* low2, const double* off, const double* w, const double* den);
+            const double* low2, const double* off, const double* w, const double* den, int batch = 1, long si = 0,
+            long so = 0);
 
 // Fourier derivative on split planes: X[2m, cols] -> Y, (i k)^times / sc^times
 // (r2c.rs:88-99; num-complex division by (sc, 0) as space2.rs:256-262).
@@ -77,8 +83,10 @@ void ew_nuvol(hipStream_t s, double* out, const double* d, const double* t, cons
 // rfft_cols: real -> split planes [2(n/2+1), cols], rows k >= cut set to zero;
 // irfft_cols: split planes -> real, with X_k first multiplied by (i k)^times / sc.
 bool fft_supported(int n);
-void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut);
-void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc);
+void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch = 1, long sx = 0,
+               long sy = 0);
+void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch = 1,
+                long sx = 0, long sy = 0);
 
 // complex layout conversion: interleaved (re, im) [m, c] <-> split planes [2m, c]
 void interleaved_to_split(hipStream_t s, const double* in, double* out, int m, int c);
