@@ -38,7 +38,8 @@ WORKLOAD_NAMES = {
     "c3": "Navier2D periodic RBC 256x257 Ra1e7 Pr1 aspect2 dt2e-3 (FP64)",
     "c1": "Navier2D confined RBC 64x64 Ra1e5 Pr1 aspect1 dt2e-2 adiabatic (FP64)",
 }
-KERNEL_CLASSES = {0: "gemm_mfma_kernel", 1: "fdma_rows_wave_kernel", 2: "fft_cols_kernels", 3: "banded_kernels"}
+KERNEL_CLASSES = {0: "gemm_mfma_kernel", 1: "fdma_rows_wave_kernel", 2: "fft_cols_kernels", 3: "banded_kernels",
+                  4: "cheb_y_kernels"}
 
 
 def make(cfg, lib_mod):

@@ -107,22 +107,29 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *   RBC_OPT_GRAPH  replay update() as a captured HIP graph (default 1)
  *   RBC_OPT_MFMA   dense transforms on the FP64 matrix cores (default 1; 0 = FMA GEMM)
  *   RBC_OPT_FFT    Fourier x transforms by FFT kernels (default 1; 0 = dense DFT)
- *   RBC_OPT_BANDS  stencil / preconditioner as banded kernels (default 1; 0 = dense) */
+ *   RBC_OPT_BANDS  stencil / preconditioner as banded kernels (default 1; 0 = dense)
+ *   RBC_OPT_DCT    Chebyshev y transforms by FFT-based DCT-I kernels when ny - 1 is a
+ *                  power of two (default 1; 0 = dense transform GEMMs) */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2
 #define RBC_OPT_BANDS 3
+#define RBC_OPT_DCT 4
 int rbc_set_option(rbc_navier* h, int option, int value);
 
 /* Kernel timing of a handle (no counterpart in the reference, whose timing protocol is
  * the external criterion bench benches/benchmark_navier.rs:23-40).  When enabled, HIP
  * events are recorded on the handle's stream around each launch of a kernel class;
  * rbc_timing_read returns the launches, their summed duration and summed algorithmic
- * work (flops for GEMMs, bytes for the memory-bound classes) since enabling. */
+ * work (flops for GEMMs, bytes for the memory-bound classes) since enabling.  Classes:
+ * RBC_K_GEMM dense transform GEMMs, RBC_K_ROWSOLVE banded row solves, RBC_K_FFT Fourier x
+ * transforms and derivatives, RBC_K_ELEMENTWISE banded stencils, scans and elementwise
+ * kernels, RBC_K_DCT Chebyshev y transforms. */
 #define RBC_K_GEMM 0
 #define RBC_K_ROWSOLVE 1
 #define RBC_K_FFT 2
 #define RBC_K_ELEMENTWISE 3
+#define RBC_K_DCT 4
 int rbc_timing_enable(rbc_navier* h, int on);
 int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work);
 

@@ -335,7 +335,7 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
 // the launch's algorithmic work (flops for GEMMs, bytes for memory-bound kernels).
 class KernelTimer {
  public:
-  static constexpr int NCAT = 4;  // RBC_K_GEMM, RBC_K_ROWSOLVE, RBC_K_FFT, RBC_K_ELEMENTWISE
+  static constexpr int NCAT = 5;  // RBC_K_GEMM, RBC_K_ROWSOLVE, RBC_K_FFT, RBC_K_ELEMENTWISE, RBC_K_DCT
   bool on = false;
   struct Rec { int cat; double work; hipEvent_t a, b; };
   ~KernelTimer() { clear(); for (auto e : pool_) (void)hipEventDestroy(e); }
@@ -381,8 +381,8 @@ class KernelTimer {
   }
   std::vector<Rec> recs_;
   std::vector<hipEvent_t> pool_;
-  long long n_[NCAT] = {0, 0, 0, 0};
-  double ms_[NCAT] = {0, 0, 0, 0}, work_[NCAT] = {0, 0, 0, 0};
+  long long n_[NCAT] = {0, 0, 0, 0, 0};
+  double ms_[NCAT] = {0, 0, 0, 0, 0}, work_[NCAT] = {0, 0, 0, 0, 0};
 };
 
 // ----------------------------------------------------------------------- Navier2D
@@ -410,6 +410,7 @@ class Navier {
   bool use_fft = true;
   bool use_mfma = true;   // FP64 GEMMs on the matrix cores
   bool use_graph = true;  // replay update() as a captured HIP graph
+  bool use_dct = true;    // Chebyshev y transforms by FFT-based DCT-I kernels
   hipGraphExec_t gexec = nullptr;
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -625,6 +626,23 @@ class Navier {
                   beta);
       timer.end(st);
       return out;
+    }
+    if (use_dct && (b.k == CHEB || b.k == CD || b.k == CN) && cheb_y_supported(b.n)) {
+      if (name == "BS" || name == "BDS") {
+        const DBand* band = b.k == CHEB ? nullptr : &stencil_band(b);
+        const int times = name == "BDS" ? 1 : 0;
+        timer.begin(st, 4, 8.0 * (double)rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
+        cheb_y_backward(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr,
+                        band ? band->mask.p : nullptr, times, 1.0 / scale[1], U, b.n, beta);
+        timer.end(st);
+        return out;
+      }
+      if ((name == "Fdl" || name == "FS") && b.k == CHEB && !U && beta == 0.0) {
+        timer.begin(st, 4, 8.0 * (double)rows * 2.0 * b.n);
+        cheb_y_forward(st, in, b.n, out, b.n, rows, b.n, name == "Fdl" ? b.n * 2 / 3 : b.n);
+        timer.end(st);
+        return out;
+      }
     }
     if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_y_supported(b.m)) {
       const DTdma& t = tdma_tables(b);
@@ -982,8 +1000,24 @@ class Space {
     gemm(st, op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 1.0, 0.0);
     return out;
   }
+  std::unique_ptr<DBand> band;  // stencil taps of a composite axis-1 basis
   const double* yop(const std::string& name, const double* in, int rows, double* out) {
     if (b1.k == CHEB && (name == "S" || name == "FO")) return in;
+    if ((b1.k == CHEB || b1.k == CD || b1.k == CN) && cheb_y_supported(b1.n)) {  // FFT-based DCT-I
+      if (name == "BS") {
+        if (b1.k != CHEB && !band) {
+          band = std::make_unique<DBand>();
+          make_stencil_band(b1, *band);
+        }
+        cheb_y_backward(st, in, b1.m, b1.m, out, b1.n, rows, b1.n, band ? band->coef.p : nullptr,
+                        band ? band->mask.p : nullptr, 0, 1.0, nullptr, 0, 0.0);
+        return out;
+      }
+      if (name == "FS" && b1.k == CHEB) {
+        cheb_y_forward(st, in, b1.n, out, b1.n, rows, b1.n, b1.n);
+        return out;
+      }
+    }
     const DOp& op = Y.get(b1, name);
     gemm(st, rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, 1.0, 0.0);
     return out;
@@ -1347,6 +1381,7 @@ int rbc_set_option(rbc_navier* h, int option, int value) {
       case RBC_OPT_MFMA: n.use_mfma = value != 0; break;
       case RBC_OPT_FFT: n.use_fft = value != 0; break;
       case RBC_OPT_BANDS: n.use_bands = value != 0; break;
+      case RBC_OPT_DCT: n.use_dct = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
   });

@@ -88,6 +88,18 @@ void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int c
 void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch = 1,
                 long sx = 0, long sy = 0);
 
+// Chebyshev transforms along axis 1 (rows of length n, n - 1 a power of two in [16, 2048])
+// by FFT-based DCT-I.  cheb_y_backward: in [rows, mi] coefficients (composite basis when
+// coef / mask give its stencil taps, else Chebyshev, mi <= n), `times` (0 or 1)
+// derivatives scaled by inv_sc, out [rows, n] = x .* U + beta out (U may be null).
+// cheb_y_forward: in [rows, n] values -> out [rows, n] Chebyshev coefficients, k >= cut zero.
+bool cheb_y_supported(int n);
+void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
+                     const double* coef, const unsigned char* mask, int times, double inv_sc, const double* U, int ldu,
+                     double beta, int batch = 1, long si = 0, long so = 0, long su = 0);
+void cheb_y_forward(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, int cut,
+                    int batch = 1, long si = 0, long so = 0);
+
 // complex layout conversion: interleaved (re, im) [m, c] <-> split planes [2m, c]
 void interleaved_to_split(hipStream_t s, const double* in, double* out, int m, int c);
 void split_to_interleaved(hipStream_t s, const double* in, double* out, int m, int c);

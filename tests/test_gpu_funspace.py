@@ -1,7 +1,8 @@
 """GPU funspace Space2 operations (fs_* C entry points) against the oracle.
 
 Bases and operations of funspace/src/space2.rs:136-356; tolerance 1e-12 relative to the
-max-norm of the oracle's result (FP64 operators applied as GEMMs vs lane recurrences).
+max-norm of the oracle's result (FP64 operators applied as GEMMs or, along axis 1 for
+n1 - 1 a power of two, as FFT-based DCT-I kernels, vs the oracle's lane recurrences).
 """
 import numpy as np
 import pytest
@@ -18,6 +19,12 @@ CASES = [  # (kind0, n0, kind1, n1)
     (4, 16, 2, 17),   # fourier_r2c x cheb_neumann
     (4, 64, 0, 65),   # fourier_r2c x chebyshev
     (0, 18, 3, 18),   # chebyshev x cheb_dirichlet_bc
+]
+
+DCT_CASES = [  # long axis-1 rows: transforms by the FFT-based DCT-I kernels
+    (0, 12, 0, 513),   # chebyshev x chebyshev
+    (1, 10, 2, 1025),  # cheb_dirichlet x cheb_neumann
+    (4, 8, 1, 2049),   # fourier_r2c x cheb_dirichlet
 ]
 
 
@@ -59,3 +66,27 @@ def test_space2_batched(oracle):
     got = sp.forward(v)
     for b in range(3):
         assert rel_err(got[b], oracle.space_apply(4, 32, 1, 17, "forward", v[b])) < 1e-12
+
+
+@pytest.mark.parametrize("case", DCT_CASES)
+def test_space2_dct_rows(oracle, case):
+    """forward / backward with long Chebyshev rows against the oracle, and the round trip."""
+    from rustpde_amd import Space2
+    k0, n0, k1, n1 = case
+    print("case fs dct", case, flush=True)
+    sp = Space2((k0, n0), (k1, n1))
+    cplx = sp.is_complex
+    rng = np.random.default_rng(7)
+    # a smooth field (decaying spectrum), as the transforms see in the time step
+    vh = _rand(sp.shape_spectral, cplx, 8) / (1.0 + np.arange(sp.shape_spectral[1]))[None, :] ** 2
+    if cplx:  # the real transform keeps no imaginary part at k = 0 and k = n0 / 2
+        vh[0] = vh[0].real
+        vh[-1] = vh[-1].real
+    v = sp.backward(vh)
+    assert rel_err(v, oracle.space_apply(k0, n0, k1, n1, "backward", vh)) < 1e-12
+    f = sp.forward(v)
+    assert rel_err(f, oracle.space_apply(k0, n0, k1, n1, "forward", v)) < 1e-12
+    assert rel_err(f, vh) < 1e-12  # forward(backward(vh)) = vh
+    w = rng.standard_normal(sp.shape_physical)
+    assert rel_err(sp.forward(w), oracle.space_apply(k0, n0, k1, n1, "forward", w)) < 1e-12
+

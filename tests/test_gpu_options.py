@@ -1,7 +1,9 @@
 """Equivalent implementations of the GPU step agree with the oracle and with each other.
 
 Options (rbc_set_option): HIP-graph replay, FP64 MFMA GEMMs, FFT kernels for the
-Fourier axis, banded kernels for stencils/preconditioners.  Tolerances as in
+Fourier axis, banded kernels for stencils/preconditioners, FFT-based DCT-I kernels for
+the Chebyshev y axis (active when ny - 1 is a power of two: the periodic 128x129 case
+and the confined 48x65 case below).  Tolerances as in
 test_gpu_navier.py (1e-12 for T/ux/uy/p after the steps taken).
 """
 import numpy as np
@@ -11,7 +13,7 @@ from helpers import compare, init_pair
 
 pytestmark = pytest.mark.gpu
 
-OPTS = {"graph": 0, "mfma": 1, "fft": 2, "bands": 3}
+OPTS = {"graph": 0, "mfma": 1, "fft": 2, "bands": 3, "dct": 4}
 
 
 def _pair(oracle, periodic):
@@ -22,7 +24,7 @@ def _pair(oracle, periodic):
         o = oracle.Navier2D.new_periodic(nx, ny, 1e6, 1.0, 0.004, 2.0)
         init_pair(g, o, nx, ny, True, 1e-3, 11, "gaussian", velocity=(0.05, 2.0, 1.0))
     else:
-        nx, ny = 48, 40
+        nx, ny = 48, 65
         g = Navier2D.new(nx, ny, 1e5, 1.0, 0.01, 1.0, True)
         o = oracle.Navier2D.new(nx, ny, 1e5, 1.0, 0.01, 1.0, True)
         init_pair(g, o, nx, ny, False, 0.05, 11, "uniform", velocity=(0.2, 1.0, 1.0))
@@ -30,7 +32,7 @@ def _pair(oracle, periodic):
 
 
 @pytest.mark.parametrize("periodic", [False, True])
-@pytest.mark.parametrize("off", [None, "graph", "mfma", "fft", "bands"])
+@pytest.mark.parametrize("off", [None, "graph", "mfma", "fft", "bands", "dct"])
 def test_option_parity(oracle, periodic, off):
     print("case options periodic=%s off=%s" % (periodic, off), flush=True)
     g, o = _pair(oracle, periodic)
