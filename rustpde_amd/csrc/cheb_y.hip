@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "fft_lds.hpp"
 #include "ops.hpp"
 
 namespace rbc {
@@ -35,11 +36,12 @@ void check(const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// e^{-2 pi i t / L}, t < L, interleaved (re, im), per (device, L)
 std::mutex g_mutex;
 std::map<std::pair<int, int>, double*> g_tables;
 
-const double* table(int L) {
+}  // namespace
+
+const double* fft_table(int L) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lock(g_mutex);
@@ -61,6 +63,9 @@ const double* table(int L) {
   return d;
 }
 
+namespace {
+using fftlds::lds_dft;
+
 // block geometry for an FFT length L: E = P L complex values per block (P row pairs),
 // T = E / 8 threads, so every pass gives each thread 8 values
 template <int L>
@@ -72,132 +77,34 @@ struct Geo {
   static constexpr int n = N + 1;
 };
 
-constexpr double SQH = 0.70710678118654752440;  // sqrt(1/2)
-
-// forward DFTs of 2, 4, 8 points (sign -1), in place, natural order
-__device__ __forceinline__ void dft2(double* r, double* i) {
-  const double ar = r[0], ai = i[0];
-  r[0] = ar + r[1];
-  i[0] = ai + i[1];
-  r[1] = ar - r[1];
-  i[1] = ai - i[1];
-}
-__device__ __forceinline__ void dft4(double& r0, double& i0, double& r1, double& i1, double& r2, double& i2, double& r3,
-                                     double& i3) {
-  const double t0r = r0 + r2, t0i = i0 + i2, t1r = r0 - r2, t1i = i0 - i2;
-  const double t2r = r1 + r3, t2i = i1 + i3;
-  const double t3r = i1 - i3, t3i = r3 - r1;  // (x1 - x3) (-i)
-  r0 = t0r + t2r;
-  i0 = t0i + t2i;
-  r2 = t0r - t2r;
-  i2 = t0i - t2i;
-  r1 = t1r + t3r;
-  i1 = t1i + t3i;
-  r3 = t1r - t3r;
-  i3 = t1i - t3i;
-}
-__device__ __forceinline__ void dft8(double* r, double* i) {
-  dft4(r[0], i[0], r[2], i[2], r[4], i[4], r[6], i[6]);  // E_k in slots 0, 2, 4, 6
-  dft4(r[1], i[1], r[3], i[3], r[5], i[5], r[7], i[7]);  // O_k in slots 1, 3, 5, 7
-  // O_1 *= e^{-i pi/4}, O_2 *= -i, O_3 *= e^{-3 i pi/4}
-  double o1r = SQH * (r[3] + i[3]), o1i = SQH * (i[3] - r[3]);
-  double o2r = i[5], o2i = -r[5];
-  double o3r = SQH * (i[7] - r[7]), o3i = -SQH * (r[7] + i[7]);
-  const double e0r = r[0], e0i = i[0], e1r = r[2], e1i = i[2], e2r = r[4], e2i = i[4], e3r = r[6], e3i = i[6];
-  const double o0r = r[1], o0i = i[1];
-  r[0] = e0r + o0r; i[0] = e0i + o0i;
-  r[4] = e0r - o0r; i[4] = e0i - o0i;
-  r[1] = e1r + o1r; i[1] = e1i + o1i;
-  r[5] = e1r - o1r; i[5] = e1i - o1i;
-  r[2] = e2r + o2r; i[2] = e2i + o2i;
-  r[6] = e2r - o2r; i[6] = e2i - o2i;
-  r[3] = e3r + o3r; i[3] = e3i + o3i;
-  r[7] = e3r - o3r; i[7] = e3i - o3i;
-}
-
-// one Stockham pass of radix R over the P transforms of length L held in (re, im)
-template <int L, int R>
-__device__ __forceinline__ void stockham_pass(double* re, double* im, int Ns, const double* __restrict__ tw) {
-  using G = Geo<L>;
-  constexpr int NB = L / R;            // butterflies per transform
-  constexpr int PER = G::E / (R * G::T);  // butterflies per thread (8 / R)
-  double xr[PER][R], xi[PER][R];
-  int dst[PER];
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int g = threadIdx.x + q * G::T;
-    const int p = g / NB, j = g - p * NB;
-    const int k = j & (Ns - 1);
-    const int base = p * L + j;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      xr[q][r] = re[base + r * NB];
-      xi[q][r] = im[base + r * NB];
-    }
-    const int step = L / (Ns * R);  // twiddle e^{-2 pi i r k / (Ns R)} = table[r k step]
-#pragma unroll
-    for (int r = 1; r < R; ++r) {
-      const int t = r * k * step;
-      const double wr = tw[2 * t], wi = tw[2 * t + 1];
-      const double ar = xr[q][r], ai = xi[q][r];
-      xr[q][r] = ar * wr - ai * wi;
-      xi[q][r] = ar * wi + ai * wr;
-    }
-    if constexpr (R == 8) dft8(xr[q], xi[q]);
-    else if constexpr (R == 4) dft4(xr[q][0], xi[q][0], xr[q][1], xi[q][1], xr[q][2], xi[q][2], xr[q][3], xi[q][3]);
-    else dft2(xr[q], xi[q]);
-    dst[q] = p * L + (j / Ns) * Ns * R + k;
+// stencil value s_k = d_k c_k + l_k c_{k-2} of one row (composite_stencil.rs:207-229);
+// d = coef[0, n), l = coef[n, 2n) hold zeros where the stencil has no entry, which leaves
+// the sum of the present terms unchanged.  Without coef the row is Chebyshev already.
+struct StencilLoad {
+  double c0, c2, d, l;
+};
+__device__ __forceinline__ StencilLoad stencil_load(const double* __restrict__ row, bool valid, int k, int mi, int n,
+                                                    const double* __restrict__ coef) {
+  StencilLoad v{0.0, 0.0, 1.0, 0.0};
+  if (k >= n) return v;
+  if (valid && k < mi) v.c0 = row[k];
+  if (coef) {
+    v.d = coef[k];
+    v.l = coef[n + k];
+    if (valid && k >= 2 && k - 2 < mi) v.c2 = row[k - 2];
   }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PER; ++q)
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      re[dst[q] + r * Ns] = xr[q][r];
-      im[dst[q] + r * Ns] = xi[q][r];
-    }
-  __syncthreads();
+  return v;
 }
-
-constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
-
-template <int L>
-__device__ __forceinline__ void lds_dft(double* re, double* im, const double* __restrict__ tw) {
-  constexpr int NUM8 = ilog2(L) / 3, REM = ilog2(L) % 3;
-  int Ns = 1;
-#pragma unroll
-  for (int s = 0; s < NUM8; ++s) {
-    stockham_pass<L, 8>(re, im, Ns, tw);
-    Ns *= 8;
-  }
-  if constexpr (REM == 2) stockham_pass<L, 4>(re, im, Ns, tw);
-  else if constexpr (REM == 1) stockham_pass<L, 2>(re, im, Ns, tw);
-}
-
-// stencil value s_k of one row (taps at offsets 0, -2, +2, +4, summed in that order)
-__device__ __forceinline__ double stencil_at(const double* __restrict__ row, int k, int mi, int n,
-                                             const double* __restrict__ coef, const unsigned char* __restrict__ mask) {
+__device__ __forceinline__ double stencil_value(const StencilLoad& v, bool composite) {
 #pragma clang fp contract(off)
-  if (!coef) return k < mi ? row[k] : 0.0;
-  const unsigned mk = mask[k];
-  constexpr int off[4] = {0, -2, 2, 4};
-  double acc = 0.0;
-  bool first = true;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    if (!(mk & (1u << t))) continue;
-    const double term = row[k + off[t]] * coef[t * n + k];
-    acc = first ? term : acc + term;
-    first = false;
-  }
-  return acc;
+  return composite ? v.d * v.c0 + v.l * v.c2 : v.c0;
 }
 
 template <int L>
 __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     const double* __restrict__ in, int ldi, int mi, double* __restrict__ out, int ldo, int rows,
-    const double* __restrict__ coef, const unsigned char* __restrict__ mask, int times, double inv_sc,
-    const double* __restrict__ U, int ldu, double beta, const double* __restrict__ tw, long si, long so, long su) {
+    const double* __restrict__ coef, int times, double inv_sc, const double* __restrict__ U, int ldu, double beta,
+    const double* __restrict__ tw, long si, long so, long su) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
   __shared__ double re[G::E], im[G::E];
@@ -206,16 +113,31 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
   if (U) U += su * blockIdx.y;
   const int tid = threadIdx.x, row0 = 2 * P * blockIdx.x;
   // (1) Chebyshev coefficients of both rows, or g_k of the derivative recurrence:
-  //     g_0 = s_1, g_k = 2 (k + 1) s_{k+1} (1 <= k <= n - 2), g_{n-1} = 0
-  for (int e = tid; e < P * n; e += T) {
+  //     g_0 = s_1, g_k = 2 (k + 1) s_{k+1} (1 <= k <= n - 2), g_{n-1} = 0.
+  //     All loads of a thread are issued before any is used.
+  constexpr int Q = (P * n + T - 1) / T;
+  StencilLoad la[Q], lb[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + q * T;
     const int p = e / n, k = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
-    const int kk = times ? k + 1 : k;
-    double sa = 0.0, sb = 0.0;
-    if (kk < n && (!times || k < n - 1)) {
-      if (ra < rows) sa = stencil_at(in + (size_t)ra * ldi, kk, mi, n, coef, mask);
-      if (rb < rows) sb = stencil_at(in + (size_t)rb * ldi, kk, mi, n, coef, mask);
-      if (times && k > 0) {
+    const int kk = (e < P * n) ? (times ? k + 1 : k) : n;  // kk = n loads nothing
+    la[q] = stencil_load(in + (size_t)ra * ldi, ra < rows, kk, mi, n, coef);
+    lb[q] = stencil_load(in + (size_t)rb * ldi, rb < rows, kk, mi, n, coef);
+  }
+  const bool comp = coef != nullptr;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + q * T;
+    if (e >= P * n) continue;
+    const int p = e / n, k = e - p * n;
+    double sa = stencil_value(la[q], comp), sb = stencil_value(lb[q], comp);
+    if (times) {
+      if (k == n - 1) {
+        sa = 0.0;
+        sb = 0.0;
+      } else if (k > 0) {
         const double f = 2.0 * (double)(k + 1);
         sa = f * sa;
         sb = f * sb;
@@ -280,21 +202,37 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     }
   }
   __syncthreads();
-  lds_dft<L>(re, im, tw);
-  // (4) x_j = Re / Im Z_j, j <= N; out = x .* U + beta out
-  for (int e = tid; e < P * n; e += T) {
-    const int p = e / n, j = e - p * n;
-    const int ra = row0 + 2 * p;
+  lds_dft<L, T, G::E>(re, im, tw);
+  // (4) x_j = Re / Im Z_j, j <= N; out = x .* U + beta out (loads first)
+  double ua[Q], ub[Q], oa[Q], ob[Q];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = ra + h;
-      if (r >= rows) continue;
-      double v = h ? im[p * L + j] : re[p * L + j];
-      double* o = out + (size_t)r * ldo + j;
-      if (U) v = v * U[(size_t)r * ldu + j];
-      if (beta != 0.0) v = v + beta * (*o);
-      *o = v;
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + q * T;
+    const int p = e / n, j = e - p * n;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    const bool ok = e < P * n;
+    ua[q] = (U && ok && ra < rows) ? U[(size_t)ra * ldu + j] : 1.0;
+    ub[q] = (U && ok && rb < rows) ? U[(size_t)rb * ldu + j] : 1.0;
+    oa[q] = (beta != 0.0 && ok && ra < rows) ? out[(size_t)ra * ldo + j] : 0.0;
+    ob[q] = (beta != 0.0 && ok && rb < rows) ? out[(size_t)rb * ldo + j] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + q * T;
+    if (e >= P * n) continue;
+    const int p = e / n, j = e - p * n;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    double va = re[p * L + j], vb = im[p * L + j];
+    if (U) {
+      va = va * ua[q];
+      vb = vb * ub[q];
     }
+    if (beta != 0.0) {
+      va = va + beta * oa[q];
+      vb = vb + beta * ob[q];
+    }
+    if (ra < rows) out[(size_t)ra * ldo + j] = va;
+    if (rb < rows) out[(size_t)rb * ldo + j] = vb;
   }
 }
 
@@ -308,11 +246,23 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const double* __r
   in += si * blockIdx.y;
   out += so * blockIdx.y;
   const int tid = threadIdx.x, row0 = 2 * P * blockIdx.x;
-  for (int e = tid; e < P * n; e += T) {
+  constexpr int Q = (P * n + T - 1) / T;
+  double va[Q], vb[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + q * T;
     const int p = e / n, k = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
-    const double a = ra < rows ? in[(size_t)ra * ldi + k] : 0.0;
-    const double b = rb < rows ? in[(size_t)rb * ldi + k] : 0.0;
+    const bool ok = e < P * n;
+    va[q] = (ok && ra < rows) ? in[(size_t)ra * ldi + k] : 0.0;
+    vb[q] = (ok && rb < rows) ? in[(size_t)rb * ldi + k] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + q * T;
+    if (e >= P * n) continue;
+    const int p = e / n, k = e - p * n;
+    const double a = va[q], b = vb[q];
     re[p * L + k] = a;
     im[p * L + k] = b;
     if (k > 0 && k < N) {
@@ -321,7 +271,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const double* __r
     }
   }
   __syncthreads();
-  lds_dft<L>(re, im, tw);
+  lds_dft<L, T, G::E>(re, im, tw);
   constexpr double invN = 1.0 / (double)N;  // exact: N is a power of two
   for (int e = tid; e < P * n; e += T) {
     const int p = e / n, k = e - p * n;
@@ -343,8 +293,9 @@ void launch_bwd(hipStream_t s, const double* in, int ldi, int mi, double* out, i
                 long si, long so, long su) {
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
+  (void)mask;  // the zero coefficients of absent taps stand for the mask
   hipLaunchKernelGGL(cheb_y_bwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
-                     mask, times, inv_sc, U, ldu, beta, table(L), si, so, su);
+                     times, inv_sc, U, ldu, beta, fft_table(L), si, so, su);
 }
 
 template <int L>
@@ -353,7 +304,7 @@ void launch_fwd(hipStream_t s, const double* in, int ldi, double* out, int ldo, 
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
   hipLaunchKernelGGL(cheb_y_fwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, out, ldo, rows, cut,
-                     table(L), si, so);
+                     fft_table(L), si, so);
 }
 
 }  // namespace

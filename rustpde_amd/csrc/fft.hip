@@ -1,166 +1,124 @@
 // Batched real FFTs along axis 0 (the strided x axis) for the Fourier (fourier_r2c)
 // direction of the periodic step, FP64.
 //
-// A real transform of length n = 2N is done as one complex FFT of length N of
-// z_t = x_{2t} + i x_{2t+1} (radix-2 decimation in frequency, in LDS, bit-reversed
-// output) plus the usual split step.  Each workgroup owns C consecutive columns; loads
-// and stores of a row segment are contiguous.  Spectral data are split planes
-// (rows [0, N] real parts, rows [N+1, 2N+1] imaginary parts).
-//   r2c: X_k = sum_j x_j e^{-2 pi i jk/n}, k = 0..N   (r2c.rs:160-175, unnormalised)
-//   c2r: x_j = 1/n sum_k X_k e^{+2 pi i jk/n}         (r2c.rs:211-226), Im X_0 and Im X_N
+// Two real columns a, b of length n (a power of two) go through one complex DFT of
+// z = a + i b (radix-8 Stockham passes in LDS, fft_lds.hpp); their spectra follow from
+// Z by A_k = (Z_k + conj Z_{n-k}) / 2, B_k = (Z_k - conj Z_{n-k}) / (2i).  The inverse
+// builds Z_k = A_k + i B_k from the two half spectra (Hermitian extension) and runs the
+// forward DFT on conj Z.  A workgroup owns C adjacent columns; its loads and stores are
+// row segments of C doubles, and blocks are remapped so that neighbouring column groups
+// share an XCD.  Spectral data are split planes (rows [0, n/2] real parts, rows
+// [n/2 + 1, n + 1] imaginary parts).
+//   r2c: X_k = sum_j x_j e^{-2 pi i jk/n}, k = 0..n/2   (r2c.rs:160-175, unnormalised)
+//   c2r: x_j = 1/n sum_k X_k e^{+2 pi i jk/n}         (r2c.rs:211-226), Im X_0 and Im X_{n/2}
 //        are ignored (half-spectrum convention); an optional (i k)^d / sc is applied to
 //        X_k first (r2c.rs:88-99 with the num-complex division of space2.rs:256-262).
 #include <hip/hip_runtime.h>
 
-#include <cmath>
-#include <map>
-#include <memory>
-#include <mutex>
+#include <algorithm>
 #include <stdexcept>
 #include <string>
-#include <vector>
 
+#include "fft_lds.hpp"
 #include "ops.hpp"
 
 namespace rbc {
 
 namespace {
 
-constexpr int FFT_THREADS = 256;  // threads per workgroup
-// columns per workgroup: LDS use 2 * C * (n/2 + 1) doubles stays <= 64 KB
-inline int fft_cols_per_block(int n) { return n <= 1024 ? 4 : (n <= 2048 ? 2 : 1); }
+using fftlds::lds_dft;
+using fftlds::xcd_remap;
 
-struct Twiddles {
-  double* stage = nullptr;  // e^{-2 pi i t/N}, t < N/2, interleaved (re, im)
-  double* post = nullptr;   // e^{-2 pi i k/n}, k <= N, interleaved
+// geometry for length L: PP column pairs per block, E = PP L complex values, T = E / 8
+template <int L>
+struct CGeo {
+  static constexpr int PP = L >= 2048 ? 1 : 2048 / L;
+  static constexpr int E = PP * L;
+  static constexpr int T = E / 8;
+  static constexpr int C = 2 * PP;  // columns per block
+  static constexpr int N = L / 2;
+  static constexpr int m = N + 1;  // half-spectrum length
 };
 
-std::mutex g_tw_mutex;
-std::map<std::pair<int, int>, Twiddles> g_tw;  // (device, n) -> tables
-
-const Twiddles& twiddles(int n) {
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lock(g_tw_mutex);
-  auto key = std::make_pair(dev, n);
-  auto it = g_tw.find(key);
-  if (it != g_tw.end()) return it->second;
-  const int N = n / 2;
-  const long double pi = 3.141592653589793238462643383279502884L;
-  std::vector<double> st(2 * std::max(N / 2, 1)), po(2 * (N + 1));
-  for (int t = 0; t < N / 2; ++t) {
-    long double a = -2.0L * pi * (long double)t / (long double)N;
-    st[2 * t] = (double)cosl(a);
-    st[2 * t + 1] = (double)sinl(a);
-  }
-  for (int k = 0; k <= N; ++k) {
-    long double a = -2.0L * pi * (long double)k / (long double)n;
-    po[2 * k] = (double)cosl(a);
-    po[2 * k + 1] = (double)sinl(a);
-  }
-  Twiddles tw;
-  if (hipMalloc(&tw.stage, st.size() * sizeof(double)) != hipSuccess ||
-      hipMalloc(&tw.post, po.size() * sizeof(double)) != hipSuccess)
-    throw std::runtime_error("fft: twiddle allocation failed");
-  if (hipMemcpy(tw.stage, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(tw.post, po.data(), po.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-    throw std::runtime_error("fft: twiddle upload failed");
-  return g_tw.emplace(key, tw).first->second;
-}
-
-__device__ __forceinline__ int bitrev(int v, int bits) { return (int)(__brev((unsigned)v) >> (32 - bits)); }
-
-// in-place radix-2 DIF FFT of C columns of length N held in LDS (re, im), sign -1 (fwd)
-// or +1 (inverse, conjugate twiddles); output in bit-reversed order
-template <int C>
-__device__ void lds_fft(double* re, double* im, int N, int ldc, const double* __restrict__ tw, bool inverse) {
-#pragma clang fp contract(off)
-  const int tid = threadIdx.x;
-  for (int h = N / 2; h >= 1; h >>= 1) {
-    const int stride = N / (2 * h);
-    for (int e = tid; e < C * (N / 2); e += FFT_THREADS) {
-      const int c = e % C, q = e / C;
-      const int blk = q / h, i = q - blk * h;
-      const int i0 = c * ldc + blk * 2 * h + i, i1 = i0 + h;
-      const double wr = tw[2 * (i * stride)];
-      const double wi = inverse ? -tw[2 * (i * stride) + 1] : tw[2 * (i * stride) + 1];
-      const double ar = re[i0], ai = im[i0], br = re[i1], bi = im[i1];
-      re[i0] = ar + br;
-      im[i0] = ai + bi;
-      const double dr = ar - br, di = ai - bi;
-      re[i1] = dr * wr - di * wi;
-      im[i1] = dr * wi + di * wr;
-    }
-    __syncthreads();
-  }
-}
-
-template <int C>
-__global__ __launch_bounds__(FFT_THREADS) void rfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
-                                                                 int n, int cols, int cut,
-                                                                 const double* __restrict__ tw,
-                                                                 const double* __restrict__ post, long sx, long sy) {
-#pragma clang fp contract(off)
+template <int L>
+__global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
+                                                               int cols, int cut, const double* __restrict__ tw,
+                                                               long sx, long sy) {
+  using G = CGeo<L>;
+  constexpr int C = G::C, T = G::T, m = G::m;
+  __shared__ double re[G::E], im[G::E];
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
-  extern __shared__ double lds[];
-  const int N = n / 2, m = N + 1, bits = __ffs(N) - 1;
-  double* re = lds;
-  double* im = lds + C * N;
-  const int j0 = blockIdx.x * C, tid = threadIdx.x;
-  for (int e = tid; e < C * N; e += FFT_THREADS) {
-    const int c = e % C, t = e / C, j = j0 + c;
-    double a = 0.0, b = 0.0;
-    if (j < cols) {
-      a = X[(size_t)(2 * t) * cols + j];
-      b = X[(size_t)(2 * t + 1) * cols + j];
-    }
-    re[c * N + t] = a;
-    im[c * N + t] = b;
+  const int tid = threadIdx.x;
+  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
+  // load: row t, column c -> z_t of pair c / 2 (real part for even c, imaginary for odd)
+  constexpr int QL = L * C / T;
+  double v[QL];
+#pragma unroll
+  for (int q = 0; q < QL; ++q) {
+    const int e = tid + q * T, c = e % C, t = e / C, j = j0 + c;
+    v[q] = j < cols ? X[(size_t)t * cols + j] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < QL; ++q) {
+    const int e = tid + q * T, c = e % C, t = e / C;
+    ((c & 1) ? im : re)[(c >> 1) * L + t] = v[q];
   }
   __syncthreads();
-  lds_fft<C>(re, im, N, N, tw, false);
-  for (int e = tid; e < C * m; e += FFT_THREADS) {
+  lds_dft<L, T, G::E>(re, im, tw);
+  // split: A_k = (Z_k + conj Z_{L-k}) / 2, B_k = (Z_k - conj Z_{L-k}) / (2i)
+  for (int e = tid; e < m * C; e += T) {
     const int c = e % C, k = e / C, j = j0 + c;
     if (j >= cols) continue;
-    double xr = 0.0, xi = 0.0;
-    if (k < cut) {
-      const int p = c * N + bitrev(k % N, bits), q = c * N + bitrev((N - k) % N, bits);
-      const double ar = re[p], ai = im[p], br = re[q], bi = -im[q];  // B = conj(Z_{N-k})
-      const double er = 0.5 * (ar + br), ei = 0.5 * (ai + bi);
-      const double dr = ar - br, di = ai - bi;
-      const double orr = 0.5 * di, oi = -0.5 * dr;  // O = D / (2i)
-      const double wr = post[2 * k], wi = post[2 * k + 1];
-      xr = er + (orr * wr - oi * wi);
-      xi = ei + (orr * wi + oi * wr);
+    const int p = (c >> 1) * L, k2 = (L - k) & (L - 1);
+    const double zr = re[p + k], zi = im[p + k], yr = re[p + k2], yi = im[p + k2];
+    double xr, xi;
+    if (c & 1) {
+      xr = 0.5 * (zi + yi);
+      xi = 0.5 * (yr - zr);
+    } else {
+      xr = 0.5 * (zr + yr);
+      xi = 0.5 * (zi - yi);
+    }
+    if (k >= cut) {
+      xr = 0.0;
+      xi = 0.0;
     }
     Y[(size_t)k * cols + j] = xr;
     Y[(size_t)(m + k) * cols + j] = xi;
   }
 }
 
-template <int C>
-__global__ __launch_bounds__(FFT_THREADS) void irfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
-                                                                  int n, int cols, int times, double sc,
-                                                                  const double* __restrict__ tw,
-                                                                  const double* __restrict__ post, long sx, long sy) {
+template <int L>
+__global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
+                                                                int cols, int times, double sc,
+                                                                const double* __restrict__ tw, long sx, long sy) {
 #pragma clang fp contract(off)
+  using G = CGeo<L>;
+  constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
+  __shared__ double re[G::E], im[G::E];
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
-  extern __shared__ double lds[];
-  const int N = n / 2, m = N + 1, bits = __ffs(N) - 1, S = m;  // column stride S = N + 1
-  double* re = lds;
-  double* im = lds + C * S;
-  const int j0 = blockIdx.x * C, tid = threadIdx.x;
-  for (int e = tid; e < C * m; e += FFT_THREADS) {
-    const int c = e % C, k = e / C, j = j0 + c;
-    double a = 0.0, b = 0.0;
-    if (j < cols) {
-      a = X[(size_t)k * cols + j];
-      b = X[(size_t)(m + k) * cols + j];
-    }
+  const int tid = threadIdx.x;
+  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
+  // (1) half spectra, (i k)^times / sc applied, packed per pair: re[k] = Re A_k (k <= N),
+  //     re[N + k] = Im A_k (0 < k < N), the same for B in im
+  constexpr int QS = (m * C + T - 1) / T;
+  double va[QS], vb[QS];
+#pragma unroll
+  for (int q = 0; q < QS; ++q) {
+    const int e = tid + q * T, c = e % C, k = e / C, j = j0 + c;
+    const bool ok = e < m * C && j < cols;
+    va[q] = ok ? X[(size_t)k * cols + j] : 0.0;
+    vb[q] = ok ? X[(size_t)(m + k) * cols + j] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < QS; ++q) {
+    const int e = tid + q * T, c = e % C, k = e / C;
+    if (e >= m * C) continue;
+    double a = va[q], b = vb[q];
     const double kk = (double)k;
-    for (int q = 0; q < times; ++q) {
+    for (int r = 0; r < times; ++r) {
       const double na = a * 0.0 - b * kk, nb = a * kk + b * 0.0;
       a = na;
       b = nb;
@@ -171,45 +129,48 @@ __global__ __launch_bounds__(FFT_THREADS) void irfft_cols_kernel(const double* _
       a = na;
       b = nb;
     }
-    if (k == 0 || k == N) b = 0.0;
-    re[c * S + k] = a;
-    im[c * S + k] = b;
+    double* z = ((c & 1) ? im : re) + (c >> 1) * L;
+    z[k] = a;
+    if (k > 0 && k < N) z[N + k] = b;
   }
   __syncthreads();
-  // Z_k = E_k + i O_k with E_k = (X_k + conj X_{N-k}) / 2, O_k = (X_k - conj X_{N-k}) e^{+2 pi i k/n} / 2.
-  // One thread owns the pair (k, N-k) and overwrites both in place (k = 0 pairs with X_N).
-  for (int e = tid; e < C * (N / 2 + 1); e += FFT_THREADS) {
-    const int c = e % C, k = e / C, k2 = N - k;
-    const double xr = re[c * S + k], xi = im[c * S + k];
-    const double yr = re[c * S + k2], yi = im[c * S + k2];
-    {  // Z_k from (X_k, X_{N-k})
-      const double br = yr, bi = -yi;
-      const double er = 0.5 * (xr + br), ei = 0.5 * (xi + bi);
-      const double dr = 0.5 * (xr - br), di = 0.5 * (xi - bi);
-      const double wr = post[2 * k], wi = -post[2 * k + 1];
-      const double orr = dr * wr - di * wi, oi = dr * wi + di * wr;
-      re[c * S + k] = er - oi;
-      im[c * S + k] = ei + orr;
-    }
-    if (k2 != k && k2 < N) {  // Z_{N-k} from (X_{N-k}, X_k)
-      const double br = xr, bi = -xi;
-      const double er = 0.5 * (yr + br), ei = 0.5 * (yi + bi);
-      const double dr = 0.5 * (yr - br), di = 0.5 * (yi - bi);
-      const double wr = post[2 * k2], wi = -post[2 * k2 + 1];
-      const double orr = dr * wr - di * wi, oi = dr * wi + di * wr;
-      re[c * S + k2] = er - oi;
-      im[c * S + k2] = ei + orr;
+  // (2) conj Z_k = (Ar - Bi) - i (Ai + Br),  conj Z_{L-k} = (Ar + Bi) + i (Ai - Br)
+  constexpr int QZ = (m * G::PP + T - 1) / T;
+  double ar[QZ], ai[QZ], br[QZ], bi[QZ];
+#pragma unroll
+  for (int q = 0; q < QZ; ++q) {
+    const int e = tid + q * T, p = e / m, k = e - p * m;
+    ar[q] = ai[q] = br[q] = bi[q] = 0.0;
+    if (e < m * G::PP) {
+      ar[q] = re[p * L + k];
+      br[q] = im[p * L + k];
+      if (k > 0 && k < N) {
+        ai[q] = re[p * L + N + k];
+        bi[q] = im[p * L + N + k];
+      }
     }
   }
   __syncthreads();
-  lds_fft<C>(re, im, N, S, tw, true);
-  const double inv = 1.0 / (double)N;
-  for (int e = tid; e < C * N; e += FFT_THREADS) {
+#pragma unroll
+  for (int q = 0; q < QZ; ++q) {
+    const int e = tid + q * T, p = e / m, k = e - p * m;
+    if (e >= m * G::PP) continue;
+    re[p * L + k] = ar[q] - bi[q];
+    im[p * L + k] = -(ai[q] + br[q]);
+    if (k > 0 && k < N) {
+      re[p * L + L - k] = ar[q] + bi[q];
+      im[p * L + L - k] = ai[q] - br[q];
+    }
+  }
+  __syncthreads();
+  lds_dft<L, T, G::E>(re, im, tw);
+  // (3) a_j = Re W_j / L, b_j = -Im W_j / L with W = DFT(conj Z)
+  constexpr double inv = 1.0 / (double)L;
+  for (int e = tid; e < L * C; e += T) {
     const int c = e % C, t = e / C, j = j0 + c;
     if (j >= cols) continue;
-    const int p = c * S + bitrev(t, bits);
-    Y[(size_t)(2 * t) * cols + j] = re[p] * inv;
-    Y[(size_t)(2 * t + 1) * cols + j] = im[p] * inv;
+    const int p = (c >> 1) * L;
+    Y[(size_t)t * cols + j] = (c & 1) ? -im[p + t] * inv : re[p + t] * inv;
   }
 }
 
@@ -223,50 +184,58 @@ void check_n(int n) {
   if (n > 4096) throw std::invalid_argument("fft: length above 4096 exceeds the LDS budget");
 }
 
+template <int L>
+void launch_rfft(hipStream_t s, const double* X, double* Y, int cols, int cut, int batch, long sx, long sy) {
+  using G = CGeo<L>;
+  const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
+  hipLaunchKernelGGL(rfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut, fft_table(L), sx,
+                     sy);
+}
+template <int L>
+void launch_irfft(hipStream_t s, const double* X, double* Y, int cols, int times, double sc, int batch, long sx,
+                  long sy) {
+  using G = CGeo<L>;
+  const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
+  hipLaunchKernelGGL(irfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
+                     fft_table(L), sx, sy);
+}
+
 }  // namespace
 
 bool fft_supported(int n) { return n >= 8 && n <= 4096 && (n & (n - 1)) == 0; }
 
-template <int C>
-static void launch_rfft(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, const Twiddles& tw,
-                        int batch, long sx, long sy) {
-  const size_t lds = sizeof(double) * 2 * C * (n / 2);
-  const unsigned blocks = (unsigned)((cols + C - 1) / C);
-  hipLaunchKernelGGL(rfft_cols_kernel<C>, dim3(blocks, batch), dim3(FFT_THREADS), lds, s, X, Y, n, cols, cut,
-                     tw.stage, tw.post, sx, sy);
-}
-template <int C>
-static void launch_irfft(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc,
-                         const Twiddles& tw, int batch, long sx, long sy) {
-  const size_t lds = sizeof(double) * 2 * C * (n / 2 + 1);
-  const unsigned blocks = (unsigned)((cols + C - 1) / C);
-  hipLaunchKernelGGL(irfft_cols_kernel<C>, dim3(blocks, batch), dim3(FFT_THREADS), lds, s, X, Y, n, cols, times, sc,
-                     tw.stage, tw.post, sx, sy);
-}
+#define RBC_FFT_SWITCH(NN, CALL)                                            \
+  switch (NN) {                                                             \
+    case 8: CALL(8); break;                                                 \
+    case 16: CALL(16); break;                                               \
+    case 32: CALL(32); break;                                               \
+    case 64: CALL(64); break;                                               \
+    case 128: CALL(128); break;                                             \
+    case 256: CALL(256); break;                                             \
+    case 512: CALL(512); break;                                             \
+    case 1024: CALL(1024); break;                                           \
+    case 2048: CALL(2048); break;                                           \
+    case 4096: CALL(4096); break;                                           \
+    default: throw std::invalid_argument("fft: unsupported length");        \
+  }
 
 void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch, long sx, long sy) {
   check_n(n);
-  if (batch < 1 || batch > 65535) throw std::invalid_argument("rfft_cols: bad batch");
-  const Twiddles& tw = twiddles(n);
+  if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("rfft_cols: bad batch or cols");
   cut = std::min(cut, n / 2 + 1);
-  switch (fft_cols_per_block(n)) {
-    case 4: launch_rfft<4>(s, X, Y, n, cols, cut, tw, batch, sx, sy); break;
-    case 2: launch_rfft<2>(s, X, Y, n, cols, cut, tw, batch, sx, sy); break;
-    default: launch_rfft<1>(s, X, Y, n, cols, cut, tw, batch, sx, sy); break;
-  }
+#define RBC_RFFT(LL) launch_rfft<LL>(s, X, Y, cols, cut, batch, sx, sy)
+  RBC_FFT_SWITCH(n, RBC_RFFT)
+#undef RBC_RFFT
   check("rfft_cols");
 }
 
 void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch, long sx,
                 long sy) {
   check_n(n);
-  if (batch < 1 || batch > 65535) throw std::invalid_argument("irfft_cols: bad batch");
-  const Twiddles& tw = twiddles(n);
-  switch (fft_cols_per_block(n)) {
-    case 4: launch_irfft<4>(s, X, Y, n, cols, times, sc, tw, batch, sx, sy); break;
-    case 2: launch_irfft<2>(s, X, Y, n, cols, times, sc, tw, batch, sx, sy); break;
-    default: launch_irfft<1>(s, X, Y, n, cols, times, sc, tw, batch, sx, sy); break;
-  }
+  if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("irfft_cols: bad batch or cols");
+#define RBC_IRFFT(LL) launch_irfft<LL>(s, X, Y, cols, times, sc, batch, sx, sy)
+  RBC_FFT_SWITCH(n, RBC_IRFFT)
+#undef RBC_IRFFT
   check("irfft_cols");
 }
 

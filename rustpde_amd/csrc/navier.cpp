@@ -989,6 +989,11 @@ class Space {
   const double* xop(const std::string& name, const double* in, int cols, double* out, double sc) {
     if (b0.k == FO) {
       if (name == "S" || name == "FO") return in;
+      if (fft_supported(b0.n) && (name == "BS" || name == "FS")) {  // real FFT kernels
+        if (name == "BS") irfft_cols(st, in, out, b0.n, cols, 0, 1.0);
+        else rfft_cols(st, in, out, b0.n, cols, b0.m);
+        return out;
+      }
       if (name == "DS" || name == "DDS") {
         fourier_diff(st, in, out, b0.m, cols, cols, name == "DS" ? 1 : 2, sc);
         return out;
