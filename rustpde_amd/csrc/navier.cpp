@@ -108,7 +108,7 @@ static void make_stencil_band(const Basis& b, DBand& band) {
 }
 
 // MatVecFdma of an (m x n) matrix (matvec.rs:142-193): taps as the reference's conditions
-static void make_matvec_band(const LMat& a, DBand& band) {
+static void make_matvec_band(const LMat& a, DBand& band, std::vector<double>* taps = nullptr) {
   const int m = a.r;
   std::vector<double> c(4 * (size_t)m, 0.0);
   std::vector<unsigned char> mk(m, 0);
@@ -123,6 +123,27 @@ static void make_matvec_band(const LMat& a, DBand& band) {
   band.mi = a.c;
   band.coef.upload(c);
   band.mask.upload(mk);
+  if (taps) *taps = c;  // absent taps are zero
+}
+
+// Per-set arrays in the slice order of fdma_rows_lds: element i = p + 2 t of chain p,
+// t = lane * per + q, stored at [array][p][q][lane]; `get(a, i, t, p)` gives the value of
+// array a for element i (zero for absent entries).
+template <class F>
+static std::vector<double> slice_order(int nsets, int m, int narrays, F get) {
+  const int per = fdma_rows_lds_per(m), len[2] = {(m + 1) / 2, m / 2};
+  // tail padding: the kernel loads q < 16 for every array of every set
+  std::vector<double> out((size_t)nsets * narrays * 2 * per * 64 + 16 * 64, 0.0);
+  for (int s = 0; s < nsets; ++s)
+    for (int a = 0; a < narrays; ++a)
+      for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < per; ++q)
+          for (int l = 0; l < 64; ++l) {
+            const int t = l * per + q;
+            if (t >= len[p]) continue;
+            out[(size_t)s * narrays * 2 * per * 64 + ((size_t)(a * 2 + p) * per + q) * 64 + l] = get(s, a, p + 2 * t, t, p);
+          }
+  return out;
 }
 
 // from_ortho tables of a composite basis: stencil and the tdma factors of linalg.rs:14-57
@@ -233,7 +254,9 @@ struct Solver {
   std::vector<double> px;             // P_x (m x n) in double, for composing fwd * P_x
   DOp pf, pb, py;
   DBand pyb;  // P_y as a banded matvec
+  std::vector<double> pyb_taps;  // its taps on the host, [4][m]
   DBuf low, dia, up1, up2;
+  DBuf diagp, pybp;  // the same in slice order for fdma_rows_lds
   int nsets = 0, m = 0;
   Band4 fa, fc;
   double alpha = 0.0;
@@ -258,6 +281,20 @@ struct Solver {
     dia.upload(D);
     up1.upload(U1);
     up2.upload(U2);
+    const int len[2] = {(m + 1) / 2, m / 2};
+    diagp.upload(slice_order(nsets, m, 4, [&](int st, int a, int i, int t, int p) -> double {
+      const size_t o = (size_t)st * m;
+      switch (a) {
+        case 0: return t >= 1 ? L[o + i - 2] : 0.0;
+        case 1: return D[o + i];
+        case 2: return t + 1 < len[p] ? U1[o + i] : 0.0;
+        default: return t + 2 < len[p] ? U2[o + i] : 0.0;
+      }
+    }));
+    if (!pyb_taps.empty()) {
+      const int mo = pyb.mo;
+      pybp.upload(slice_order(1, m, 4, [&](int, int a, int i, int, int) { return pyb_taps[(size_t)a * mo + i]; }));
+    }
     if (has_fb) {
       const int mm = (int)lam.size();
       LMat F(mm, mm), P((int)px.size() / bx.n, bx.n), Q(mm, mm);
@@ -325,7 +362,7 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
   s->fc = Band4::from_matrix(C1, m1);
   if (g0.has_p) s->px = to_vec(g0.p);
   upload_op(s->py, g1.p.t());  // MatVecFdma(precond) along y as a right GEMM
-  make_matvec_band(g1.p, s->pyb);
+  make_matvec_band(g1.p, s->pyb, &s->pyb_taps);
   s->build_rows();
   return s;
 }
@@ -463,6 +500,18 @@ class Navier {
     if (fdma_rows_wave_supported(m)) fdma_rows_wave(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
     else fdma_rows(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
     timer.end(st);
+  }
+  // P_y matvec then the per-row solves: R [nrows, ny] -> X [nrows, my]
+  void ROWSF(const double* R, int ldr, double* X, int ld, int nrows, const Solver& s) {
+    const int m = s.m, nr = s.pyb.mi;
+    if (use_bands && fdma_rows_lds_supported(m, nr)) {
+      timer.begin(st, 1, 8.0 * ((double)nrows * (nr + m) + 4.0 * (double)s.nsets * m));
+      fdma_rows_lds(st, R, ldr, nr, X, ld, nrows, m, s.nsets, s.pybp.p, s.diagp.p);
+      timer.end(st);
+      return;
+    }
+    BANDY(s.pyb, R, ldr, X, ld, nrows, 0.0);
+    ROWS(X, ld, nrows, m, s.nsets, s);
   }
   void BANDY(const DBand& b, const double* in, int ldi, double* out, int ldo, int rows, double beta) {
     timer.begin(st, 3, 8.0 * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
@@ -718,14 +767,12 @@ class Navier {
       return;
     }
     if (!s.has_fb) {  // Fourier x: P_y then the per-wavenumber row systems
-      BANDY(s.pyb, rhs, ny, out, my, 2 * s.bx.m, 0.0);
-      ROWS(out, my, 2 * s.bx.m, my, s.nsets, s);
+      ROWSF(rhs, ny, out, my, 2 * s.bx.m, s);
       return;
     }
-    xmat(s.pf, rhs, ny, W1.p);               // fwd * P_x : [m0, ny]
-    BANDY(s.pyb, W1.p, ny, W2.p, my, s.pf.rows, 0.0);  // P_y : [m0, my]
-    ROWS(W2.p, my, s.pf.rows, my, s.nsets, s);
-    xmat(s.pb, W2.p, my, out);               // bwd       : [m0, my]
+    xmat(s.pf, rhs, ny, W1.p);              // fwd * P_x : [m0, ny]
+    ROWSF(W1.p, ny, W2.p, my, s.pf.rows, s);  // P_y and the row systems : [m0, my]
+    xmat(s.pb, W2.p, my, out);              // bwd       : [m0, my]
   }
 
   // convective term of q with x-basis bxq (y basis cheb_dirichlet), into CONV (physical)
@@ -1134,13 +1181,11 @@ class SolverDev {
                         (long)sv.hx.rows * o1, batch, 1.0, 0.0);
       gemm_mfma(st, batch * sv.hx.rows, m1, sv.hy.rows, W1.p, sv.hy.rows, sv.hy.b.p, sv.hy.cols, y, m1, 1.0, 0.0);
     } else if (!sv.has_fb) {
-      band_y(st, x, o1, y, m1, batch * ro, sv.pyb.mo, sv.pyb.coef.p, sv.pyb.mask.p, 0.0);
-      fdma_rows_launch(y, m1, batch * ro, m1, sv);
+      rows_launch(x, o1, y, m1, batch * ro, sv);
     } else {
       gemm_mfma_batched(st, sv.pf.rows, o1, sv.pf.cols, sv.pf.b.p, sv.pf.cols, 0, x, o1, (long)in_sz, W1.p, o1,
                         (long)sv.pf.rows * o1, batch, 1.0, 0.0);
-      band_y(st, W1.p, o1, W2.p, m1, batch * sv.pf.rows, sv.pyb.mo, sv.pyb.coef.p, sv.pyb.mask.p, 0.0);
-      fdma_rows_launch(W2.p, m1, batch * sv.pf.rows, m1, sv);
+      rows_launch(W1.p, o1, W2.p, m1, batch * sv.pf.rows, sv);
       gemm_mfma_batched(st, sv.pb.rows, m1, sv.pb.cols, sv.pb.b.p, sv.pb.cols, 0, W2.p, m1, (long)sv.pf.rows * m1, y, m1,
                         (long)out_sz, batch, 1.0, 0.0);
     }
@@ -1148,7 +1193,14 @@ class SolverDev {
       for (int b = 0; b < batch; ++b) split_to_interleaved(st, y + out_sz * b, out + 2 * (size_t)m0 * m1 * b, m0, m1);
     HIPCHECK(hipStreamSynchronize(st));
   }
-  void fdma_rows_launch(double* X, int ld, int nrows, int m, const Solver& sv) {
+  // P_y matvec and the row solves: R [nrows, ldr] -> X [nrows, ld]
+  void rows_launch(const double* R, int ldr, double* X, int ld, int nrows, const Solver& sv) {
+    const int m = sv.m, nr = sv.pyb.mi;
+    if (fdma_rows_lds_supported(m, nr)) {
+      fdma_rows_lds(st, R, ldr, nr, X, ld, nrows, m, sv.nsets, sv.pybp.p, sv.diagp.p);
+      return;
+    }
+    band_y(st, R, ldr, X, ld, nrows, sv.pyb.mo, sv.pyb.coef.p, sv.pyb.mask.p, 0.0);
     if (fdma_rows_wave_supported(m)) fdma_rows_wave(st, X, ld, nrows, m, sv.nsets, sv.low.p, sv.dia.p, sv.up1.p, sv.up2.p);
     else fdma_rows(st, X, ld, nrows, m, sv.nsets, sv.low.p, sv.dia.p, sv.up1.p, sv.up2.p);
   }

@@ -88,6 +88,15 @@ void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int c
 void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch = 1,
                 long sx = 0, long sy = 0);
 
+// Banded matvec (taps at offsets 0, -2, +2, +4) then the row solves of fdma_rows, one
+// wavefront per row, R [nrows, nr] -> X [nrows, m]; `bandp` and `diagp` hold the taps and
+// the swept diagonal sets in slice order (fdma_rows_lds_per(m) values per lane and
+// chain).  Without a band R must hold m values per row (in place allowed).
+int fdma_rows_lds_per(int m);
+bool fdma_rows_lds_supported(int m, int nr);
+void fdma_rows_lds(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m, int nsets,
+                   const double* bandp, const double* diagp, int batch = 1, long sr = 0, long sx = 0);
+
 // Chebyshev transforms along axis 1 (rows of length n, n - 1 a power of two in [16, 2048])
 // by FFT-based DCT-I.  cheb_y_backward: in [rows, mi] coefficients (composite basis when
 // coef / mask give its stencil taps, else Chebyshev, mi <= n), `times` (0 or 1)
