@@ -109,12 +109,15 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *   RBC_OPT_FFT    Fourier x transforms by FFT kernels (default 1; 0 = dense DFT)
  *   RBC_OPT_BANDS  stencil / preconditioner as banded kernels (default 1; 0 = dense)
  *   RBC_OPT_DCT    Chebyshev y transforms by FFT-based DCT-I kernels when ny - 1 is a
- *                  power of two (default 1; 0 = dense transform GEMMs) */
+ *                  power of two (default 1; 0 = dense transform GEMMs)
+ *   RBC_OPT_BATCH  periodic step: the transforms of ux, uy and temp and the ux/uy solves
+ *                  run as batched launches (default 1; needs the FFT, DCT and band paths) */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2
 #define RBC_OPT_BANDS 3
 #define RBC_OPT_DCT 4
+#define RBC_OPT_BATCH 5
 int rbc_set_option(rbc_navier* h, int option, int value);
 
 /* Kernel timing of a handle (no counterpart in the reference, whose timing protocol is

@@ -39,14 +39,22 @@ struct NumericError : std::runtime_error {
 struct DBuf {
   double* p = nullptr;
   size_t n = 0;
+  bool owned = true;  // false: a view into another buffer
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = true;
+  }
+  void view(double* at, size_t count) {
+    release();
+    p = at;
+    n = count;
+    owned = false;
   }
   void alloc(size_t count) {
     release();
@@ -448,6 +456,10 @@ class Navier {
   bool use_mfma = true;   // FP64 GEMMs on the matrix cores
   bool use_graph = true;  // replay update() as a captured HIP graph
   bool use_dct = true;    // Chebyshev y transforms by FFT-based DCT-I kernels
+  bool use_batch = true;  // periodic step: the three fields' transforms and solves batched
+  bool vel_same = true;   // the ux and uy solvers hold identical tables
+  size_t FS = 0;          // slot stride of the batched buffers
+  DBuf STATE, PHYS2;      // [ux, uy, temp] spectral and [ux, uy] physical, one slot each
   hipGraphExec_t gexec = nullptr;
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -572,8 +584,19 @@ class Navier {
     HIPCHECK(hipStreamCreate(&st));
     const int R = std::max(nx, periodic ? 2 * mx() : nx);
     const size_t full = (size_t)R * ny;
-    for (DBuf* b : {&T, &UX, &UY, &P0, &P1, &Tv, &UXv, &UYv, &P0v, &P1v, &THAT_BC, &DTBCX, &DTBCY, &GXX, &GYY, &THAT,
-                    &UXP, &UYP, &A, &G, &CONV, &CV, &RHS, &DIV, &W1, &W2, &W3, &W4, &W9})
+    FS = full;
+    // slot-strided buffers of the batched step (slot k at p + k FS)
+    STATE.alloc(3 * full);
+    UX.view(STATE.p, full);
+    UY.view(STATE.p + full, full);
+    T.view(STATE.p + 2 * full, full);
+    PHYS2.alloc(2 * full);
+    UXP.view(PHYS2.p, full);
+    UYP.view(PHYS2.p + full, full);
+    for (DBuf* b : {&A, &CONV, &CV, &RHS, &W1, &W2}) b->alloc(3 * full);
+    G.alloc(2 * full);
+    for (DBuf* b : {&P0, &P1, &Tv, &UXv, &UYv, &P0v, &P1v, &THAT_BC, &DTBCX, &DTBCY, &GXX, &GYY, &THAT, &DIV, &W3,
+                    &W4, &W9})
       b->alloc(full);
     BC.alloc((size_t)R * 2);
     BCv.alloc((size_t)nx * ny);
@@ -851,7 +874,85 @@ class Navier {
     const double* a = xop(bx_u, "BS", U, by_u.m, W1.p, W9.p);
     yop(by_u, "BS", a, nx, out);
   }
+  // ----------------------------------------------------------- batched periodic step
+  bool batch_ok() const {
+    return use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
+           use_bands && fdma_rows_lds_supported(by_u.m, by_u.n);
+  }
+  // irfft of `batch` slots [2 mx, cols] -> [nx, cols], (i k)^times / sc first
+  void IRFFT_B(const double* in, double* out, int cols, int times, double sc, int batch) {
+    timer.begin(st, 2, 8.0 * batch * cols * ((double)nx + 2.0 * mx()));
+    irfft_cols(st, in, out, nx, cols, times, sc, batch, (long)FS, (long)FS);
+    timer.end(st);
+  }
+  void RFFT_B(const double* in, double* out, int cols, int cut, int batch) {
+    timer.begin(st, 2, 8.0 * batch * cols * ((double)nx + 2.0 * mx()));
+    rfft_cols(st, in, out, nx, cols, cut, batch, (long)FS, (long)FS);
+    timer.end(st);
+  }
+  // Chebyshev y backward of composite by (times derivatives): out = x .* U + beta out
+  void CHEBY_BWD_B(const Basis& b, const double* in, double* out, int rows, int times, const double* U, double beta,
+                   int batch) {
+    const DBand& band = stencil_band(b);
+    timer.begin(st, 4, 8.0 * batch * rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
+    cheb_y_backward(st, in, b.m, b.m, out, b.n, rows, b.n, band.coef.p, band.mask.p, times, 1.0 / scale[1], U, b.n,
+                    beta, batch, (long)FS, (long)FS, 0);
+    timer.end(st);
+  }
+  void CHEBY_FWD_B(const double* in, double* out, int rows, int cut, int batch) {
+    timer.begin(st, 4, 8.0 * batch * rows * 2.0 * ny);
+    cheb_y_forward(st, in, ny, out, ny, rows, ny, cut, batch, (long)FS, (long)FS);
+    timer.end(st);
+  }
+  void update_batched() {  // navier.rs:737-765 with the three fields' work batched
+    const long no = (long)rows_ortho(bx_f) * ny, np = (long)nx * ny;
+    const int my = by_u.m, R2 = 2 * mx();
+    const size_t F = FS;
+    // buoyancy: temp to ortho space plus fieldbc (navier.rs:740-742)
+    to_ortho(bx_t, by_u, T.p, W3.p);
+    ew_add(st, THAT.p, W3.p, THAT_BC.p, no);
+    // ux, uy in physical space
+    IRFFT_B(UX.p, W1.p, my, 0, 1.0, 2);
+    CHEBY_BWD_B(by_u, W1.p, UXP.p, nx, 0, nullptr, 0.0, 2);
+    // convection of ux, uy, temp: dq/dx ux + dq/dy uy (conv_term.rs:35-41)
+    IRFFT_B(UX.p, W1.p, my, 1, scale[0], 3);
+    CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 0, UXP.p, 0.0, 3);
+    IRFFT_B(UX.p, W1.p, my, 0, 1.0, 3);
+    CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 1, UYP.p, 1.0, 3);
+    ew_acc_prod(st, CONV.p + 2 * F, UXP.p, DTBCX.p, np);  // + u . grad(fieldbc) for temp
+    ew_acc_prod(st, CONV.p + 2 * F, UYP.p, DTBCY.p, np);
+    CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
+    RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
+    // the three fields in ortho space
+    timer.begin(st, 3, 8.0 * 3.0 * R2 * ((double)my + ny));
+    const DBand& sb = stencil_band(by_u);
+    band_y(st, UX.p, my, A.p, ny, R2, sb.mo, sb.coef.p, sb.mask.p, 0.0, 3, (long)F, (long)F);
+    timer.end(st);
+    // pressure gradients (navier.rs:622-654)
+    xop(bx_p, "DS", P0.p, ny, G.p, W9.p);
+    yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
+    ew_rhs_vel(st, RHS.p, A.p, G.p, nullptr, CV.p, dt, no);
+    ew_rhs_vel(st, RHS.p + F, A.p + F, G.p + F, THAT.p, CV.p + F, dt, no);
+    ew_rhs_temp(st, RHS.p + 2 * F, A.p + 2 * F, GXX.p, GYY.p, CV.p + 2 * F, dt, no);  // navier.rs:660-674
+    // implicit solves: ux and uy share one solver; temp
+    const Solver& sv = *solver[0];
+    timer.begin(st, 1, 2.0 * 8.0 * ((double)R2 * (ny + my) + 4.0 * (double)sv.nsets * my));
+    fdma_rows_lds(st, RHS.p, ny, ny, UX.p, my, R2, my, sv.nsets, sv.pybp.p, sv.diagp.p, 2, (long)F, (long)F);
+    timer.end(st);
+    ROWSF(RHS.p + 2 * F, ny, T.p, my, R2, *solver[2]);
+    // projection (navier.rs:683-721)
+    divergence(DIV.p);
+    solve_pres();
+    project_velocity();
+    update_pres();
+    time += dt;
+  }
+
   void update() {  // navier.rs:737-765
+    if (batch_ok()) {
+      update_batched();
+      return;
+    }
     const long no = (long)rows_ortho(bx_f) * ny;
     to_ortho(bx_t, by_u, T.p, W3.p);
     ew_add(st, THAT.p, W3.p, THAT_BC.p, no);
@@ -1424,6 +1525,8 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
       std::memcpy(sv.bwd.data(), bwd, sv.bwd.size() * sizeof(double));
     }
     sv.build_rows();
+    const rbc::Solver &s0 = *h->n->solver[0], &s1 = *h->n->solver[1];
+    h->n->vel_same = s0.lam == s1.lam && s0.fwd == s1.fwd && s0.bwd == s1.bwd;
   });
 }
 
@@ -1439,6 +1542,7 @@ int rbc_set_option(rbc_navier* h, int option, int value) {
       case RBC_OPT_FFT: n.use_fft = value != 0; break;
       case RBC_OPT_BANDS: n.use_bands = value != 0; break;
       case RBC_OPT_DCT: n.use_dct = value != 0; break;
+      case RBC_OPT_BATCH: n.use_batch = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
   });
