@@ -104,7 +104,7 @@ template <int L>
 __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     const double* __restrict__ in, int ldi, int mi, double* __restrict__ out, int ldo, int rows,
     const double* __restrict__ coef, int times, double inv_sc, const double* __restrict__ U, int ldu, double beta,
-    const double* __restrict__ tw, long si, long so, long su) {
+    const double* __restrict__ tw, long si, long so, long su, bool transform) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
   __shared__ double re[G::E], im[G::E];
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     __syncthreads();
   }
   // (3) u_k = (-1)^k w_k c_k / sc^times and the even extension u_{L-k} = u_k
-  for (int e = tid; e < P * n; e += T) {
+  for (int e = tid; e < P * n && transform; e += T) {
     const int p = e / n, k = e - p * n;
     double a = re[p * L + k], b = im[p * L + k];
     if (times && inv_sc != 1.0) {
@@ -201,9 +201,12 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
       im[p * L + L - k] = b;
     }
   }
-  __syncthreads();
-  lds_dft<L, T, G::E>(re, im, tw);
-  // (4) x_j = Re / Im Z_j, j <= N; out = x .* U + beta out (loads first)
+  if (transform) {
+    __syncthreads();
+    lds_dft<L, T, G::E>(re, im, tw);
+  }
+  // (4) x_j = Re / Im Z_j, j <= N, or the coefficients d_j / sc without the transform;
+  //     out = x .* U + beta out (loads first)
   double ua[Q], ub[Q], oa[Q], ob[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -223,6 +226,10 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     double va = re[p * L + j], vb = im[p * L + j];
+    if (!transform && inv_sc != 1.0) {
+      va = va * inv_sc;
+      vb = vb * inv_sc;
+    }
     if (U) {
       va = va * ua[q];
       vb = vb * ub[q];
@@ -290,12 +297,12 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const double* __r
 template <int L>
 void launch_bwd(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, const double* coef,
                 const unsigned char* mask, int times, double inv_sc, const double* U, int ldu, double beta, int batch,
-                long si, long so, long su) {
+                long si, long so, long su, bool transform) {
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
   (void)mask;  // the zero coefficients of absent taps stand for the mask
   hipLaunchKernelGGL(cheb_y_bwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
-                     times, inv_sc, U, ldu, beta, fft_table(L), si, so, su);
+                     times, inv_sc, U, ldu, beta, fft_table(L), si, so, su, transform);
 }
 
 template <int L>
@@ -334,10 +341,23 @@ void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* o
   if (times < 0 || times > 1) throw std::invalid_argument("cheb_y_backward: times must be 0 or 1");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_backward: bad batch or rows");
   if (mi > n || (!coef && mi < 1)) throw std::invalid_argument("cheb_y_backward: bad input length");
-#define RBC_BWD(LL) launch_bwd<LL>(s, in, ldi, mi, out, ldo, rows, coef, mask, times, inv_sc, U, ldu, beta, batch, si, so, su)
+#define RBC_BWD(LL) \
+  launch_bwd<LL>(s, in, ldi, mi, out, ldo, rows, coef, mask, times, inv_sc, U, ldu, beta, batch, si, so, su, true)
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_BWD)
 #undef RBC_BWD
   check("cheb_y_backward");
+}
+
+void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
+                 const double* coef, double inv_sc, double beta, int batch, long si, long so) {
+  if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_diff: unsupported length");
+  if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_diff: bad batch or rows");
+  if (mi > n || (!coef && mi < 1)) throw std::invalid_argument("cheb_y_diff: bad input length");
+#define RBC_DIFF(LL) \
+  launch_bwd<LL>(s, in, ldi, mi, out, ldo, rows, coef, nullptr, 1, inv_sc, nullptr, 0, beta, batch, si, so, 0, false)
+  RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_DIFF)
+#undef RBC_DIFF
+  check("cheb_y_diff");
 }
 
 void cheb_y_forward(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, int cut, int batch,

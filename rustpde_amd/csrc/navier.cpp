@@ -157,6 +157,7 @@ static std::vector<double> slice_order(int nsets, int m, int narrays, F get) {
 // from_ortho tables of a composite basis: stencil and the tdma factors of linalg.rs:14-57
 struct DTdma {
   DBuf diag, low2, off, w, den;
+  DBuf tabp;  // diag, low2, off_{i-2}, den, w in the slice order of tdma_rows_lds
   int m = 0;
 };
 static void make_tdma(const Basis& b, DTdma& t) {
@@ -178,6 +179,15 @@ static void make_tdma(const Basis& b, DTdma& t) {
   t.off.upload(off);
   t.w.upload(w);
   t.den.upload(den);
+  t.tabp.upload(slice_order(1, m, 5, [&](int, int a, int i, int tt, int) -> double {
+    switch (a) {
+      case 0: return b.diag[i];
+      case 1: return b.low2[i];
+      case 2: return tt >= 1 ? off[i - 2] : 0.0;
+      case 3: return den[i];
+      default: return i < m - 2 ? w[i] : 0.0;
+    }
+  }));
 }
 
 static ld ipow(ld x, int e) {
@@ -690,6 +700,14 @@ class Navier {
       BANDY(stencil_band(b), in, b.m, out, b.n, rows, beta);
       return out;
     }
+    if (use_dct && use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) &&
+        cheb_y_supported(b.n)) {
+      const DBand* band = b.k != CHEB ? &stencil_band(b) : nullptr;
+      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
+      cheb_y_diff(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr, 1.0 / scale[1], beta);
+      timer.end(st);
+      return out;
+    }
     if (use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) && cheb_diff_y_supported(b.n)) {
       const bool comp = b.k != CHEB;
       const DBand* band = comp ? &stencil_band(b) : nullptr;
@@ -715,6 +733,13 @@ class Navier {
         timer.end(st);
         return out;
       }
+    }
+    if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_rows_lds_supported(b.m)) {
+      const DTdma& t = tdma_tables(b);
+      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.n + b.m));
+      tdma_rows_lds(st, in, b.n, out, b.m, rows, b.m, t.tabp.p);
+      timer.end(st);
+      return out;
     }
     if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_y_supported(b.m)) {
       const DTdma& t = tdma_tables(b);

@@ -97,6 +97,12 @@ bool fdma_rows_lds_supported(int m, int nr);
 void fdma_rows_lds(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m, int nsets,
                    const double* bandp, const double* diagp, int batch = 1, long sr = 0, long sx = 0);
 
+// from_ortho of every row as tdma_y, one wavefront per row with LDS staging and the
+// tables (diag, low2, off_{i-2}, den, w) in slice order: R [rows, m + 2] -> X [rows, m].
+bool tdma_rows_lds_supported(int m);
+void tdma_rows_lds(hipStream_t s, const double* R, int ldr, double* X, int ld, int rows, int m, const double* tabp,
+                   int batch = 1, long sr = 0, long sx = 0);
+
 // Chebyshev transforms along axis 1 (rows of length n, n - 1 a power of two in [16, 2048])
 // by FFT-based DCT-I.  cheb_y_backward: in [rows, mi] coefficients (composite basis when
 // coef / mask give its stencil taps, else Chebyshev, mi <= n), `times` (0 or 1)
@@ -108,6 +114,10 @@ void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* o
                      double beta, int batch = 1, long si = 0, long so = 0, long su = 0);
 void cheb_y_forward(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, int cut,
                     int batch = 1, long si = 0, long so = 0);
+// cheb_y_diff: d/dy in Chebyshev space (ortho.rs:107-125) with the composite stencil
+// fused on load: out [rows, n] = d / sc + beta out, by the same LDS suffix scans.
+void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
+                 const double* coef, double inv_sc, double beta, int batch = 1, long si = 0, long so = 0);
 
 // complex layout conversion: interleaved (re, im) [m, c] <-> split planes [2m, c]
 void interleaved_to_split(hipStream_t s, const double* in, double* out, int m, int c);
