@@ -959,12 +959,13 @@ class Navier {
     ew_rhs_vel(st, RHS.p, A.p, G.p, nullptr, CV.p, dt, no);
     ew_rhs_vel(st, RHS.p + F, A.p + F, G.p + F, THAT.p, CV.p + F, dt, no);
     ew_rhs_temp(st, RHS.p + 2 * F, A.p + 2 * F, GXX.p, GYY.p, CV.p + 2 * F, dt, no);  // navier.rs:660-674
-    // implicit solves: ux and uy share one solver; temp
-    const Solver& sv = *solver[0];
-    timer.begin(st, 1, 2.0 * 8.0 * ((double)R2 * (ny + my) + 4.0 * (double)sv.nsets * my));
-    fdma_rows_lds(st, RHS.p, ny, ny, UX.p, my, R2, my, sv.nsets, sv.pybp.p, sv.diagp.p, 2, (long)F, (long)F);
+    // implicit solves of ux, uy (one solver for both) and temp in one launch
+    const Solver &sv = *solver[0], &stt = *solver[2];
+    const double* bands[3] = {sv.pybp.p, sv.pybp.p, stt.pybp.p};
+    const double* diags[3] = {sv.diagp.p, sv.diagp.p, stt.diagp.p};
+    timer.begin(st, 1, 8.0 * (3.0 * (double)R2 * (ny + my) + 2.0 * 4.0 * (double)sv.nsets * my));
+    fdma_rows_lds_multi(st, RHS.p, ny, ny, UX.p, my, R2, my, sv.nsets, bands, diags, 3, (long)F, (long)F);
     timer.end(st);
-    ROWSF(RHS.p + 2 * F, ny, T.p, my, R2, *solver[2]);
     // projection (navier.rs:683-721)
     divergence(DIV.p);
     solve_pres();

@@ -96,6 +96,10 @@ int fdma_rows_lds_per(int m);
 bool fdma_rows_lds_supported(int m, int nr);
 void fdma_rows_lds(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m, int nsets,
                    const double* bandp, const double* diagp, int batch = 1, long sr = 0, long sx = 0);
+// The same for up to 4 batch entries with tables of their own (entry k: bandps[k], diagps[k]).
+void fdma_rows_lds_multi(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m,
+                         int nsets, const double* const* bandps, const double* const* diagps, int batch, long sr,
+                         long sx);
 
 // from_ortho of every row as tdma_y, one wavefront per row with LDS staging and the
 // tables (diag, low2, off_{i-2}, den, w) in slice order: R [rows, m + 2] -> X [rows, m].

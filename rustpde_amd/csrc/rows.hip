@@ -2,9 +2,9 @@
 // of fdma.rs:101-118), FP64, optionally fused with the preconditioner matvec that
 // precedes them (MatVecFdma along y, matvec.rs:172-193).
 //
-// One wavefront solves one row.  The offsets -2, 0, 2, 4 of the 4-diagonal systems couple
-// only indices of equal parity, so a row is two independent chains; lane l owns chain
-// positions [l per, (l + 1) per) of both.  A pass over a chain is a linear recurrence:
+// The offsets -2, 0, 2, 4 of the 4-diagonal systems couple only indices of equal parity,
+// so a row is two independent chains, each solved by one wavefront; lane l owns chain
+// positions [l per, (l + 1) per).  A pass over a chain is a linear recurrence:
 // each lane composes the affine map of its slice, the maps are scanned across the
 // wavefront with shuffles, and each lane replays its slice with the reference's
 // arithmetic from its incoming state.
@@ -34,96 +34,93 @@ __host__ __device__ __forceinline__ int pad(int j) { return j + (j >> 4); }
 // LDS doubles per wavefront for rows of nr values (two guard zeros on each side)
 inline int buf_len(int nr) { return pad(nr + 6) + 1; }
 
+// Tables of up to four batch entries (blockIdx.y), or one shared by all entries.
+struct RowTables {
+  const double* band[4];
+  const double* diag[4];
+  int shared;
+};
+
+// One wavefront per (row, parity chain); the two chains of a row share its LDS buffer.
 template <int SLICE>
-__global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(
-    const double* __restrict__ R, int ldr, int nr, double* __restrict__ X, int ld, int nrows, int m, int nsets,
-    int per, int bl, const double* __restrict__ bandp, const double* __restrict__ diagp, long sr, long sx) {
+__global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double* __restrict__ R, int ldr, int nr,
+                                                                    double* __restrict__ X, int ld, int nrows, int m,
+                                                                    int nsets, int per, int bl, RowTables tb, long sr,
+                                                                    long sx) {
 #pragma clang fp contract(off)
   extern __shared__ double lds[];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int slot = w >> 1, p = w & 1;  // row slot of the block, chain parity
+  const int ti = tb.shared ? 0 : (int)blockIdx.y;
+  const double* __restrict__ bandp = tb.band[ti];
+  const double* __restrict__ diagp = tb.diag[ti];
   R += sr * blockIdx.y;
   X += sx * blockIdx.y;
-  double* buf = lds + w * bl;
-  // rows r and r + nsets share a diagonal set: keep them in one block when nrows = 2 nsets
+  double* buf = lds + slot * bl;
+  // rows r and r + nsets share a diagonal set: one block per set when nrows = 2 nsets
   int r, set;
   bool valid;
   if (nrows == 2 * nsets) {
-    set = blockIdx.x * (WAVES / 2) + (w >> 1);
-    r = set + (w & 1) * nsets;
-    valid = set < nsets;
-    if (!valid) set = 0;  // idle wavefront: reads set 0, stores nothing
+    set = blockIdx.x;
+    r = set + slot * nsets;
+    valid = true;
   } else {
-    r = blockIdx.x * WAVES + w;
+    r = blockIdx.x * (WAVES / 2) + slot;
     valid = r < nrows;
     set = valid ? r % nsets : 0;
   }
   // (1) row -> LDS: element i at pad(i + 2); i in [-2, 0) and [nr, nr + 4) read as zero
-  for (int j = lane; j < nr + 6; j += WAVE) {
+  for (int j = p * WAVE + lane; j < nr + 6; j += 2 * WAVE) {
     const int i = j - 2;
     buf[pad(j)] = (valid && i >= 0 && i < nr) ? R[(size_t)r * ldr + i] : 0.0;
   }
   __syncthreads();
-  const int len[2] = {(m + 1) / 2, m / 2};
+  const int len = p == 0 ? (m + 1) / 2 : m / 2;
   const int t0 = lane * per;
-  int cnt[2];
-  double v[2][SLICE];
+  const int cnt = max(0, min(per, len - t0));
+  double v[SLICE];
   // The slice-ordered arrays carry SLICE * WAVE doubles of tail padding, so the loads of
   // all q < SLICE are in bounds and are issued unconditionally, ahead of their use.
   const size_t setbase = (size_t)set * (8 * (size_t)per * WAVE);
-#define DG(a, p, q) diagp[setbase + ((size_t)((a) * 2 + (p)) * per + (q)) * WAVE + lane]
+#define DG(a, q) diagp[setbase + ((size_t)((a) * 2 + p) * per + (q)) * WAVE + lane]
   if (bandp) {  // taps at offsets 0, -2, +2, +4, summed in that order
     const size_t tp = (size_t)2 * per * WAVE;
-    double c[4][2][SLICE];
+    double c[4][SLICE];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+      for (int q = 0; q < SLICE; ++q) c[k][q] = bandp[k * tp + ((size_t)p * per + q) * WAVE + lane];
 #pragma unroll
-        for (int q = 0; q < SLICE; ++q) c[k][p][q] = bandp[k * tp + ((size_t)p * per + q) * WAVE + lane];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      cnt[p] = max(0, min(per, len[p] - t0));
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        double val = 0.0;
-        if (q < cnt[p]) {
-          const int j = p + 2 * (t0 + q) + 2;
-          val = c[0][p][q] * buf[pad(j)];
-          val = val + c[1][p][q] * buf[pad(j - 2)];
-          val = val + c[2][p][q] * buf[pad(j + 2)];
-          val = val + c[3][p][q] * buf[pad(j + 4)];
-        }
-        v[p][q] = val;
+    for (int q = 0; q < SLICE; ++q) {
+      double val = 0.0;
+      if (q < cnt) {
+        const int j = p + 2 * (t0 + q) + 2;
+        val = c[0][q] * buf[pad(j)];
+        val = val + c[1][q] * buf[pad(j - 2)];
+        val = val + c[2][q] * buf[pad(j + 2)];
+        val = val + c[3][q] * buf[pad(j + 4)];
       }
+      v[q] = val;
     }
   } else {
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      cnt[p] = max(0, min(per, len[p] - t0));
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) v[p][q] = q < cnt[p] ? buf[pad(p + 2 * (t0 + q) + 2)] : 0.0;
-    }
+    for (int q = 0; q < SLICE; ++q) v[q] = q < cnt ? buf[pad(p + 2 * (t0 + q) + 2)] : 0.0;
   }
   // ---- forward elimination: y_t = x_t - y_{t-1} low[i-2]  (i = p + 2t >= 2)
-  double lo[2][SLICE];
+  {
+    double lo[SLICE];
 #pragma unroll
-  for (int p = 0; p < 2; ++p)
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) lo[p][q] = DG(0, p, q);
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
+    for (int q = 0; q < SLICE; ++q) lo[q] = DG(0, q);
     double A = 1.0, B = 0.0;  // slice map y_out = A y_in + B
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) {
-      if (q < cnt[p]) {
-        const int t = t0 + q;
-        if (t == 0) {
+      if (q < cnt) {
+        if (t0 + q == 0) {
           A = 0.0;
-          B = v[p][q];
+          B = v[q];
         } else {
-          const double l = lo[p][q];
-          A = -A * l;
-          B = v[p][q] - B * l;
+          A = -A * lo[q];
+          B = v[q] - B * lo[q];
         }
       }
     }
@@ -139,32 +136,28 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(
     if (lane == 0) y = 0.0;
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) {
-      if (q < cnt[p]) {
-        const int t = t0 + q;
-        y = (t == 0) ? v[p][q] : v[p][q] - y * lo[p][q];
-        v[p][q] = y;
+      if (q < cnt) {
+        y = (t0 + q == 0) ? v[q] : v[q] - y * lo[q];
+        v[q] = y;
       }
     }
   }
   // ---- back substitution: x_t = (y_t - x_{t+1} up1[i] - x_{t+2} up2[i]) / dia[i]
-  double dg[2][SLICE], g1[2][SLICE], g2[2][SLICE];
-#pragma unroll
-  for (int p = 0; p < 2; ++p)
+  {
+    double dg[SLICE], g1[SLICE], g2[SLICE];
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) {
-      dg[p][q] = DG(1, p, q);
-      g1[p][q] = DG(2, p, q);
-      g2[p][q] = DG(3, p, q);
+      dg[q] = DG(1, q);
+      g1[q] = DG(2, q);
+      g2[q] = DG(3, q);
     }
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
     // slice map on the state s = (x_{t+1}, x_{t+2}):  s' = M s + w
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0, w0 = 0.0, w1 = 0.0;
 #pragma unroll
     for (int q = SLICE - 1; q >= 0; --q) {
-      if (q >= cnt[p]) continue;
-      const double d = dg[p][q], u1 = g1[p][q], u2 = g2[p][q];
-      const double c0 = v[p][q] / d, c1 = -u1 / d, c2 = -u2 / d;
+      if (q >= cnt) continue;
+      const double d = dg[q];
+      const double c0 = v[q] / d, c1 = -g1[q] / d, c2 = -g2[q] / d;
       const double n00 = c1 * m00 + c2 * m10, n01 = c1 * m01 + c2 * m11, nw0 = c1 * w0 + c2 * w1 + c0;
       m10 = m00;
       m11 = m01;
@@ -196,28 +189,26 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(
     }
 #pragma unroll
     for (int q = SLICE - 1; q >= 0; --q) {
-      if (q >= cnt[p]) continue;
+      if (q >= cnt) continue;
       const int t = t0 + q;
-      double val = v[p][q];
-      if (t + 1 < len[p]) val = val - s0 * g1[p][q];
-      if (t + 2 < len[p]) val = val - s1 * g2[p][q];
-      val = val / dg[p][q];
+      double val = v[q];
+      if (t + 1 < len) val = val - s0 * g1[q];
+      if (t + 2 < len) val = val - s1 * g2[q];
+      val = val / dg[q];
       s1 = s0;
       s0 = val;
-      v[p][q] = val;
+      v[q] = val;
     }
   }
 #undef DG
   // (3) slices -> LDS -> row, unit stride
   __syncthreads();
 #pragma unroll
-  for (int p = 0; p < 2; ++p)
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q)
-      if (q < cnt[p]) buf[pad(p + 2 * (t0 + q) + 2)] = v[p][q];
+  for (int q = 0; q < SLICE; ++q)
+    if (q < cnt) buf[pad(p + 2 * (t0 + q) + 2)] = v[q];
   __syncthreads();
   if (valid)
-    for (int i = lane; i < m; i += WAVE) X[(size_t)r * ld + i] = buf[pad(i + 2)];
+    for (int i = p * WAVE + lane; i < m; i += 2 * WAVE) X[(size_t)r * ld + i] = buf[pad(i + 2)];
 }
 
 // from_ortho of a composite basis per row (composite_stencil.rs:250-276, linalg.rs:14-57):
@@ -373,29 +364,52 @@ int fdma_rows_lds_per(int m) { return ((m + 1) / 2 + WAVE - 1) / WAVE; }
 
 bool fdma_rows_lds_supported(int m, int nr) {
   return m >= 5 && fdma_rows_lds_per(m) <= 16 && nr >= m &&
-         (size_t)WAVES * buf_len(nr) * sizeof(double) <= 65536;
+         (size_t)(WAVES / 2) * buf_len(nr) * sizeof(double) <= 65536;
+}
+
+static void launch_rows(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m,
+                        int nsets, const RowTables& tb, int batch, long sr, long sx) {
+  if (!fdma_rows_lds_supported(m, nr)) throw std::invalid_argument("fdma_rows_lds: unsupported row length");
+  if (nsets < 1 || nrows < 1 || batch < 1 || batch > 65535) throw std::invalid_argument("fdma_rows_lds: bad sizes");
+  if (!tb.shared && batch > 4) throw std::invalid_argument("fdma_rows_lds: at most 4 entries with own tables");
+  for (int k = 0; k < (tb.shared ? 1 : batch); ++k)
+    if (!tb.band[k] && nr != m) throw std::invalid_argument("fdma_rows_lds: rhs length must equal m without a band");
+  const int per = fdma_rows_lds_per(m), bl = buf_len(nr);
+  const unsigned blocks = nrows == 2 * nsets ? (unsigned)nsets : (unsigned)((nrows + 1) / 2);
+  const size_t lds = (size_t)(WAVES / 2) * bl * sizeof(double);
+  const dim3 grid(blocks, batch), block(WAVE * WAVES);
+  if (per <= 4)
+    hipLaunchKernelGGL(fdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
+                       sr, sx);
+  else if (per <= 8)
+    hipLaunchKernelGGL(fdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
+                       sr, sx);
+  else
+    hipLaunchKernelGGL(fdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
+                       sr, sx);
+  check("fdma_rows_lds");
 }
 
 void fdma_rows_lds(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m, int nsets,
                    const double* bandp, const double* diagp, int batch, long sr, long sx) {
-  if (!fdma_rows_lds_supported(m, nr)) throw std::invalid_argument("fdma_rows_lds: unsupported row length");
-  if (nsets < 1 || nrows < 1 || batch < 1 || batch > 65535) throw std::invalid_argument("fdma_rows_lds: bad sizes");
-  if (!bandp && nr != m) throw std::invalid_argument("fdma_rows_lds: rhs length must equal m without a band");
-  const int per = fdma_rows_lds_per(m), bl = buf_len(nr);
-  const unsigned blocks =
-      nrows == 2 * nsets ? (unsigned)((nsets + WAVES / 2 - 1) / (WAVES / 2)) : (unsigned)((nrows + WAVES - 1) / WAVES);
-  const size_t lds = (size_t)WAVES * bl * sizeof(double);
-  const dim3 grid(blocks, batch), block(WAVE * WAVES);
-  if (per <= 4)
-    hipLaunchKernelGGL(fdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, bandp,
-                       diagp, sr, sx);
-  else if (per <= 8)
-    hipLaunchKernelGGL(fdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, bandp,
-                       diagp, sr, sx);
-  else
-    hipLaunchKernelGGL(fdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl,
-                       bandp, diagp, sr, sx);
-  check("fdma_rows_lds");
+  RowTables tb{};
+  tb.band[0] = bandp;
+  tb.diag[0] = diagp;
+  tb.shared = 1;
+  launch_rows(s, R, ldr, nr, X, ld, nrows, m, nsets, tb, batch, sr, sx);
+}
+
+void fdma_rows_lds_multi(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m,
+                         int nsets, const double* const* bandps, const double* const* diagps, int batch, long sr,
+                         long sx) {
+  if (batch < 1 || batch > 4) throw std::invalid_argument("fdma_rows_lds_multi: 1 to 4 entries");
+  RowTables tb{};
+  for (int k = 0; k < batch; ++k) {
+    tb.band[k] = bandps[k];
+    tb.diag[k] = diagps[k];
+  }
+  tb.shared = 0;
+  launch_rows(s, R, ldr, nr, X, ld, nrows, m, nsets, tb, batch, sr, sx);
 }
 
 bool tdma_rows_lds_supported(int m) {
