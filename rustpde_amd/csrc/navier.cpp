@@ -902,7 +902,8 @@ class Navier {
   // ----------------------------------------------------------- batched periodic step
   bool batch_ok() const {
     return use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
-           use_bands && fdma_rows_lds_supported(by_u.m, by_u.n);
+           use_bands && fdma_rows_lds_supported(by_u.m, by_u.n) && tdma_rows_lds_supported(by_u.m) &&
+           by_phi.n == by_u.n;
   }
   // irfft of `batch` slots [2 mx, cols] -> [nx, cols], (i k)^times / sc first
   void IRFFT_B(const double* in, double* out, int cols, int times, double sc, int batch) {
@@ -933,9 +934,12 @@ class Navier {
     const long no = (long)rows_ortho(bx_f) * ny, np = (long)nx * ny;
     const int my = by_u.m, R2 = 2 * mx();
     const size_t F = FS;
-    // buoyancy: temp to ortho space plus fieldbc (navier.rs:740-742)
-    to_ortho(bx_t, by_u, T.p, W3.p);
-    ew_add(st, THAT.p, W3.p, THAT_BC.p, no);
+    // ux, uy, temp in ortho space; buoyancy = temp plus fieldbc (navier.rs:740-742)
+    timer.begin(st, 3, 8.0 * 3.0 * R2 * ((double)my + ny));
+    const DBand& sb = stencil_band(by_u);
+    band_y(st, UX.p, my, A.p, ny, R2, sb.mo, sb.coef.p, sb.mask.p, 0.0, 3, (long)F, (long)F);
+    timer.end(st);
+    ew_add(st, THAT.p, A.p + 2 * F, THAT_BC.p, no);
     // ux, uy in physical space
     IRFFT_B(UX.p, W1.p, my, 0, 1.0, 2);
     CHEBY_BWD_B(by_u, W1.p, UXP.p, nx, 0, nullptr, 0.0, 2);
@@ -944,21 +948,13 @@ class Navier {
     CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 0, UXP.p, 0.0, 3);
     IRFFT_B(UX.p, W1.p, my, 0, 1.0, 3);
     CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 1, UYP.p, 1.0, 3);
-    ew_acc_prod(st, CONV.p + 2 * F, UXP.p, DTBCX.p, np);  // + u . grad(fieldbc) for temp
-    ew_acc_prod(st, CONV.p + 2 * F, UYP.p, DTBCY.p, np);
+    ew_acc_prod2(st, CONV.p + 2 * F, UXP.p, DTBCX.p, UYP.p, DTBCY.p, np);  // + u . grad(fieldbc) for temp
     CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
     RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
-    // the three fields in ortho space
-    timer.begin(st, 3, 8.0 * 3.0 * R2 * ((double)my + ny));
-    const DBand& sb = stencil_band(by_u);
-    band_y(st, UX.p, my, A.p, ny, R2, sb.mo, sb.coef.p, sb.mask.p, 0.0, 3, (long)F, (long)F);
-    timer.end(st);
-    // pressure gradients (navier.rs:622-654)
+    // pressure gradients (navier.rs:622-654) and the right-hand sides (navier.rs:660-674)
     xop(bx_p, "DS", P0.p, ny, G.p, W9.p);
     yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
-    ew_rhs_vel(st, RHS.p, A.p, G.p, nullptr, CV.p, dt, no);
-    ew_rhs_vel(st, RHS.p + F, A.p + F, G.p + F, THAT.p, CV.p + F, dt, no);
-    ew_rhs_temp(st, RHS.p + 2 * F, A.p + 2 * F, GXX.p, GYY.p, CV.p + 2 * F, dt, no);  // navier.rs:660-674
+    ew_rhs3(st, RHS.p, A.p, G.p, THAT.p, CV.p, GXX.p, GYY.p, dt, no, (long)F);
     // implicit solves of ux, uy (one solver for both) and temp in one launch
     const Solver &sv = *solver[0], &stt = *solver[2];
     const double* bands[3] = {sv.pybp.p, sv.pybp.p, stt.pybp.p};
@@ -969,9 +965,23 @@ class Navier {
     // projection (navier.rs:683-721)
     divergence(DIV.p);
     solve_pres();
-    project_velocity();
+    project_velocity_batched();
     update_pres();
     time += dt;
+  }
+  // ux -= from_ortho(d/dx S pseu), uy -= from_ortho(d/dy S pseu) (navier.rs:683-695, c = 1)
+  void project_velocity_batched() {
+    const int myp = by_phi.m, ro = rows_ortho(bx_phi);
+    const size_t F = FS;
+    const double* a = xop(bx_phi, "DS", P1.p, myp, W1.p, W9.p);
+    const double* g = yop(by_phi, "S", a, ro, W2.p);
+    if (g != W2.p) throw std::logic_error("project: stencil output expected in place");
+    const double* g2 = yop(by_phi, "DS", P1.p, ro, W2.p + F);
+    if (g2 != W2.p + F) throw std::logic_error("project: derivative output expected in place");
+    const DTdma& t = tdma_tables(by_u);
+    timer.begin(st, 3, 8.0 * 2.0 * (double)ro * ((double)by_u.n + 2.0 * by_u.m));
+    tdma_rows_lds(st, W2.p, by_u.n, UX.p, by_u.m, ro, by_u.m, t.tabp.p, 2, (long)F, (long)F, true);
+    timer.end(st);
   }
 
   void update() {  // navier.rs:737-765

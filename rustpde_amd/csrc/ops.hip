@@ -180,6 +180,27 @@ EW_KERNEL(ew_rhs_temp, (double* out, const double* a, const double* bx, const do
           out[i] = ((a[i] + bx[i]) + by[i]) - c[i] * dt)
 EW_KERNEL(ew_acc_prod, (double* out, const double* a, const double* b, long n), out[i] = out[i] + a[i] * b[i])
 EW_KERNEL(ew_project, (double* u, const double* f, double c, long n), u[i] = f[i] * (-c) + u[i])
+EW_KERNEL(ew_acc_prod2, (double* out, const double* a, const double* b, const double* c, const double* d, long n),
+          {
+            const double t = out[i] + a[i] * b[i];
+            out[i] = t + c[i] * d[i];
+          })
+
+__global__ void ew_rhs3_kernel(double* out, const double* a, const double* g, const double* buoy, const double* c,
+                               const double* bx, const double* by, double dt, long n, long F) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = blockIdx.y;
+  const long j = k * F + i;
+  if (k == 2) {
+    out[j] = ((a[j] + bx[i]) + by[i]) - c[j] * dt;
+  } else {
+    double r = a[j] - g[j] * dt;
+    if (k == 1) r = r + buoy[i] * dt;
+    out[j] = r - c[j] * dt;
+  }
+}
 EW_KERNEL(ew_update_pres, (double* p, const double* d, const double* phi, double nu, double inv_dt, long n),
           {
             double q = p[i] - d[i] * nu;
@@ -215,6 +236,17 @@ void ew_acc_prod(hipStream_t s, double* out, const double* a, const double* b, l
   EW_LAUNCH(ew_acc_prod, n, out, a, b);
 }
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n) { EW_LAUNCH(ew_project, n, u, f, c); }
+void ew_acc_prod2(hipStream_t s, double* out, const double* a, const double* b, const double* c, const double* d,
+                  long n) {
+  EW_LAUNCH(ew_acc_prod2, n, out, a, b, c, d);
+}
+void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
+             const double* bx, const double* by, double dt, long n, long F) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(ew_rhs3_kernel, dim3(blocks_for(n, 256), 3), dim3(256), 0, s, out, a, g, buoy, c, bx, by, dt, n,
+                     F);
+  check_launch("ew_rhs3");
+}
 void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt, long n) {
   EW_LAUNCH(ew_update_pres, n, p, div, phi, nu, inv_dt);
 }

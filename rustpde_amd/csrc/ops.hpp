@@ -62,6 +62,12 @@ void ew_rhs_vel(hipStream_t s, double* out, const double* a, const double* g, co
 void ew_rhs_temp(hipStream_t s, double* out, const double* a, const double* bx, const double* by,
                  const double* c, double dt, long n);  // ((a + bx) + by) - c dt
 void ew_acc_prod(hipStream_t s, double* out, const double* a, const double* b, long n);     // out + a b
+void ew_acc_prod2(hipStream_t s, double* out, const double* a, const double* b, const double* c, const double* d,
+                  long n);  // (out + a b) + c d
+// The right-hand sides of ux, uy, temp in slots 0, 1, 2 (slot stride F) in one launch:
+// slot 0 ((a - g dt)) - c dt, slot 1 ((a - g dt) + buoy dt) - c dt, slot 2 ((a + bx) + by) - c dt
+void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
+             const double* bx, const double* by, double dt, long n, long F);
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n);              // f (-c) + u
 void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt,
                     long n);  // (p - div nu) + phi inv_dt
@@ -104,8 +110,9 @@ void fdma_rows_lds_multi(hipStream_t s, const double* R, int ldr, int nr, double
 // from_ortho of every row as tdma_y, one wavefront per row with LDS staging and the
 // tables (diag, low2, off_{i-2}, den, w) in slice order: R [rows, m + 2] -> X [rows, m].
 bool tdma_rows_lds_supported(int m);
+// With project the result x is subtracted instead: X = x (-1) + X.
 void tdma_rows_lds(hipStream_t s, const double* R, int ldr, double* X, int ld, int rows, int m, const double* tabp,
-                   int batch = 1, long sr = 0, long sx = 0);
+                   int batch = 1, long sr = 0, long sx = 0, bool project = false);
 
 // Chebyshev transforms along axis 1 (rows of length n, n - 1 a power of two in [16, 2048])
 // by FFT-based DCT-I.  cheb_y_backward: in [rows, mi] coefficients (composite basis when

@@ -213,144 +213,131 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double
 
 // from_ortho of a composite basis per row (composite_stencil.rs:250-276, linalg.rs:14-57):
 // d_i = p_i diag_i + p_{i+2} low2_i, then g_i = (d_i - g_{i-2} off_{i-2}) / den_i and
-// x_i = g_i - x_{i+2} w_i (x_i = g_i for the last two i), arithmetic as tdma_y.
+// x_i = g_i - x_{i+2} w_i (x_i = g_i for the last two i), arithmetic as tdma_y.  One
+// wavefront per (row, parity chain).  With `project` the result is subtracted from X as
+// the velocity projection does (navier.rs:683-695): X = x (-1) + X.
 template <int SLICE>
 __global__ __launch_bounds__(WAVE* WAVES) void tdma_rows_lds_kernel(const double* __restrict__ R, int ldr,
                                                                     double* __restrict__ X, int ld, int rows, int m,
                                                                     int per, int bl, const double* __restrict__ tab,
-                                                                    long sr, long sx) {
+                                                                    long sr, long sx, int project) {
 #pragma clang fp contract(off)
   extern __shared__ double lds[];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const int slot = w >> 1, p = w & 1;
   R += sr * blockIdx.y;
   X += sx * blockIdx.y;
-  double* buf = lds + w * bl;
-  const int r = blockIdx.x * WAVES + w, nr = m + 2;
+  double* buf = lds + slot * bl;
+  const int r = blockIdx.x * (WAVES / 2) + slot, nr = m + 2;
   const bool valid = r < rows;
-  for (int j = lane; j < nr + 6; j += WAVE) {
+  for (int j = p * WAVE + lane; j < nr + 6; j += 2 * WAVE) {
     const int i = j - 2;
     buf[pad(j)] = (valid && i >= 0 && i < nr) ? R[(size_t)r * ldr + i] : 0.0;
   }
   __syncthreads();
-  const int len[2] = {(m + 1) / 2, m / 2};
+  const int len = p == 0 ? (m + 1) / 2 : m / 2;
   const int t0 = lane * per;
-  int cnt[2];
-  double v[2][SLICE];
-#define TB(a, p, q) tab[((size_t)((a) * 2 + (p)) * per + (q)) * WAVE + lane]
+  const int cnt = max(0, min(per, len - t0));
+  double v[SLICE];
+#define TB(a, q) tab[((size_t)((a) * 2 + p) * per + (q)) * WAVE + lane]
   {
-    double dg[2][SLICE], l2[2][SLICE];
+    double dg[SLICE], l2[SLICE];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < SLICE; ++q) {
+      dg[q] = TB(0, q);
+      l2[q] = TB(1, q);
+    }
 #pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        dg[p][q] = TB(0, p, q);
-        l2[p][q] = TB(1, p, q);
-      }
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      cnt[p] = max(0, min(per, len[p] - t0));
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        const int j = p + 2 * (t0 + q) + 2;
-        v[p][q] = q < cnt[p] ? buf[pad(j)] * dg[p][q] + buf[pad(j + 2)] * l2[p][q] : 0.0;
-      }
+    for (int q = 0; q < SLICE; ++q) {
+      const int j = p + 2 * (t0 + q) + 2;
+      v[q] = q < cnt ? buf[pad(j)] * dg[q] + buf[pad(j + 2)] * l2[q] : 0.0;
     }
   }
   {  // forward: g_i = (d_i - g_{i-2} off_{i-2}) / den_i  (i >= 2), g_i = d_i / den_i (i < 2)
-    double of[2][SLICE], dn[2][SLICE];
+    double of[SLICE], dn[SLICE];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < SLICE; ++q) {
+      of[q] = TB(2, q);
+      dn[q] = TB(3, q);
+    }
+    double A = 1.0, B = 0.0;
 #pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        of[p][q] = TB(2, p, q);
-        dn[p][q] = TB(3, p, q);
-      }
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      double A = 1.0, B = 0.0;
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        if (q < cnt[p]) {
-          if (t0 + q == 0) {
-            A = 0.0;
-            B = v[p][q] / dn[p][q];
-          } else {
-            A = (-of[p][q] / dn[p][q]) * A;
-            B = (v[p][q] - B * of[p][q]) / dn[p][q];
-          }
+    for (int q = 0; q < SLICE; ++q) {
+      if (q < cnt) {
+        if (t0 + q == 0) {
+          A = 0.0;
+          B = v[q] / dn[q];
+        } else {
+          A = (-of[q] / dn[q]) * A;
+          B = (v[q] - B * of[q]) / dn[q];
         }
       }
-      double pa = A, pb = B;
-      for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
-        const double qa = __shfl_up(pa, o2), qb = __shfl_up(pb, o2);
-        if (lane >= o2) {
-          pb = pa * qb + pb;
-          pa = pa * qa;
-        }
+    }
+    double pa = A, pb = B;
+    for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
+      const double qa = __shfl_up(pa, o2), qb = __shfl_up(pb, o2);
+      if (lane >= o2) {
+        pb = pa * qb + pb;
+        pa = pa * qa;
       }
-      double g = __shfl_up(pb, 1);
-      if (lane == 0) g = 0.0;
+    }
+    double g = __shfl_up(pb, 1);
+    if (lane == 0) g = 0.0;
 #pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        if (q < cnt[p]) {
-          g = (t0 + q == 0) ? v[p][q] / dn[p][q] : (v[p][q] - g * of[p][q]) / dn[p][q];
-          v[p][q] = g;
-        }
+    for (int q = 0; q < SLICE; ++q) {
+      if (q < cnt) {
+        g = (t0 + q == 0) ? v[q] / dn[q] : (v[q] - g * of[q]) / dn[q];
+        v[q] = g;
       }
     }
   }
   {  // back: x_i = g_i for i >= m - 2, else x_i = g_i - x_{i+2} w_i
-    double wv[2][SLICE];
+    double wv[SLICE];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < SLICE; ++q) wv[q] = TB(4, q);
+    double A = 1.0, B = 0.0;
 #pragma unroll
-      for (int q = 0; q < SLICE; ++q) wv[p][q] = TB(4, p, q);
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      double A = 1.0, B = 0.0;
-#pragma unroll
-      for (int q = SLICE - 1; q >= 0; --q) {
-        if (q < cnt[p]) {
-          const int i = p + 2 * (t0 + q);
-          if (i >= m - 2) {
-            A = 0.0;
-            B = v[p][q];
-          } else {
-            A = -wv[p][q] * A;
-            B = v[p][q] - B * wv[p][q];
-          }
+    for (int q = SLICE - 1; q >= 0; --q) {
+      if (q < cnt) {
+        if (p + 2 * (t0 + q) >= m - 2) {
+          A = 0.0;
+          B = v[q];
+        } else {
+          A = -wv[q] * A;
+          B = v[q] - B * wv[q];
         }
       }
-      double pa = A, pb = B;
-      for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
-        const double qa = __shfl_down(pa, o2), qb = __shfl_down(pb, o2);
-        if (lane + o2 < WAVE) {
-          pb = pa * qb + pb;
-          pa = pa * qa;
-        }
+    }
+    double pa = A, pb = B;
+    for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
+      const double qa = __shfl_down(pa, o2), qb = __shfl_down(pb, o2);
+      if (lane + o2 < WAVE) {
+        pb = pa * qb + pb;
+        pa = pa * qa;
       }
-      double x = __shfl_down(pb, 1);
-      if (lane == WAVE - 1) x = 0.0;
+    }
+    double x = __shfl_down(pb, 1);
+    if (lane == WAVE - 1) x = 0.0;
 #pragma unroll
-      for (int q = SLICE - 1; q >= 0; --q) {
-        if (q < cnt[p]) {
-          const int i = p + 2 * (t0 + q);
-          x = (i >= m - 2) ? v[p][q] : v[p][q] - x * wv[p][q];
-          v[p][q] = x;
-        }
+    for (int q = SLICE - 1; q >= 0; --q) {
+      if (q < cnt) {
+        x = (p + 2 * (t0 + q) >= m - 2) ? v[q] : v[q] - x * wv[q];
+        v[q] = x;
       }
     }
   }
 #undef TB
   __syncthreads();
 #pragma unroll
-  for (int p = 0; p < 2; ++p)
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q)
-      if (q < cnt[p]) buf[pad(p + 2 * (t0 + q) + 2)] = v[p][q];
+  for (int q = 0; q < SLICE; ++q)
+    if (q < cnt) buf[pad(p + 2 * (t0 + q) + 2)] = v[q];
   __syncthreads();
   if (valid)
-    for (int i = lane; i < m; i += WAVE) X[(size_t)r * ld + i] = buf[pad(i + 2)];
+    for (int i = p * WAVE + lane; i < m; i += 2 * WAVE) {
+      double* o = X + (size_t)r * ld + i;
+      const double x = buf[pad(i + 2)];
+      *o = project ? x * (-1.0) + *o : x;
+    }
 }
 
 void check(const char* what) {
@@ -413,22 +400,23 @@ void fdma_rows_lds_multi(hipStream_t s, const double* R, int ldr, int nr, double
 }
 
 bool tdma_rows_lds_supported(int m) {
-  return m >= 4 && fdma_rows_lds_per(m) <= 16 && (size_t)WAVES * buf_len(m + 2) * sizeof(double) <= 65536;
+  return m >= 4 && fdma_rows_lds_per(m) <= 16 && (size_t)(WAVES / 2) * buf_len(m + 2) * sizeof(double) <= 65536;
 }
 
 void tdma_rows_lds(hipStream_t s, const double* R, int ldr, double* X, int ld, int rows, int m, const double* tabp,
-                   int batch, long sr, long sx) {
+                   int batch, long sr, long sx, bool project) {
   if (!tdma_rows_lds_supported(m)) throw std::invalid_argument("tdma_rows_lds: unsupported row length");
   if (rows < 1 || batch < 1 || batch > 65535) throw std::invalid_argument("tdma_rows_lds: bad sizes");
-  const int per = fdma_rows_lds_per(m), bl = buf_len(m + 2);
-  const dim3 grid((unsigned)((rows + WAVES - 1) / WAVES), batch), block(WAVE * WAVES);
-  const size_t lds = (size_t)WAVES * bl * sizeof(double);
+  const int per = fdma_rows_lds_per(m), bl = buf_len(m + 2), pj = project ? 1 : 0;
+  const dim3 grid((unsigned)((rows + 1) / 2), batch), block(WAVE * WAVES);
+  const size_t lds = (size_t)(WAVES / 2) * bl * sizeof(double);
   if (per <= 4)
-    hipLaunchKernelGGL(tdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx);
+    hipLaunchKernelGGL(tdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx, pj);
   else if (per <= 8)
-    hipLaunchKernelGGL(tdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx);
+    hipLaunchKernelGGL(tdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx, pj);
   else
-    hipLaunchKernelGGL(tdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx);
+    hipLaunchKernelGGL(tdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx,
+                       pj);
   check("tdma_rows_lds");
 }
 
