@@ -124,15 +124,23 @@ int rbc_set_option(rbc_navier* h, int option, int value);
  * the external criterion bench benches/benchmark_navier.rs:23-40).  When enabled, HIP
  * events are recorded on the handle's stream around each launch of a kernel class;
  * rbc_timing_read returns the launches, their summed duration and summed algorithmic
- * work (flops for GEMMs, bytes for the memory-bound classes) since enabling.  Classes:
- * RBC_K_GEMM dense transform GEMMs, RBC_K_ROWSOLVE banded row solves, RBC_K_FFT Fourier x
- * transforms and derivatives, RBC_K_ELEMENTWISE banded stencils, scans and elementwise
- * kernels, RBC_K_DCT Chebyshev y transforms. */
+ * work (flops for GEMMs, bytes for the memory-bound classes) since enabling.  Each class
+ * but RBC_K_ELEMENTWISE is one kernel:
+ *   RBC_K_GEMM gemm_mfma_kernel (dense transforms), RBC_K_ROWSOLVE fdma_rows_lds_kernel
+ *   (Hholtz/Poisson rows), RBC_K_FFT irfft_cols_kernel, RBC_K_RFFT rfft_cols_kernel,
+ *   RBC_K_DCT cheb_y_bwd_kernel, RBC_K_DCT_FWD cheb_y_fwd_kernel, RBC_K_DIFF_Y
+ *   cheb_y_diff_kernel, RBC_K_TDMA tdma_rows_lds_kernel, RBC_K_BAND band_y_kernel;
+ *   RBC_K_ELEMENTWISE the remaining timed kernels (Fourier derivative, fallbacks). */
 #define RBC_K_GEMM 0
 #define RBC_K_ROWSOLVE 1
 #define RBC_K_FFT 2
 #define RBC_K_ELEMENTWISE 3
 #define RBC_K_DCT 4
+#define RBC_K_RFFT 5
+#define RBC_K_DCT_FWD 6
+#define RBC_K_DIFF_Y 7
+#define RBC_K_TDMA 8
+#define RBC_K_BAND 9
 int rbc_timing_enable(rbc_navier* h, int on);
 int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work);
 

@@ -100,11 +100,12 @@ __device__ __forceinline__ double stencil_value(const StencilLoad& v, bool compo
   return composite ? v.d * v.c0 + v.l * v.c2 : v.c0;
 }
 
-template <int L>
-__global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
-    const double* __restrict__ in, int ldi, int mi, double* __restrict__ out, int ldo, int rows,
-    const double* __restrict__ coef, int times, double inv_sc, const double* __restrict__ U, int ldu, double beta,
-    const double* __restrict__ tw, long si, long so, long su, bool transform) {
+template <int L, bool transform>
+__device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, int ldi, int mi,
+                                                double* __restrict__ out, int ldo, int rows,
+                                                const double* __restrict__ coef, int times, double inv_sc,
+                                                const double* __restrict__ U, int ldu, double beta,
+                                                const double* __restrict__ tw, long si, long so, long su) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
   __shared__ double re[G::E], im[G::E];
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
       im[p * L + L - k] = b;
     }
   }
-  if (transform) {
+  if constexpr (transform) {
     __syncthreads();
     lds_dft<L, T, G::E>(re, im, tw);
   }
@@ -241,6 +242,22 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     if (ra < rows) out[(size_t)ra * ldo + j] = va;
     if (rb < rows) out[(size_t)rb * ldo + j] = vb;
   }
+}
+
+template <int L>
+__global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
+    const double* __restrict__ in, int ldi, int mi, double* __restrict__ out, int ldo, int rows,
+    const double* __restrict__ coef, int times, double inv_sc, const double* __restrict__ U, int ldu, double beta,
+    const double* __restrict__ tw, long si, long so, long su) {
+  cheb_y_bwd_body<L, true>(in, ldi, mi, out, ldo, rows, coef, times, inv_sc, U, ldu, beta, tw, si, so, su);
+}
+// the same without the transform: Chebyshev coefficients of the derivative
+template <int L>
+__global__ __launch_bounds__(Geo<L>::T) void cheb_y_diff_kernel(const double* __restrict__ in, int ldi, int mi,
+                                                                double* __restrict__ out, int ldo, int rows,
+                                                                const double* __restrict__ coef, double inv_sc,
+                                                                double beta, long si, long so) {
+  cheb_y_bwd_body<L, false>(in, ldi, mi, out, ldo, rows, coef, 1, inv_sc, nullptr, 0, beta, nullptr, si, so, 0);
 }
 
 template <int L>
@@ -301,8 +318,12 @@ void launch_bwd(hipStream_t s, const double* in, int ldi, int mi, double* out, i
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
   (void)mask;  // the zero coefficients of absent taps stand for the mask
-  hipLaunchKernelGGL(cheb_y_bwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
-                     times, inv_sc, U, ldu, beta, fft_table(L), si, so, su, transform);
+  if (transform)
+    RBC_LAUNCH(cheb_y_bwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
+                       times, inv_sc, U, ldu, beta, fft_table(L), si, so, su);
+  else
+    RBC_LAUNCH(cheb_y_diff_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
+                       inv_sc, beta, si, so);
 }
 
 template <int L>
@@ -310,7 +331,7 @@ void launch_fwd(hipStream_t s, const double* in, int ldi, double* out, int ldo, 
                 long so) {
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
-  hipLaunchKernelGGL(cheb_y_fwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, out, ldo, rows, cut,
+  RBC_LAUNCH(cheb_y_fwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, out, ldo, rows, cut,
                      fft_table(L), si, so);
 }
 

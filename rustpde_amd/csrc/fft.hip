@@ -188,7 +188,7 @@ template <int L>
 void launch_rfft(hipStream_t s, const double* X, double* Y, int cols, int cut, int batch, long sx, long sy) {
   using G = CGeo<L>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
-  hipLaunchKernelGGL(rfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut, fft_table(L), sx,
+  RBC_LAUNCH(rfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut, fft_table(L), sx,
                      sy);
 }
 template <int L>
@@ -196,7 +196,7 @@ void launch_irfft(hipStream_t s, const double* X, double* Y, int cols, int times
                   long sy) {
   using G = CGeo<L>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
-  hipLaunchKernelGGL(irfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
+  RBC_LAUNCH(irfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
                      fft_table(L), sx, sy);
 }
 

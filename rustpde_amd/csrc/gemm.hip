@@ -106,7 +106,7 @@ void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int 
   if (K <= 0) throw std::runtime_error("gemm_mfma: K must be positive");
   if (batch > 65535) throw std::runtime_error("gemm_mfma: batch exceeds the grid limit");
   dim3 grid((N + 63) / 64, (M + 63) / 64, batch);
-  hipLaunchKernelGGL((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
+  RBC_LAUNCH((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
                      alpha, beta, U, ldu, sU);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("gemm_mfma: ") + hipGetErrorString(e));

@@ -390,7 +390,7 @@ void cheb_diff_y(hipStream_t s, const double* in, int ldi, double* out, int ldo,
                  const unsigned char* mask, double inv_sc, double beta, int batch, long si, long so) {
   if (!cheb_diff_y_supported(n)) throw std::invalid_argument("cheb_diff_y: unsupported length");
   const unsigned blocks = (unsigned)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-  hipLaunchKernelGGL(cheb_diff_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo,
+  RBC_LAUNCH(cheb_diff_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo,
                      rows, n, coef, mask, inv_sc, beta, si, so);
   check("cheb_diff_y");
 }
@@ -399,7 +399,7 @@ void tdma_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int 
             const double* low2, const double* off, const double* w, const double* den, int batch, long si, long so) {
   if (!tdma_y_supported(m)) throw std::invalid_argument("tdma_y: unsupported length");
   const unsigned blocks = (unsigned)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-  hipLaunchKernelGGL(tdma_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo, rows, m,
+  RBC_LAUNCH(tdma_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo, rows, m,
                      diag, low2, off, w, den, si, so);
   check("tdma_y");
 }
@@ -410,7 +410,7 @@ void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nset
                     const double* up1, const double* up2, int batch, long sx) {
   if (!fdma_rows_wave_supported(m)) throw std::invalid_argument("fdma_rows_wave: unsupported row length");
   const unsigned blocks = (unsigned)((nrows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-  hipLaunchKernelGGL(fdma_rows_wave_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, X, ld, nrows, m,
+  RBC_LAUNCH(fdma_rows_wave_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, X, ld, nrows, m,
                      nsets, low, dia, up1, up2, sx);
   check("fdma_rows_wave");
 }
@@ -418,7 +418,7 @@ void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nset
 void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int mo, const double* coef,
             const unsigned char* mask, double beta, int batch, long si, long so) {
   const long n = (long)rows * mo;
-  hipLaunchKernelGGL(band_y_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, ldo, rows,
+  RBC_LAUNCH(band_y_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, ldo, rows,
                      mo, coef, mask, beta, si, so);
   check("band_y");
 }
@@ -426,7 +426,7 @@ void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int 
 void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
             const unsigned char* mask, double beta, int batch, long si, long so) {
   const long n = (long)cols * mo;
-  hipLaunchKernelGGL(band_x_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, out, cols, mo, coef,
+  RBC_LAUNCH(band_x_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, out, cols, mo, coef,
                      mask, beta, si, so);
   check("band_x");
 }

@@ -390,19 +390,26 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
 // the launch's algorithmic work (flops for GEMMs, bytes for memory-bound kernels).
 class KernelTimer {
  public:
-  static constexpr int NCAT = 5;  // RBC_K_GEMM, RBC_K_ROWSOLVE, RBC_K_FFT, RBC_K_ELEMENTWISE, RBC_K_DCT
+  static constexpr int NCAT = 10;  // the RBC_K_* classes of the C ABI
   bool on = false;
   struct Rec { int cat; double work; hipEvent_t a, b; };
   ~KernelTimer() { clear(); for (auto e : pool_) (void)hipEventDestroy(e); }
+  // arms the launch hook: the next launch records the events in its own dispatch
   void begin(hipStream_t st, int cat, double work) {
+    (void)st;
     if (!on) return;
     Rec r{cat, work, get(), get()};
-    HIPCHECK(hipEventRecord(r.a, st));
+    launch_timing() = LaunchTiming{r.a, r.b};
     recs_.push_back(r);
   }
   void end(hipStream_t st) {
     if (!on) return;
-    HIPCHECK(hipEventRecord(recs_.back().b, st));
+    LaunchTiming& t = launch_timing();
+    if (t.start) {  // nothing was launched: an empty interval
+      HIPCHECK(hipEventRecord(t.start, st));
+      HIPCHECK(hipEventRecord(t.stop, st));
+      t = LaunchTiming{};
+    }
   }
   // accumulate finished records; returns per category (launches, ms, work)
   void collect() {
@@ -436,8 +443,8 @@ class KernelTimer {
   }
   std::vector<Rec> recs_;
   std::vector<hipEvent_t> pool_;
-  long long n_[NCAT] = {0, 0, 0, 0, 0};
-  double ms_[NCAT] = {0, 0, 0, 0, 0}, work_[NCAT] = {0, 0, 0, 0, 0};
+  long long n_[NCAT] = {};
+  double ms_[NCAT] = {}, work_[NCAT] = {};
 };
 
 // ----------------------------------------------------------------------- Navier2D
@@ -518,7 +525,7 @@ class Navier {
   }
   void ROWS(double* X, int ld, int nrows, int m, int nsets, const Solver& s) {
     // bytes: the rows read and written once, the 4 diagonals of every set read once
-    timer.begin(st, 1, 8.0 * (2.0 * nrows * m + 4.0 * (double)nsets * m));
+    timer.begin(st, 3, 8.0 * (2.0 * nrows * m + 4.0 * (double)nsets * m));
     if (fdma_rows_wave_supported(m)) fdma_rows_wave(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
     else fdma_rows(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
     timer.end(st);
@@ -536,7 +543,7 @@ class Navier {
     ROWS(X, ld, nrows, m, s.nsets, s);
   }
   void BANDY(const DBand& b, const double* in, int ldi, double* out, int ldo, int rows, double beta) {
-    timer.begin(st, 3, 8.0 * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
+    timer.begin(st, 9, 8.0 * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
     band_y(st, in, ldi, out, ldo, rows, b.mo, b.coef.p, b.mask.p, beta);
     timer.end(st);
   }
@@ -570,7 +577,7 @@ class Navier {
     return ref;
   }
   void FDIFF(const double* X, double* Y, int m, int cols, int ld, int times, double sc) {
-    timer.begin(st, 2, 8.0 * 4.0 * m * cols);
+    timer.begin(st, 3, 8.0 * 4.0 * m * cols);
     fourier_diff(st, X, Y, m, cols, ld, times, sc);
     timer.end(st);
   }
@@ -658,7 +665,7 @@ class Navier {
       if (name == "DS") { FDIFF(in, out, b.m, cols, cols, 1, scale[0]); return out; }
       if (name == "DDS") { FDIFF(in, out, b.m, cols, cols, 2, scale[0] * scale[0]); return out; }
       const double work = 8.0 * cols * ((double)b.n + 2.0 * b.m);
-      timer.begin(st, 2, work);
+      timer.begin(st, (name == "BS" || name == "BDS") ? 2 : 5, work);
       if (name == "BS") irfft_cols(st, in, out, b.n, cols, 0, 1.0);
       else if (name == "BDS") irfft_cols(st, in, out, b.n, cols, 1, scale[0]);
       else if (name == "FS") rfft_cols(st, in, out, b.n, cols, b.m);
@@ -703,7 +710,7 @@ class Navier {
     if (use_dct && use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) &&
         cheb_y_supported(b.n)) {
       const DBand* band = b.k != CHEB ? &stencil_band(b) : nullptr;
-      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
+      timer.begin(st, 7, 8.0 * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
       cheb_y_diff(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr, 1.0 / scale[1], beta);
       timer.end(st);
       return out;
@@ -728,7 +735,7 @@ class Navier {
         return out;
       }
       if ((name == "Fdl" || name == "FS") && b.k == CHEB && !U && beta == 0.0) {
-        timer.begin(st, 4, 8.0 * (double)rows * 2.0 * b.n);
+        timer.begin(st, 6, 8.0 * (double)rows * 2.0 * b.n);
         cheb_y_forward(st, in, b.n, out, b.n, rows, b.n, name == "Fdl" ? b.n * 2 / 3 : b.n);
         timer.end(st);
         return out;
@@ -736,7 +743,7 @@ class Navier {
     }
     if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_rows_lds_supported(b.m)) {
       const DTdma& t = tdma_tables(b);
-      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.n + b.m));
+      timer.begin(st, 8, 8.0 * (double)rows * ((double)b.n + b.m));
       tdma_rows_lds(st, in, b.n, out, b.m, rows, b.m, t.tabp.p);
       timer.end(st);
       return out;
@@ -912,7 +919,7 @@ class Navier {
     timer.end(st);
   }
   void RFFT_B(const double* in, double* out, int cols, int cut, int batch) {
-    timer.begin(st, 2, 8.0 * batch * cols * ((double)nx + 2.0 * mx()));
+    timer.begin(st, 5, 8.0 * batch * cols * ((double)nx + 2.0 * mx()));
     rfft_cols(st, in, out, nx, cols, cut, batch, (long)FS, (long)FS);
     timer.end(st);
   }
@@ -926,7 +933,7 @@ class Navier {
     timer.end(st);
   }
   void CHEBY_FWD_B(const double* in, double* out, int rows, int cut, int batch) {
-    timer.begin(st, 4, 8.0 * batch * rows * 2.0 * ny);
+    timer.begin(st, 6, 8.0 * batch * rows * 2.0 * ny);
     cheb_y_forward(st, in, ny, out, ny, rows, ny, cut, batch, (long)FS, (long)FS);
     timer.end(st);
   }
@@ -935,7 +942,7 @@ class Navier {
     const int my = by_u.m, R2 = 2 * mx();
     const size_t F = FS;
     // ux, uy, temp in ortho space; buoyancy = temp plus fieldbc (navier.rs:740-742)
-    timer.begin(st, 3, 8.0 * 3.0 * R2 * ((double)my + ny));
+    timer.begin(st, 9, 8.0 * 3.0 * R2 * ((double)my + ny));
     const DBand& sb = stencil_band(by_u);
     band_y(st, UX.p, my, A.p, ny, R2, sb.mo, sb.coef.p, sb.mask.p, 0.0, 3, (long)F, (long)F);
     timer.end(st);
@@ -979,7 +986,7 @@ class Navier {
     const double* g2 = yop(by_phi, "DS", P1.p, ro, W2.p + F);
     if (g2 != W2.p + F) throw std::logic_error("project: derivative output expected in place");
     const DTdma& t = tdma_tables(by_u);
-    timer.begin(st, 3, 8.0 * 2.0 * (double)ro * ((double)by_u.n + 2.0 * by_u.m));
+    timer.begin(st, 8, 8.0 * 2.0 * (double)ro * ((double)by_u.n + 2.0 * by_u.m));
     tdma_rows_lds(st, W2.p, by_u.n, UX.p, by_u.m, ro, by_u.m, t.tabp.p, 2, (long)F, (long)F, true);
     timer.end(st);
   }

@@ -8,6 +8,12 @@
 
 namespace rbc {
 
+LaunchTiming& launch_timing() {
+  thread_local LaunchTiming t;
+  return t;
+}
+
+
 static void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -83,11 +89,11 @@ void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const do
   if (K <= 0) throw std::runtime_error("gemm: K must be positive");
   if ((long)M * N >= 256L * 64 * 64) {
     dim3 grid(blocks_for(N, 64), blocks_for(M, 64));
-    hipLaunchKernelGGL((gemm_kernel<64, 64, 16, 4, 4>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
+    RBC_LAUNCH((gemm_kernel<64, 64, 16, 4, 4>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
                        beta, U, ldu);
   } else {
     dim3 grid(blocks_for(N, 32), blocks_for(M, 32));
-    hipLaunchKernelGGL((gemm_kernel<32, 32, 16, 2, 2>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
+    RBC_LAUNCH((gemm_kernel<32, 32, 16, 2, 2>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, alpha,
                        beta, U, ldu);
   }
   check_launch("gemm");
@@ -124,7 +130,7 @@ void fdma_rows(hipStream_t s, double* X, int ld, int nrows, int m, int nsets, co
                const double* up1, const double* up2) {
   if (m < 5) throw std::runtime_error("fdma_rows: system too small");
   const long nt = 2L * nrows;
-  hipLaunchKernelGGL(fdma_rows_kernel, dim3(blocks_for(nt, 64)), dim3(64), 0, s, X, ld, nrows, m, nsets, low, dia, up1,
+  RBC_LAUNCH(fdma_rows_kernel, dim3(blocks_for(nt, 64)), dim3(64), 0, s, X, ld, nrows, m, nsets, low, dia, up1,
                      up2);
   check_launch("fdma_rows");
 }
@@ -157,7 +163,7 @@ __global__ void fourier_diff_kernel(const double* __restrict__ X, double* __rest
 
 void fourier_diff(hipStream_t s, const double* X, double* Y, int m, int cols, int ld, int times, double sc) {
   const long n = (long)m * cols;
-  hipLaunchKernelGGL(fourier_diff_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, X, Y, m, cols, ld, times, sc);
+  RBC_LAUNCH(fourier_diff_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, X, Y, m, cols, ld, times, sc);
   check_launch("fourier_diff");
 }
 
@@ -218,7 +224,7 @@ EW_KERNEL(ew_nuvol, (double* out, const double* d, const double* t, const double
 #define EW_LAUNCH(name, n, ...)                                                                          \
   do {                                                                                                  \
     if ((n) > 0) {                                                                                      \
-      hipLaunchKernelGGL(name##_kernel, dim3(blocks_for((n), 256)), dim3(256), 0, s, __VA_ARGS__, (n)); \
+      RBC_LAUNCH(name##_kernel, dim3(blocks_for((n), 256)), dim3(256), 0, s, __VA_ARGS__, (n)); \
       check_launch(#name);                                                                              \
     }                                                                                                   \
   } while (0)
@@ -243,7 +249,7 @@ void ew_acc_prod2(hipStream_t s, double* out, const double* a, const double* b, 
 void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
              const double* bx, const double* by, double dt, long n, long F) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(ew_rhs3_kernel, dim3(blocks_for(n, 256), 3), dim3(256), 0, s, out, a, g, buoy, c, bx, by, dt, n,
+  RBC_LAUNCH(ew_rhs3_kernel, dim3(blocks_for(n, 256), 3), dim3(256), 0, s, out, a, g, buoy, c, bx, by, dt, n,
                      F);
   check_launch("ew_rhs3");
 }
@@ -266,7 +272,7 @@ __global__ void set_zero_kernel(double* x, int n_idx, long i0, long i1) {
   }
 }
 void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1) {
-  hipLaunchKernelGGL(set_zero_kernel, dim3(1), dim3(64), 0, s, x, n_idx, i0, i1);
+  RBC_LAUNCH(set_zero_kernel, dim3(1), dim3(64), 0, s, x, n_idx, i0, i1);
   check_launch("set_zero_entries");
 }
 
@@ -281,7 +287,7 @@ __global__ void col_weighted_sum_kernel(const double* __restrict__ v, int n0, in
   out[j] = s;
 }
 void col_weighted_sum(hipStream_t s, const double* v, int n0, int n1, const double* w, double* out) {
-  hipLaunchKernelGGL(col_weighted_sum_kernel, dim3(blocks_for(n1, 64)), dim3(64), 0, s, v, n0, n1, w, out);
+  RBC_LAUNCH(col_weighted_sum_kernel, dim3(blocks_for(n1, 64)), dim3(64), 0, s, v, n0, n1, w, out);
   check_launch("col_weighted_sum");
 }
 
@@ -328,15 +334,15 @@ __global__ void sum_kernel(const double* __restrict__ part, int np, double* __re
 }
 void dot_and_norm(hipStream_t s, const double* a, const double* w, int n, const double* b, long nb, double* out) {
   if (a) {
-    hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(256), 0, s, a, w, n, out);
+    RBC_LAUNCH(dot_kernel, dim3(1), dim3(256), 0, s, a, w, n, out);
     check_launch("dot");
   }
   if (b) {
     // partial sums land in out[2 .. 2 + 256); the caller's buffer holds >= 258 doubles
     const int np = 256;
-    hipLaunchKernelGGL(sumsq_partial_kernel, dim3(np), dim3(256), 0, s, b, nb, out + 2);
+    RBC_LAUNCH(sumsq_partial_kernel, dim3(np), dim3(256), 0, s, b, nb, out + 2);
     check_launch("sumsq_partial");
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, out + 2, np, out + 1);
+    RBC_LAUNCH(sum_kernel, dim3(1), dim3(256), 0, s, out + 2, np, out + 1);
     check_launch("sum");
   }
 }
@@ -355,11 +361,11 @@ __global__ void s2i_kernel(const double* __restrict__ in, double* __restrict__ o
   out[2 * t + 1] = in[(long)m * c + t];
 }
 void interleaved_to_split(hipStream_t s, const double* in, double* out, int m, int c) {
-  hipLaunchKernelGGL(i2s_kernel, dim3(blocks_for((long)m * c, 256)), dim3(256), 0, s, in, out, m, c);
+  RBC_LAUNCH(i2s_kernel, dim3(blocks_for((long)m * c, 256)), dim3(256), 0, s, in, out, m, c);
   check_launch("interleaved_to_split");
 }
 void split_to_interleaved(hipStream_t s, const double* in, double* out, int m, int c) {
-  hipLaunchKernelGGL(s2i_kernel, dim3(blocks_for((long)m * c, 256)), dim3(256), 0, s, in, out, m, c);
+  RBC_LAUNCH(s2i_kernel, dim3(blocks_for((long)m * c, 256)), dim3(256), 0, s, in, out, m, c);
   check_launch("split_to_interleaved");
 }
 

@@ -2,9 +2,30 @@
 // FP64 and row-major.  Complex spectral fields are stored as split planes: a complex
 // [m, c] array is the real array [2m, c] whose first m rows hold the real parts.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 namespace rbc {
+
+// Live kernel timing: when armed (per host thread), the next RBC_LAUNCH records the two
+// events at the start and the end of that kernel's dispatch (hipExtLaunchKernelGGL), so
+// the measured duration is the kernel's own, as a profiler's kernel trace reports it.
+struct LaunchTiming {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+LaunchTiming& launch_timing();
+
+#define RBC_LAUNCH(kernel, grid, block, shm, stream, ...)                                       \
+  do {                                                                                         \
+    ::rbc::LaunchTiming& lt_ = ::rbc::launch_timing();                                         \
+    if (lt_.start) {                                                                           \
+      hipEvent_t a_ = lt_.start, b_ = lt_.stop;                                                \
+      lt_.start = lt_.stop = nullptr;                                                          \
+      hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, a_, b_, 0, __VA_ARGS__);         \
+    } else {                                                                                   \
+      hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);                       \
+    }                                                                                          \
+  } while (0)
 
 // C[M,N] = alpha * (A[M,K] B[K,N]) (.* U if U) + beta * C
 void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,

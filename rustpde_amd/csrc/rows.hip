@@ -366,13 +366,13 @@ static void launch_rows(hipStream_t s, const double* R, int ldr, int nr, double*
   const size_t lds = (size_t)(WAVES / 2) * bl * sizeof(double);
   const dim3 grid(blocks, batch), block(WAVE * WAVES);
   if (per <= 4)
-    hipLaunchKernelGGL(fdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
+    RBC_LAUNCH(fdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
                        sr, sx);
   else if (per <= 8)
-    hipLaunchKernelGGL(fdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
+    RBC_LAUNCH(fdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
                        sr, sx);
   else
-    hipLaunchKernelGGL(fdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
+    RBC_LAUNCH(fdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
                        sr, sx);
   check("fdma_rows_lds");
 }
@@ -411,11 +411,11 @@ void tdma_rows_lds(hipStream_t s, const double* R, int ldr, double* X, int ld, i
   const dim3 grid((unsigned)((rows + 1) / 2), batch), block(WAVE * WAVES);
   const size_t lds = (size_t)(WAVES / 2) * bl * sizeof(double);
   if (per <= 4)
-    hipLaunchKernelGGL(tdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx, pj);
+    RBC_LAUNCH(tdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx, pj);
   else if (per <= 8)
-    hipLaunchKernelGGL(tdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx, pj);
+    RBC_LAUNCH(tdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx, pj);
   else
-    hipLaunchKernelGGL(tdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx,
+    RBC_LAUNCH(tdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, X, ld, rows, m, per, bl, tabp, sr, sx,
                        pj);
   check("tdma_rows_lds");
 }
