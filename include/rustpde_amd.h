@@ -111,13 +111,16 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *   RBC_OPT_DCT    Chebyshev y transforms by FFT-based DCT-I kernels when ny - 1 is a
  *                  power of two (default 1; 0 = dense transform GEMMs)
  *   RBC_OPT_BATCH  periodic step: the transforms of ux, uy and temp and the ux/uy solves
- *                  run as batched launches (default 1; needs the FFT, DCT and band paths) */
+ *                  run as batched launches (default 1; needs the FFT, DCT and band paths)
+ *   RBC_OPT_FUSE   periodic batched step: divergence, Poisson rows, projection and
+ *                  pressure update as one kernel per x-wavenumber (default 1) */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2
 #define RBC_OPT_BANDS 3
 #define RBC_OPT_DCT 4
 #define RBC_OPT_BATCH 5
+#define RBC_OPT_FUSE 6
 int rbc_set_option(rbc_navier* h, int option, int value);
 
 /* Kernel timing of a handle (no counterpart in the reference, whose timing protocol is
@@ -130,6 +133,7 @@ int rbc_set_option(rbc_navier* h, int option, int value);
  *   (Hholtz/Poisson rows), RBC_K_FFT irfft_cols_kernel, RBC_K_RFFT rfft_cols_kernel,
  *   RBC_K_DCT cheb_y_bwd_kernel, RBC_K_DCT_FWD cheb_y_fwd_kernel, RBC_K_DIFF_Y
  *   cheb_y_diff_kernel, RBC_K_TDMA tdma_rows_lds_kernel, RBC_K_BAND band_y_kernel;
+ *   RBC_K_PRESSURE pressure_rows_kernel (the fused pressure stage);
  *   RBC_K_ELEMENTWISE the remaining timed kernels (Fourier derivative, fallbacks). */
 #define RBC_K_GEMM 0
 #define RBC_K_ROWSOLVE 1
@@ -141,6 +145,7 @@ int rbc_set_option(rbc_navier* h, int option, int value);
 #define RBC_K_DIFF_Y 7
 #define RBC_K_TDMA 8
 #define RBC_K_BAND 9
+#define RBC_K_PRESSURE 10
 int rbc_timing_enable(rbc_navier* h, int on);
 int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work);
 

@@ -390,7 +390,7 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
 // the launch's algorithmic work (flops for GEMMs, bytes for memory-bound kernels).
 class KernelTimer {
  public:
-  static constexpr int NCAT = 10;  // the RBC_K_* classes of the C ABI
+  static constexpr int NCAT = 11;  // the RBC_K_* classes of the C ABI
   bool on = false;
   struct Rec { int cat; double work; hipEvent_t a, b; };
   ~KernelTimer() { clear(); for (auto e : pool_) (void)hipEventDestroy(e); }
@@ -474,6 +474,7 @@ class Navier {
   bool use_graph = true;  // replay update() as a captured HIP graph
   bool use_dct = true;    // Chebyshev y transforms by FFT-based DCT-I kernels
   bool use_batch = true;  // periodic step: the three fields' transforms and solves batched
+  bool use_fuse = true;   // periodic step: the pressure stage as one fused kernel
   bool vel_same = true;   // the ux and uy solvers hold identical tables
   size_t FS = 0;          // slot stride of the batched buffers
   DBuf STATE, PHYS2;      // [ux, uy, temp] spectral and [ux, uy] physical, one slot each
@@ -970,11 +971,27 @@ class Navier {
     fdma_rows_lds_multi(st, RHS.p, ny, ny, UX.p, my, R2, my, sv.nsets, bands, diags, 3, (long)F, (long)F);
     timer.end(st);
     // projection (navier.rs:683-721)
-    divergence(DIV.p);
-    solve_pres();
-    project_velocity_batched();
-    update_pres();
+    if (pressure_fused_ok()) {
+      const Solver& sp = *solver[3];
+      PressureArgs pa{UX.p, UY.p, P0.p, P1.p, stencil_band(by_u).coef.p, stencil_band(by_phi).coef.p, sp.pybp.p,
+                      sp.diagp.p, tdma_tables(by_u).tabp.p, mx(), ny, by_u.m, 0, 0, scale[0], 1.0 / scale[1], nu,
+                      1.0 / dt};
+      const double rows = 2.0 * mx();
+      timer.begin(st, 10, 8.0 * rows * (2.0 * by_u.m + ny) + 8.0 * rows * (3.0 * by_u.m + ny) +
+                              8.0 * 4.0 * mx() * (double)by_phi.m);
+      pressure_rows(st, pa);
+      timer.end(st);
+    } else {
+      divergence(DIV.p);
+      solve_pres();
+      project_velocity_batched();
+      update_pres();
+    }
     time += dt;
+  }
+  bool pressure_fused_ok() {
+    return use_fuse && pressure_rows_supported(ny, by_u.m) && by_phi.m == by_u.m && !solver[3]->has_fb &&
+           solver[3]->nsets == mx();
   }
   // ux -= from_ortho(d/dx S pseu), uy -= from_ortho(d/dy S pseu) (navier.rs:683-695, c = 1)
   void project_velocity_batched() {
@@ -1586,6 +1603,7 @@ int rbc_set_option(rbc_navier* h, int option, int value) {
       case RBC_OPT_BANDS: n.use_bands = value != 0; break;
       case RBC_OPT_DCT: n.use_dct = value != 0; break;
       case RBC_OPT_BATCH: n.use_batch = value != 0; break;
+      case RBC_OPT_FUSE: n.use_fuse = value != 0; break;
       default: throw std::invalid_argument("unknown option");
     }
   });

@@ -135,6 +135,21 @@ bool tdma_rows_lds_supported(int m);
 void tdma_rows_lds(hipStream_t s, const double* R, int ldr, double* X, int ld, int rows, int m, const double* tabp,
                    int batch = 1, long sr = 0, long sx = 0, bool project = false);
 
+// The pressure stage of the periodic step fused per x-wavenumber (one workgroup per k):
+// divergence, Poisson rows (pseu(0,0) = 0), projection of ux and uy, pressure update.
+// Fields are split-plane spectral arrays: ux, uy [2 mx, mu], pres [2 mx, ny], pseu
+// [2 mx, mu]; su / sp are the stencil taps [2][ny] of the velocity / pseudo-pressure y
+// bases; pybp / pdiag the Poisson solver's P_y taps and diagonal sets and ttab the tdma
+// tables of the velocity y basis, all in slice order.  per and bl are set by the call.
+struct PressureArgs {
+  double *UX, *UY, *P0, *P1;
+  const double *su, *sp, *pybp, *pdiag, *ttab;
+  int mx, ny, mu, per, bl;
+  double kscale, inv_scy, nu, inv_dt;
+};
+bool pressure_rows_supported(int ny, int mu);
+void pressure_rows(hipStream_t s, const PressureArgs& a);
+
 // Chebyshev transforms along axis 1 (rows of length n, n - 1 a power of two in [16, 2048])
 // by FFT-based DCT-I.  cheb_y_backward: in [rows, mi] coefficients (composite basis when
 // coef / mask give its stencil taps, else Chebyshev, mi <= n), `times` (0 or 1)

@@ -1,0 +1,339 @@
+// The pressure stage of the periodic step (navier.rs:683-721) fused per x-wavenumber, FP64.
+//
+// With a Fourier x basis every operation of the stage acts on the two spectral rows
+// (real and imaginary plane) of one wavenumber k: the divergence
+//     div = S_u(ik ux / sc) + d/dy S_u(uy)                          (navier.rs:698-703)
+// the Poisson rows  P_y div -> pseu (Fdma of set k, fdma_tensor.rs:195-234), pseu(0,0) = 0,
+// the projection    ux -= FO_u(S_p(ik pseu / sc)),  uy -= FO_u(d/dy S_p(pseu))
+// and the update    pres = (pres - div nu) + S_p(pseu) / dt.
+// One workgroup owns wavenumber k: four wavefronts, one per (plane, parity chain); the rows
+// live in padded LDS buffers between the phases, so the stage reads ux, uy, pres once and
+// writes ux, uy, pres, pseu once.  Every value is formed with the arithmetic of the unfused
+// kernels it replaces (band stencils, the num-complex Fourier derivative, the derivative
+// suffix sums, the Fdma and tdma recurrences of chains.hpp).
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "chains.hpp"
+#include "ops.hpp"
+
+namespace rbc {
+
+namespace {
+
+using chains::buf_len;
+using chains::pad;
+using chains::WAVE;
+
+// (a + ib)(0 + ik) then / (sc + 0i), as num-complex forms them (fourier_diff, times = 1)
+__device__ __forceinline__ void fdiff(double a, double b, double kk, double sc, double& ra, double& rb) {
+#pragma clang fp contract(off)
+  double na = a * 0.0 - b * kk, nb = a * kk + b * 0.0;
+  if (sc != 1.0) {
+    const double ns = sc * sc + 0.0 * 0.0;
+    const double qa = (na * sc + nb * 0.0) / ns, qb = (nb * sc - na * 0.0) / ns;
+    na = qa;
+    nb = qb;
+  }
+  ra = na;
+  rb = nb;
+}
+
+template <int SLICE>
+__global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+  const int r = w >> 1, p = w & 1;  // plane (0 real, 1 imaginary) and chain parity of this wavefront
+  const int k = blockIdx.x, mx = a.mx, ny = a.ny, mu = a.mu, N = ny - 1, bl = a.bl;
+  const double kk = (double)k;
+  // LDS rows (index = plane): XA ux, later projection results; YB uy, later pseu; DV div
+#define XA(pl) (lds + (pl) * bl)
+#define YB(pl) (lds + (2 + (pl)) * bl)
+#define DV(pl) (lds + (4 + (pl)) * bl)
+#define ROWU(pl) ((size_t)((pl) ? mx + k : k) * mu)
+#define ROWN(pl) ((size_t)((pl) ? mx + k : k) * ny)
+  const double* __restrict__ su0 = a.su;
+  const double* __restrict__ su1 = a.su + ny;
+  const double* __restrict__ sp0 = a.sp;
+  const double* __restrict__ sp1 = a.sp + ny;
+
+  // (1) ux, uy rows -> LDS (element i at pad(i + 2), zero guards at i in [-2, 0), [mu, mu + 4))
+  for (int e = tid; e < 4 * (mu + 6); e += 256) {
+    const int row = e / (mu + 6), j = e - row * (mu + 6), i = j - 2;
+    const double* src = row < 2 ? a.UX : a.UY;
+    double v = 0.0;
+    if (i >= 0 && i < mu) v = src[ROWU(row & 1) + i];
+    lds[(row < 2 ? (row & 1) : 2 + (row & 1)) * bl + pad(j)] = v;
+  }
+  __syncthreads();
+  // (2a) div = S_u(ik ux / sc) for i < ny (taps i, i - 2), zero guards around it
+  for (int j = tid; j < ny + 6; j += 256) {
+    const int i = j - 2;
+    double d0 = 0.0, d1 = 0.0;
+    if (i >= 0 && i < ny) {
+      double c0r, c0i, c2r, c2i;
+      fdiff(XA(0)[pad(j)], XA(1)[pad(j)], kk, a.kscale, c0r, c0i);
+      fdiff(XA(0)[pad(j - 2)], XA(1)[pad(j - 2)], kk, a.kscale, c2r, c2i);
+      d0 = c0r * su0[i] + c2r * su1[i];
+      d1 = c0i * su0[i] + c2i * su1[i];
+    }
+    DV(0)[pad(j)] = d0;
+    DV(1)[pad(j)] = d1;
+  }
+  __syncthreads();
+  // (2b) div += d/dy S_u(uy): g_i = 2 (i + 1) s_{i+1} summed from the top along the chains
+  //      1, 3, .., N - 1 and 2, 4, .., N; d_0 = s_1 + d_2 / 2 (ortho.rs:107-125)
+  const int Cd = N / 2, perd = (Cd + WAVE - 1) / WAVE, t0d = lane * perd;
+  const int cntd = max(0, min(perd, Cd - t0d));
+  {
+    double v[SLICE];
+    const double* Y = YB(r);
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      v[q] = 0.0;
+      if (q < cntd) {
+        const int i = (p ? 1 : 2) + 2 * (t0d + q);
+        if (i <= N - 1) {
+          const double s = Y[pad(i + 3)] * su0[i + 1] + Y[pad(i + 1)] * su1[i + 1];
+          v[q] = (2.0 * (double)(i + 1)) * s;
+        }
+      }
+    }
+    chains::suffix_sum(v, lane, cntd);
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      if (q < cntd) {
+        const int i = (p ? 1 : 2) + 2 * (t0d + q);
+        double d = v[q];
+        if (a.inv_scy != 1.0) d = d * a.inv_scy;
+        DV(r)[pad(i + 2)] = d + 1.0 * DV(r)[pad(i + 2)];
+      }
+    }
+    if (p == 0 && lane == 0) {
+      const double s1 = Y[pad(3)] * su0[1] + Y[pad(1)] * su1[1];
+      double d = s1 + v[0] / 2.0;
+      if (a.inv_scy != 1.0) d = d * a.inv_scy;
+      DV(r)[pad(2)] = d + 1.0 * DV(r)[pad(2)];
+    }
+  }
+  __syncthreads();
+  // (3) pseu = Poisson rows of div (P_y taps, then the Fdma of set k); pseu(0,0) = 0
+  const int per = a.per, t0 = lane * per;
+  const int lenu = p == 0 ? (mu + 1) / 2 : mu / 2;
+  const int cnt = max(0, min(per, lenu - t0));
+  {
+    double v[SLICE];
+    const size_t tp = (size_t)2 * per * WAVE;
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      const size_t b = ((size_t)p * per + q) * WAVE + lane;
+      const double c0 = a.pybp[b], c1 = a.pybp[tp + b], c2 = a.pybp[2 * tp + b], c3 = a.pybp[3 * tp + b];
+      double val = 0.0;
+      if (q < cnt) {
+        const int j = p + 2 * (t0 + q) + 2;
+        const double* D = DV(r);
+        val = c0 * D[pad(j)];
+        val = val + c1 * D[pad(j - 2)];
+        val = val + c2 * D[pad(j + 2)];
+        val = val + c3 * D[pad(j + 4)];
+      }
+      v[q] = val;
+    }
+    const size_t sb = (size_t)k * (8 * (size_t)per * WAVE);
+#define DG(arr, q) a.pdiag[sb + ((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
+    {
+      double lo[SLICE];
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) lo[q] = DG(0, q);
+      chains::fdma_forward(v, lo, lane, t0, cnt);
+    }
+    {
+      double dg[SLICE], g1[SLICE], g2[SLICE];
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        dg[q] = DG(1, q);
+        g1[q] = DG(2, q);
+        g2[q] = DG(3, q);
+      }
+      chains::fdma_backward(v, dg, g1, g2, lane, t0, cnt, lenu);
+    }
+#undef DG
+    if (k == 0 && p == 0 && lane == 0) v[0] = 0.0;  // navier.rs:710-715
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q)
+      if (q < cnt) YB(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
+  }
+  __syncthreads();
+  // (4a) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt
+  for (int e = tid; e < 2 * ny; e += 256) {
+    const int rr = e / ny, i = e - rr * ny;
+    const double* Pp = YB(rr);
+    if (i < mu) a.P1[ROWU(rr) + i] = Pp[pad(i + 2)];
+    const double w3 = Pp[pad(i + 2)] * sp0[i] + Pp[pad(i)] * sp1[i];
+    double* pr = a.P0 + ROWN(rr) + i;
+    const double q = *pr - DV(rr)[pad(i + 2)] * a.nu;
+    *pr = q + w3 * a.inv_dt;
+  }
+  // (4b) ux projection: d_i = g_i diag_i + g_{i+2} low2_i with g = S_p(ik pseu / sc), then
+  //      the tdma recurrences of from_ortho
+#define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
+  {
+    double v[SLICE];
+    {
+      double dg[SLICE], l2[SLICE];
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        dg[q] = TB(0, q);
+        l2[q] = TB(1, q);
+      }
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        v[q] = 0.0;
+        if (q < cnt) {
+          const int i = p + 2 * (t0 + q);
+          double c[3];  // plane r of ik pseu / sc at i - 2, i, i + 2
+#pragma unroll
+          for (int o = 0; o < 3; ++o) {
+            double cre, cim;
+            fdiff(YB(0)[pad(i + 2 * o)], YB(1)[pad(i + 2 * o)], kk, a.kscale, cre, cim);
+            c[o] = r ? cim : cre;
+          }
+          const double g0 = c[1] * sp0[i] + c[0] * sp1[i];
+          const double g2 = c[2] * sp0[i + 2] + c[1] * sp1[i + 2];
+          v[q] = g0 * dg[q] + g2 * l2[q];
+        }
+      }
+    }
+    {
+      double of[SLICE], dn[SLICE], wv[SLICE];
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        of[q] = TB(2, q);
+        dn[q] = TB(3, q);
+        wv[q] = TB(4, q);
+      }
+      chains::tdma_forward(v, of, dn, lane, t0, cnt);
+      chains::tdma_backward(v, wv, lane, t0, cnt, p, mu);
+    }
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q)
+      if (q < cnt) XA(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
+  }
+  __syncthreads();
+  // (5a) ux = x (-1) + ux
+  for (int e = tid; e < 2 * mu; e += 256) {
+    const int rr = e / mu, i = e - rr * mu;
+    double* u = a.UX + ROWU(rr) + i;
+    *u = XA(rr)[pad(i + 2)] * (-1.0) + *u;
+  }
+  // (5b) d/dy S_p(pseu) into DV (the chains of (2b))
+  {
+    double v[SLICE];
+    const double* Pp = YB(r);
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      v[q] = 0.0;
+      if (q < cntd) {
+        const int i = (p ? 1 : 2) + 2 * (t0d + q);
+        if (i <= N - 1) {
+          const double s = Pp[pad(i + 3)] * sp0[i + 1] + Pp[pad(i + 1)] * sp1[i + 1];
+          v[q] = (2.0 * (double)(i + 1)) * s;
+        }
+      }
+    }
+    chains::suffix_sum(v, lane, cntd);
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      if (q < cntd) {
+        const int i = (p ? 1 : 2) + 2 * (t0d + q);
+        double d = v[q];
+        if (a.inv_scy != 1.0) d = d * a.inv_scy;
+        DV(r)[pad(i + 2)] = d;
+      }
+    }
+    if (p == 0 && lane == 0) {
+      const double s1 = Pp[pad(3)] * sp0[1] + Pp[pad(1)] * sp1[1];
+      double d = s1 + v[0] / 2.0;
+      if (a.inv_scy != 1.0) d = d * a.inv_scy;
+      DV(r)[pad(2)] = d;
+    }
+  }
+  __syncthreads();
+  // (6) uy projection: tdma of d_i = g_i diag_i + g_{i+2} low2_i with g = DV
+  {
+    double v[SLICE];
+    {
+      double dg[SLICE], l2[SLICE];
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        dg[q] = TB(0, q);
+        l2[q] = TB(1, q);
+      }
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        const int j = p + 2 * (t0 + q) + 2;
+        v[q] = q < cnt ? DV(r)[pad(j)] * dg[q] + DV(r)[pad(j + 2)] * l2[q] : 0.0;
+      }
+    }
+    {
+      double of[SLICE], dn[SLICE], wv[SLICE];
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        of[q] = TB(2, q);
+        dn[q] = TB(3, q);
+        wv[q] = TB(4, q);
+      }
+      chains::tdma_forward(v, of, dn, lane, t0, cnt);
+      chains::tdma_backward(v, wv, lane, t0, cnt, p, mu);
+    }
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q)
+      if (q < cnt) XA(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
+  }
+#undef TB
+  __syncthreads();
+  // (7) uy = x (-1) + uy
+  for (int e = tid; e < 2 * mu; e += 256) {
+    const int rr = e / mu, i = e - rr * mu;
+    double* u = a.UY + ROWU(rr) + i;
+    *u = XA(rr)[pad(i + 2)] * (-1.0) + *u;
+  }
+#undef XA
+#undef YB
+#undef DV
+#undef ROWU
+#undef ROWN
+}
+
+void check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+bool pressure_rows_supported(int ny, int mu) {
+  const int per = fdma_rows_lds_per(mu), perd = ((ny - 1) / 2 + WAVE - 1) / WAVE;
+  return ny >= 9 && (ny & 1) == 1 && mu == ny - 2 && per <= 8 && perd <= 8 &&
+         (size_t)6 * buf_len(ny) * sizeof(double) <= 65536;
+}
+
+void pressure_rows(hipStream_t s, const PressureArgs& args) {
+  if (!pressure_rows_supported(args.ny, args.mu)) throw std::invalid_argument("pressure_rows: unsupported sizes");
+  if (args.mx < 1) throw std::invalid_argument("pressure_rows: bad mx");
+  PressureArgs a = args;
+  a.per = fdma_rows_lds_per(a.mu);
+  a.bl = buf_len(a.ny);
+  const int perd = ((a.ny - 1) / 2 + WAVE - 1) / WAVE, sl = a.per > perd ? a.per : perd;
+  const size_t lds = (size_t)6 * a.bl * sizeof(double);
+  if (sl <= 4)
+    RBC_LAUNCH(pressure_rows_kernel<4>, dim3(a.mx), dim3(256), lds, s, a);
+  else
+    RBC_LAUNCH(pressure_rows_kernel<8>, dim3(a.mx), dim3(256), lds, s, a);
+  check("pressure_rows");
+}
+
+}  // namespace rbc

@@ -20,19 +20,17 @@
 #include <stdexcept>
 #include <string>
 
+#include "chains.hpp"
 #include "ops.hpp"
 
 namespace rbc {
 
 namespace {
 
-constexpr int WAVE = 64;
-constexpr int WAVES = 4;  // wavefronts (rows) per block
-
-__host__ __device__ __forceinline__ int pad(int j) { return j + (j >> 4); }
-
-// LDS doubles per wavefront for rows of nr values (two guard zeros on each side)
-inline int buf_len(int nr) { return pad(nr + 6) + 1; }
+using chains::buf_len;
+using chains::pad;
+using chains::WAVE;
+constexpr int WAVES = 4;  // wavefronts per block
 
 // Tables of up to four batch entries (blockIdx.y), or one shared by all entries.
 struct RowTables {
@@ -106,43 +104,13 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) v[q] = q < cnt ? buf[pad(p + 2 * (t0 + q) + 2)] : 0.0;
   }
-  // ---- forward elimination: y_t = x_t - y_{t-1} low[i-2]  (i = p + 2t >= 2)
+  // ---- forward elimination and back substitution (diagonal tables of this set)
   {
     double lo[SLICE];
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) lo[q] = DG(0, q);
-    double A = 1.0, B = 0.0;  // slice map y_out = A y_in + B
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) {
-      if (q < cnt) {
-        if (t0 + q == 0) {
-          A = 0.0;
-          B = v[q];
-        } else {
-          A = -A * lo[q];
-          B = v[q] - B * lo[q];
-        }
-      }
-    }
-    double pa = A, pb = B;  // inclusive scan, lower lanes first
-    for (int off = 1; off < WAVE; off <<= 1) {
-      const double qa = __shfl_up(pa, off), qb = __shfl_up(pb, off);
-      if (lane >= off) {
-        pb = pa * qb + pb;
-        pa = pa * qa;
-      }
-    }
-    double y = __shfl_up(pb, 1);
-    if (lane == 0) y = 0.0;
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) {
-      if (q < cnt) {
-        y = (t0 + q == 0) ? v[q] : v[q] - y * lo[q];
-        v[q] = y;
-      }
-    }
+    chains::fdma_forward(v, lo, lane, t0, cnt);
   }
-  // ---- back substitution: x_t = (y_t - x_{t+1} up1[i] - x_{t+2} up2[i]) / dia[i]
   {
     double dg[SLICE], g1[SLICE], g2[SLICE];
 #pragma unroll
@@ -151,54 +119,7 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double
       g1[q] = DG(2, q);
       g2[q] = DG(3, q);
     }
-    // slice map on the state s = (x_{t+1}, x_{t+2}):  s' = M s + w
-    double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0, w0 = 0.0, w1 = 0.0;
-#pragma unroll
-    for (int q = SLICE - 1; q >= 0; --q) {
-      if (q >= cnt) continue;
-      const double d = dg[q];
-      const double c0 = v[q] / d, c1 = -g1[q] / d, c2 = -g2[q] / d;
-      const double n00 = c1 * m00 + c2 * m10, n01 = c1 * m01 + c2 * m11, nw0 = c1 * w0 + c2 * w1 + c0;
-      m10 = m00;
-      m11 = m01;
-      w1 = w0;
-      m00 = n00;
-      m01 = n01;
-      w0 = nw0;
-    }
-    double a00 = m00, a01 = m01, a10 = m10, a11 = m11, b0 = w0, b1 = w1;  // suffix scan
-    for (int off = 1; off < WAVE; off <<= 1) {
-      const double q00 = __shfl_down(a00, off), q01 = __shfl_down(a01, off), q10 = __shfl_down(a10, off),
-                   q11 = __shfl_down(a11, off), r0 = __shfl_down(b0, off), r1 = __shfl_down(b1, off);
-      if (lane + off < WAVE) {
-        const double n00 = a00 * q00 + a01 * q10, n01 = a00 * q01 + a01 * q11;
-        const double n10 = a10 * q00 + a11 * q10, n11 = a10 * q01 + a11 * q11;
-        const double nb0 = a00 * r0 + a01 * r1 + b0, nb1 = a10 * r0 + a11 * r1 + b1;
-        a00 = n00;
-        a01 = n01;
-        a10 = n10;
-        a11 = n11;
-        b0 = nb0;
-        b1 = nb1;
-      }
-    }
-    double s0 = __shfl_down(b0, 1), s1 = __shfl_down(b1, 1);
-    if (lane == WAVE - 1) {
-      s0 = 0.0;
-      s1 = 0.0;
-    }
-#pragma unroll
-    for (int q = SLICE - 1; q >= 0; --q) {
-      if (q >= cnt) continue;
-      const int t = t0 + q;
-      double val = v[q];
-      if (t + 1 < len) val = val - s0 * g1[q];
-      if (t + 2 < len) val = val - s1 * g2[q];
-      val = val / dg[q];
-      s1 = s0;
-      s0 = val;
-      v[q] = val;
-    }
+    chains::fdma_backward(v, dg, g1, g2, lane, t0, cnt, len);
   }
 #undef DG
   // (3) slices -> LDS -> row, unit stride
@@ -260,71 +181,13 @@ __global__ __launch_bounds__(WAVE* WAVES) void tdma_rows_lds_kernel(const double
       of[q] = TB(2, q);
       dn[q] = TB(3, q);
     }
-    double A = 1.0, B = 0.0;
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) {
-      if (q < cnt) {
-        if (t0 + q == 0) {
-          A = 0.0;
-          B = v[q] / dn[q];
-        } else {
-          A = (-of[q] / dn[q]) * A;
-          B = (v[q] - B * of[q]) / dn[q];
-        }
-      }
-    }
-    double pa = A, pb = B;
-    for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
-      const double qa = __shfl_up(pa, o2), qb = __shfl_up(pb, o2);
-      if (lane >= o2) {
-        pb = pa * qb + pb;
-        pa = pa * qa;
-      }
-    }
-    double g = __shfl_up(pb, 1);
-    if (lane == 0) g = 0.0;
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) {
-      if (q < cnt) {
-        g = (t0 + q == 0) ? v[q] / dn[q] : (v[q] - g * of[q]) / dn[q];
-        v[q] = g;
-      }
-    }
+    chains::tdma_forward(v, of, dn, lane, t0, cnt);
   }
   {  // back: x_i = g_i for i >= m - 2, else x_i = g_i - x_{i+2} w_i
     double wv[SLICE];
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) wv[q] = TB(4, q);
-    double A = 1.0, B = 0.0;
-#pragma unroll
-    for (int q = SLICE - 1; q >= 0; --q) {
-      if (q < cnt) {
-        if (p + 2 * (t0 + q) >= m - 2) {
-          A = 0.0;
-          B = v[q];
-        } else {
-          A = -wv[q] * A;
-          B = v[q] - B * wv[q];
-        }
-      }
-    }
-    double pa = A, pb = B;
-    for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
-      const double qa = __shfl_down(pa, o2), qb = __shfl_down(pb, o2);
-      if (lane + o2 < WAVE) {
-        pb = pa * qb + pb;
-        pa = pa * qa;
-      }
-    }
-    double x = __shfl_down(pb, 1);
-    if (lane == WAVE - 1) x = 0.0;
-#pragma unroll
-    for (int q = SLICE - 1; q >= 0; --q) {
-      if (q < cnt) {
-        x = (p + 2 * (t0 + q) >= m - 2) ? v[q] : v[q] - x * wv[q];
-        v[q] = x;
-      }
-    }
+    chains::tdma_backward(v, wv, lane, t0, cnt, p, m);
   }
 #undef TB
   __syncthreads();

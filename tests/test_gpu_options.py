@@ -13,7 +13,7 @@ from helpers import compare, init_pair
 
 pytestmark = pytest.mark.gpu
 
-OPTS = {"graph": 0, "mfma": 1, "fft": 2, "bands": 3, "dct": 4, "batch": 5}
+OPTS = {"graph": 0, "mfma": 1, "fft": 2, "bands": 3, "dct": 4, "batch": 5, "fuse": 6}
 
 
 def _pair(oracle, periodic):
@@ -32,7 +32,7 @@ def _pair(oracle, periodic):
 
 
 @pytest.mark.parametrize("periodic", [False, True])
-@pytest.mark.parametrize("off", [None, "graph", "mfma", "fft", "bands", "dct", "batch"])
+@pytest.mark.parametrize("off", [None, "graph", "mfma", "fft", "bands", "dct", "batch", "fuse"])
 def test_option_parity(oracle, periodic, off):
     print("case options periodic=%s off=%s" % (periodic, off), flush=True)
     g, o = _pair(oracle, periodic)
