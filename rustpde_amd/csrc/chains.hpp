@@ -17,6 +17,50 @@ __host__ __device__ __forceinline__ int pad(int j) { return j + (j >> 4); }
 // LDS doubles for a row of nr values with two guard zeros on each side
 inline int buf_len(int nr) { return pad(nr + 6) + 1; }
 
+// A row of nr values -> LDS (element i at pad(i + 2), zeros at i in [-2, 0) and
+// [nr, nr + 4)) by the nthr threads of a group, thread tid; all loads are issued before the
+// first store.  Needs MAXQ * nthr >= nr + 6.
+template <int MAXQ>
+__device__ __forceinline__ void stage_row(double* buf, const double* __restrict__ row, bool valid, int nr, int tid,
+                                          int nthr) {
+  double v[MAXQ];
+#pragma unroll
+  for (int q = 0; q < MAXQ; ++q) {
+    const int i = tid + q * nthr - 2;
+    v[q] = (valid && i >= 0 && i < nr) ? row[i] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < MAXQ; ++q) {
+    const int j = tid + q * nthr;
+    if (j < nr + 6) buf[pad(j)] = v[q];
+  }
+}
+
+// LDS row -> HBM: row[i] = x_i, or row[i] = x_i (-1) + row[i] with `subtract` (old values
+// loaded first).  Needs MAXQ * nthr >= m.
+template <int MAXQ>
+__device__ __forceinline__ void unstage_row(const double* buf, double* __restrict__ row, bool valid, int m, int tid,
+                                            int nthr, bool subtract) {
+#pragma clang fp contract(off)
+  if (!valid) return;
+  double old[MAXQ];
+  if (subtract) {
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+      const int i = tid + q * nthr;
+      old[q] = i < m ? row[i] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MAXQ; ++q) {
+    const int i = tid + q * nthr;
+    if (i < m) {
+      const double x = buf[pad(i + 2)];
+      row[i] = subtract ? x * (-1.0) + old[q] : x;
+    }
+  }
+}
+
 // Fdma forward elimination (fdma.rs:101-118): y_t = x_t - y_{t-1} lo_t (lo_t = low[i - 2])
 template <int SLICE>
 __device__ __forceinline__ void fdma_forward(double (&v)[SLICE], const double (&lo)[SLICE], int lane, int t0,

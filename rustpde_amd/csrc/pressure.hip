@@ -60,28 +60,39 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
   const double* __restrict__ sp0 = a.sp;
   const double* __restrict__ sp1 = a.sp + ny;
 
-  // (1) ux, uy rows -> LDS (element i at pad(i + 2), zero guards at i in [-2, 0), [mu, mu + 4))
-  for (int e = tid; e < 4 * (mu + 6); e += 256) {
-    const int row = e / (mu + 6), j = e - row * (mu + 6), i = j - 2;
-    const double* src = row < 2 ? a.UX : a.UY;
-    double v = 0.0;
-    if (i >= 0 && i < mu) v = src[ROWU(row & 1) + i];
-    lds[(row < 2 ? (row & 1) : 2 + (row & 1)) * bl + pad(j)] = v;
-  }
+  // rows of up to 2 SLICE WAVE + 1 values: unrolled staging bounds (all loads before stores)
+  constexpr int QW = (2 * SLICE * WAVE + 8 + WAVE - 1) / WAVE;  // per wavefront
+  constexpr int QH = (2 * SLICE * WAVE + 8 + 127) / 128;       // per half block
+  constexpr int QB = (2 * SLICE * WAVE + 8 + 255) / 256;       // per block
+  const int h = tid >> 7, th = tid & 127;                      // half block: plane h
+  // (1) ux, uy rows -> LDS (element i at pad(i + 2), zero guards at i in [-2, 0), [mu, mu + 4)):
+  //     wavefront w stages ux re, ux im, uy re, uy im
+  chains::stage_row<QW>(lds + w * bl, (w < 2 ? a.UX : a.UY) + ROWU(w & 1), true, mu, lane, WAVE);
   __syncthreads();
   // (2a) div = S_u(ik ux / sc) for i < ny (taps i, i - 2), zero guards around it
-  for (int j = tid; j < ny + 6; j += 256) {
-    const int i = j - 2;
-    double d0 = 0.0, d1 = 0.0;
-    if (i >= 0 && i < ny) {
-      double c0r, c0i, c2r, c2i;
-      fdiff(XA(0)[pad(j)], XA(1)[pad(j)], kk, a.kscale, c0r, c0i);
-      fdiff(XA(0)[pad(j - 2)], XA(1)[pad(j - 2)], kk, a.kscale, c2r, c2i);
-      d0 = c0r * su0[i] + c2r * su1[i];
-      d1 = c0i * su0[i] + c2i * su1[i];
+  {
+    double s0[QB], s1[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int i = tid + q * 256 - 2;
+      s0[q] = (i >= 0 && i < ny) ? su0[i] : 0.0;
+      s1[q] = (i >= 0 && i < ny) ? su1[i] : 0.0;
     }
-    DV(0)[pad(j)] = d0;
-    DV(1)[pad(j)] = d1;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const int j = tid + q * 256, i = j - 2;
+      if (j >= ny + 6) continue;
+      double d0 = 0.0, d1 = 0.0;
+      if (i >= 0 && i < ny) {
+        double c0r, c0i, c2r, c2i;
+        fdiff(XA(0)[pad(j)], XA(1)[pad(j)], kk, a.kscale, c0r, c0i);
+        fdiff(XA(0)[pad(j - 2)], XA(1)[pad(j - 2)], kk, a.kscale, c2r, c2i);
+        d0 = c0r * s0[q] + c2r * s1[q];
+        d1 = c0i * s0[q] + c2i * s1[q];
+      }
+      DV(0)[pad(j)] = d0;
+      DV(1)[pad(j)] = d1;
+    }
   }
   __syncthreads();
   // (2b) div += d/dy S_u(uy): g_i = 2 (i + 1) s_{i+1} summed from the top along the chains
@@ -167,15 +178,27 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
       if (q < cnt) YB(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
   }
   __syncthreads();
-  // (4a) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt
-  for (int e = tid; e < 2 * ny; e += 256) {
-    const int rr = e / ny, i = e - rr * ny;
-    const double* Pp = YB(rr);
-    if (i < mu) a.P1[ROWU(rr) + i] = Pp[pad(i + 2)];
-    const double w3 = Pp[pad(i + 2)] * sp0[i] + Pp[pad(i)] * sp1[i];
-    double* pr = a.P0 + ROWN(rr) + i;
-    const double q = *pr - DV(rr)[pad(i + 2)] * a.nu;
-    *pr = q + w3 * a.inv_dt;
+  // (4a) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt (half block h: plane h)
+  {
+    double pv[QH], c0[QH], c1[QH];
+    double* prow = a.P0 + ROWN(h);
+#pragma unroll
+    for (int q = 0; q < QH; ++q) {
+      const int i = th + q * 128;
+      pv[q] = i < ny ? prow[i] : 0.0;
+      c0[q] = i < ny ? sp0[i] : 0.0;
+      c1[q] = i < ny ? sp1[i] : 0.0;
+    }
+    const double* Pp = YB(h);
+#pragma unroll
+    for (int q = 0; q < QH; ++q) {
+      const int i = th + q * 128;
+      if (i >= ny) continue;
+      if (i < mu) a.P1[ROWU(h) + i] = Pp[pad(i + 2)];
+      const double w3 = Pp[pad(i + 2)] * c0[q] + Pp[pad(i)] * c1[q];
+      const double t = pv[q] - DV(h)[pad(i + 2)] * a.nu;
+      prow[i] = t + w3 * a.inv_dt;
+    }
   }
   // (4b) ux projection: d_i = g_i diag_i + g_{i+2} low2_i with g = S_p(ik pseu / sc), then
   //      the tdma recurrences of from_ortho
@@ -224,11 +247,7 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
   }
   __syncthreads();
   // (5a) ux = x (-1) + ux
-  for (int e = tid; e < 2 * mu; e += 256) {
-    const int rr = e / mu, i = e - rr * mu;
-    double* u = a.UX + ROWU(rr) + i;
-    *u = XA(rr)[pad(i + 2)] * (-1.0) + *u;
-  }
+  chains::unstage_row<QH>(XA(h), a.UX + ROWU(h), true, mu, th, 128, true);
   // (5b) d/dy S_p(pseu) into DV (the chains of (2b))
   {
     double v[SLICE];
@@ -296,11 +315,7 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
 #undef TB
   __syncthreads();
   // (7) uy = x (-1) + uy
-  for (int e = tid; e < 2 * mu; e += 256) {
-    const int rr = e / mu, i = e - rr * mu;
-    double* u = a.UY + ROWU(rr) + i;
-    *u = XA(rr)[pad(i + 2)] * (-1.0) + *u;
-  }
+  chains::unstage_row<QH>(XA(h), a.UY + ROWU(h), true, mu, th, 128, true);
 #undef XA
 #undef YB
 #undef DV

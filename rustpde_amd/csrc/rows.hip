@@ -68,10 +68,8 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double
     set = valid ? r % nsets : 0;
   }
   // (1) row -> LDS: element i at pad(i + 2); i in [-2, 0) and [nr, nr + 4) read as zero
-  for (int j = p * WAVE + lane; j < nr + 6; j += 2 * WAVE) {
-    const int i = j - 2;
-    buf[pad(j)] = (valid && i >= 0 && i < nr) ? R[(size_t)r * ldr + i] : 0.0;
-  }
+  constexpr int MAXQ = (2 * SLICE * WAVE + 8 + 2 * WAVE - 1) / (2 * WAVE);
+  chains::stage_row<MAXQ>(buf, R + (size_t)r * ldr, valid, nr, p * WAVE + lane, 2 * WAVE);
   __syncthreads();
   const int len = p == 0 ? (m + 1) / 2 : m / 2;
   const int t0 = lane * per;
@@ -128,8 +126,7 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double
   for (int q = 0; q < SLICE; ++q)
     if (q < cnt) buf[pad(p + 2 * (t0 + q) + 2)] = v[q];
   __syncthreads();
-  if (valid)
-    for (int i = p * WAVE + lane; i < m; i += 2 * WAVE) X[(size_t)r * ld + i] = buf[pad(i + 2)];
+  chains::unstage_row<MAXQ>(buf, X + (size_t)r * ld, valid, m, p * WAVE + lane, 2 * WAVE, false);
 }
 
 // from_ortho of a composite basis per row (composite_stencil.rs:250-276, linalg.rs:14-57):
@@ -151,10 +148,8 @@ __global__ __launch_bounds__(WAVE* WAVES) void tdma_rows_lds_kernel(const double
   double* buf = lds + slot * bl;
   const int r = blockIdx.x * (WAVES / 2) + slot, nr = m + 2;
   const bool valid = r < rows;
-  for (int j = p * WAVE + lane; j < nr + 6; j += 2 * WAVE) {
-    const int i = j - 2;
-    buf[pad(j)] = (valid && i >= 0 && i < nr) ? R[(size_t)r * ldr + i] : 0.0;
-  }
+  constexpr int MAXQ = (2 * SLICE * WAVE + 8 + 2 * WAVE - 1) / (2 * WAVE);
+  chains::stage_row<MAXQ>(buf, R + (size_t)r * ldr, valid, nr, p * WAVE + lane, 2 * WAVE);
   __syncthreads();
   const int len = p == 0 ? (m + 1) / 2 : m / 2;
   const int t0 = lane * per;
@@ -195,12 +190,7 @@ __global__ __launch_bounds__(WAVE* WAVES) void tdma_rows_lds_kernel(const double
   for (int q = 0; q < SLICE; ++q)
     if (q < cnt) buf[pad(p + 2 * (t0 + q) + 2)] = v[q];
   __syncthreads();
-  if (valid)
-    for (int i = p * WAVE + lane; i < m; i += 2 * WAVE) {
-      double* o = X + (size_t)r * ld + i;
-      const double x = buf[pad(i + 2)];
-      *o = project ? x * (-1.0) + *o : x;
-    }
+  chains::unstage_row<MAXQ>(buf, X + (size_t)r * ld, valid, m, p * WAVE + lane, 2 * WAVE, project != 0);
 }
 
 void check(const char* what) {
