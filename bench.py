@@ -141,6 +141,9 @@ def main():
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS) + ["c2"])
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device to device 3 times: the known-size copy that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -211,9 +214,44 @@ def main():
         "kernel_classes": {KERNEL_CLASSES[k]: {"launches": v[0], "ms": v[1]} for k, v in stats.items()},
         "roofline": roof,
     }
+    traffic = committed_traffic(args.config, roof["kernel"])
+    if traffic is not None:
+        roof["traffic"] = traffic["bytes_per_launch"]
+        roof["traffic_source"] = traffic["source"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_budget)
+    if args.copy_probe > 0:
+        copy_probe(L, args.copy_probe)
     print(json.dumps(out), flush=True)
+
+
+def committed_traffic(cfg, kernel):
+    """HBM bytes per launch of `kernel` on workload `cfg` from the committed counter summary
+    (profiles/hbm_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, calibrated on a copy)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    entry = table.get(cfg, {}).get(kernel)
+    if not entry:
+        return None
+    return {"bytes_per_launch": entry["bytes_per_launch"], "source": "profiles/hbm_traffic.json"}
+
+
+def copy_probe(L, mib):
+    """mib MiB copied device to device three times (hipMemcpy: one copy kernel per call)."""
+    import ctypes
+
+    from rustpde_amd import _lib
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(L.rbc_dev_malloc(ctypes.byref(a), n))
+    _lib.check(L.rbc_dev_malloc(ctypes.byref(b), n))
+    for _ in range(3):
+        _lib.check(L.rbc_memcpy_d2d(b, a, n))
+    _lib.check(L.rbc_dev_free(a))
+    _lib.check(L.rbc_dev_free(b))
 
 
 if __name__ == "__main__":

@@ -199,6 +199,7 @@ int rbc_dev_malloc(void** out, unsigned long long bytes);
 int rbc_dev_free(void* p);
 int rbc_memcpy_h2d(void* dst, const void* src, unsigned long long bytes);
 int rbc_memcpy_d2h(void* dst, const void* src, unsigned long long bytes);
+int rbc_memcpy_d2d(void* dst, const void* src, unsigned long long bytes);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,7 @@ SIGNATURES = {
     "rbc_dev_free": (_i, [_vp]),
     "rbc_memcpy_h2d": (_i, [_vp, _vp, ctypes.c_ulonglong]),
     "rbc_memcpy_d2h": (_i, [_vp, _vp, ctypes.c_ulonglong]),
+    "rbc_memcpy_d2d": (_i, [_vp, _vp, ctypes.c_ulonglong]),
 }
 
 _lib = None

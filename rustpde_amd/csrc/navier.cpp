@@ -1757,5 +1757,8 @@ int rbc_memcpy_h2d(void* dst, const void* src, unsigned long long bytes) {
 int rbc_memcpy_d2h(void* dst, const void* src, unsigned long long bytes) {
   return guarded([&] { HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
 }
+int rbc_memcpy_d2d(void* dst, const void* src, unsigned long long bytes) {
+  return guarded([&] { HIPCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
+}
 
 }  // extern "C"
