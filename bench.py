@@ -139,6 +139,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS) + ["c2"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
+                    help="mixed: convective terms in FP32, the rest FP64 (periodic configs); the FP64 step is "
+                         "timed beside it and the field / Nu differences after 10 steps are reported")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -159,19 +162,32 @@ def main():
     from rustpde_amd import _lib
     L = _lib.load()
     periodic, nx, ny = CONFIGS[args.config][:3]
+    mixed = args.precision == "mixed" and periodic  # the mixed step is the periodic batched step's
+
+    def timed_steps(nav):
+        nav.update(args.warmup)
+        nav.synchronize()
+        t0 = time.perf_counter()  # K production steps (the captured step replayed as a HIP graph)
+        nav.update(args.steps)
+        nav.synchronize()
+        el = time.perf_counter() - t0
+        if nav.exit():
+            print("bench.py: divergence is NaN after the timed steps", file=sys.stderr)
+            sys.exit(3)
+        return el
 
     nav = make(args.config, rustpde_amd)
-    nav.update(args.warmup)
-    nav.synchronize()
-
-    # timed region: K production steps (the captured step replayed as a HIP graph)
-    t0 = time.perf_counter()
-    nav.update(args.steps)
-    nav.synchronize()
-    elapsed = time.perf_counter() - t0
-    if nav.exit():
-        print("bench.py: divergence is NaN after the timed steps", file=sys.stderr)
-        sys.exit(3)
+    if mixed:
+        nav.set_option(nav.OPT_MIXED, 1)
+    elapsed = timed_steps(nav)
+    extra = {}
+    if mixed:
+        nav64 = make(args.config, rustpde_amd)
+        el64 = timed_steps(nav64)
+        del nav64
+        extra["fp64_steps_per_s"] = args.steps / el64
+        extra["mixed_speedup_vs_fp64"] = el64 / elapsed
+        extra["mixed_error_vs_fp64_after_10_steps"] = mixed_errors(args.config, rustpde_amd)
 
     # the same K steps launched one kernel at a time with HIP events recorded on the
     # handle's stream around every launch: per-kernel-class durations for the roofline
@@ -206,14 +222,15 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": WORKLOAD_NAMES[args.config], "nx": nx, "ny": ny, "precision": "fp64",
-                   "parallelism": "single-gpu"},
+        "vs_baseline": None, "dtype": "f32+f64 mixed" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": WORKLOAD_NAMES[args.config].replace("(FP64)", "(mixed FP32/FP64)" if mixed else "(FP64)"),
+                   "nx": nx, "ny": ny, "precision": "mixed" if mixed else "fp64", "parallelism": "single-gpu"},
         "grid_point_updates_per_s": value * nx * ny,
         "ms_per_step_with_kernel_events": 1e3 * elapsed_events / args.steps,
         "kernel_classes": {KERNEL_CLASSES[k]: {"launches": v[0], "ms": v[1]} for k, v in stats.items()},
         "roofline": roof,
     }
+    out.update(extra)
     traffic = committed_traffic(args.config, roof["kernel"])
     if traffic is not None:
         roof["traffic"] = traffic["bytes_per_launch"]
@@ -223,6 +240,26 @@ def main():
     if args.copy_probe > 0:
         copy_probe(L, args.copy_probe)
     print(json.dumps(out), flush=True)
+
+
+def mixed_errors(cfg, lib_mod):
+    """Relative max-norm differences of the mixed step from the FP64 step after 10 steps from
+    the same seeded fields (spectral coefficients), and of Nu."""
+    import numpy as np
+    a = make(cfg, lib_mod)
+    b = make(cfg, lib_mod)
+    b.set_option(b.OPT_MIXED, 1)
+    a.update(10)
+    b.update(10)
+    err = {}
+    for name in ("temp", "ux", "uy"):
+        x, y = getattr(a, name).vhat, getattr(b, name).vhat
+        err[name] = float(np.abs(x - y).max() / np.abs(x).max())
+    x, y = a.pres[0].vhat, b.pres[0].vhat
+    err["pres"] = float(np.abs(x - y).max() / np.abs(x).max())
+    nua, nub = a.eval_nu(), b.eval_nu()
+    err["nu"] = abs(nua - nub) / abs(nua)
+    return err
 
 
 def committed_traffic(cfg, kernel):

@@ -113,7 +113,10 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *   RBC_OPT_BATCH  periodic step: the transforms of ux, uy and temp and the ux/uy solves
  *                  run as batched launches (default 1; needs the FFT, DCT and band paths)
  *   RBC_OPT_FUSE   periodic batched step: divergence, Poisson rows, projection and
- *                  pressure update as one kernel per x-wavenumber (default 1) */
+ *                  pressure update as one kernel per x-wavenumber (default 1)
+ *   RBC_OPT_MIXED  mixed precision (periodic batched step): the convective terms (their
+ *                  x/y transforms and products) in FP32, everything else in FP64
+ *                  (default 0; fields agree with the FP64 step to ~1e-9 after 10 steps) */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2
@@ -121,6 +124,7 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
 #define RBC_OPT_DCT 4
 #define RBC_OPT_BATCH 5
 #define RBC_OPT_FUSE 6
+#define RBC_OPT_MIXED 7
 int rbc_set_option(rbc_navier* h, int option, int value);
 
 /* Kernel timing of a handle (no counterpart in the reference, whose timing protocol is

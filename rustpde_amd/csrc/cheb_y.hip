@@ -22,6 +22,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fft_lds.hpp"
@@ -80,35 +81,39 @@ struct Geo {
 // stencil value s_k = d_k c_k + l_k c_{k-2} of one row (composite_stencil.rs:207-229);
 // d = coef[0, n), l = coef[n, 2n) hold zeros where the stencil has no entry, which leaves
 // the sum of the present terms unchanged.  Without coef the row is Chebyshev already.
+template <typename V>
 struct StencilLoad {
-  double c0, c2, d, l;
+  V c0, c2, d, l;
 };
-__device__ __forceinline__ StencilLoad stencil_load(const double* __restrict__ row, bool valid, int k, int mi, int n,
-                                                    const double* __restrict__ coef) {
-  StencilLoad v{0.0, 0.0, 1.0, 0.0};
+template <typename V, typename Tin>
+__device__ __forceinline__ StencilLoad<V> stencil_load(const Tin* __restrict__ row, bool valid, int k, int mi, int n,
+                                                       const double* __restrict__ coef) {
+  StencilLoad<V> v{0, 0, 1, 0};
   if (k >= n) return v;
-  if (valid && k < mi) v.c0 = row[k];
+  if (valid && k < mi) v.c0 = (V)row[k];
   if (coef) {
-    v.d = coef[k];
-    v.l = coef[n + k];
-    if (valid && k >= 2 && k - 2 < mi) v.c2 = row[k - 2];
+    v.d = (V)coef[k];
+    v.l = (V)coef[n + k];
+    if (valid && k >= 2 && k - 2 < mi) v.c2 = (V)row[k - 2];
   }
   return v;
 }
-__device__ __forceinline__ double stencil_value(const StencilLoad& v, bool composite) {
+template <typename V>
+__device__ __forceinline__ V stencil_value(const StencilLoad<V>& v, bool composite) {
 #pragma clang fp contract(off)
   return composite ? v.d * v.c0 + v.l * v.c2 : v.c0;
 }
 
-template <int L, bool transform>
-__device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, int ldi, int mi,
-                                                double* __restrict__ out, int ldo, int rows,
-                                                const double* __restrict__ coef, int times, double inv_sc,
-                                                const double* __restrict__ U, int ldu, double beta,
+// Tin: input type, V: arithmetic (and LDS) type, Tout: type of out and U
+template <int L, bool transform, typename Tin, typename V, typename Tout>
+__device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out,
+                                                int ldo, int rows, const double* __restrict__ coef, int times,
+                                                double inv_sc_d, const Tout* __restrict__ U, int ldu, double beta_d,
                                                 const double* __restrict__ tw, long si, long so, long su) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
-  __shared__ double re[G::E], im[G::E];
+  __shared__ V re[G::E], im[G::E];
+  const V inv_sc = (V)inv_sc_d, beta = (V)beta_d;
   in += si * blockIdx.y;
   out += so * blockIdx.y;
   if (U) U += su * blockIdx.y;
@@ -117,15 +122,15 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
   //     g_0 = s_1, g_k = 2 (k + 1) s_{k+1} (1 <= k <= n - 2), g_{n-1} = 0.
   //     All loads of a thread are issued before any is used.
   constexpr int Q = (P * n + T - 1) / T;
-  StencilLoad la[Q], lb[Q];
+  StencilLoad<V> la[Q], lb[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int e = tid + q * T;
     const int p = e / n, k = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     const int kk = (e < P * n) ? (times ? k + 1 : k) : n;  // kk = n loads nothing
-    la[q] = stencil_load(in + (size_t)ra * ldi, ra < rows, kk, mi, n, coef);
-    lb[q] = stencil_load(in + (size_t)rb * ldi, rb < rows, kk, mi, n, coef);
+    la[q] = stencil_load<V>(in + (size_t)ra * ldi, ra < rows, kk, mi, n, coef);
+    lb[q] = stencil_load<V>(in + (size_t)rb * ldi, rb < rows, kk, mi, n, coef);
   }
   const bool comp = coef != nullptr;
 #pragma unroll
@@ -133,13 +138,13 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
     const int e = tid + q * T;
     if (e >= P * n) continue;
     const int p = e / n, k = e - p * n;
-    double sa = stencil_value(la[q], comp), sb = stencil_value(lb[q], comp);
+    V sa = stencil_value(la[q], comp), sb = stencil_value(lb[q], comp);
     if (times) {
       if (k == n - 1) {
-        sa = 0.0;
-        sb = 0.0;
+        sa = 0;
+        sb = 0;
       } else if (k > 0) {
-        const double f = 2.0 * (double)(k + 1);
+        const V f = (V)(2.0 * (double)(k + 1));
         sa = f * sa;
         sb = f * sb;
       }
@@ -155,20 +160,20 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
     const int lane = tid & 63, wave = tid >> 6;
     for (int ch = wave; ch < NCH; ch += T / 64) {
       const int p = ch >> 2, comp = (ch >> 1) & 1, par = ch & 1;
-      double* z = (comp ? im : re) + p * L + (par ? 1 : 2);
+      V* z = (comp ? im : re) + p * L + (par ? 1 : 2);
       const int t0 = lane * SEG;
-      double tot = 0.0;
+      V tot = 0;
 #pragma unroll
       for (int q = SEG - 1; q >= 0; --q)
         if (t0 + q < C) tot = z[2 * (t0 + q)] + tot;
-      double inc = tot;  // inclusive suffix sum over lanes
+      V inc = tot;  // inclusive suffix sum over lanes
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
-        const double o = __shfl_down(inc, off);
+        const V o = __shfl_down(inc, off);
         if (lane + off < 64) inc = inc + o;
       }
-      double acc = __shfl_down(inc, 1);
-      if (lane == 63) acc = 0.0;
+      V acc = __shfl_down(inc, 1);
+      if (lane == 63) acc = 0;
 #pragma unroll
       for (int q = SEG - 1; q >= 0; --q) {
         const int t = t0 + q;
@@ -178,8 +183,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
         }
       }
       if (par == 0 && lane == 0) {  // acc = d_2
-        double* z0 = (comp ? im : re) + p * L;
-        z0[0] = z0[0] + acc / 2.0;
+        V* z0 = (comp ? im : re) + p * L;
+        z0[0] = z0[0] + acc / (V)2;
       }
     }
     __syncthreads();
@@ -187,12 +192,12 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
   // (3) u_k = (-1)^k w_k c_k / sc^times and the even extension u_{L-k} = u_k
   for (int e = tid; e < P * n && transform; e += T) {
     const int p = e / n, k = e - p * n;
-    double a = re[p * L + k], b = im[p * L + k];
-    if (times && inv_sc != 1.0) {
+    V a = re[p * L + k], b = im[p * L + k];
+    if (times && inv_sc != (V)1) {
       a = a * inv_sc;
       b = b * inv_sc;
     }
-    const double w = ((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5);
+    const V w = (V)(((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5));
     a = a * w;
     b = b * w;
     re[p * L + k] = a;
@@ -208,17 +213,17 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
   }
   // (4) x_j = Re / Im Z_j, j <= N, or the coefficients d_j / sc without the transform;
   //     out = x .* U + beta out (loads first)
-  double ua[Q], ub[Q], oa[Q], ob[Q];
+  V ua[Q], ub[Q], oa[Q], ob[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int e = tid + q * T;
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     const bool ok = e < P * n;
-    ua[q] = (U && ok && ra < rows) ? U[(size_t)ra * ldu + j] : 1.0;
-    ub[q] = (U && ok && rb < rows) ? U[(size_t)rb * ldu + j] : 1.0;
-    oa[q] = (beta != 0.0 && ok && ra < rows) ? out[(size_t)ra * ldo + j] : 0.0;
-    ob[q] = (beta != 0.0 && ok && rb < rows) ? out[(size_t)rb * ldo + j] : 0.0;
+    ua[q] = (U && ok && ra < rows) ? (V)U[(size_t)ra * ldu + j] : (V)1;
+    ub[q] = (U && ok && rb < rows) ? (V)U[(size_t)rb * ldu + j] : (V)1;
+    oa[q] = (beta != (V)0 && ok && ra < rows) ? (V)out[(size_t)ra * ldo + j] : (V)0;
+    ob[q] = (beta != (V)0 && ok && rb < rows) ? (V)out[(size_t)rb * ldo + j] : (V)0;
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -226,8 +231,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
     if (e >= P * n) continue;
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
-    double va = re[p * L + j], vb = im[p * L + j];
-    if (!transform && inv_sc != 1.0) {
+    V va = re[p * L + j], vb = im[p * L + j];
+    if (!transform && inv_sc != (V)1) {
       va = va * inv_sc;
       vb = vb * inv_sc;
     }
@@ -235,21 +240,22 @@ __device__ __forceinline__ void cheb_y_bwd_body(const double* __restrict__ in, i
       va = va * ua[q];
       vb = vb * ub[q];
     }
-    if (beta != 0.0) {
+    if (beta != (V)0) {
       va = va + beta * oa[q];
       vb = vb + beta * ob[q];
     }
-    if (ra < rows) out[(size_t)ra * ldo + j] = va;
-    if (rb < rows) out[(size_t)rb * ldo + j] = vb;
+    if (ra < rows) out[(size_t)ra * ldo + j] = (Tout)va;
+    if (rb < rows) out[(size_t)rb * ldo + j] = (Tout)vb;
   }
 }
 
-template <int L>
+template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
-    const double* __restrict__ in, int ldi, int mi, double* __restrict__ out, int ldo, int rows,
-    const double* __restrict__ coef, int times, double inv_sc, const double* __restrict__ U, int ldu, double beta,
+    const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out, int ldo, int rows,
+    const double* __restrict__ coef, int times, double inv_sc, const Tout* __restrict__ U, int ldu, double beta,
     const double* __restrict__ tw, long si, long so, long su) {
-  cheb_y_bwd_body<L, true>(in, ldi, mi, out, ldo, rows, coef, times, inv_sc, U, ldu, beta, tw, si, so, su);
+  cheb_y_bwd_body<L, true, Tin, V, Tout>(in, ldi, mi, out, ldo, rows, coef, times, inv_sc, U, ldu, beta, tw, si, so,
+                                         su);
 }
 // the same without the transform: Chebyshev coefficients of the derivative
 template <int L>
@@ -257,36 +263,37 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_diff_kernel(const double* __
                                                                 double* __restrict__ out, int ldo, int rows,
                                                                 const double* __restrict__ coef, double inv_sc,
                                                                 double beta, long si, long so) {
-  cheb_y_bwd_body<L, false>(in, ldi, mi, out, ldo, rows, coef, 1, inv_sc, nullptr, 0, beta, nullptr, si, so, 0);
+  cheb_y_bwd_body<L, false, double, double, double>(in, ldi, mi, out, ldo, rows, coef, 1, inv_sc, nullptr, 0, beta,
+                                                     nullptr, si, so, 0);
 }
 
-template <int L>
-__global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const double* __restrict__ in, int ldi,
-                                                               double* __restrict__ out, int ldo, int rows, int cut,
+template <int L, typename Tin, typename V, typename Tout>
+__global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __restrict__ in, int ldi,
+                                                               Tout* __restrict__ out, int ldo, int rows, int cut,
                                                                const double* __restrict__ tw, long si, long so) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
-  __shared__ double re[G::E], im[G::E];
+  __shared__ V re[G::E], im[G::E];
   in += si * blockIdx.y;
   out += so * blockIdx.y;
   const int tid = threadIdx.x, row0 = 2 * P * blockIdx.x;
   constexpr int Q = (P * n + T - 1) / T;
-  double va[Q], vb[Q];
+  V va[Q], vb[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int e = tid + q * T;
     const int p = e / n, k = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     const bool ok = e < P * n;
-    va[q] = (ok && ra < rows) ? in[(size_t)ra * ldi + k] : 0.0;
-    vb[q] = (ok && rb < rows) ? in[(size_t)rb * ldi + k] : 0.0;
+    va[q] = (ok && ra < rows) ? (V)in[(size_t)ra * ldi + k] : (V)0;
+    vb[q] = (ok && rb < rows) ? (V)in[(size_t)rb * ldi + k] : (V)0;
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int e = tid + q * T;
     if (e >= P * n) continue;
     const int p = e / n, k = e - p * n;
-    const double a = va[q], b = vb[q];
+    const V a = va[q], b = vb[q];
     re[p * L + k] = a;
     im[p * L + k] = b;
     if (k > 0 && k < N) {
@@ -300,39 +307,43 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const double* __r
   for (int e = tid; e < P * n; e += T) {
     const int p = e / n, k = e - p * n;
     const int ra = row0 + 2 * p;
-    const double s = ((k & 1) ? -invN : invN) * ((k == 0 || k == N) ? 0.5 : 1.0);
+    const V s = (V)(((k & 1) ? -invN : invN) * ((k == 0 || k == N) ? 0.5 : 1.0));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int r = ra + h;
       if (r >= rows) continue;
-      const double v = h ? im[p * L + k] : re[p * L + k];
-      out[(size_t)r * ldo + k] = k < cut ? v * s : 0.0;
+      const V v = h ? im[p * L + k] : re[p * L + k];
+      out[(size_t)r * ldo + k] = (Tout)(k < cut ? v * s : (V)0);
     }
   }
 }
 
-template <int L>
-void launch_bwd(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, const double* coef,
-                const unsigned char* mask, int times, double inv_sc, const double* U, int ldu, double beta, int batch,
+template <int L, typename Tin, typename V, typename Tout>
+void launch_bwd(hipStream_t s, const Tin* in, int ldi, int mi, Tout* out, int ldo, int rows, const double* coef,
+                const unsigned char* mask, int times, double inv_sc, const Tout* U, int ldu, double beta, int batch,
                 long si, long so, long su, bool transform) {
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
   (void)mask;  // the zero coefficients of absent taps stand for the mask
-  if (transform)
-    RBC_LAUNCH(cheb_y_bwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
-                       times, inv_sc, U, ldu, beta, fft_table(L), si, so, su);
-  else
-    RBC_LAUNCH(cheb_y_diff_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
-                       inv_sc, beta, si, so);
+  if (transform) {
+    RBC_LAUNCH((cheb_y_bwd_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo,
+               rows, coef, times, inv_sc, U, ldu, beta, fft_table(L), si, so, su);
+  } else {
+    if constexpr (std::is_same<Tin, double>::value && std::is_same<Tout, double>::value)
+      RBC_LAUNCH(cheb_y_diff_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
+                 inv_sc, beta, si, so);
+    else
+      throw std::invalid_argument("cheb_y_diff: FP64 only");
+  }
 }
 
-template <int L>
-void launch_fwd(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int cut, int batch, long si,
+template <int L, typename Tin, typename V, typename Tout>
+void launch_fwd(hipStream_t s, const Tin* in, int ldi, Tout* out, int ldo, int rows, int cut, int batch, long si,
                 long so) {
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
-  RBC_LAUNCH(cheb_y_fwd_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, out, ldo, rows, cut,
-                     fft_table(L), si, so);
+  RBC_LAUNCH((cheb_y_fwd_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, out, ldo, rows, cut,
+             fft_table(L), si, so);
 }
 
 }  // namespace
@@ -362,8 +373,9 @@ void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* o
   if (times < 0 || times > 1) throw std::invalid_argument("cheb_y_backward: times must be 0 or 1");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_backward: bad batch or rows");
   if (mi > n || (!coef && mi < 1)) throw std::invalid_argument("cheb_y_backward: bad input length");
-#define RBC_BWD(LL) \
-  launch_bwd<LL>(s, in, ldi, mi, out, ldo, rows, coef, mask, times, inv_sc, U, ldu, beta, batch, si, so, su, true)
+#define RBC_BWD(LL)                                                                                          \
+  launch_bwd<LL, double, double, double>(s, in, ldi, mi, out, ldo, rows, coef, mask, times, inv_sc, U, ldu, beta, \
+                                         batch, si, so, su, true)
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_BWD)
 #undef RBC_BWD
   check("cheb_y_backward");
@@ -374,8 +386,9 @@ void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, 
   if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_diff: unsupported length");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_diff: bad batch or rows");
   if (mi > n || (!coef && mi < 1)) throw std::invalid_argument("cheb_y_diff: bad input length");
-#define RBC_DIFF(LL) \
-  launch_bwd<LL>(s, in, ldi, mi, out, ldo, rows, coef, nullptr, 1, inv_sc, nullptr, 0, beta, batch, si, so, 0, false)
+#define RBC_DIFF(LL)                                                                                             \
+  launch_bwd<LL, double, double, double>(s, in, ldi, mi, out, ldo, rows, coef, nullptr, 1, inv_sc, nullptr, 0, beta, \
+                                         batch, si, so, 0, false)
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_DIFF)
 #undef RBC_DIFF
   check("cheb_y_diff");
@@ -385,10 +398,35 @@ void cheb_y_forward(hipStream_t s, const double* in, int ldi, double* out, int l
                     long si, long so) {
   if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_forward: unsupported length");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_forward: bad batch or rows");
-#define RBC_FWD(LL) launch_fwd<LL>(s, in, ldi, out, ldo, rows, cut, batch, si, so)
+#define RBC_FWD(LL) launch_fwd<LL, double, double, double>(s, in, ldi, out, ldo, rows, cut, batch, si, so)
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_FWD)
 #undef RBC_FWD
   check("cheb_y_forward");
+}
+
+void cheb_y_backward_f32(hipStream_t s, const float* in, int ldi, int mi, float* out, int ldo, int rows, int n,
+                         const double* coef, int times, double inv_sc, const float* U, int ldu, double beta, int batch,
+                         long si, long so, long su) {
+  if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_backward_f32: unsupported length");
+  if (times < 0 || times > 1) throw std::invalid_argument("cheb_y_backward_f32: times must be 0 or 1");
+  if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_backward_f32: bad batch or rows");
+  if (mi > n || (!coef && mi < 1)) throw std::invalid_argument("cheb_y_backward_f32: bad input length");
+#define RBC_BWD32(LL)                                                                                              \
+  launch_bwd<LL, float, float, float>(s, in, ldi, mi, out, ldo, rows, coef, nullptr, times, inv_sc, U, ldu, beta, \
+                                      batch, si, so, su, true)
+  RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_BWD32)
+#undef RBC_BWD32
+  check("cheb_y_backward_f32");
+}
+
+void cheb_y_forward_f32(hipStream_t s, const float* in, int ldi, float* out, int ldo, int rows, int n, int cut,
+                        int batch, long si, long so) {
+  if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_forward_f32: unsupported length");
+  if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_forward_f32: bad batch or rows");
+#define RBC_FWD32(LL) launch_fwd<LL, float, float, float>(s, in, ldi, out, ldo, rows, cut, batch, si, so)
+  RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_FWD32)
+#undef RBC_FWD32
+  check("cheb_y_forward_f32");
 }
 
 }  // namespace rbc

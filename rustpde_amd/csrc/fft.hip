@@ -40,24 +40,25 @@ struct CGeo {
   static constexpr int m = N + 1;  // half-spectrum length
 };
 
-template <int L>
-__global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
+// Tin: input type, V: arithmetic (and LDS) type, Tout: output type
+template <int L, typename Tin, typename V, typename Tout>
+__global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
                                                                int cols, int cut, const double* __restrict__ tw,
                                                                long sx, long sy) {
   using G = CGeo<L>;
   constexpr int C = G::C, T = G::T, m = G::m;
-  __shared__ double re[G::E], im[G::E];
+  __shared__ V re[G::E], im[G::E];
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
   const int tid = threadIdx.x;
   const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
   // load: row t, column c -> z_t of pair c / 2 (real part for even c, imaginary for odd)
   constexpr int QL = L * C / T;
-  double v[QL];
+  V v[QL];
 #pragma unroll
   for (int q = 0; q < QL; ++q) {
     const int e = tid + q * T, c = e % C, t = e / C, j = j0 + c;
-    v[q] = j < cols ? X[(size_t)t * cols + j] : 0.0;
+    v[q] = j < cols ? (V)X[(size_t)t * cols + j] : (V)0;
   }
 #pragma unroll
   for (int q = 0; q < QL; ++q) {
@@ -71,32 +72,34 @@ __global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const double* __r
     const int c = e % C, k = e / C, j = j0 + c;
     if (j >= cols) continue;
     const int p = (c >> 1) * L, k2 = (L - k) & (L - 1);
-    const double zr = re[p + k], zi = im[p + k], yr = re[p + k2], yi = im[p + k2];
-    double xr, xi;
+    const V zr = re[p + k], zi = im[p + k], yr = re[p + k2], yi = im[p + k2];
+    const V half = (V)0.5;
+    V xr, xi;
     if (c & 1) {
-      xr = 0.5 * (zi + yi);
-      xi = 0.5 * (yr - zr);
+      xr = half * (zi + yi);
+      xi = half * (yr - zr);
     } else {
-      xr = 0.5 * (zr + yr);
-      xi = 0.5 * (zi - yi);
+      xr = half * (zr + yr);
+      xi = half * (zi - yi);
     }
     if (k >= cut) {
-      xr = 0.0;
-      xi = 0.0;
+      xr = 0;
+      xi = 0;
     }
-    Y[(size_t)k * cols + j] = xr;
-    Y[(size_t)(m + k) * cols + j] = xi;
+    Y[(size_t)k * cols + j] = (Tout)xr;
+    Y[(size_t)(m + k) * cols + j] = (Tout)xi;
   }
 }
 
-template <int L>
-__global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const double* __restrict__ X, double* __restrict__ Y,
-                                                                int cols, int times, double sc,
+template <int L, typename Tin, typename V, typename Tout>
+__global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
+                                                                int cols, int times, double sc_d,
                                                                 const double* __restrict__ tw, long sx, long sy) {
 #pragma clang fp contract(off)
   using G = CGeo<L>;
   constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
-  __shared__ double re[G::E], im[G::E];
+  __shared__ V re[G::E], im[G::E];
+  const V sc = (V)sc_d;
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
   const int tid = threadIdx.x;
@@ -104,43 +107,43 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const double* __
   // (1) half spectra, (i k)^times / sc applied, packed per pair: re[k] = Re A_k (k <= N),
   //     re[N + k] = Im A_k (0 < k < N), the same for B in im
   constexpr int QS = (m * C + T - 1) / T;
-  double va[QS], vb[QS];
+  V va[QS], vb[QS];
 #pragma unroll
   for (int q = 0; q < QS; ++q) {
     const int e = tid + q * T, c = e % C, k = e / C, j = j0 + c;
     const bool ok = e < m * C && j < cols;
-    va[q] = ok ? X[(size_t)k * cols + j] : 0.0;
-    vb[q] = ok ? X[(size_t)(m + k) * cols + j] : 0.0;
+    va[q] = ok ? (V)X[(size_t)k * cols + j] : (V)0;
+    vb[q] = ok ? (V)X[(size_t)(m + k) * cols + j] : (V)0;
   }
 #pragma unroll
   for (int q = 0; q < QS; ++q) {
     const int e = tid + q * T, c = e % C, k = e / C;
     if (e >= m * C) continue;
-    double a = va[q], b = vb[q];
-    const double kk = (double)k;
+    V a = va[q], b = vb[q];
+    const V kk = (V)k, zero = 0;
     for (int r = 0; r < times; ++r) {
-      const double na = a * 0.0 - b * kk, nb = a * kk + b * 0.0;
+      const V na = a * zero - b * kk, nb = a * kk + b * zero;
       a = na;
       b = nb;
     }
-    if (sc != 1.0) {
-      const double ns = sc * sc + 0.0 * 0.0;
-      const double na = (a * sc + b * 0.0) / ns, nb = (b * sc - a * 0.0) / ns;
+    if (sc != (V)1) {
+      const V ns = sc * sc + zero * zero;
+      const V na = (a * sc + b * zero) / ns, nb = (b * sc - a * zero) / ns;
       a = na;
       b = nb;
     }
-    double* z = ((c & 1) ? im : re) + (c >> 1) * L;
+    V* z = ((c & 1) ? im : re) + (c >> 1) * L;
     z[k] = a;
     if (k > 0 && k < N) z[N + k] = b;
   }
   __syncthreads();
   // (2) conj Z_k = (Ar - Bi) - i (Ai + Br),  conj Z_{L-k} = (Ar + Bi) + i (Ai - Br)
   constexpr int QZ = (m * G::PP + T - 1) / T;
-  double ar[QZ], ai[QZ], br[QZ], bi[QZ];
+  V ar[QZ], ai[QZ], br[QZ], bi[QZ];
 #pragma unroll
   for (int q = 0; q < QZ; ++q) {
     const int e = tid + q * T, p = e / m, k = e - p * m;
-    ar[q] = ai[q] = br[q] = bi[q] = 0.0;
+    ar[q] = ai[q] = br[q] = bi[q] = 0;
     if (e < m * G::PP) {
       ar[q] = re[p * L + k];
       br[q] = im[p * L + k];
@@ -165,12 +168,12 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const double* __
   __syncthreads();
   lds_dft<L, T, G::E>(re, im, tw);
   // (3) a_j = Re W_j / L, b_j = -Im W_j / L with W = DFT(conj Z)
-  constexpr double inv = 1.0 / (double)L;
+  const V inv = (V)(1.0 / (double)L);
   for (int e = tid; e < L * C; e += T) {
     const int c = e % C, t = e / C, j = j0 + c;
     if (j >= cols) continue;
     const int p = (c >> 1) * L;
-    Y[(size_t)t * cols + j] = (c & 1) ? -im[p + t] * inv : re[p + t] * inv;
+    Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -im[p + t] * inv : re[p + t] * inv);
   }
 }
 
@@ -184,20 +187,19 @@ void check_n(int n) {
   if (n > 4096) throw std::invalid_argument("fft: length above 4096 exceeds the LDS budget");
 }
 
-template <int L>
-void launch_rfft(hipStream_t s, const double* X, double* Y, int cols, int cut, int batch, long sx, long sy) {
+template <int L, typename Tin, typename V, typename Tout>
+void launch_rfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int cut, int batch, long sx, long sy) {
   using G = CGeo<L>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
-  RBC_LAUNCH(rfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut, fft_table(L), sx,
-                     sy);
+  RBC_LAUNCH((rfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut,
+             fft_table(L), sx, sy);
 }
-template <int L>
-void launch_irfft(hipStream_t s, const double* X, double* Y, int cols, int times, double sc, int batch, long sx,
-                  long sy) {
+template <int L, typename Tin, typename V, typename Tout>
+void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, double sc, int batch, long sx, long sy) {
   using G = CGeo<L>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
-  RBC_LAUNCH(irfft_cols_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
-                     fft_table(L), sx, sy);
+  RBC_LAUNCH((irfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
+             fft_table(L), sx, sy);
 }
 
 }  // namespace
@@ -223,7 +225,7 @@ void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int c
   check_n(n);
   if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("rfft_cols: bad batch or cols");
   cut = std::min(cut, n / 2 + 1);
-#define RBC_RFFT(LL) launch_rfft<LL>(s, X, Y, cols, cut, batch, sx, sy)
+#define RBC_RFFT(LL) launch_rfft<LL, double, double, double>(s, X, Y, cols, cut, batch, sx, sy)
   RBC_FFT_SWITCH(n, RBC_RFFT)
 #undef RBC_RFFT
   check("rfft_cols");
@@ -233,10 +235,30 @@ void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int 
                 long sy) {
   check_n(n);
   if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("irfft_cols: bad batch or cols");
-#define RBC_IRFFT(LL) launch_irfft<LL>(s, X, Y, cols, times, sc, batch, sx, sy)
+#define RBC_IRFFT(LL) launch_irfft<LL, double, double, double>(s, X, Y, cols, times, sc, batch, sx, sy)
   RBC_FFT_SWITCH(n, RBC_IRFFT)
 #undef RBC_IRFFT
   check("irfft_cols");
+}
+
+void rfft_cols_f32(hipStream_t s, const float* X, double* Y, int n, int cols, int cut, int batch, long sx, long sy) {
+  check_n(n);
+  if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("rfft_cols_f32: bad batch or cols");
+  cut = std::min(cut, n / 2 + 1);
+#define RBC_RFFT32(LL) launch_rfft<LL, float, float, double>(s, X, Y, cols, cut, batch, sx, sy)
+  RBC_FFT_SWITCH(n, RBC_RFFT32)
+#undef RBC_RFFT32
+  check("rfft_cols_f32");
+}
+
+void irfft_cols_f32(hipStream_t s, const double* X, float* Y, int n, int cols, int times, double sc, int batch,
+                    long sx, long sy) {
+  check_n(n);
+  if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("irfft_cols_f32: bad batch or cols");
+#define RBC_IRFFT32(LL) launch_irfft<LL, double, float, float>(s, X, Y, cols, times, sc, batch, sx, sy)
+  RBC_FFT_SWITCH(n, RBC_IRFFT32)
+#undef RBC_IRFFT32
+  check("irfft_cols_f32");
 }
 
 }  // namespace rbc

@@ -1,4 +1,5 @@
-// In-LDS complex DFTs for the FFT-based kernels (FP64): radix-8 Stockham passes plus one
+// In-LDS complex DFTs for the FFT-based kernels (FP64, or FP32 in the mixed-precision
+// step; T is the arithmetic type): radix-8 Stockham passes plus one
 // radix-4 or radix-2 pass, natural order in and out, forward sign (e^{-2 pi i jk/L}).
 // A block holds E / L transforms of length L at stride L in two arrays (re, im); each of
 // its T threads handles E / T = 8 values per pass.  Twiddles come from a table of
@@ -9,21 +10,22 @@
 namespace rbc {
 namespace fftlds {
 
-constexpr double SQH = 0.70710678118654752440;  // sqrt(1/2)
+constexpr double SQH = 0.70710678118654752440;  // sqrt(1/2), rounded to T where used
 
 // forward DFTs of 2, 4, 8 points (sign -1), in place, natural order
-__device__ __forceinline__ void dft2(double* r, double* i) {
-  const double ar = r[0], ai = i[0];
+template <typename T>
+__device__ __forceinline__ void dft2(T* r, T* i) {
+  const T ar = r[0], ai = i[0];
   r[0] = ar + r[1];
   i[0] = ai + i[1];
   r[1] = ar - r[1];
   i[1] = ai - i[1];
 }
-__device__ __forceinline__ void dft4(double& r0, double& i0, double& r1, double& i1, double& r2, double& i2, double& r3,
-                                     double& i3) {
-  const double t0r = r0 + r2, t0i = i0 + i2, t1r = r0 - r2, t1i = i0 - i2;
-  const double t2r = r1 + r3, t2i = i1 + i3;
-  const double t3r = i1 - i3, t3i = r3 - r1;  // (x1 - x3) (-i)
+template <typename T>
+__device__ __forceinline__ void dft4(T& r0, T& i0, T& r1, T& i1, T& r2, T& i2, T& r3, T& i3) {
+  const T t0r = r0 + r2, t0i = i0 + i2, t1r = r0 - r2, t1i = i0 - i2;
+  const T t2r = r1 + r3, t2i = i1 + i3;
+  const T t3r = i1 - i3, t3i = r3 - r1;  // (x1 - x3) (-i)
   r0 = t0r + t2r;
   i0 = t0i + t2i;
   r2 = t0r - t2r;
@@ -33,15 +35,17 @@ __device__ __forceinline__ void dft4(double& r0, double& i0, double& r1, double&
   r3 = t1r - t3r;
   i3 = t1i - t3i;
 }
-__device__ __forceinline__ void dft8(double* r, double* i) {
+template <typename T>
+__device__ __forceinline__ void dft8(T* r, T* i) {
+  const T h = (T)SQH;
   dft4(r[0], i[0], r[2], i[2], r[4], i[4], r[6], i[6]);  // E_k in slots 0, 2, 4, 6
   dft4(r[1], i[1], r[3], i[3], r[5], i[5], r[7], i[7]);  // O_k in slots 1, 3, 5, 7
   // O_1 *= e^{-i pi/4}, O_2 *= -i, O_3 *= e^{-3 i pi/4}
-  double o1r = SQH * (r[3] + i[3]), o1i = SQH * (i[3] - r[3]);
-  double o2r = i[5], o2i = -r[5];
-  double o3r = SQH * (i[7] - r[7]), o3i = -SQH * (r[7] + i[7]);
-  const double e0r = r[0], e0i = i[0], e1r = r[2], e1i = i[2], e2r = r[4], e2i = i[4], e3r = r[6], e3i = i[6];
-  const double o0r = r[1], o0i = i[1];
+  T o1r = h * (r[3] + i[3]), o1i = h * (i[3] - r[3]);
+  T o2r = i[5], o2i = -r[5];
+  T o3r = h * (i[7] - r[7]), o3i = -h * (r[7] + i[7]);
+  const T e0r = r[0], e0i = i[0], e1r = r[2], e1i = i[2], e2r = r[4], e2i = i[4], e3r = r[6], e3i = i[6];
+  const T o0r = r[1], o0i = i[1];
   r[0] = e0r + o0r; i[0] = e0i + o0i;
   r[4] = e0r - o0r; i[4] = e0i - o0i;
   r[1] = e1r + o1r; i[1] = e1i + o1i;
@@ -53,12 +57,12 @@ __device__ __forceinline__ void dft8(double* r, double* i) {
 }
 
 // one Stockham pass of radix R over the P transforms of length L held in (re, im)
-template <int L, int R, int T, int E>
-__device__ __forceinline__ void stockham_pass(double* re, double* im, int Ns, const double* __restrict__ tw) {
+template <int L, int R, int T, int E, typename V>
+__device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double* __restrict__ tw) {
   static_assert(E % (R * T) == 0 && E % L == 0, "block geometry");
   constexpr int NB = L / R;          // butterflies per transform
   constexpr int PER = E / (R * T);   // butterflies per thread
-  double xr[PER][R], xi[PER][R];
+  V xr[PER][R], xi[PER][R];
   int dst[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -74,23 +78,23 @@ __device__ __forceinline__ void stockham_pass(double* re, double* im, int Ns, co
     // twiddles e^{-2 pi i r k / (Ns R)} = table[r k L / (Ns R)]: powers 1, 2, 4 from the
     // table, the others as products of two of them
     const int t1 = k * (L / (Ns * R));
-    double wr[R], wi[R];
-    wr[0] = 1.0;
-    wi[0] = 0.0;
-    if constexpr (R >= 2) { wr[1] = tw[2 * t1]; wi[1] = tw[2 * t1 + 1]; }
+    V wr[R], wi[R];
+    wr[0] = 1;
+    wi[0] = 0;
+    if constexpr (R >= 2) { wr[1] = (V)tw[2 * t1]; wi[1] = (V)tw[2 * t1 + 1]; }
     if constexpr (R >= 4) {
-      wr[2] = tw[4 * t1]; wi[2] = tw[4 * t1 + 1];
+      wr[2] = (V)tw[4 * t1]; wi[2] = (V)tw[4 * t1 + 1];
       wr[3] = wr[1] * wr[2] - wi[1] * wi[2]; wi[3] = wr[1] * wi[2] + wi[1] * wr[2];
     }
     if constexpr (R == 8) {
-      wr[4] = tw[8 * t1]; wi[4] = tw[8 * t1 + 1];
+      wr[4] = (V)tw[8 * t1]; wi[4] = (V)tw[8 * t1 + 1];
       wr[5] = wr[1] * wr[4] - wi[1] * wi[4]; wi[5] = wr[1] * wi[4] + wi[1] * wr[4];
       wr[6] = wr[2] * wr[4] - wi[2] * wi[4]; wi[6] = wr[2] * wi[4] + wi[2] * wr[4];
       wr[7] = wr[3] * wr[4] - wi[3] * wi[4]; wi[7] = wr[3] * wi[4] + wi[3] * wr[4];
     }
 #pragma unroll
     for (int r = 1; r < R; ++r) {
-      const double ar = xr[q][r], ai = xi[q][r];
+      const V ar = xr[q][r], ai = xi[q][r];
       xr[q][r] = ar * wr[r] - ai * wi[r];
       xi[q][r] = ar * wi[r] + ai * wr[r];
     }
@@ -112,8 +116,8 @@ __device__ __forceinline__ void stockham_pass(double* re, double* im, int Ns, co
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
-template <int L, int T, int E>
-__device__ __forceinline__ void lds_dft(double* re, double* im, const double* __restrict__ tw) {
+template <int L, int T, int E, typename V>
+__device__ __forceinline__ void lds_dft(V* re, V* im, const double* __restrict__ tw) {
   constexpr int NUM8 = ilog2(L) / 3, REM = ilog2(L) % 3;
   int Ns = 1;
 #pragma unroll

@@ -475,6 +475,8 @@ class Navier {
   bool use_dct = true;    // Chebyshev y transforms by FFT-based DCT-I kernels
   bool use_batch = true;  // periodic step: the three fields' transforms and solves batched
   bool use_fuse = true;   // periodic step: the pressure stage as one fused kernel
+  bool mixed = false;     // periodic batched step: convective terms in FP32, the rest FP64
+  DBuf MIX;               // FP32 slots of the mixed convective pipeline (11 slots of FS floats)
   bool vel_same = true;   // the ux and uy solvers hold identical tables
   size_t FS = 0;          // slot stride of the batched buffers
   DBuf STATE, PHYS2;      // [ux, uy, temp] spectral and [ux, uy] physical, one slot each
@@ -948,6 +950,23 @@ class Navier {
     band_y(st, UX.p, my, A.p, ny, R2, sb.mo, sb.coef.p, sb.mask.p, 0.0, 3, (long)F, (long)F);
     timer.end(st);
     ew_add(st, THAT.p, A.p + 2 * F, THAT_BC.p, no);
+    if (mixed) {
+      conv_mixed();
+    } else {
+      conv_fp64();
+    }
+    // pressure gradients (navier.rs:622-654) and the right-hand sides (navier.rs:660-674)
+    xop(bx_p, "DS", P0.p, ny, G.p, W9.p);
+    yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
+    ew_rhs3(st, RHS.p, A.p, G.p, THAT.p, CV.p, GXX.p, GYY.p, dt, no, (long)F);
+    solves_and_pressure();
+  }
+  // convective terms of ux, uy, temp into CV (FP64): dq/dx ux + dq/dy uy (conv_term.rs:35-41),
+  // dealiased forward transforms
+  void conv_fp64() {
+    const long np = (long)nx * ny;
+    const int my = by_u.m;
+    const size_t F = FS;
     // ux, uy in physical space
     IRFFT_B(UX.p, W1.p, my, 0, 1.0, 2);
     CHEBY_BWD_B(by_u, W1.p, UXP.p, nx, 0, nullptr, 0.0, 2);
@@ -959,10 +978,48 @@ class Navier {
     ew_acc_prod2(st, CONV.p + 2 * F, UXP.p, DTBCX.p, UYP.p, DTBCY.p, np);  // + u . grad(fieldbc) for temp
     CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
     RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
-    // pressure gradients (navier.rs:622-654) and the right-hand sides (navier.rs:660-674)
-    xop(bx_p, "DS", P0.p, ny, G.p, W9.p);
-    yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
-    ew_rhs3(st, RHS.p, A.p, G.p, THAT.p, CV.p, GXX.p, GYY.p, dt, no, (long)F);
+  }
+  // the same pipeline with FP32 values and arithmetic (FP64 spectra in, FP64 CV out)
+  void conv_mixed() {
+    const long np = (long)nx * ny;
+    const int my = by_u.m, mx2 = 2 * mx();
+    const long F = (long)FS;
+    if (MIX.n < (11 * FS + 1) / 2) throw std::logic_error("mixed buffers not allocated");
+    float* W1f = reinterpret_cast<float*>(MIX.p);
+    float* UXPf = W1f + 3 * F;
+    float* UYPf = UXPf + F;
+    float* CONVf = UXPf + 2 * F;
+    float* W2f = CONVf + 3 * F;
+    const DBand& band = stencil_band(by_u);
+    const double fx = 4.0 * ((double)nx + mx2);  // bytes per column: FP64 spectrum in, FP32 values out
+    auto irfft = [&](int times, double sc, int batch) {
+      timer.begin(st, 2, (double)batch * my * fx);
+      irfft_cols_f32(st, UX.p, W1f, nx, my, times, sc, batch, F, F);
+      timer.end(st);
+    };
+    auto bwd = [&](float* out, int times, const float* U, double beta, int batch) {
+      timer.begin(st, 4, 4.0 * batch * nx * ((double)my + ny * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
+      cheb_y_backward_f32(st, W1f, my, my, out, ny, nx, ny, band.coef.p, times, 1.0 / scale[1], U, ny, beta, batch, F,
+                          F, 0);
+      timer.end(st);
+    };
+    irfft(0, 1.0, 2);
+    bwd(UXPf, 0, nullptr, 0.0, 2);
+    irfft(1, scale[0], 3);
+    bwd(CONVf, 0, UXPf, 0.0, 3);
+    irfft(0, 1.0, 3);
+    bwd(CONVf, 1, UYPf, 1.0, 3);
+    ew_acc_prod2_f32(st, CONVf + 2 * F, UXPf, DTBCX.p, UYPf, DTBCY.p, np);
+    timer.begin(st, 6, 4.0 * 3.0 * nx * 2.0 * ny);
+    cheb_y_forward_f32(st, CONVf, ny, W2f, ny, nx, ny, ny * 2 / 3, 3, F, F);
+    timer.end(st);
+    timer.begin(st, 5, 3.0 * ny * (4.0 * nx + 8.0 * mx2));
+    rfft_cols_f32(st, W2f, CV.p, nx, ny, bx_f.m * 2 / 3, 3, F, F);
+    timer.end(st);
+  }
+  void solves_and_pressure() {
+    const int my = by_u.m, R2 = 2 * mx();
+    const size_t F = FS;
     // implicit solves of ux, uy (one solver for both) and temp in one launch
     const Solver &sv = *solver[0], &stt = *solver[2];
     const double* bands[3] = {sv.pybp.p, sv.pybp.p, stt.pybp.p};
@@ -1604,6 +1661,10 @@ int rbc_set_option(rbc_navier* h, int option, int value) {
       case RBC_OPT_DCT: n.use_dct = value != 0; break;
       case RBC_OPT_BATCH: n.use_batch = value != 0; break;
       case RBC_OPT_FUSE: n.use_fuse = value != 0; break;
+      case RBC_OPT_MIXED:
+        n.mixed = value != 0;
+        if (n.mixed && n.MIX.n < (11 * n.FS + 1) / 2) n.MIX.alloc((11 * n.FS + 1) / 2);
+        break;
       default: throw std::invalid_argument("unknown option");
     }
   });

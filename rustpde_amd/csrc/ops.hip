@@ -246,6 +246,17 @@ void ew_acc_prod2(hipStream_t s, double* out, const double* a, const double* b, 
                   long n) {
   EW_LAUNCH(ew_acc_prod2, n, out, a, b, c, d);
 }
+__global__ void ew_acc_prod2_f32_kernel(float* out, const float* a, const double* b, const float* c, const double* d,
+                                        long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = out[i] + a[i] * (float)b[i];
+  out[i] = t + c[i] * (float)d[i];
+}
+void ew_acc_prod2_f32(hipStream_t s, float* out, const float* a, const double* b, const float* c, const double* d,
+                      long n) {
+  EW_LAUNCH(ew_acc_prod2_f32, n, out, a, b, c, d);
+}
 void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
              const double* bx, const double* by, double dt, long n, long F) {
   if (n <= 0) return;

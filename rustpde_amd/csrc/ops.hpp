@@ -85,6 +85,8 @@ void ew_rhs_temp(hipStream_t s, double* out, const double* a, const double* bx, 
 void ew_acc_prod(hipStream_t s, double* out, const double* a, const double* b, long n);     // out + a b
 void ew_acc_prod2(hipStream_t s, double* out, const double* a, const double* b, const double* c, const double* d,
                   long n);  // (out + a b) + c d
+void ew_acc_prod2_f32(hipStream_t s, float* out, const float* a, const double* b, const float* c, const double* d,
+                      long n);  // the same in FP32 (b, d rounded to FP32)
 // The right-hand sides of ux, uy, temp in slots 0, 1, 2 (slot stride F) in one launch:
 // slot 0 ((a - g dt)) - c dt, slot 1 ((a - g dt) + buoy dt) - c dt, slot 2 ((a + bx) + by) - c dt
 void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
@@ -110,6 +112,12 @@ void ew_nuvol(hipStream_t s, double* out, const double* d, const double* t, cons
 // rfft_cols: real -> split planes [2(n/2+1), cols], rows k >= cut set to zero;
 // irfft_cols: split planes -> real, with X_k first multiplied by (i k)^times / sc.
 bool fft_supported(int n);
+// FP32 variants for the mixed-precision convective terms: irfft from FP64 spectra to FP32
+// values, rfft from FP32 values to FP64 spectra (arithmetic in FP32).
+void rfft_cols_f32(hipStream_t s, const float* X, double* Y, int n, int cols, int cut, int batch = 1, long sx = 0,
+                   long sy = 0);
+void irfft_cols_f32(hipStream_t s, const double* X, float* Y, int n, int cols, int times, double sc, int batch = 1,
+                    long sx = 0, long sy = 0);
 void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch = 1, long sx = 0,
                long sy = 0);
 void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch = 1,
@@ -161,6 +169,12 @@ void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* o
                      double beta, int batch = 1, long si = 0, long so = 0, long su = 0);
 void cheb_y_forward(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, int cut,
                     int batch = 1, long si = 0, long so = 0);
+// FP32 variants (FP32 in, arithmetic and out; tables in FP64, rounded on load)
+void cheb_y_backward_f32(hipStream_t s, const float* in, int ldi, int mi, float* out, int ldo, int rows, int n,
+                         const double* coef, int times, double inv_sc, const float* U, int ldu, double beta,
+                         int batch = 1, long si = 0, long so = 0, long su = 0);
+void cheb_y_forward_f32(hipStream_t s, const float* in, int ldi, float* out, int ldo, int rows, int n, int cut,
+                        int batch = 1, long si = 0, long so = 0);
 // cheb_y_diff: d/dy in Chebyshev space (ortho.rs:107-125) with the composite stencil
 // fused on load: out [rows, n] = d / sc + beta out, by the same LDS suffix scans.
 void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
