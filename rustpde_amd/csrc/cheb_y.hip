@@ -66,6 +66,8 @@ const double* fft_table(int L) {
 
 namespace {
 using fftlds::lds_dft;
+using fftlds::lp;
+using fftlds::lp_size;
 
 // block geometry for an FFT length L: E = P L complex values per block (P row pairs),
 // T = E / 8 threads, so every pass gives each thread 8 values
@@ -112,7 +114,7 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
                                                 const double* __restrict__ tw, long si, long so, long su) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
-  __shared__ V re[G::E], im[G::E];
+  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   const V inv_sc = (V)inv_sc_d, beta = (V)beta_d;
   in += si * blockIdx.y;
   out += so * blockIdx.y;
@@ -149,8 +151,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
         sb = f * sb;
       }
     }
-    re[p * L + k] = sa;
-    im[p * L + k] = sb;
+    re[lp(p * L + k)] = sa;
+    im[lp(p * L + k)] = sb;
   }
   __syncthreads();
   if (times) {
@@ -160,12 +162,13 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     const int lane = tid & 63, wave = tid >> 6;
     for (int ch = wave; ch < NCH; ch += T / 64) {
       const int p = ch >> 2, comp = (ch >> 1) & 1, par = ch & 1;
-      V* z = (comp ? im : re) + p * L + (par ? 1 : 2);
+      V* z = comp ? im : re;
+      const int zb = p * L + (par ? 1 : 2);  // chain element t at z[lp(zb + 2 t)]
       const int t0 = lane * SEG;
       V tot = 0;
 #pragma unroll
       for (int q = SEG - 1; q >= 0; --q)
-        if (t0 + q < C) tot = z[2 * (t0 + q)] + tot;
+        if (t0 + q < C) tot = z[lp(zb + 2 * (t0 + q))] + tot;
       V inc = tot;  // inclusive suffix sum over lanes
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
@@ -178,13 +181,12 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
       for (int q = SEG - 1; q >= 0; --q) {
         const int t = t0 + q;
         if (t < C) {
-          acc = z[2 * t] + acc;
-          z[2 * t] = acc;
+          acc = z[lp(zb + 2 * t)] + acc;
+          z[lp(zb + 2 * t)] = acc;
         }
       }
       if (par == 0 && lane == 0) {  // acc = d_2
-        V* z0 = (comp ? im : re) + p * L;
-        z0[0] = z0[0] + acc / (V)2;
+        z[lp(p * L)] = z[lp(p * L)] + acc / (V)2;
       }
     }
     __syncthreads();
@@ -192,7 +194,7 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   // (3) u_k = (-1)^k w_k c_k / sc^times and the even extension u_{L-k} = u_k
   for (int e = tid; e < P * n && transform; e += T) {
     const int p = e / n, k = e - p * n;
-    V a = re[p * L + k], b = im[p * L + k];
+    V a = re[lp(p * L + k)], b = im[lp(p * L + k)];
     if (times && inv_sc != (V)1) {
       a = a * inv_sc;
       b = b * inv_sc;
@@ -200,11 +202,11 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     const V w = (V)(((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5));
     a = a * w;
     b = b * w;
-    re[p * L + k] = a;
-    im[p * L + k] = b;
+    re[lp(p * L + k)] = a;
+    im[lp(p * L + k)] = b;
     if (k > 0 && k < N) {
-      re[p * L + L - k] = a;
-      im[p * L + L - k] = b;
+      re[lp(p * L + L - k)] = a;
+      im[lp(p * L + L - k)] = b;
     }
   }
   if constexpr (transform) {
@@ -231,7 +233,7 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     if (e >= P * n) continue;
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
-    V va = re[p * L + j], vb = im[p * L + j];
+    V va = re[lp(p * L + j)], vb = im[lp(p * L + j)];
     if (!transform && inv_sc != (V)1) {
       va = va * inv_sc;
       vb = vb * inv_sc;
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
                                                                const double* __restrict__ tw, long si, long so) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
-  __shared__ V re[G::E], im[G::E];
+  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   in += si * blockIdx.y;
   out += so * blockIdx.y;
   const int tid = threadIdx.x, row0 = 2 * P * blockIdx.x;
@@ -294,11 +296,11 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
     if (e >= P * n) continue;
     const int p = e / n, k = e - p * n;
     const V a = va[q], b = vb[q];
-    re[p * L + k] = a;
-    im[p * L + k] = b;
+    re[lp(p * L + k)] = a;
+    im[lp(p * L + k)] = b;
     if (k > 0 && k < N) {
-      re[p * L + L - k] = a;
-      im[p * L + L - k] = b;
+      re[lp(p * L + L - k)] = a;
+      im[lp(p * L + L - k)] = b;
     }
   }
   __syncthreads();
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
     for (int h = 0; h < 2; ++h) {
       const int r = ra + h;
       if (r >= rows) continue;
-      const V v = h ? im[p * L + k] : re[p * L + k];
+      const V v = h ? im[lp(p * L + k)] : re[lp(p * L + k)];
       out[(size_t)r * ldo + k] = (Tout)(k < cut ? v * s : (V)0);
     }
   }
@@ -348,9 +350,9 @@ void launch_fwd(hipStream_t s, const Tin* in, int ldi, Tout* out, int ldo, int r
 
 }  // namespace
 
-bool cheb_y_supported(int n) {
+bool cheb_y_supported(int n) {  // the padded LDS buffers of L = 2 (n - 1) <= 2048 stay within 64 KB
   const int N = n - 1;
-  return N >= 16 && N <= 2048 && (N & (N - 1)) == 0;
+  return N >= 16 && N <= 1024 && (N & (N - 1)) == 0;
 }
 
 #define RBC_CHEB_Y_SWITCH(L2, CALL)                                   \
@@ -362,7 +364,6 @@ bool cheb_y_supported(int n) {
     case 512: CALL(512); break;                                       \
     case 1024: CALL(1024); break;                                     \
     case 2048: CALL(2048); break;                                     \
-    case 4096: CALL(4096); break;                                     \
     default: throw std::invalid_argument("cheb_y: unsupported length"); \
   }
 

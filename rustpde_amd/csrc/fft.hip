@@ -27,6 +27,8 @@ namespace rbc {
 namespace {
 
 using fftlds::lds_dft;
+using fftlds::lp;
+using fftlds::lp_size;
 using fftlds::xcd_remap;
 
 // geometry for length L: PP column pairs per block, E = PP L complex values, T = E / 8
@@ -47,7 +49,7 @@ __global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const Tin* __rest
                                                                long sx, long sy) {
   using G = CGeo<L>;
   constexpr int C = G::C, T = G::T, m = G::m;
-  __shared__ V re[G::E], im[G::E];
+  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
   const int tid = threadIdx.x;
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const Tin* __rest
 #pragma unroll
   for (int q = 0; q < QL; ++q) {
     const int e = tid + q * T, c = e % C, t = e / C;
-    ((c & 1) ? im : re)[(c >> 1) * L + t] = v[q];
+    ((c & 1) ? im : re)[lp((c >> 1) * L + t)] = v[q];
   }
   __syncthreads();
   lds_dft<L, T, G::E>(re, im, tw);
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const Tin* __rest
     const int c = e % C, k = e / C, j = j0 + c;
     if (j >= cols) continue;
     const int p = (c >> 1) * L, k2 = (L - k) & (L - 1);
-    const V zr = re[p + k], zi = im[p + k], yr = re[p + k2], yi = im[p + k2];
+    const V zr = re[lp(p + k)], zi = im[lp(p + k)], yr = re[lp(p + k2)], yi = im[lp(p + k2)];
     const V half = (V)0.5;
     V xr, xi;
     if (c & 1) {
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __res
 #pragma clang fp contract(off)
   using G = CGeo<L>;
   constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
-  __shared__ V re[G::E], im[G::E];
+  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   const V sc = (V)sc_d;
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
@@ -132,9 +134,10 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __res
       a = na;
       b = nb;
     }
-    V* z = ((c & 1) ? im : re) + (c >> 1) * L;
-    z[k] = a;
-    if (k > 0 && k < N) z[N + k] = b;
+    V* z = (c & 1) ? im : re;
+    const int zb = (c >> 1) * L;
+    z[lp(zb + k)] = a;
+    if (k > 0 && k < N) z[lp(zb + N + k)] = b;
   }
   __syncthreads();
   // (2) conj Z_k = (Ar - Bi) - i (Ai + Br),  conj Z_{L-k} = (Ar + Bi) + i (Ai - Br)
@@ -145,11 +148,11 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __res
     const int e = tid + q * T, p = e / m, k = e - p * m;
     ar[q] = ai[q] = br[q] = bi[q] = 0;
     if (e < m * G::PP) {
-      ar[q] = re[p * L + k];
-      br[q] = im[p * L + k];
+      ar[q] = re[lp(p * L + k)];
+      br[q] = im[lp(p * L + k)];
       if (k > 0 && k < N) {
-        ai[q] = re[p * L + N + k];
-        bi[q] = im[p * L + N + k];
+        ai[q] = re[lp(p * L + N + k)];
+        bi[q] = im[lp(p * L + N + k)];
       }
     }
   }
@@ -158,11 +161,11 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __res
   for (int q = 0; q < QZ; ++q) {
     const int e = tid + q * T, p = e / m, k = e - p * m;
     if (e >= m * G::PP) continue;
-    re[p * L + k] = ar[q] - bi[q];
-    im[p * L + k] = -(ai[q] + br[q]);
+    re[lp(p * L + k)] = ar[q] - bi[q];
+    im[lp(p * L + k)] = -(ai[q] + br[q]);
     if (k > 0 && k < N) {
-      re[p * L + L - k] = ar[q] + bi[q];
-      im[p * L + L - k] = ai[q] - br[q];
+      re[lp(p * L + L - k)] = ar[q] + bi[q];
+      im[lp(p * L + L - k)] = ai[q] - br[q];
     }
   }
   __syncthreads();
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __res
     const int c = e % C, t = e / C, j = j0 + c;
     if (j >= cols) continue;
     const int p = (c >> 1) * L;
-    Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -im[p + t] * inv : re[p + t] * inv);
+    Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -im[lp(p + t)] * inv : re[lp(p + t)] * inv);
   }
 }
 
@@ -184,7 +187,7 @@ void check(const char* what) {
 
 void check_n(int n) {
   if (n < 8 || (n & (n - 1)) != 0) throw std::invalid_argument("fft: length must be a power of two >= 8");
-  if (n > 4096) throw std::invalid_argument("fft: length above 4096 exceeds the LDS budget");
+  if (n > 2048) throw std::invalid_argument("fft: length above 2048 exceeds the LDS budget");
 }
 
 template <int L, typename Tin, typename V, typename Tout>
@@ -204,7 +207,7 @@ void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, dou
 
 }  // namespace
 
-bool fft_supported(int n) { return n >= 8 && n <= 4096 && (n & (n - 1)) == 0; }
+bool fft_supported(int n) { return n >= 8 && n <= 2048 && (n & (n - 1)) == 0; }
 
 #define RBC_FFT_SWITCH(NN, CALL)                                            \
   switch (NN) {                                                             \
@@ -217,7 +220,6 @@ bool fft_supported(int n) { return n >= 8 && n <= 4096 && (n & (n - 1)) == 0; }
     case 512: CALL(512); break;                                             \
     case 1024: CALL(1024); break;                                           \
     case 2048: CALL(2048); break;                                           \
-    case 4096: CALL(4096); break;                                           \
     default: throw std::invalid_argument("fft: unsupported length");        \
   }
 

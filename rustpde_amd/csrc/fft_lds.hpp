@@ -10,6 +10,12 @@
 namespace rbc {
 namespace fftlds {
 
+// LDS index of element a of the transform buffers: one pad element per 16 spreads the
+// strided accesses of the passes and of the chain scans over the banks
+__host__ __device__ constexpr int lp(int a) { return a + (a >> 4); }
+// LDS elements of a buffer of E values
+__host__ __device__ constexpr int lp_size(int E) { return lp(E) + 1; }
+
 constexpr double SQH = 0.70710678118654752440;  // sqrt(1/2), rounded to T where used
 
 // forward DFTs of 2, 4, 8 points (sign -1), in place, natural order
@@ -72,8 +78,8 @@ __device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double
     const int base = p * L + j;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      xr[q][r] = re[base + r * NB];
-      xi[q][r] = im[base + r * NB];
+      xr[q][r] = re[lp(base + r * NB)];
+      xi[q][r] = im[lp(base + r * NB)];
     }
     // twiddles e^{-2 pi i r k / (Ns R)} = table[r k L / (Ns R)]: powers 1, 2, 4 from the
     // table, the others as products of two of them
@@ -108,8 +114,8 @@ __device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double
   for (int q = 0; q < PER; ++q)
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      re[dst[q] + r * Ns] = xr[q][r];
-      im[dst[q] + r * Ns] = xi[q][r];
+      re[lp(dst[q] + r * Ns)] = xr[q][r];
+      im[lp(dst[q] + r * Ns)] = xi[q][r];
     }
   __syncthreads();
 }

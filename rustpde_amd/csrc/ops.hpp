@@ -158,7 +158,7 @@ struct PressureArgs {
 bool pressure_rows_supported(int ny, int mu);
 void pressure_rows(hipStream_t s, const PressureArgs& a);
 
-// Chebyshev transforms along axis 1 (rows of length n, n - 1 a power of two in [16, 2048])
+// Chebyshev transforms along axis 1 (rows of length n, n - 1 a power of two in [16, 1024])
 // by FFT-based DCT-I.  cheb_y_backward: in [rows, mi] coefficients (composite basis when
 // coef / mask give its stencil taps, else Chebyshev, mi <= n), `times` (0 or 1)
 // derivatives scaled by inv_sc, out [rows, n] = x .* U + beta out (U may be null).

@@ -21,10 +21,10 @@ CASES = [  # (kind0, n0, kind1, n1)
     (0, 18, 3, 18),   # chebyshev x cheb_dirichlet_bc
 ]
 
-DCT_CASES = [  # long axis-1 rows: transforms by the FFT-based DCT-I kernels
+DCT_CASES = [  # long axis-1 rows: FFT-based DCT-I kernels up to 1025, dense operators beyond
     (0, 12, 0, 513),   # chebyshev x chebyshev
     (1, 10, 2, 1025),  # cheb_dirichlet x cheb_neumann
-    (4, 8, 1, 2049),   # fourier_r2c x cheb_dirichlet
+    (4, 8, 1, 2049),   # fourier_r2c x cheb_dirichlet (dense GEMM path)
 ]
 
 
