@@ -107,11 +107,24 @@ __device__ __forceinline__ V stencil_value(const StencilLoad<V>& v, bool composi
 }
 
 // Tin: input type, V: arithmetic (and LDS) type, Tout: type of out and U
+// Extra products added to one batch entry after the epilogue: out += E1 F1, then out += E2 F2
+// (temp's u . grad(fieldbc) terms, navier.rs:547-550); E of the output type, F FP64, both
+// [rows, ldu], shared by the batch.
+template <typename Tout>
+struct ExtraProducts {
+  const Tout* e1;
+  const double* f1;
+  const Tout* e2;
+  const double* f2;
+  int slot;  // batch entry that receives them (-1: none)
+};
+
 template <int L, bool transform, typename Tin, typename V, typename Tout>
 __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out,
                                                 int ldo, int rows, const double* __restrict__ coef, int times,
                                                 double inv_sc_d, const Tout* __restrict__ U, int ldu, double beta_d,
-                                                const double* __restrict__ tw, long si, long so, long su) {
+                                                const double* __restrict__ tw, long si, long so, long su,
+                                                ExtraProducts<Tout> xp) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
@@ -214,7 +227,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     lds_dft<L, T, G::E>(re, im, tw);
   }
   // (4) x_j = Re / Im Z_j, j <= N, or the coefficients d_j / sc without the transform;
-  //     out = x .* U + beta out (loads first)
+  //     out = x .* U + beta out (loads first), plus the extra products on one entry
+  const bool extra = xp.slot == (int)blockIdx.y;
   V ua[Q], ub[Q], oa[Q], ob[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -246,6 +260,18 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
       va = va + beta * oa[q];
       vb = vb + beta * ob[q];
     }
+    if (extra) {
+      if (ra < rows) {
+        const size_t o = (size_t)ra * ldu + j;
+        va = va + (V)xp.e1[o] * (V)xp.f1[o];
+        va = va + (V)xp.e2[o] * (V)xp.f2[o];
+      }
+      if (rb < rows) {
+        const size_t o = (size_t)rb * ldu + j;
+        vb = vb + (V)xp.e1[o] * (V)xp.f1[o];
+        vb = vb + (V)xp.e2[o] * (V)xp.f2[o];
+      }
+    }
     if (ra < rows) out[(size_t)ra * ldo + j] = (Tout)va;
     if (rb < rows) out[(size_t)rb * ldo + j] = (Tout)vb;
   }
@@ -255,9 +281,9 @@ template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out, int ldo, int rows,
     const double* __restrict__ coef, int times, double inv_sc, const Tout* __restrict__ U, int ldu, double beta,
-    const double* __restrict__ tw, long si, long so, long su) {
+    const double* __restrict__ tw, long si, long so, long su, ExtraProducts<Tout> xp) {
   cheb_y_bwd_body<L, true, Tin, V, Tout>(in, ldi, mi, out, ldo, rows, coef, times, inv_sc, U, ldu, beta, tw, si, so,
-                                         su);
+                                         su, xp);
 }
 // the same without the transform: Chebyshev coefficients of the derivative
 template <int L>
@@ -266,7 +292,8 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_diff_kernel(const double* __
                                                                 const double* __restrict__ coef, double inv_sc,
                                                                 double beta, long si, long so) {
   cheb_y_bwd_body<L, false, double, double, double>(in, ldi, mi, out, ldo, rows, coef, 1, inv_sc, nullptr, 0, beta,
-                                                     nullptr, si, so, 0);
+                                                     nullptr, si, so, 0,
+                                                     ExtraProducts<double>{nullptr, nullptr, nullptr, nullptr, -1});
 }
 
 template <int L, typename Tin, typename V, typename Tout>
@@ -323,13 +350,14 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
 template <int L, typename Tin, typename V, typename Tout>
 void launch_bwd(hipStream_t s, const Tin* in, int ldi, int mi, Tout* out, int ldo, int rows, const double* coef,
                 const unsigned char* mask, int times, double inv_sc, const Tout* U, int ldu, double beta, int batch,
-                long si, long so, long su, bool transform) {
+                long si, long so, long su, bool transform,
+                ExtraProducts<Tout> xp = ExtraProducts<Tout>{nullptr, nullptr, nullptr, nullptr, -1}) {
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
   (void)mask;  // the zero coefficients of absent taps stand for the mask
   if (transform) {
     RBC_LAUNCH((cheb_y_bwd_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo,
-               rows, coef, times, inv_sc, U, ldu, beta, fft_table(L), si, so, su);
+               rows, coef, times, inv_sc, U, ldu, beta, fft_table(L), si, so, su, xp);
   } else {
     if constexpr (std::is_same<Tin, double>::value && std::is_same<Tout, double>::value)
       RBC_LAUNCH(cheb_y_diff_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
@@ -369,14 +397,15 @@ bool cheb_y_supported(int n) {  // the padded LDS buffers of L = 2 (n - 1) <= 20
 
 void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
                      const double* coef, const unsigned char* mask, int times, double inv_sc, const double* U, int ldu,
-                     double beta, int batch, long si, long so, long su) {
+                     double beta, int batch, long si, long so, long su, const double* e1, const double* f1,
+                     const double* e2, const double* f2, int xslot) {
   if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_backward: unsupported length");
   if (times < 0 || times > 1) throw std::invalid_argument("cheb_y_backward: times must be 0 or 1");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_backward: bad batch or rows");
   if (mi > n || (!coef && mi < 1)) throw std::invalid_argument("cheb_y_backward: bad input length");
 #define RBC_BWD(LL)                                                                                          \
   launch_bwd<LL, double, double, double>(s, in, ldi, mi, out, ldo, rows, coef, mask, times, inv_sc, U, ldu, beta, \
-                                         batch, si, so, su, true)
+                                         batch, si, so, su, true, ExtraProducts<double>{e1, f1, e2, f2, xslot})
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_BWD)
 #undef RBC_BWD
   check("cheb_y_backward");
@@ -407,14 +436,15 @@ void cheb_y_forward(hipStream_t s, const double* in, int ldi, double* out, int l
 
 void cheb_y_backward_f32(hipStream_t s, const float* in, int ldi, int mi, float* out, int ldo, int rows, int n,
                          const double* coef, int times, double inv_sc, const float* U, int ldu, double beta, int batch,
-                         long si, long so, long su) {
+                         long si, long so, long su, const float* e1, const double* f1, const float* e2,
+                         const double* f2, int xslot) {
   if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_backward_f32: unsupported length");
   if (times < 0 || times > 1) throw std::invalid_argument("cheb_y_backward_f32: times must be 0 or 1");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_backward_f32: bad batch or rows");
   if (mi > n || (!coef && mi < 1)) throw std::invalid_argument("cheb_y_backward_f32: bad input length");
 #define RBC_BWD32(LL)                                                                                              \
   launch_bwd<LL, float, float, float>(s, in, ldi, mi, out, ldo, rows, coef, nullptr, times, inv_sc, U, ldu, beta, \
-                                      batch, si, so, su, true)
+                                      batch, si, so, su, true, ExtraProducts<float>{e1, f1, e2, f2, xslot})
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_BWD32)
 #undef RBC_BWD32
   check("cheb_y_backward_f32");

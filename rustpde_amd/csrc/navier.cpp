@@ -927,12 +927,16 @@ class Navier {
     timer.end(st);
   }
   // Chebyshev y backward of composite by (times derivatives): out = x .* U + beta out
+  // extra = true: temp's slot (2) also gets + ux dTbc/dx + uy dTbc/dy (navier.rs:547-550)
   void CHEBY_BWD_B(const Basis& b, const double* in, double* out, int rows, int times, const double* U, double beta,
-                   int batch) {
+                   int batch, bool extra = false) {
     const DBand& band = stencil_band(b);
-    timer.begin(st, 4, 8.0 * batch * rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
+    const double xb = extra ? 8.0 * 4.0 * rows * b.n : 0.0;
+    timer.begin(st, 4,
+                8.0 * batch * rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))) + xb);
     cheb_y_backward(st, in, b.m, b.m, out, b.n, rows, b.n, band.coef.p, band.mask.p, times, 1.0 / scale[1], U, b.n,
-                    beta, batch, (long)FS, (long)FS, 0);
+                    beta, batch, (long)FS, (long)FS, 0, extra ? UXP.p : nullptr, DTBCX.p, extra ? UYP.p : nullptr,
+                    DTBCY.p, extra ? 2 : -1);
     timer.end(st);
   }
   void CHEBY_FWD_B(const double* in, double* out, int rows, int cut, int batch) {
@@ -941,32 +945,24 @@ class Navier {
     timer.end(st);
   }
   void update_batched() {  // navier.rs:737-765 with the three fields' work batched
-    const long no = (long)rows_ortho(bx_f) * ny, np = (long)nx * ny;
-    const int my = by_u.m, R2 = 2 * mx();
+    const int my = by_u.m;
     const size_t F = FS;
-    // ux, uy, temp in ortho space; buoyancy = temp plus fieldbc (navier.rs:740-742)
-    timer.begin(st, 9, 8.0 * 3.0 * R2 * ((double)my + ny));
-    const DBand& sb = stencil_band(by_u);
-    band_y(st, UX.p, my, A.p, ny, R2, sb.mo, sb.coef.p, sb.mask.p, 0.0, 3, (long)F, (long)F);
-    timer.end(st);
-    ew_add(st, THAT.p, A.p + 2 * F, THAT_BC.p, no);
     if (mixed) {
       conv_mixed();
     } else {
       conv_fp64();
     }
-    // pressure gradients (navier.rs:622-654) and the right-hand sides (navier.rs:660-674)
-    xop(bx_p, "DS", P0.p, ny, G.p, W9.p);
+    // d/dy pres, then the right-hand sides (navier.rs:622-674) with to_ortho of ux, uy, temp,
+    // the buoyancy temp + fieldbc (navier.rs:740-742) and d/dx pres formed in the same kernel
     yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
-    ew_rhs3(st, RHS.p, A.p, G.p, THAT.p, CV.p, GXX.p, GYY.p, dt, no, (long)F);
+    rhs3_fused(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p, CV.p, GXX.p, GYY.p, RHS.p, mx(), ny,
+               scale[0], dt, (long)F);
     solves_and_pressure();
   }
   // convective terms of ux, uy, temp into CV (FP64): dq/dx ux + dq/dy uy (conv_term.rs:35-41),
   // dealiased forward transforms
   void conv_fp64() {
-    const long np = (long)nx * ny;
     const int my = by_u.m;
-    const size_t F = FS;
     // ux, uy in physical space
     IRFFT_B(UX.p, W1.p, my, 0, 1.0, 2);
     CHEBY_BWD_B(by_u, W1.p, UXP.p, nx, 0, nullptr, 0.0, 2);
@@ -974,14 +970,12 @@ class Navier {
     IRFFT_B(UX.p, W1.p, my, 1, scale[0], 3);
     CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 0, UXP.p, 0.0, 3);
     IRFFT_B(UX.p, W1.p, my, 0, 1.0, 3);
-    CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 1, UYP.p, 1.0, 3);
-    ew_acc_prod2(st, CONV.p + 2 * F, UXP.p, DTBCX.p, UYP.p, DTBCY.p, np);  // + u . grad(fieldbc) for temp
+    CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 1, UYP.p, 1.0, 3, true);  // + u . grad(fieldbc) for temp
     CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
     RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
   }
   // the same pipeline with FP32 values and arithmetic (FP64 spectra in, FP64 CV out)
   void conv_mixed() {
-    const long np = (long)nx * ny;
     const int my = by_u.m, mx2 = 2 * mx();
     const long F = (long)FS;
     if (MIX.n < (11 * FS + 1) / 2) throw std::logic_error("mixed buffers not allocated");
@@ -997,19 +991,20 @@ class Navier {
       irfft_cols_f32(st, UX.p, W1f, nx, my, times, sc, batch, F, F);
       timer.end(st);
     };
-    auto bwd = [&](float* out, int times, const float* U, double beta, int batch) {
-      timer.begin(st, 4, 4.0 * batch * nx * ((double)my + ny * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
+    auto bwd = [&](float* out, int times, const float* U, double beta, int batch, bool extra) {
+      const double xb = extra ? 24.0 * nx * ny : 0.0;
+      timer.begin(st, 4,
+                  4.0 * batch * nx * ((double)my + ny * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))) + xb);
       cheb_y_backward_f32(st, W1f, my, my, out, ny, nx, ny, band.coef.p, times, 1.0 / scale[1], U, ny, beta, batch, F,
-                          F, 0);
+                          F, 0, extra ? UXPf : nullptr, DTBCX.p, extra ? UYPf : nullptr, DTBCY.p, extra ? 2 : -1);
       timer.end(st);
     };
     irfft(0, 1.0, 2);
-    bwd(UXPf, 0, nullptr, 0.0, 2);
+    bwd(UXPf, 0, nullptr, 0.0, 2, false);
     irfft(1, scale[0], 3);
-    bwd(CONVf, 0, UXPf, 0.0, 3);
+    bwd(CONVf, 0, UXPf, 0.0, 3, false);
     irfft(0, 1.0, 3);
-    bwd(CONVf, 1, UYPf, 1.0, 3);
-    ew_acc_prod2_f32(st, CONVf + 2 * F, UXPf, DTBCX.p, UYPf, DTBCY.p, np);
+    bwd(CONVf, 1, UYPf, 1.0, 3, true);  // + u . grad(fieldbc) for temp
     timer.begin(st, 6, 4.0 * 3.0 * nx * 2.0 * ny);
     cheb_y_forward_f32(st, CONVf, ny, W2f, ny, nx, ny, ny * 2 / 3, 3, F, F);
     timer.end(st);

@@ -207,6 +207,59 @@ __global__ void ew_rhs3_kernel(double* out, const double* a, const double* g, co
     out[j] = r - c[j] * dt;
   }
 }
+// rhs of ux, uy, temp in ortho space for the periodic step, one thread per ortho entry:
+//   a_q = S U_q (to_ortho stencil, taps i and i - 2), buoy = a_temp + fieldbc,
+//   ux: ((a - dP/dx dt)) - c dt, dP/dx = (i k P / sc) as fourier_diff forms it,
+//   uy: ((a - g1 dt) + buoy dt) - c dt,  temp: ((a + gxx) + gyy) - c dt
+__global__ void rhs3_fused_kernel(const double* __restrict__ U, int my, const double* __restrict__ sc0,
+                                  const double* __restrict__ sc1, const double* __restrict__ P0,
+                                  const double* __restrict__ g1, const double* __restrict__ thbc,
+                                  const double* __restrict__ cv, const double* __restrict__ gxx,
+                                  const double* __restrict__ gyy, double* __restrict__ out, int mx, int ny, double kscale,
+                                  double dt, long F) {
+#pragma clang fp contract(off)
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)2 * mx * ny) return;
+  const int r = (int)(e / ny), i = (int)(e - (long)r * ny);
+  const bool imag = r >= mx;
+  const int k = imag ? r - mx : r;
+  const double c0 = sc0[i], c1 = sc1[i];
+  const bool t0 = i < my, t1 = i >= 2 && i - 2 < my;
+  double a[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double* u = U + q * F + (size_t)r * my;
+    const double x0 = t0 ? u[i] : 0.0, x2 = t1 ? u[i - 2] : 0.0;
+    a[q] = x0 * c0 + x2 * c1;
+  }
+  // d/dx pres of this entry (split planes: real row k, imaginary row mx + k)
+  const double pa = P0[(size_t)k * ny + i], pb = P0[(size_t)(mx + k) * ny + i], kk = (double)k;
+  double na = pa * 0.0 - pb * kk, nb = pa * kk + pb * 0.0;
+  if (kscale != 1.0) {
+    const double ns = kscale * kscale + 0.0 * 0.0;
+    const double qa = (na * kscale + nb * 0.0) / ns, qb = (nb * kscale - na * 0.0) / ns;
+    na = qa;
+    nb = qb;
+  }
+  const double gx = imag ? nb : na;
+  const long j = (long)r * ny + i;
+  const double buoy = a[2] + thbc[j];
+  double r0 = a[0] - gx * dt;
+  out[j] = r0 - cv[j] * dt;
+  double r1 = a[1] - g1[j] * dt;
+  r1 = r1 + buoy * dt;
+  out[F + j] = r1 - cv[F + j] * dt;
+  out[2 * F + j] = ((a[2] + gxx[j]) + gyy[j]) - cv[2 * F + j] * dt;
+}
+void rhs3_fused(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
+                const double* thbc, const double* cv, const double* gxx, const double* gyy, double* out, int mx, int ny,
+                double kscale, double dt, long F) {
+  const long n = (long)2 * mx * ny;
+  RBC_LAUNCH(rhs3_fused_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, U, my, stencil, stencil + ny, P0, g1, thbc,
+             cv, gxx, gyy, out, mx, ny, kscale, dt, F);
+  check_launch("rhs3_fused");
+}
+
 EW_KERNEL(ew_update_pres, (double* p, const double* d, const double* phi, double nu, double inv_dt, long n),
           {
             double q = p[i] - d[i] * nu;

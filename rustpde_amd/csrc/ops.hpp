@@ -91,6 +91,12 @@ void ew_acc_prod2_f32(hipStream_t s, float* out, const float* a, const double* b
 // slot 0 ((a - g dt)) - c dt, slot 1 ((a - g dt) + buoy dt) - c dt, slot 2 ((a + bx) + by) - c dt
 void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
              const double* bx, const double* by, double dt, long n, long F);
+// The same with the to_ortho stencil of ux, uy, temp (U: slots of [2 mx, my], taps
+// stencil[0, ny) and [ny, 2 ny)), the buoyancy temp + fieldbc (thbc) and d/dx pres (from P0,
+// Fourier x) formed on the fly; g1 = d/dy pres; all other arrays [2 mx, ny], slot stride F.
+void rhs3_fused(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
+                const double* thbc, const double* cv, const double* gxx, const double* gyy, double* out, int mx, int ny,
+                double kscale, double dt, long F);
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n);              // f (-c) + u
 void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt,
                     long n);  // (p - div nu) + phi inv_dt
@@ -164,15 +170,21 @@ void pressure_rows(hipStream_t s, const PressureArgs& a);
 // derivatives scaled by inv_sc, out [rows, n] = x .* U + beta out (U may be null).
 // cheb_y_forward: in [rows, n] values -> out [rows, n] Chebyshev coefficients, k >= cut zero.
 bool cheb_y_supported(int n);
+// Optional extra products on batch entry xslot: out += e1 f1, then out += e2 f2 ([rows, ldu],
+// shared by the batch).
 void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
                      const double* coef, const unsigned char* mask, int times, double inv_sc, const double* U, int ldu,
-                     double beta, int batch = 1, long si = 0, long so = 0, long su = 0);
+                     double beta, int batch = 1, long si = 0, long so = 0, long su = 0, const double* e1 = nullptr,
+                     const double* f1 = nullptr, const double* e2 = nullptr, const double* f2 = nullptr,
+                     int xslot = -1);
 void cheb_y_forward(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, int cut,
                     int batch = 1, long si = 0, long so = 0);
 // FP32 variants (FP32 in, arithmetic and out; tables in FP64, rounded on load)
 void cheb_y_backward_f32(hipStream_t s, const float* in, int ldi, int mi, float* out, int ldo, int rows, int n,
                          const double* coef, int times, double inv_sc, const float* U, int ldu, double beta,
-                         int batch = 1, long si = 0, long so = 0, long su = 0);
+                         int batch = 1, long si = 0, long so = 0, long su = 0, const float* e1 = nullptr,
+                         const double* f1 = nullptr, const float* e2 = nullptr, const double* f2 = nullptr,
+                         int xslot = -1);
 void cheb_y_forward_f32(hipStream_t s, const float* in, int ldi, float* out, int ldo, int rows, int n, int cut,
                         int batch = 1, long si = 0, long so = 0);
 // cheb_y_diff: d/dy in Chebyshev space (ortho.rs:107-125) with the composite stencil
