@@ -87,16 +87,35 @@ void fft_pow2(std::vector<cd>& a, int sign) {
   }
 }
 
+// cos(pi * r / q) and sin(pi * r / q) for r in [0, 2q), tabulated once per q (per thread):
+// the direct sums below take the same values as calling std::cos / std::sin each time
+struct TrigTable {
+  std::vector<double> c, s;
+};
+const TrigTable& trig_table(long long q) {
+  thread_local std::vector<std::pair<long long, std::unique_ptr<TrigTable>>> cache;
+  for (auto& e : cache)
+    if (e.first == q) return *e.second;
+  auto t = std::make_unique<TrigTable>();
+  t->c.resize(2 * q);
+  t->s.resize(2 * q);
+  for (long long r = 0; r < 2 * q; ++r) {
+    t->c[r] = std::cos(M_PI * (double)r / (double)q);
+    t->s[r] = std::sin(M_PI * (double)r / (double)q);
+  }
+  cache.emplace_back(q, std::move(t));
+  return *cache.back().second;
+}
 // cos(pi * p / q) with exact integer reduction of p modulo 2q
 inline double cospi_ratio(long long p, long long q) {
   long long r = p % (2 * q);
   if (r < 0) r += 2 * q;
-  return std::cos(M_PI * (double)r / (double)q);
+  return trig_table(q).c[r];
 }
 inline double sinpi_ratio(long long p, long long q) {
   long long r = p % (2 * q);
   if (r < 0) r += 2 * q;
-  return std::sin(M_PI * (double)r / (double)q);
+  return trig_table(q).s[r];
 }
 
 // DCT-I as used by ndrustfft::nddct1 (rustdct 0.6):
@@ -111,9 +130,16 @@ void dct1(const double* x, double* y, int n) {
     for (int k = 0; k < n; ++k) y[k] = a[k].real();
     return;
   }
+  const TrigTable& tb = trig_table(n - 1);
+  const long long q2 = 2LL * (n - 1);
   for (int k = 0; k < n; ++k) {
     double s = 0.0;
-    for (int j = 1; j < n - 1; ++j) s += x[j] * cospi_ratio((long long)j * k, n - 1);
+    long long r = 0;  // r = jk mod 2(n-1): cos(pi jk / (n-1)) = tb.c[r]
+    for (int j = 1; j < n - 1; ++j) {
+      r += k;
+      if (r >= q2) r -= q2;
+      s += x[j] * tb.c[r];
+    }
     y[k] = x[0] + ((k & 1) ? -x[n - 1] : x[n - 1]) + 2.0 * s;
   }
 }
@@ -128,11 +154,16 @@ void rfft(const double* v, cd* X, int n) {
     for (int k = 0; k < m; ++k) X[k] = a[k];
     return;
   }
+  const TrigTable& tb = trig_table(n);
   for (int k = 0; k < m; ++k) {
     double sr = 0.0, si = 0.0;
+    const long long step = (2LL * k) % (2LL * n);
+    long long r = 0;  // r = 2jk mod 2n: cos / sin(2 pi jk / n)
     for (int j = 0; j < n; ++j) {
-      sr += v[j] * cospi_ratio(2LL * j * k, n);
-      si -= v[j] * sinpi_ratio(2LL * j * k, n);
+      sr += v[j] * tb.c[r];
+      si -= v[j] * tb.s[r];
+      r += step;
+      if (r >= 2LL * n) r -= 2LL * n;
     }
     X[k] = cd(sr, si);
   }
@@ -152,10 +183,16 @@ void irfft(const cd* X, double* v, int n) {
     for (int j = 0; j < n; ++j) v[j] = a[j].real() * (1.0 / n);
     return;
   }
+  const TrigTable& tb = trig_table(n);
   for (int j = 0; j < n; ++j) {
     double s = 0.0;
-    for (int k = 0; k < n; ++k)
-      s += a[k].real() * cospi_ratio(2LL * j * k, n) - a[k].imag() * sinpi_ratio(2LL * j * k, n);
+    const long long step = (2LL * j) % (2LL * n);
+    long long r = 0;  // r = 2jk mod 2n
+    for (int k = 0; k < n; ++k) {
+      s += a[k].real() * tb.c[r] - a[k].imag() * tb.s[r];
+      r += step;
+      if (r >= 2LL * n) r -= 2LL * n;
+    }
     v[j] = s * (1.0 / n);
   }
 }
