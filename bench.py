@@ -133,12 +133,69 @@ def bench_hholtz_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def c4_descs():
+    """96 confined instances: N in {64, 96, 128, 192}, Ra log-uniform in [1e4, 1e7], Pr = 1,
+    aspect 1, adiabatic sidewalls, dt = 0.01 * 64 / N (seeded draws)."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    sizes = rng.choice([64, 96, 128, 192], size=96)
+    ras = 10.0 ** rng.uniform(4.0, 7.0, size=96)
+    return [(int(nn), int(nn), float(ra), 1.0, 0.01 * 64.0 / nn, 1.0, True) for nn, ra in zip(sizes, ras)]
+
+
+def bench_ensemble(args):
+    """Ensemble of 96 confined instances advanced together (rbc_create_ensemble)."""
+    import ctypes
+
+    from rustpde_amd import Ensemble, _lib, seeded
+    L = _lib.load()
+    descs = c4_descs()
+    ens = Ensemble(descs)
+    for i, d in enumerate(descs):
+        nav = ens[i]
+        nav.set_velocity(0.2, 1.0, 1.0)
+        nav.temp.v = seeded.interior_perturbation(d[0], d[1], 1e-2, 1000 + i, kind="uniform", periodic=False)
+        nav.temp.forward()
+    ens.update(args.warmup)
+    t0 = time.perf_counter()
+    ens.update(args.steps)
+    el = time.perf_counter() - t0
+    n = len(descs)
+    points = sum(d[0] * d[1] for d in descs)
+    out = {"metric": "ensemble instance-steps/s (96 confined RBC instances, FP64)", "value": n * args.steps / el,
+           "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "ensemble of 96 confined RBC instances, N in {64,96,128,192}, Ra log-uniform "
+                                  "[1e4,1e7], dt = 0.01*64/N (FP64)", "instances": n, "groups": ens.groups,
+                      "sizes": {str(k): sum(1 for d in descs if d[0] == k) for k in (64, 96, 128, 192)}},
+           "grid_point_updates_per_s": points * args.steps / el}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        O.set_threads(threads)
+        per = {}
+        for nn in (64, 96, 128, 192):
+            o = O.Navier2D.new(nn, nn, 1e5, 1.0, 0.01 * 64.0 / nn, 1.0, True)
+            o.update(1)
+            k = 3
+            t0 = time.perf_counter()
+            o.update(k)
+            per[nn] = (time.perf_counter() - t0) / k
+        cpu_step = sum(per[d[0]] for d in descs)  # seconds for one step of all 96 instances
+        out["cpu_baseline"] = {"value": n / cpu_step, "unit": "instance-steps/s", "cores": threads, "kind": "port",
+                               "sample": "3 oracle steps per size (64, 96, 128, 192), weighted by the ensemble's "
+                                         "size counts"}
+    print(json.dumps(out), flush=True)
+    ens.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS) + ["c2"])
+    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS) + ["c2", "c4"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="mixed: convective terms in FP32, the rest FP64 (periodic configs); the FP64 step is "
                          "timed beside it and the field / Nu differences after 10 steps are reported")
@@ -154,6 +211,9 @@ def main():
 
     if args.config == "c2":
         bench_hholtz_batch(args)
+        return
+    if args.config == "c4":
+        bench_ensemble(args)
         return
 
     import ctypes

@@ -197,6 +197,28 @@ int rbc_solver_tensor_info(rbc_solver* h, int* n, int* has_fb);
 int rbc_solver_get_tensor(rbc_solver* h, double* lam, double* fwd, double* bwd);
 int rbc_solver_set_tensor(rbc_solver* h, const double* lam, const double* fwd, const double* bwd);
 
+/* ----------------------------------------------------------------- ensembles
+ * Many independent confined instances advanced together (no counterpart in the reference,
+ * which runs one Navier2D per process; each instance is exactly Navier2D::new with its own
+ * parameters).  Instances with equal (nx, ny, pr, dt, aspect, adiabatic) are advanced by
+ * one set of batched launches; Ra (and the seeded fields) may differ per instance.
+ * rbc_ensemble_instance returns a handle owned by the ensemble: every per-handle entry
+ * point (fields, forward/backward, set_velocity, diagnostics, tensors) applies to that
+ * instance; rbc_update and rbc_destroy refuse it (use rbc_ensemble_update / _destroy). */
+typedef struct rbc_desc {
+  int nx, ny;
+  double ra, pr, dt, aspect;
+  int adiabatic;
+} rbc_desc;
+typedef struct rbc_ensemble rbc_ensemble;
+int rbc_create_ensemble(int n, const rbc_desc* desc, rbc_ensemble** out);
+int rbc_ensemble_destroy(rbc_ensemble* h);
+int rbc_ensemble_size(rbc_ensemble* h, int* n, int* groups);
+int rbc_ensemble_instance(rbc_ensemble* h, int inst, rbc_navier** out);
+/* Integrate::update of every instance, nsteps times (blocks until done). */
+int rbc_ensemble_update(rbc_ensemble* h, int nsteps);
+int rbc_ensemble_synchronize(rbc_ensemble* h);
+
 /* ------------------------------------------------------------ device memory helpers
  * For callers without their own HIP runtime binding (e.g. ctypes). */
 int rbc_dev_malloc(void** out, unsigned long long bytes);

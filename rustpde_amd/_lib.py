@@ -63,9 +63,20 @@ SIGNATURES = {
     "rbc_memcpy_h2d": (_i, [_vp, _vp, ctypes.c_ulonglong]),
     "rbc_memcpy_d2h": (_i, [_vp, _vp, ctypes.c_ulonglong]),
     "rbc_memcpy_d2d": (_i, [_vp, _vp, ctypes.c_ulonglong]),
+    "rbc_create_ensemble": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    "rbc_ensemble_destroy": (_i, [_vp]),
+    "rbc_ensemble_size": (_i, [_vp, _ip, _ip]),
+    "rbc_ensemble_instance": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
+    "rbc_ensemble_update": (_i, [_vp, _i]),
+    "rbc_ensemble_synchronize": (_i, [_vp]),
 }
 
 _lib = None
+
+
+class RbcDesc(ctypes.Structure):
+    """rbc_desc of include/rustpde_amd.h: one confined instance of an ensemble."""
+    _fields_ = [("nx", _i), ("ny", _i), ("ra", _d), ("pr", _d), ("dt", _d), ("aspect", _d), ("adiabatic", _i)]
 
 
 class RbcError(RuntimeError):

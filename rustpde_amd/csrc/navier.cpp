@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -457,7 +458,7 @@ class Navier {
   hipStream_t st = nullptr;
   Basis bx_u, bx_t, bx_p, bx_phi, bx_f, by_u, by_p, by_phi, by_bc;
   OpCache X, Y;
-  std::unique_ptr<Solver> solver[4];
+  std::shared_ptr<Solver> solver[4];
   // state
   DBuf T, UX, UY, P0, P1, BC;        // spectral (split planes when periodic)
   DBuf Tv, UXv, UYv, P0v, P1v, BCv;  // physical
@@ -475,6 +476,14 @@ class Navier {
   bool use_dct = true;    // Chebyshev y transforms by FFT-based DCT-I kernels
   bool use_batch = true;  // periodic step: the three fields' transforms and solves batched
   bool use_fuse = true;   // periodic step: the pressure stage as one fused kernel
+  // ensemble group: this handle (the group's leader) advances nb instances of the same size
+  // whose buffers are slots nb * bs apart; per-instance nu and ADI operators in slabs
+  int nb = 1;
+  long bs = 0;
+  DBuf NUARR;
+  DBuf ADIH[3][2];
+  long adi_s[3][2] = {};
+  long nbn(long n) const { return (long)(nb - 1) * bs + n; }  // element-wise extent over the group
   bool mixed = false;     // periodic batched step: convective terms in FP32, the rest FP64
   DBuf MIX;               // FP32 slots of the mixed convective pipeline (11 slots of FS floats)
   bool vel_same = true;   // the ux and uy solvers hold identical tables
@@ -519,26 +528,36 @@ class Navier {
   }  // Fourier x transforms by the FFT kernels (dense DFT GEMMs otherwise)
 
   // instrumented launches
+  // sA / sB: operand strides between the instances of a group (0: shared operator)
   void GEMM(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc, double beta = 0.0,
-         const double* U = nullptr, int ldu = 0) {
-    timer.begin(st, 0, 2.0 * M * N * K);
-    if (use_mfma) gemm_mfma(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
-    else gemm(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
+            const double* U = nullptr, int ldu = 0, long sA = 0, long sB = 0) {
+    timer.begin(st, 0, 2.0 * M * N * K * nb);
+    if (nb > 1) {
+      gemm_mfma_batched(st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, bs, nb, 1.0, beta, U, ldu, U ? bs : 0);
+    } else if (use_mfma) {
+      gemm_mfma(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
+    } else {
+      gemm(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
+    }
     timer.end(st);
   }
   void ROWS(double* X, int ld, int nrows, int m, int nsets, const Solver& s) {
     // bytes: the rows read and written once, the 4 diagonals of every set read once
-    timer.begin(st, 3, 8.0 * (2.0 * nrows * m + 4.0 * (double)nsets * m));
-    if (fdma_rows_wave_supported(m)) fdma_rows_wave(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
-    else fdma_rows(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+    timer.begin(st, 3, 8.0 * nb * (2.0 * nrows * m + 4.0 * (double)nsets * m));
+    if (fdma_rows_wave_supported(m)) {
+      fdma_rows_wave(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p, nb, bs);
+    } else {
+      for (int b = 0; b < nb; ++b)
+        fdma_rows(st, X + b * bs, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+    }
     timer.end(st);
   }
   // P_y matvec then the per-row solves: R [nrows, ny] -> X [nrows, my]
   void ROWSF(const double* R, int ldr, double* X, int ld, int nrows, const Solver& s) {
     const int m = s.m, nr = s.pyb.mi;
     if (use_bands && fdma_rows_lds_supported(m, nr)) {
-      timer.begin(st, 1, 8.0 * ((double)nrows * (nr + m) + 4.0 * (double)s.nsets * m));
-      fdma_rows_lds(st, R, ldr, nr, X, ld, nrows, m, s.nsets, s.pybp.p, s.diagp.p);
+      timer.begin(st, 1, 8.0 * nb * ((double)nrows * (nr + m) + 4.0 * (double)s.nsets * m));
+      fdma_rows_lds(st, R, ldr, nr, X, ld, nrows, m, s.nsets, s.pybp.p, s.diagp.p, nb, bs, bs);
       timer.end(st);
       return;
     }
@@ -546,13 +565,13 @@ class Navier {
     ROWS(X, ld, nrows, m, s.nsets, s);
   }
   void BANDY(const DBand& b, const double* in, int ldi, double* out, int ldo, int rows, double beta) {
-    timer.begin(st, 9, 8.0 * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
-    band_y(st, in, ldi, out, ldo, rows, b.mo, b.coef.p, b.mask.p, beta);
+    timer.begin(st, 9, 8.0 * nb * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
+    band_y(st, in, ldi, out, ldo, rows, b.mo, b.coef.p, b.mask.p, beta, nb, bs, bs);
     timer.end(st);
   }
   void BANDX(const DBand& b, const double* in, double* out, int cols, double beta) {
-    timer.begin(st, 3, 8.0 * (double)cols * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
-    band_x(st, in, out, cols, b.mo, b.coef.p, b.mask.p, beta);
+    timer.begin(st, 3, 8.0 * nb * (double)cols * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
+    band_x(st, in, out, cols, b.mo, b.coef.p, b.mask.p, beta, nb, bs, bs);
     timer.end(st);
   }
   // cached stencil bands per basis
@@ -580,8 +599,8 @@ class Navier {
     return ref;
   }
   void FDIFF(const double* X, double* Y, int m, int cols, int ld, int times, double sc) {
-    timer.begin(st, 3, 8.0 * 4.0 * m * cols);
-    fourier_diff(st, X, Y, m, cols, ld, times, sc);
+    timer.begin(st, 3, 8.0 * 4.0 * m * cols * nb);
+    for (int b = 0; b < nb; ++b) fourier_diff(st, X + b * bs, Y + b * bs, m, cols, ld, times, sc);
     timer.end(st);
   }
 
@@ -589,7 +608,19 @@ class Navier {
   int rows_spec(const Basis& b) const { return periodic ? 2 * b.m : b.m; }
   int rows_ortho(const Basis& b) const { return periodic ? 2 * b.m : b.n; }
 
-  Navier(bool per, int nx_, int ny_, double ra_, double pr_, double dt_, double aspect_, bool adiabatic)
+  // Ensemble groups: `slabs` places every field and work buffer of this instance in slot `slot`
+  // of group-owned slabs (slot stride = the buffer size), `ext` is the group's stream and
+  // `share` an instance of the same shape whose Poisson solver (Ra-independent) is reused.
+  struct Slabs {
+    static constexpr int NK = 29;
+    size_t F = 0;
+    DBuf slab[NK];
+    double* at(int k, int slot) { return slab[k].p + (size_t)slot * F; }
+  };
+  bool owns_stream = true;
+
+  Navier(bool per, int nx_, int ny_, double ra_, double pr_, double dt_, double aspect_, bool adiabatic,
+         Slabs* slabs = nullptr, int slot = 0, hipStream_t ext = nullptr, const Navier* share = nullptr)
       : periodic(per), nx(nx_), ny(ny_), ra(ra_), pr(pr_), dt(dt_), aspect(aspect_),
         bx_u(per ? FO : CD, nx_), bx_t(per ? FO : (adiabatic ? CN : CD), nx_), bx_p(per ? FO : CHEB, nx_),
         bx_phi(per ? FO : CN, nx_), bx_f(per ? FO : CHEB, nx_), by_u(CD, ny_), by_p(CHEB, ny_), by_phi(CN, ny_),
@@ -601,23 +632,36 @@ class Navier {
     // get_nu / get_ka with height 2 (navier.rs:46-55, 231-232)
     nu = std::sqrt(pr / (ra / std::pow(scale[1] * 2.0, 3.0)));
     ka = std::sqrt(1.0 / ((ra / std::pow(scale[1] * 2.0, 3.0)) * pr));
-    HIPCHECK(hipStreamCreate(&st));
+    if (ext) {
+      st = ext;
+      owns_stream = false;
+    } else {
+      HIPCHECK(hipStreamCreate(&st));
+    }
     const int R = std::max(nx, periodic ? 2 * mx() : nx);
     const size_t full = (size_t)R * ny;
     FS = full;
-    // slot-strided buffers of the batched step (slot k at p + k FS)
-    STATE.alloc(3 * full);
-    UX.view(STATE.p, full);
-    UY.view(STATE.p + full, full);
-    T.view(STATE.p + 2 * full, full);
-    PHYS2.alloc(2 * full);
-    UXP.view(PHYS2.p, full);
-    UYP.view(PHYS2.p + full, full);
-    for (DBuf* b : {&A, &CONV, &CV, &RHS, &W1, &W2}) b->alloc(3 * full);
-    G.alloc(2 * full);
-    for (DBuf* b : {&P0, &P1, &Tv, &UXv, &UYv, &P0v, &P1v, &THAT_BC, &DTBCX, &DTBCY, &GXX, &GYY, &THAT, &DIV, &W3,
-                    &W4, &W9})
-      b->alloc(full);
+    if (slabs) {  // ensemble member: one slot of every group slab
+      if (per) throw std::invalid_argument("ensemble instances are confined");
+      if (slabs->F != full) throw std::invalid_argument("ensemble slab stride mismatch");
+      DBuf* all[Slabs::NK] = {&UX, &UY, &T, &UXP, &UYP, &A, &CONV, &CV, &RHS, &W1, &W2, &G, &P0, &P1, &Tv,
+                              &UXv, &UYv, &P0v, &P1v, &THAT_BC, &DTBCX, &DTBCY, &GXX, &GYY, &THAT, &DIV, &W3, &W4, &W9};
+      for (int k = 0; k < Slabs::NK; ++k) all[k]->view(slabs->at(k, slot), full);
+    } else {
+      // slot-strided buffers of the batched step (slot k at p + k FS)
+      STATE.alloc(3 * full);
+      UX.view(STATE.p, full);
+      UY.view(STATE.p + full, full);
+      T.view(STATE.p + 2 * full, full);
+      PHYS2.alloc(2 * full);
+      UXP.view(PHYS2.p, full);
+      UYP.view(PHYS2.p + full, full);
+      for (DBuf* b : {&A, &CONV, &CV, &RHS, &W1, &W2}) b->alloc(3 * full);
+      G.alloc(2 * full);
+      for (DBuf* b : {&P0, &P1, &Tv, &UXv, &UYv, &P0v, &P1v, &THAT_BC, &DTBCX, &DTBCY, &GXX, &GYY, &THAT, &DIV, &W3,
+                      &W4, &W9})
+        b->alloc(full);
+    }
     BC.alloc((size_t)R * 2);
     BCv.alloc((size_t)nx * ny);
     RED.alloc(std::max(ny, 8) + 300);
@@ -629,7 +673,7 @@ class Navier {
     solver[0] = make_solver(hk, bx_u, by_u, cv);
     solver[1] = make_solver(hk, bx_u, by_u, cv);
     solver[2] = make_solver(hk, bx_t, by_u, ck);
-    solver[3] = make_solver(POISSON, bx_phi, by_phi, cp);
+    solver[3] = share ? share->solver[3] : std::shared_ptr<Solver>(make_solver(POISSON, bx_phi, by_phi, cp));
     // averaging weights of `field` (its coordinates are not rescaled, navier.rs:500-515)
     wx_dx = cell_widths(bx_f.coords(), bx_f.fourier());
     wy_dx = cell_widths(by_p.coords(), false);
@@ -642,7 +686,7 @@ class Navier {
   }
   ~Navier() {
     drop_graph();
-    if (st) (void)hipStreamDestroy(st);
+    if (st && owns_stream) (void)hipStreamDestroy(st);
   }
 
   static std::vector<double> cell_widths(const std::vector<double>& x, bool periodic) {  // field.rs:135-163
@@ -685,7 +729,7 @@ class Navier {
       if (name == "BDS") {
         FDIFF(in, tmp, m, cols, cols, 1, scale[0]);
         const DOp& op = X.get(b, "BS");
-        GEMM(op.rows, cols, op.cols, op.b.p, op.cols, tmp, cols, out, cols);
+        GEMM(op.rows, cols, op.cols, op.b.p, op.cols, tmp, cols, out, cols, 0.0, nullptr, 0, 0, bs);
         return out;
       }
     } else if (b.k == CHEB && (name == "S" || name == "FO")) {
@@ -696,11 +740,13 @@ class Navier {
       return out;
     }
     const DOp& op = X.get(b, name);
-    GEMM(op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols);
+    GEMM(op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 0.0, nullptr, 0, 0, bs);
     return out;
   }
-  void xmat(const DOp& op, const double* in, int cols, double* out) {
-    GEMM(op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols);
+  void xmat(const DOp& op, const double* in, int cols, double* out, const double* per_inst = nullptr,
+            long sop = 0) {  // per_inst: the group's operators, sop apart
+    GEMM(op.rows, cols, op.cols, per_inst ? per_inst : op.b.p, op.cols, in, cols, out, cols, 0.0, nullptr, 0,
+         per_inst ? sop : 0, bs);
   }
   // y (axis 1): in [rows, cin] -> out [rows, cout]; out = (in Y^T) .* U + beta out
   const double* yop(const Basis& b, const std::string& name, const double* in, int rows, double* out,
@@ -713,17 +759,18 @@ class Navier {
     if (use_dct && use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) &&
         cheb_y_supported(b.n)) {
       const DBand* band = b.k != CHEB ? &stencil_band(b) : nullptr;
-      timer.begin(st, 7, 8.0 * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
-      cheb_y_diff(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr, 1.0 / scale[1], beta);
+      timer.begin(st, 7, 8.0 * nb * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
+      cheb_y_diff(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr, 1.0 / scale[1], beta, nb, bs,
+                  bs);
       timer.end(st);
       return out;
     }
     if (use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) && cheb_diff_y_supported(b.n)) {
       const bool comp = b.k != CHEB;
       const DBand* band = comp ? &stencil_band(b) : nullptr;
-      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
+      timer.begin(st, 3, 8.0 * nb * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
       cheb_diff_y(st, in, b.m, out, b.n, rows, b.n, comp ? band->coef.p : nullptr, comp ? band->mask.p : nullptr, 1.0,
-                  beta);
+                  beta, nb, bs, bs);
       timer.end(st);
       return out;
     }
@@ -731,39 +778,41 @@ class Navier {
       if (name == "BS" || name == "BDS") {
         const DBand* band = b.k == CHEB ? nullptr : &stencil_band(b);
         const int times = name == "BDS" ? 1 : 0;
-        timer.begin(st, 4, 8.0 * (double)rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
+        timer.begin(st, 4,
+                    8.0 * nb * (double)rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
         cheb_y_backward(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr,
-                        band ? band->mask.p : nullptr, times, 1.0 / scale[1], U, b.n, beta);
+                        band ? band->mask.p : nullptr, times, 1.0 / scale[1], U, b.n, beta, nb, bs, bs, U ? bs : 0);
         timer.end(st);
         return out;
       }
       if ((name == "Fdl" || name == "FS") && b.k == CHEB && !U && beta == 0.0) {
-        timer.begin(st, 6, 8.0 * (double)rows * 2.0 * b.n);
-        cheb_y_forward(st, in, b.n, out, b.n, rows, b.n, name == "Fdl" ? b.n * 2 / 3 : b.n);
+        timer.begin(st, 6, 8.0 * nb * (double)rows * 2.0 * b.n);
+        cheb_y_forward(st, in, b.n, out, b.n, rows, b.n, name == "Fdl" ? b.n * 2 / 3 : b.n, nb, bs, bs);
         timer.end(st);
         return out;
       }
     }
     if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_rows_lds_supported(b.m)) {
       const DTdma& t = tdma_tables(b);
-      timer.begin(st, 8, 8.0 * (double)rows * ((double)b.n + b.m));
-      tdma_rows_lds(st, in, b.n, out, b.m, rows, b.m, t.tabp.p);
+      timer.begin(st, 8, 8.0 * nb * (double)rows * ((double)b.n + b.m));
+      tdma_rows_lds(st, in, b.n, out, b.m, rows, b.m, t.tabp.p, nb, bs, bs);
       timer.end(st);
       return out;
     }
     if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_y_supported(b.m)) {
       const DTdma& t = tdma_tables(b);
-      timer.begin(st, 3, 8.0 * (double)rows * ((double)b.n + b.m));
-      tdma_y(st, in, b.n, out, b.m, rows, b.m, t.diag.p, t.low2.p, t.off.p, t.w.p, t.den.p);
+      timer.begin(st, 3, 8.0 * nb * (double)rows * ((double)b.n + b.m));
+      tdma_y(st, in, b.n, out, b.m, rows, b.m, t.diag.p, t.low2.p, t.off.p, t.w.p, t.den.p, nb, bs, bs);
       timer.end(st);
       return out;
     }
     const DOp& op = Y.get(b, name);
-    GEMM(rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, beta, U, op.cols);
+    GEMM(rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, beta, U, op.cols, bs, 0);
     return out;
   }
-  void ymat(const DOp& op, const double* in, int rows, double* out) {
-    GEMM(rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols);
+  void ymat(const DOp& op, const double* in, int rows, double* out, const double* per_inst = nullptr, long sop = 0) {
+    GEMM(rows, op.cols, op.rows, in, op.rows, per_inst ? per_inst : op.b.p, op.cols, out, op.cols, 0.0, nullptr, 0, bs,
+         per_inst ? sop : 0);
   }
 
   // ------------------------------------------------------------------ set-up pieces
@@ -820,8 +869,9 @@ class Navier {
     Solver& s = *solver[which];
     const int my = s.by.m;
     if (s.kind == ADI) {
-      xmat(s.hx, rhs, ny, W1.p);         // [mx, ny]
-      ymat(s.hy, W1.p, s.hx.rows, out);  // [mx, my]
+      const bool grp = nb > 1 && which < 3;  // the group's per-instance operators
+      xmat(s.hx, rhs, ny, W1.p, grp ? ADIH[which][0].p : nullptr, adi_s[which][0]);         // [mx, ny]
+      ymat(s.hy, W1.p, s.hx.rows, out, grp ? ADIH[which][1].p : nullptr, adi_s[which][1]);  // [mx, my]
       return;
     }
     if (!s.has_fb) {  // Fourier x: P_y then the per-wavenumber row systems
@@ -848,7 +898,8 @@ class Navier {
   void to_ortho(const Basis& bx, const Basis& by, const double* q, double* out) {
     const double* a = xop(bx, "S", q, by.m, W1.p, W9.p);
     const double* r = yop(by, "S", a, rows_ortho(bx), out);
-    if (r != out) HIPCHECK(hipMemcpyAsync(out, r, sizeof(double) * rows_ortho(bx) * by.n, hipMemcpyDeviceToDevice, st));
+    if (r != out)
+      HIPCHECK(hipMemcpyAsync(out, r, sizeof(double) * nbn((long)rows_ortho(bx) * by.n), hipMemcpyDeviceToDevice, st));
   }
 
   void solve_vel(int which, double* U, const double* buoy) {  // navier.rs:622-654
@@ -858,7 +909,7 @@ class Navier {
     else yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p);      // d/dy pres
     conv_pair(bx_u, U);
     forward_dealias_conv();
-    ew_rhs_vel(st, RHS.p, A.p, G.p, buoy, CV.p, dt, no);
+    ew_rhs_vel(st, RHS.p, A.p, G.p, buoy, CV.p, dt, nbn(no));
     solve(which, RHS.p, U);
   }
   void divergence(double* out) {  // navier.rs:698-703
@@ -876,33 +927,34 @@ class Navier {
     const double* g = yop(by_phi, "S", a, ro, W2.p);
     const double* f = xop(bx_u, "FO", g, ny, W3.p, W9.p);
     yop(by_u, "FO", f, rows_spec(bx_u), W4.p);
-    ew_project(st, UX.p, W4.p, 1.0, ns);
+    ew_project(st, UX.p, W4.p, 1.0, nbn(ns));
     // uy
     a = xop(bx_phi, "S", P1.p, myp, W1.p, W9.p);
     g = yop(by_phi, "DS", a, ro, W2.p);
     f = xop(bx_u, "FO", g, ny, W3.p, W9.p);
     yop(by_u, "FO", f, rows_spec(bx_u), W4.p);
-    ew_project(st, UY.p, W4.p, 1.0, ns);
+    ew_project(st, UY.p, W4.p, 1.0, nbn(ns));
   }
   void solve_pres() {  // navier.rs:710-715
     solve(3, DIV.p, P1.p);
-    if (periodic) set_zero_entries(st, P1.p, 2, 0, (long)bx_phi.m * by_phi.m);
-    else set_zero_entries(st, P1.p, 1, 0, 0);
+    if (periodic) set_zero_entries(st, P1.p, 2, 0, (long)bx_phi.m * by_phi.m, nb, bs);
+    else set_zero_entries(st, P1.p, 1, 0, 0, nb, bs);
   }
   void update_pres() {  // navier.rs:717-721
     const long no = (long)rows_ortho(bx_p) * ny;
     to_ortho(bx_phi, by_phi, P1.p, W3.p);
-    ew_update_pres(st, P0.p, DIV.p, W3.p, nu, 1.0 / dt, no);
+    if (nb > 1) ew_update_pres_group(st, P0.p, DIV.p, W3.p, NUARR.p, 1.0 / dt, nbn(no), bs);
+    else ew_update_pres(st, P0.p, DIV.p, W3.p, nu, 1.0 / dt, no);
   }
   void solve_temp() {  // navier.rs:660-674
     const long no = (long)rows_ortho(bx_f) * ny;
     const long np = (long)nx * ny;
     to_ortho(bx_t, by_u, T.p, A.p);
     conv_pair(bx_t, T.p);
-    ew_acc_prod(st, CONV.p, UXP.p, DTBCX.p, np);
-    ew_acc_prod(st, CONV.p, UYP.p, DTBCY.p, np);
+    ew_acc_prod(st, CONV.p, UXP.p, DTBCX.p, nbn(np));
+    ew_acc_prod(st, CONV.p, UYP.p, DTBCY.p, nbn(np));
     forward_dealias_conv();
-    ew_rhs_temp(st, RHS.p, A.p, GXX.p, GYY.p, CV.p, dt, no);
+    ew_rhs_temp(st, RHS.p, A.p, GXX.p, GYY.p, CV.p, dt, nbn(no));
     solve(2, RHS.p, T.p);
   }
   void backward_vel(const double* U, double* out) {
@@ -911,7 +963,7 @@ class Navier {
   }
   // ----------------------------------------------------------- batched periodic step
   bool batch_ok() const {
-    return use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
+    return nb == 1 && use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
            use_bands && fdma_rows_lds_supported(by_u.m, by_u.n) && tdma_rows_lds_supported(by_u.m) &&
            by_phi.n == by_u.n;
   }
@@ -1067,7 +1119,7 @@ class Navier {
     }
     const long no = (long)rows_ortho(bx_f) * ny;
     to_ortho(bx_t, by_u, T.p, W3.p);
-    ew_add(st, THAT.p, W3.p, THAT_BC.p, no);
+    ew_add(st, THAT.p, W3.p, THAT_BC.p, nbn(no));
     backward_vel(UX.p, UXP.p);
     backward_vel(UY.p, UYP.p);
     solve_vel(0, UX.p, nullptr);
@@ -1419,6 +1471,93 @@ class SolverDev {
   }
 };
 
+// ----------------------------------------------------------------------- ensembles
+// Independent confined instances (C4-style ensembles).  Instances with the same shape, Pr,
+// dt, aspect and sidewall condition form a group: their fields and work arrays are slots of
+// group slabs and the group's first instance (the leader) advances all of them with one
+// set of launches (batched GEMMs over shared operators, per-instance ADI operators and nu;
+// batched lane kernels; element-wise kernels over the whole slab), captured once as a HIP
+// graph.  Groups run on their own streams.
+class Ensemble {
+ public:
+  struct Group {
+    std::unique_ptr<Navier::Slabs> slabs;
+    std::vector<std::unique_ptr<Navier>> navs;  // navs[0] is the leader
+    hipStream_t st = nullptr;
+    ~Group() {
+      if (st) (void)hipStreamSynchronize(st);
+      navs.clear();
+      slabs.reset();
+      if (st) (void)hipStreamDestroy(st);
+    }
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  std::vector<Navier*> inst;
+  std::vector<int> group_of;
+
+  Ensemble(int n, const rbc_desc* d) {
+    if (n < 1 || !d) throw std::invalid_argument("ensemble: no instances");
+    std::map<std::tuple<int, int, double, double, double, int>, std::vector<int>> keys;
+    for (int i = 0; i < n; ++i) {
+      if (d[i].nx < 8 || d[i].ny < 8) throw std::invalid_argument("ensemble: nx and ny must be >= 8");
+      keys[std::make_tuple(d[i].nx, d[i].ny, d[i].pr, d[i].dt, d[i].aspect, d[i].adiabatic != 0)].push_back(i);
+    }
+    inst.assign(n, nullptr);
+    group_of.assign(n, -1);
+    for (auto& kv : keys) {
+      const std::vector<int>& mem = kv.second;
+      const int B = (int)mem.size();
+      const rbc_desc& d0 = d[mem[0]];
+      auto g = std::make_unique<Group>();
+      HIPCHECK(hipStreamCreate(&g->st));
+      g->slabs = std::make_unique<Navier::Slabs>();
+      g->slabs->F = (size_t)d0.nx * d0.ny;
+      for (int k = 0; k < Navier::Slabs::NK; ++k) g->slabs->slab[k].alloc(g->slabs->F * B);
+      for (int b = 0; b < B; ++b) {
+        const rbc_desc& di = d[mem[b]];
+        g->navs.push_back(std::make_unique<Navier>(false, di.nx, di.ny, di.ra, di.pr, di.dt, di.aspect,
+                                                   di.adiabatic != 0, g->slabs.get(), b, g->st,
+                                                   b > 0 ? g->navs[0].get() : nullptr));
+        inst[mem[b]] = g->navs.back().get();
+        group_of[mem[b]] = (int)groups.size();
+      }
+      Navier& L = *g->navs[0];
+      if (B > 1) {
+        L.nb = B;
+        L.bs = (long)g->slabs->F;
+        std::vector<double> nus(B);
+        for (int b = 0; b < B; ++b) nus[b] = g->navs[b]->nu;
+        L.NUARR.upload(nus);
+        for (int w = 0; w < 3; ++w) {  // ADI operators of ux, uy, temp per instance
+          for (int ax = 0; ax < 2; ++ax) {
+            const DOp& op0 = ax == 0 ? L.solver[w]->hx : L.solver[w]->hy;
+            const size_t sz = (size_t)op0.rows * op0.cols;
+            L.ADIH[w][ax].alloc(sz * B);
+            L.adi_s[w][ax] = (long)sz;
+            for (int b = 0; b < B; ++b) {
+              const DOp& op = ax == 0 ? g->navs[b]->solver[w]->hx : g->navs[b]->solver[w]->hy;
+              HIPCHECK(hipMemcpy(L.ADIH[w][ax].p + b * sz, op.b.p, sz * sizeof(double), hipMemcpyDeviceToDevice));
+            }
+          }
+        }
+      }
+      groups.push_back(std::move(g));
+    }
+  }
+  // every instance advances nsteps steps; the groups run concurrently
+  void update(int nsteps) {
+    for (auto& g : groups) g->navs[0]->update_n(nsteps);
+    for (auto& g : groups) {
+      HIPCHECK(hipStreamSynchronize(g->st));
+      const double t = g->navs[0]->time;
+      for (auto& nv : g->navs) nv->time = t;
+    }
+  }
+  void synchronize() {
+    for (auto& g : groups) HIPCHECK(hipStreamSynchronize(g->st));
+  }
+};
+
 }  // namespace rbc
 
 // =================================================================== C ABI
@@ -1426,7 +1565,13 @@ using rbc::Navier;
 using rbc::Space;
 
 struct rbc_navier {
-  std::unique_ptr<Navier> n;
+  std::unique_ptr<Navier> own;
+  Navier* n = nullptr;
+  bool member = false;  // an ensemble instance: owned and advanced by its ensemble
+};
+struct rbc_ensemble {
+  std::unique_ptr<rbc::Ensemble> e;
+  std::vector<std::unique_ptr<rbc_navier>> inst;
 };
 struct fs_space2 {
   std::unique_ptr<Space> s;
@@ -1473,7 +1618,8 @@ int rbc_create_confined(int nx, int ny, double ra, double pr, double dt, double 
     if (!out) throw std::invalid_argument("null out");
     auto* h = new rbc_navier();
     try {
-      h->n = std::make_unique<Navier>(false, nx, ny, ra, pr, dt, aspect, adiabatic != 0);
+      h->own = std::make_unique<Navier>(false, nx, ny, ra, pr, dt, aspect, adiabatic != 0);
+      h->n = h->own.get();
     } catch (...) {
       delete h;
       throw;
@@ -1486,7 +1632,8 @@ int rbc_create_periodic(int nx, int ny, double ra, double pr, double dt, double 
     if (!out) throw std::invalid_argument("null out");
     auto* h = new rbc_navier();
     try {
-      h->n = std::make_unique<Navier>(true, nx, ny, ra, pr, dt, aspect, true);
+      h->own = std::make_unique<Navier>(true, nx, ny, ra, pr, dt, aspect, true);
+      h->n = h->own.get();
     } catch (...) {
       delete h;
       throw;
@@ -1495,7 +1642,10 @@ int rbc_create_periodic(int nx, int ny, double ra, double pr, double dt, double 
   });
 }
 int rbc_destroy(rbc_navier* h) {
-  return guarded([&] { delete h; });
+  return guarded([&] {
+    if (h && h->member) throw std::invalid_argument("an ensemble instance is destroyed with its ensemble");
+    delete h;
+  });
 }
 int rbc_field_shape(rbc_navier* h, int which, int space, int* n0, int* n1, int* is_complex) {
   return guarded([&] {
@@ -1560,6 +1710,7 @@ int rbc_set_temperature(rbc_navier* h, double amp, double m, double n) {
 int rbc_update(rbc_navier* h, int nsteps) {
   return guarded([&] {
     NEED(h);
+    if (h->member) throw std::invalid_argument("ensemble instances advance with rbc_ensemble_update");
     if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
     h->n->update_n(nsteps);
   });
@@ -1798,6 +1949,51 @@ int rbc_solver_set_tensor(rbc_solver* h, const double* lam, const double* fwd, c
       std::memcpy(sv.bwd.data(), bwd, sv.bwd.size() * sizeof(double));
     }
     sv.build_rows();
+  });
+}
+
+int rbc_create_ensemble(int n, const rbc_desc* desc, rbc_ensemble** out) {
+  return guarded([&] {
+    if (!out || !desc) throw std::invalid_argument("null argument");
+    auto h = std::make_unique<rbc_ensemble>();
+    h->e = std::make_unique<rbc::Ensemble>(n, desc);
+    for (int i = 0; i < n; ++i) {
+      auto nv = std::make_unique<rbc_navier>();
+      nv->n = h->e->inst[i];
+      nv->member = true;
+      h->inst.push_back(std::move(nv));
+    }
+    *out = h.release();
+  });
+}
+int rbc_ensemble_destroy(rbc_ensemble* h) {
+  return guarded([&] { delete h; });
+}
+int rbc_ensemble_size(rbc_ensemble* h, int* n, int* groups) {
+  return guarded([&] {
+    if (!h || !h->e) throw std::invalid_argument("null handle");
+    if (n) *n = (int)h->inst.size();
+    if (groups) *groups = (int)h->e->groups.size();
+  });
+}
+int rbc_ensemble_instance(rbc_ensemble* h, int inst, rbc_navier** out) {
+  return guarded([&] {
+    if (!h || !h->e || !out) throw std::invalid_argument("null argument");
+    if (inst < 0 || inst >= (int)h->inst.size()) throw std::invalid_argument("instance index");
+    *out = h->inst[inst].get();
+  });
+}
+int rbc_ensemble_update(rbc_ensemble* h, int nsteps) {
+  return guarded([&] {
+    if (!h || !h->e) throw std::invalid_argument("null handle");
+    if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
+    h->e->update(nsteps);
+  });
+}
+int rbc_ensemble_synchronize(rbc_ensemble* h) {
+  return guarded([&] {
+    if (!h || !h->e) throw std::invalid_argument("null handle");
+    h->e->synchronize();
   });
 }
 

@@ -329,15 +329,32 @@ void ew_nuvol(hipStream_t s, double* out, const double* d, const double* t, cons
   EW_LAUNCH(ew_nuvol, n, out, d, t, uy, ka, sy);
 }
 
-__global__ void set_zero_kernel(double* x, int n_idx, long i0, long i1) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    x[i0] = 0.0;
-    if (n_idx > 1) x[i1] = 0.0;
-  }
+__global__ void set_zero_kernel(double* x, int n_idx, long i0, long i1, int batch, long stride) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  double* y = x + b * stride;
+  y[i0] = 0.0;
+  if (n_idx > 1) y[i1] = 0.0;
 }
-void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1) {
-  RBC_LAUNCH(set_zero_kernel, dim3(1), dim3(64), 0, s, x, n_idx, i0, i1);
+void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1, int batch, long stride) {
+  RBC_LAUNCH(set_zero_kernel, dim3((batch + 63) / 64), dim3(64), 0, s, x, n_idx, i0, i1, batch, stride);
   check_launch("set_zero_entries");
+}
+
+__global__ void ew_update_pres_group_kernel(double* p, const double* d, const double* phi, const double* nu,
+                                            double inv_dt, long n, long stride) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q = p[i] - d[i] * nu[i / stride];
+  p[i] = q + phi[i] * inv_dt;
+}
+void ew_update_pres_group(hipStream_t s, double* p, const double* div, const double* phi, const double* nu,
+                          double inv_dt, long n, long stride) {
+  if (n <= 0) return;
+  RBC_LAUNCH(ew_update_pres_group_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, div, phi, nu, inv_dt, n,
+             stride);
+  check_launch("ew_update_pres_group");
 }
 
 // ------------------------------------------------------------------------- reductions

@@ -101,7 +101,11 @@ void ew_project(hipStream_t s, double* u, const double* f, double c, long n);   
 void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt,
                     long n);  // (p - div nu) + phi inv_dt
 void ew_scale(hipStream_t s, double* out, const double* a, double f, long n);              // a f
-void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1);              // x[i0] = x[i1] = 0
+// x[i0] = x[i1] = 0 (n_idx = 1: only x[i0]), in each of `batch` slots `stride` apart
+void set_zero_entries(hipStream_t s, double* x, int n_idx, long i0, long i1, int batch = 1, long stride = 0);
+// pressure update of an ensemble group: nu of entry i is nu[i / stride]
+void ew_update_pres_group(hipStream_t s, double* p, const double* div, const double* phi, const double* nu,
+                          double inv_dt, long n, long stride);
 
 // Weighted column sums: out[j] = sum_i v[i, j] * w[i]  (i sequential, as average.rs:25-34)
 void col_weighted_sum(hipStream_t s, const double* v, int n0, int n1, const double* w, double* out);

@@ -99,9 +99,9 @@ class Navier2D:
         return cls(h, True, nx, ny, ra, pr, dt, aspect)
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and not getattr(self, "_borrowed", False):
             _lib.load().rbc_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
