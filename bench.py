@@ -41,7 +41,8 @@ WORKLOAD_NAMES = {
 # timing classes of rbc_timing_read; all but 3 (the remaining timed kernels) are one kernel each
 KERNEL_CLASSES = {0: "gemm_mfma_kernel", 1: "fdma_rows_lds_kernel", 2: "irfft_cols_kernel", 3: "other kernels",
                   4: "cheb_y_bwd_kernel", 5: "rfft_cols_kernel", 6: "cheb_y_fwd_kernel", 7: "cheb_y_diff_kernel",
-                  8: "tdma_rows_lds_kernel", 9: "band_y_kernel", 10: "pressure_rows_kernel"}
+                  8: "tdma_rows_lds_kernel", 9: "band_y_kernel", 10: "pressure_rows_kernel",
+                  11: "cheb_y_conv_kernel"}
 MIXED_CLASS = 3
 
 

@@ -137,7 +137,8 @@ int rbc_set_option(rbc_navier* h, int option, int value);
  *   (Hholtz/Poisson rows), RBC_K_FFT irfft_cols_kernel, RBC_K_RFFT rfft_cols_kernel,
  *   RBC_K_DCT cheb_y_bwd_kernel, RBC_K_DCT_FWD cheb_y_fwd_kernel, RBC_K_DIFF_Y
  *   cheb_y_diff_kernel, RBC_K_TDMA tdma_rows_lds_kernel, RBC_K_BAND band_y_kernel;
- *   RBC_K_PRESSURE pressure_rows_kernel (the fused pressure stage);
+ *   RBC_K_PRESSURE pressure_rows_kernel (the fused pressure stage); RBC_K_CONV
+ *   cheb_y_conv_kernel (the convective term's y transforms and products, one launch);
  *   RBC_K_ELEMENTWISE the remaining timed kernels (Fourier derivative, fallbacks). */
 #define RBC_K_GEMM 0
 #define RBC_K_ROWSOLVE 1
@@ -150,6 +151,7 @@ int rbc_set_option(rbc_navier* h, int option, int value);
 #define RBC_K_TDMA 8
 #define RBC_K_BAND 9
 #define RBC_K_PRESSURE 10
+#define RBC_K_CONV 11
 int rbc_timing_enable(rbc_navier* h, int on);
 int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work);
 

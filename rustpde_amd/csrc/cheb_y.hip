@@ -119,20 +119,30 @@ struct ExtraProducts {
   int slot;  // batch entry that receives them (-1: none)
 };
 
-template <int L, bool transform, typename Tin, typename V, typename Tout>
+// DUAL: the two components of a complex transform are row r of `in` (no derivative) and
+// row r of `in2` (one derivative, times = 1), and the epilogue forms the convective term
+//     out_r = d/dy(in2)_r .* U2_r + 1 (x_r .* U_r)
+// with the operation order of two separate launches (out = x .* U, then out = y .* U2 +
+// beta out with beta = 1).  Only the pairing of rows inside the complex transform differs
+// from those launches, which moves the results by rounding only.
+template <int L, bool transform, bool DUAL, typename Tin, typename V, typename Tout>
 __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out,
                                                 int ldo, int rows, const double* __restrict__ coef, int times,
                                                 double inv_sc_d, const Tout* __restrict__ U, int ldu, double beta_d,
                                                 const double* __restrict__ tw, long si, long so, long su,
-                                                ExtraProducts<Tout> xp) {
+                                                ExtraProducts<Tout> xp, const Tin* __restrict__ in2 = nullptr,
+                                                const Tout* __restrict__ U2 = nullptr) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   const V inv_sc = (V)inv_sc_d, beta = (V)beta_d;
   in += si * blockIdx.y;
+  if constexpr (DUAL) in2 += si * blockIdx.y;
   out += so * blockIdx.y;
   if (U) U += su * blockIdx.y;
-  const int tid = threadIdx.x, row0 = 2 * P * blockIdx.x;
+  // component a: times_a derivatives, component b: times_b
+  const int times_a = DUAL ? 0 : times, times_b = DUAL ? 1 : times;
+  const int tid = threadIdx.x, row0 = (DUAL ? 1 : 2) * P * blockIdx.x;
   // (1) Chebyshev coefficients of both rows, or g_k of the derivative recurrence:
   //     g_0 = s_1, g_k = 2 (k + 1) s_{k+1} (1 <= k <= n - 2), g_{n-1} = 0.
   //     All loads of a thread are issued before any is used.
@@ -142,10 +152,11 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   for (int q = 0; q < Q; ++q) {
     const int e = tid + q * T;
     const int p = e / n, k = e - p * n;
-    const int ra = row0 + 2 * p, rb = ra + 1;
-    const int kk = (e < P * n) ? (times ? k + 1 : k) : n;  // kk = n loads nothing
-    la[q] = stencil_load<V>(in + (size_t)ra * ldi, ra < rows, kk, mi, n, coef);
-    lb[q] = stencil_load<V>(in + (size_t)rb * ldi, rb < rows, kk, mi, n, coef);
+    const int ra = DUAL ? row0 + p : row0 + 2 * p, rb = DUAL ? ra : ra + 1;
+    const int ka = (e < P * n) ? (times_a ? k + 1 : k) : n;  // k = n loads nothing
+    const int kb = (e < P * n) ? (times_b ? k + 1 : k) : n;
+    la[q] = stencil_load<V>(in + (size_t)ra * ldi, ra < rows, ka, mi, n, coef);
+    lb[q] = stencil_load<V>((DUAL ? in2 : in) + (size_t)rb * ldi, rb < rows, kb, mi, n, coef);
   }
   const bool comp = coef != nullptr;
 #pragma unroll
@@ -154,15 +165,13 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     if (e >= P * n) continue;
     const int p = e / n, k = e - p * n;
     V sa = stencil_value(la[q], comp), sb = stencil_value(lb[q], comp);
-    if (times) {
-      if (k == n - 1) {
-        sa = 0;
-        sb = 0;
-      } else if (k > 0) {
-        const V f = (V)(2.0 * (double)(k + 1));
-        sa = f * sa;
-        sb = f * sb;
-      }
+    if (times_a) {
+      if (k == n - 1) sa = 0;
+      else if (k > 0) sa = (V)(2.0 * (double)(k + 1)) * sa;
+    }
+    if (times_b) {
+      if (k == n - 1) sb = 0;
+      else if (k > 0) sb = (V)(2.0 * (double)(k + 1)) * sb;
     }
     re[lp(p * L + k)] = sa;
     im[lp(p * L + k)] = sb;
@@ -171,10 +180,11 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   if (times) {
     // (2) d_k = g_k + d_{k+2} for k >= 1 (suffix sums along the chains 1, 3, .., N-1 and
     //     2, 4, .., N), then d_0 = g_0 + d_2 / 2.  One wavefront per chain.
-    constexpr int C = N / 2, SEG = (C + 63) / 64, NCH = 4 * P;
+    //     DUAL: the chains of component b only.
+    constexpr int C = N / 2, SEG = (C + 63) / 64, NCH = (DUAL ? 2 : 4) * P;
     const int lane = tid & 63, wave = tid >> 6;
     for (int ch = wave; ch < NCH; ch += T / 64) {
-      const int p = ch >> 2, comp = (ch >> 1) & 1, par = ch & 1;
+      const int p = DUAL ? ch >> 1 : ch >> 2, comp = DUAL ? 1 : (ch >> 1) & 1, par = ch & 1;
       V* z = comp ? im : re;
       const int zb = p * L + (par ? 1 : 2);  // chain element t at z[lp(zb + 2 t)]
       const int t0 = lane * SEG;
@@ -208,10 +218,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   for (int e = tid; e < P * n && transform; e += T) {
     const int p = e / n, k = e - p * n;
     V a = re[lp(p * L + k)], b = im[lp(p * L + k)];
-    if (times && inv_sc != (V)1) {
-      a = a * inv_sc;
-      b = b * inv_sc;
-    }
+    if (times_a && inv_sc != (V)1) a = a * inv_sc;
+    if (times_b && inv_sc != (V)1) b = b * inv_sc;
     const V w = (V)(((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5));
     a = a * w;
     b = b * w;
@@ -229,6 +237,36 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   // (4) x_j = Re / Im Z_j, j <= N, or the coefficients d_j / sc without the transform;
   //     out = x .* U + beta out (loads first), plus the extra products on one entry
   const bool extra = xp.slot == (int)blockIdx.y;
+  if constexpr (DUAL) {
+    V ua[Q], ub[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int p = e / n, j = e - p * n;
+      const int r = row0 + p;
+      const bool ok = e < P * n && r < rows;
+      ua[q] = ok ? (V)U[(size_t)r * ldu + j] : (V)0;
+      ub[q] = ok ? (V)U2[(size_t)r * ldu + j] : (V)0;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      if (e >= P * n) continue;
+      const int p = e / n, j = e - p * n;
+      const int r = row0 + p;
+      if (r >= rows) continue;
+      const V xa = (V)(Tout)(re[lp(p * L + j)] * ua[q]);  // the value the first launch stores
+      V v = im[lp(p * L + j)] * ub[q];
+      v = v + (V)1 * xa;
+      if (extra) {
+        const size_t o = (size_t)r * ldu + j;
+        v = v + (V)xp.e1[o] * (V)xp.f1[o];
+        v = v + (V)xp.e2[o] * (V)xp.f2[o];
+      }
+      out[(size_t)r * ldo + j] = (Tout)v;
+    }
+    return;
+  }
   V ua[Q], ub[Q], oa[Q], ob[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -282,8 +320,21 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out, int ldo, int rows,
     const double* __restrict__ coef, int times, double inv_sc, const Tout* __restrict__ U, int ldu, double beta,
     const double* __restrict__ tw, long si, long so, long su, ExtraProducts<Tout> xp) {
-  cheb_y_bwd_body<L, true, Tin, V, Tout>(in, ldi, mi, out, ldo, rows, coef, times, inv_sc, U, ldu, beta, tw, si, so,
-                                         su, xp);
+  cheb_y_bwd_body<L, true, false, Tin, V, Tout>(in, ldi, mi, out, ldo, rows, coef, times, inv_sc, U, ldu, beta, tw, si,
+                                                so, su, xp);
+}
+// the convective term of a batch of fields from their x-derivatives (dx) and values (val),
+// both in x-physical / y-spectral form: out = d/dy(val) .* uy + dx .* ux (conv_term.rs:35-41)
+template <int L, typename Tin, typename V, typename Tout>
+__global__ __launch_bounds__(Geo<L>::T) void cheb_y_conv_kernel(const Tin* __restrict__ dx, const Tin* __restrict__ val,
+                                                                int ldi, int mi, Tout* __restrict__ out, int ldo,
+                                                                int rows, const double* __restrict__ coef,
+                                                                double inv_sc, const Tout* __restrict__ ux,
+                                                                const Tout* __restrict__ uy, int ldu,
+                                                                const double* __restrict__ tw, long si, long so,
+                                                                ExtraProducts<Tout> xp) {
+  cheb_y_bwd_body<L, true, true, Tin, V, Tout>(dx, ldi, mi, out, ldo, rows, coef, 1, inv_sc, ux, ldu, 1.0, tw, si, so,
+                                               0, xp, val, uy);
 }
 // the same without the transform: Chebyshev coefficients of the derivative
 template <int L>
@@ -291,7 +342,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_diff_kernel(const double* __
                                                                 double* __restrict__ out, int ldo, int rows,
                                                                 const double* __restrict__ coef, double inv_sc,
                                                                 double beta, long si, long so) {
-  cheb_y_bwd_body<L, false, double, double, double>(in, ldi, mi, out, ldo, rows, coef, 1, inv_sc, nullptr, 0, beta,
+  cheb_y_bwd_body<L, false, false, double, double, double>(in, ldi, mi, out, ldo, rows, coef, 1, inv_sc, nullptr, 0, beta,
                                                      nullptr, si, so, 0,
                                                      ExtraProducts<double>{nullptr, nullptr, nullptr, nullptr, -1});
 }
@@ -368,6 +419,16 @@ void launch_bwd(hipStream_t s, const Tin* in, int ldi, int mi, Tout* out, int ld
 }
 
 template <int L, typename Tin, typename V, typename Tout>
+void launch_conv(hipStream_t s, const Tin* dx, const Tin* val, int ldi, int mi, Tout* out, int ldo, int rows,
+                 const double* coef, double inv_sc, const Tout* ux, const Tout* uy, int ldu, int batch, long si, long so,
+                 ExtraProducts<Tout> xp) {
+  using G = Geo<L>;
+  const unsigned blocks = (unsigned)((rows + G::P - 1) / G::P);
+  RBC_LAUNCH((cheb_y_conv_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, dx, val, ldi, mi, out, ldo,
+             rows, coef, inv_sc, ux, uy, ldu, fft_table(L), si, so, xp);
+}
+
+template <int L, typename Tin, typename V, typename Tout>
 void launch_fwd(hipStream_t s, const Tin* in, int ldi, Tout* out, int ldo, int rows, int cut, int batch, long si,
                 long so) {
   using G = Geo<L>;
@@ -409,6 +470,35 @@ void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* o
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_BWD)
 #undef RBC_BWD
   check("cheb_y_backward");
+}
+
+void cheb_y_conv(hipStream_t s, const double* dx, const double* val, int ldi, int mi, double* out, int ldo, int rows,
+                 int n, const double* coef, double inv_sc, const double* ux, const double* uy, int ldu, int batch,
+                 long si, long so, const double* e1, const double* f1, const double* e2, const double* f2, int xslot) {
+  if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_conv: unsupported length");
+  if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_conv: bad batch or rows");
+  if (mi > n || (!coef && mi < 1) || !ux || !uy) throw std::invalid_argument("cheb_y_conv: bad arguments");
+#define RBC_CONV(LL)                                                                                            \
+  launch_conv<LL, double, double, double>(s, dx, val, ldi, mi, out, ldo, rows, coef, inv_sc, ux, uy, ldu, batch, si, \
+                                          so, ExtraProducts<double>{e1, f1, e2, f2, xslot})
+  RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_CONV)
+#undef RBC_CONV
+  check("cheb_y_conv");
+}
+
+void cheb_y_conv_f32(hipStream_t s, const float* dx, const float* val, int ldi, int mi, float* out, int ldo, int rows,
+                     int n, const double* coef, double inv_sc, const float* ux, const float* uy, int ldu, int batch,
+                     long si, long so, const float* e1, const double* f1, const float* e2, const double* f2,
+                     int xslot) {
+  if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_conv_f32: unsupported length");
+  if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_conv_f32: bad batch or rows");
+  if (mi > n || (!coef && mi < 1) || !ux || !uy) throw std::invalid_argument("cheb_y_conv_f32: bad arguments");
+#define RBC_CONV32(LL)                                                                                       \
+  launch_conv<LL, float, float, float>(s, dx, val, ldi, mi, out, ldo, rows, coef, inv_sc, ux, uy, ldu, batch, si, \
+                                       so, ExtraProducts<float>{e1, f1, e2, f2, xslot})
+  RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_CONV32)
+#undef RBC_CONV32
+  check("cheb_y_conv_f32");
 }
 
 void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,

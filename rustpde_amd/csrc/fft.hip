@@ -93,17 +93,29 @@ __global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const Tin* __rest
   }
 }
 
+// Batch entries b < split (all of them when split = 0) apply (i k)^times / sc and write Y;
+// entries b >= split read input b - split again, apply (i k)^times1 / sc1 and write Y1 (the
+// values and the x-derivatives of the same fields in one launch).
 template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
                                                                 int cols, int times, double sc_d,
-                                                                const double* __restrict__ tw, long sx, long sy) {
+                                                                const double* __restrict__ tw, long sx, long sy,
+                                                                Tout* __restrict__ Y1, int split, int times1,
+                                                                double sc1) {
 #pragma clang fp contract(off)
   using G = CGeo<L>;
   constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
+  int b = blockIdx.y;
+  if (split > 0 && b >= split) {
+    b -= split;
+    Y = Y1;
+    times = times1;
+    sc_d = sc1;
+  }
   const V sc = (V)sc_d;
-  X += sx * blockIdx.y;
-  Y += sy * blockIdx.y;
+  X += sx * b;
+  Y += sy * b;
   const int tid = threadIdx.x;
   const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
   // (1) half spectra, (i k)^times / sc applied, packed per pair: re[k] = Re A_k (k <= N),
@@ -198,11 +210,12 @@ void launch_rfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int cut, int ba
              fft_table(L), sx, sy);
 }
 template <int L, typename Tin, typename V, typename Tout>
-void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, double sc, int batch, long sx, long sy) {
+void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, double sc, int batch, long sx, long sy,
+                  Tout* Y1 = nullptr, int split = 0, int times1 = 0, double sc1 = 1.0) {
   using G = CGeo<L>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
   RBC_LAUNCH((irfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
-             fft_table(L), sx, sy);
+             fft_table(L), sx, sy, Y1, split, times1, sc1);
 }
 
 }  // namespace
@@ -241,6 +254,28 @@ void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int 
   RBC_FFT_SWITCH(n, RBC_IRFFT)
 #undef RBC_IRFFT
   check("irfft_cols");
+}
+
+void irfft_cols_pair(hipStream_t s, const double* X, double* Y0, double* Y1, int n, int cols, int times1, double sc1,
+                     int nin, long sx, long sy) {
+  check_n(n);
+  if (nin < 1 || 2 * nin > 65535 || cols < 1 || !Y1) throw std::invalid_argument("irfft_cols_pair: bad arguments");
+#define RBC_IRFFTP(LL) \
+  launch_irfft<LL, double, double, double>(s, X, Y0, cols, 0, 1.0, 2 * nin, sx, sy, Y1, nin, times1, sc1)
+  RBC_FFT_SWITCH(n, RBC_IRFFTP)
+#undef RBC_IRFFTP
+  check("irfft_cols_pair");
+}
+
+void irfft_cols_pair_f32(hipStream_t s, const double* X, float* Y0, float* Y1, int n, int cols, int times1, double sc1,
+                         int nin, long sx, long sy) {
+  check_n(n);
+  if (nin < 1 || 2 * nin > 65535 || cols < 1 || !Y1) throw std::invalid_argument("irfft_cols_pair_f32: bad arguments");
+#define RBC_IRFFTP32(LL) \
+  launch_irfft<LL, double, float, float>(s, X, Y0, cols, 0, 1.0, 2 * nin, sx, sy, Y1, nin, times1, sc1)
+  RBC_FFT_SWITCH(n, RBC_IRFFTP32)
+#undef RBC_IRFFTP32
+  check("irfft_cols_pair_f32");
 }
 
 void rfft_cols_f32(hipStream_t s, const float* X, double* Y, int n, int cols, int cut, int batch, long sx, long sy) {

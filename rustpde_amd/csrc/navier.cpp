@@ -391,7 +391,7 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
 // the launch's algorithmic work (flops for GEMMs, bytes for memory-bound kernels).
 class KernelTimer {
  public:
-  static constexpr int NCAT = 11;  // the RBC_K_* classes of the C ABI
+  static constexpr int NCAT = 12;  // the RBC_K_* classes of the C ABI
   bool on = false;
   struct Rec { int cat; double work; hipEvent_t a, b; };
   ~KernelTimer() { clear(); for (auto e : pool_) (void)hipEventDestroy(e); }
@@ -1015,6 +1015,22 @@ class Navier {
   // dealiased forward transforms
   void conv_fp64() {
     const int my = by_u.m;
+    if (use_fuse) {
+      // values of ux, uy, temp (W2) and their x-derivatives (W1) in one launch
+      timer.begin(st, 2, 8.0 * 6.0 * my * ((double)nx + 2.0 * mx()));
+      irfft_cols_pair(st, UX.p, W2.p, W1.p, nx, my, 1, scale[0], 3, (long)FS, (long)FS);
+      timer.end(st);
+      CHEBY_BWD_B(by_u, W2.p, UXP.p, nx, 0, nullptr, 0.0, 2);  // ux, uy in physical space
+      // conv = d/dy q uy + dq/dx ux (conv_term.rs:35-41), + u . grad(fieldbc) for temp
+      const DBand& band = stencil_band(by_u);
+      timer.begin(st, 11, 8.0 * 3.0 * nx * (2.0 * my + 3.0 * ny) + 8.0 * 4.0 * nx * ny);
+      cheb_y_conv(st, W1.p, W2.p, my, my, CONV.p, ny, nx, ny, band.coef.p, 1.0 / scale[1], UXP.p, UYP.p, ny, 3,
+                  (long)FS, (long)FS, UXP.p, DTBCX.p, UYP.p, DTBCY.p, 2);
+      timer.end(st);
+      CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
+      RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
+      return;
+    }
     // ux, uy in physical space
     IRFFT_B(UX.p, W1.p, my, 0, 1.0, 2);
     CHEBY_BWD_B(by_u, W1.p, UXP.p, nx, 0, nullptr, 0.0, 2);
@@ -1051,12 +1067,28 @@ class Navier {
                           F, 0, extra ? UXPf : nullptr, DTBCX.p, extra ? UYPf : nullptr, DTBCY.p, extra ? 2 : -1);
       timer.end(st);
     };
-    irfft(0, 1.0, 2);
-    bwd(UXPf, 0, nullptr, 0.0, 2, false);
-    irfft(1, scale[0], 3);
-    bwd(CONVf, 0, UXPf, 0.0, 3, false);
-    irfft(0, 1.0, 3);
-    bwd(CONVf, 1, UYPf, 1.0, 3, true);  // + u . grad(fieldbc) for temp
+    if (use_fuse) {
+      // values (W2f) and x-derivatives (W1f) of ux, uy, temp in one launch, then ux, uy
+      // in physical space and the convective terms in one launch each
+      timer.begin(st, 2, 6.0 * my * fx);
+      irfft_cols_pair_f32(st, UX.p, W2f, W1f, nx, my, 1, scale[0], 3, F, F);
+      timer.end(st);
+      timer.begin(st, 4, 4.0 * 2.0 * nx * ((double)my + ny));
+      cheb_y_backward_f32(st, W2f, my, my, UXPf, ny, nx, ny, band.coef.p, 0, 1.0 / scale[1], nullptr, ny, 0.0, 2, F, F,
+                          0, nullptr, nullptr, nullptr, nullptr, -1);
+      timer.end(st);
+      timer.begin(st, 11, 4.0 * 3.0 * nx * (2.0 * my + 3.0 * ny) + 24.0 * nx * ny);
+      cheb_y_conv_f32(st, W1f, W2f, my, my, CONVf, ny, nx, ny, band.coef.p, 1.0 / scale[1], UXPf, UYPf, ny, 3, F, F,
+                      UXPf, DTBCX.p, UYPf, DTBCY.p, 2);
+      timer.end(st);
+    } else {
+      irfft(0, 1.0, 2);
+      bwd(UXPf, 0, nullptr, 0.0, 2, false);
+      irfft(1, scale[0], 3);
+      bwd(CONVf, 0, UXPf, 0.0, 3, false);
+      irfft(0, 1.0, 3);
+      bwd(CONVf, 1, UYPf, 1.0, 3, true);  // + u . grad(fieldbc) for temp
+    }
     timer.begin(st, 6, 4.0 * 3.0 * nx * 2.0 * ny);
     cheb_y_forward_f32(st, CONVf, ny, W2f, ny, nx, ny, ny * 2 / 3, 3, F, F);
     timer.end(st);

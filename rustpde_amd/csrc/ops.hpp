@@ -132,6 +132,12 @@ void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int c
                long sy = 0);
 void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch = 1,
                 long sx = 0, long sy = 0);
+// values (entries [0, nin): times 0, sc 1 -> Y0) and (i k)^times1 / sc1 (-> Y1) of the same
+// nin inputs in one launch of 2 nin batch entries
+void irfft_cols_pair(hipStream_t s, const double* X, double* Y0, double* Y1, int n, int cols, int times1, double sc1,
+                     int nin, long sx, long sy);
+void irfft_cols_pair_f32(hipStream_t s, const double* X, float* Y0, float* Y1, int n, int cols, int times1, double sc1,
+                         int nin, long sx, long sy);
 
 // Banded matvec (taps at offsets 0, -2, +2, +4) then the row solves of fdma_rows, one
 // wavefront per row, R [nrows, nr] -> X [nrows, m]; `bandp` and `diagp` hold the taps and
@@ -193,6 +199,16 @@ void cheb_y_forward_f32(hipStream_t s, const float* in, int ldi, float* out, int
                         int batch = 1, long si = 0, long so = 0);
 // cheb_y_diff: d/dy in Chebyshev space (ortho.rs:107-125) with the composite stencil
 // fused on load: out [rows, n] = d / sc + beta out, by the same LDS suffix scans.
+// convective term of `batch` fields from their x-derivatives and values (x physical, y composite
+// spectral, stencil coef): out = d/dy(val) .* uy + dx .* ux, one complex transform per row pair
+void cheb_y_conv(hipStream_t s, const double* dx, const double* val, int ldi, int mi, double* out, int ldo, int rows,
+                 int n, const double* coef, double inv_sc, const double* ux, const double* uy, int ldu, int batch,
+                 long si, long so, const double* e1 = nullptr, const double* f1 = nullptr, const double* e2 = nullptr,
+                 const double* f2 = nullptr, int xslot = -1);
+void cheb_y_conv_f32(hipStream_t s, const float* dx, const float* val, int ldi, int mi, float* out, int ldo, int rows,
+                     int n, const double* coef, double inv_sc, const float* ux, const float* uy, int ldu, int batch,
+                     long si, long so, const float* e1 = nullptr, const double* f1 = nullptr,
+                     const float* e2 = nullptr, const double* f2 = nullptr, int xslot = -1);
 void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
                  const double* coef, double inv_sc, double beta, int batch = 1, long si = 0, long so = 0);
 
