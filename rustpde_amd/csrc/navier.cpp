@@ -158,7 +158,7 @@ static std::vector<double> slice_order(int nsets, int m, int narrays, F get) {
 // from_ortho tables of a composite basis: stencil and the tdma factors of linalg.rs:14-57
 struct DTdma {
   DBuf diag, low2, off, w, den;
-  DBuf tabp;  // diag, low2, off_{i-2}, den, w in the slice order of tdma_rows_lds
+  DBuf tabp;  // diag, low2, off_{i-2}, 1 / den, w in the slice order of tdma_rows_lds
   int m = 0;
 };
 static void make_tdma(const Basis& b, DTdma& t) {
@@ -185,7 +185,7 @@ static void make_tdma(const Basis& b, DTdma& t) {
       case 0: return b.diag[i];
       case 1: return b.low2[i];
       case 2: return tt >= 1 ? off[i - 2] : 0.0;
-      case 3: return den[i];
+      case 3: return 1.0 / den[i];
       default: return i < m - 2 ? w[i] : 0.0;
     }
   }));
@@ -275,7 +275,7 @@ struct Solver {
   DBand pyb;  // P_y as a banded matvec
   std::vector<double> pyb_taps;  // its taps on the host, [4][m]
   DBuf low, dia, up1, up2;
-  DBuf diagp, pybp;  // the same in slice order for fdma_rows_lds
+  DBuf diagp, pybp;  // the same in slice order for fdma_rows_lds (low, 1 / dia, up1, up2)
   int nsets = 0, m = 0;
   Band4 fa, fc;
   double alpha = 0.0;
@@ -305,7 +305,7 @@ struct Solver {
       const size_t o = (size_t)st * m;
       switch (a) {
         case 0: return t >= 1 ? L[o + i - 2] : 0.0;
-        case 1: return D[o + i];
+        case 1: return 1.0 / D[o + i];
         case 2: return t + 1 < len[p] ? U1[o + i] : 0.0;
         default: return t + 2 < len[p] ? U2[o + i] : 0.0;
       }

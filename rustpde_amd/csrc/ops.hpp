@@ -141,8 +141,8 @@ void irfft_cols_pair_f32(hipStream_t s, const double* X, float* Y0, float* Y1, i
 
 // Banded matvec (taps at offsets 0, -2, +2, +4) then the row solves of fdma_rows, one
 // wavefront per row, R [nrows, nr] -> X [nrows, m]; `bandp` and `diagp` hold the taps and
-// the swept diagonal sets in slice order (fdma_rows_lds_per(m) values per lane and
-// chain).  Without a band R must hold m values per row (in place allowed).
+// the swept diagonal sets (low, 1 / dia, up1, up2) in slice order (fdma_rows_lds_per(m)
+// values per lane and chain).  Without a band R must hold m values per row (in place allowed).
 int fdma_rows_lds_per(int m);
 bool fdma_rows_lds_supported(int m, int nr);
 void fdma_rows_lds(hipStream_t s, const double* R, int ldr, int nr, double* X, int ld, int nrows, int m, int nsets,
@@ -153,7 +153,7 @@ void fdma_rows_lds_multi(hipStream_t s, const double* R, int ldr, int nr, double
                          long sx);
 
 // from_ortho of every row as tdma_y, one wavefront per row with LDS staging and the
-// tables (diag, low2, off_{i-2}, den, w) in slice order: R [rows, m + 2] -> X [rows, m].
+// tables (diag, low2, off_{i-2}, 1 / den, w) in slice order: R [rows, m + 2] -> X [rows, m].
 bool tdma_rows_lds_supported(int m);
 // With project the result x is subtracted instead: X = x (-1) + X.
 void tdma_rows_lds(hipStream_t s, const double* R, int ldr, double* X, int ld, int rows, int m, const double* tabp,
@@ -170,6 +170,7 @@ struct PressureArgs {
   const double *su, *sp, *pybp, *pdiag, *ttab;
   int mx, ny, mu, per, bl;
   double kscale, inv_scy, nu, inv_dt;
+  double isc = 0.0;  // 1 / kscale when kscale is a power of two (set by pressure_rows)
 };
 bool pressure_rows_supported(int ny, int mu);
 void pressure_rows(hipStream_t s, const PressureArgs& a);

@@ -13,6 +13,7 @@
 // suffix sums, the Fdma and tdma recurrences of chains.hpp).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <stdexcept>
 #include <string>
 
@@ -27,22 +28,38 @@ using chains::buf_len;
 using chains::pad;
 using chains::WAVE;
 
-// (a + ib)(0 + ik) then / (sc + 0i), as num-complex forms them (fourier_diff, times = 1)
-__device__ __forceinline__ void fdiff(double a, double b, double kk, double sc, double& ra, double& rb) {
+// (a + ib)(0 + ik) then / (sc + 0i), as num-complex forms them (fourier_diff, times = 1).
+// isc = 1 / sc when sc is a power of two (else 0): the division (x sc) / (sc sc) is then
+// exactly the product x isc.
+__device__ __forceinline__ void fdiff(double a, double b, double kk, double sc, double isc, double& ra, double& rb) {
 #pragma clang fp contract(off)
   double na = a * 0.0 - b * kk, nb = a * kk + b * 0.0;
   if (sc != 1.0) {
-    const double ns = sc * sc + 0.0 * 0.0;
-    const double qa = (na * sc + nb * 0.0) / ns, qb = (nb * sc - na * 0.0) / ns;
-    na = qa;
-    nb = qb;
+    if (isc != 0.0) {
+      na = na * isc;
+      nb = nb * isc;
+    } else {
+      const double ns = sc * sc + 0.0 * 0.0;
+      const double qa = (na * sc + nb * 0.0) / ns, qb = (nb * sc - na * 0.0) / ns;
+      na = qa;
+      nb = qb;
+    }
   }
   ra = na;
   rb = nb;
 }
+// plane r of the same from the other plane's value x: r = 0 gives (0 - x k) / sc (x the
+// imaginary part), r = 1 gives (x k + 0) / sc (x the real part); equal to fdiff up to the
+// sign of a zero result
+__device__ __forceinline__ double fdiff_plane(double x, int r, double kk, double sc, double isc) {
+#pragma clang fp contract(off)
+  double v = r ? x * kk : 0.0 - x * kk;
+  if (sc != 1.0) v = isc != 0.0 ? v * isc : (v * sc) / (sc * sc + 0.0 * 0.0);
+  return v;
+}
 
 template <int SLICE>
-__global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
+__global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #pragma clang fp contract(off)
   extern __shared__ double lds[];
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
@@ -85,8 +102,8 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
       double d0 = 0.0, d1 = 0.0;
       if (i >= 0 && i < ny) {
         double c0r, c0i, c2r, c2i;
-        fdiff(XA(0)[pad(j)], XA(1)[pad(j)], kk, a.kscale, c0r, c0i);
-        fdiff(XA(0)[pad(j - 2)], XA(1)[pad(j - 2)], kk, a.kscale, c2r, c2i);
+        fdiff(XA(0)[pad(j)], XA(1)[pad(j)], kk, a.kscale, a.isc, c0r, c0i);
+        fdiff(XA(0)[pad(j - 2)], XA(1)[pad(j - 2)], kk, a.kscale, a.isc, c2r, c2i);
         d0 = c0r * s0[q] + c2r * s1[q];
         d1 = c0i * s0[q] + c2i * s1[q];
       }
@@ -162,14 +179,14 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
       chains::fdma_forward(v, lo, lane, t0, cnt);
     }
     {
-      double dg[SLICE], g1[SLICE], g2[SLICE];
+      double rd[SLICE], g1[SLICE], g2[SLICE];  // 1 / dia, up1, up2
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
-        dg[q] = DG(1, q);
+        rd[q] = DG(1, q);
         g1[q] = DG(2, q);
         g2[q] = DG(3, q);
       }
-      chains::fdma_backward(v, dg, g1, g2, lane, t0, cnt, lenu);
+      chains::fdma_backward(v, rd, g1, g2, lane, t0, cnt, lenu);
     }
 #undef DG
     if (k == 0 && p == 0 && lane == 0) v[0] = 0.0;  // navier.rs:710-715
@@ -212,33 +229,38 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
         dg[q] = TB(0, q);
         l2[q] = TB(1, q);
       }
+      // plane r of ik pseu / sc at the chain elements i0 - 2 + 2 j (j <= cnt + 1) and the
+      // stencil taps at i0 + 2 j (j <= cnt), each formed once for the slice
+      const int i0 = p + 2 * t0;
+      const double* Xo = YB(1 - r);
+      double cs[SLICE + 2], ta[SLICE + 1], tb[SLICE + 1];
+#pragma unroll
+      for (int j = 0; j < SLICE + 2; ++j)
+        cs[j] = (cnt > 0 && j <= cnt + 1) ? fdiff_plane(Xo[pad(i0 + 2 * j)], r, kk, a.kscale, a.isc) : 0.0;
+#pragma unroll
+      for (int j = 0; j < SLICE + 1; ++j) {
+        ta[j] = (cnt > 0 && j <= cnt) ? sp0[i0 + 2 * j] : 0.0;
+        tb[j] = (cnt > 0 && j <= cnt) ? sp1[i0 + 2 * j] : 0.0;
+      }
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
         v[q] = 0.0;
         if (q < cnt) {
-          const int i = p + 2 * (t0 + q);
-          double c[3];  // plane r of ik pseu / sc at i - 2, i, i + 2
-#pragma unroll
-          for (int o = 0; o < 3; ++o) {
-            double cre, cim;
-            fdiff(YB(0)[pad(i + 2 * o)], YB(1)[pad(i + 2 * o)], kk, a.kscale, cre, cim);
-            c[o] = r ? cim : cre;
-          }
-          const double g0 = c[1] * sp0[i] + c[0] * sp1[i];
-          const double g2 = c[2] * sp0[i + 2] + c[1] * sp1[i + 2];
+          const double g0 = cs[q + 1] * ta[q] + cs[q] * tb[q];
+          const double g2 = cs[q + 2] * ta[q + 1] + cs[q + 1] * tb[q + 1];
           v[q] = g0 * dg[q] + g2 * l2[q];
         }
       }
     }
     {
-      double of[SLICE], dn[SLICE], wv[SLICE];
+      double of[SLICE], rn[SLICE], wv[SLICE];  // off_{i-2}, 1 / den_i, w_i
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
         of[q] = TB(2, q);
-        dn[q] = TB(3, q);
+        rn[q] = TB(3, q);
         wv[q] = TB(4, q);
       }
-      chains::tdma_forward(v, of, dn, lane, t0, cnt);
+      chains::tdma_forward(v, of, rn, lane, t0, cnt);
       chains::tdma_backward(v, wv, lane, t0, cnt, p, mu);
     }
 #pragma unroll
@@ -298,14 +320,14 @@ __global__ __launch_bounds__(256) void pressure_rows_kernel(PressureArgs a) {
       }
     }
     {
-      double of[SLICE], dn[SLICE], wv[SLICE];
+      double of[SLICE], rn[SLICE], wv[SLICE];  // off_{i-2}, 1 / den_i, w_i
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
         of[q] = TB(2, q);
-        dn[q] = TB(3, q);
+        rn[q] = TB(3, q);
         wv[q] = TB(4, q);
       }
-      chains::tdma_forward(v, of, dn, lane, t0, cnt);
+      chains::tdma_forward(v, of, rn, lane, t0, cnt);
       chains::tdma_backward(v, wv, lane, t0, cnt, p, mu);
     }
 #pragma unroll
@@ -342,6 +364,8 @@ void pressure_rows(hipStream_t s, const PressureArgs& args) {
   PressureArgs a = args;
   a.per = fdma_rows_lds_per(a.mu);
   a.bl = buf_len(a.ny);
+  int ex = 0;
+  a.isc = (a.kscale > 0.0 && std::frexp(a.kscale, &ex) == 0.5) ? 1.0 / a.kscale : 0.0;
   const int perd = ((a.ny - 1) / 2 + WAVE - 1) / WAVE, sl = a.per > perd ? a.per : perd;
   const size_t lds = (size_t)6 * a.bl * sizeof(double);
   if (sl <= 4)

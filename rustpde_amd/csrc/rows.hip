@@ -7,7 +7,8 @@
 // positions [l per, (l + 1) per).  A pass over a chain is a linear recurrence:
 // each lane composes the affine map of its slice, the maps are scanned across the
 // wavefront with shuffles, and each lane replays its slice with the reference's
-// arithmetic from its incoming state.
+// arithmetic from its incoming state (a division by a diagonal is a product with the
+// reciprocal the table holds).
 //
 // Memory layout for coalescing: the right-hand side row is loaded with unit stride into
 // a padded LDS buffer of the wavefront and the slices are read from there (results go
@@ -110,14 +111,14 @@ __global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double
     chains::fdma_forward(v, lo, lane, t0, cnt);
   }
   {
-    double dg[SLICE], g1[SLICE], g2[SLICE];
+    double rd[SLICE], g1[SLICE], g2[SLICE];  // 1 / dia, up1, up2
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) {
-      dg[q] = DG(1, q);
+      rd[q] = DG(1, q);
       g1[q] = DG(2, q);
       g2[q] = DG(3, q);
     }
-    chains::fdma_backward(v, dg, g1, g2, lane, t0, cnt, len);
+    chains::fdma_backward(v, rd, g1, g2, lane, t0, cnt, len);
   }
 #undef DG
   // (3) slices -> LDS -> row, unit stride
@@ -170,13 +171,13 @@ __global__ __launch_bounds__(WAVE* WAVES) void tdma_rows_lds_kernel(const double
     }
   }
   {  // forward: g_i = (d_i - g_{i-2} off_{i-2}) / den_i  (i >= 2), g_i = d_i / den_i (i < 2)
-    double of[SLICE], dn[SLICE];
+    double of[SLICE], rn[SLICE];  // off_{i-2}, 1 / den_i
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) {
       of[q] = TB(2, q);
-      dn[q] = TB(3, q);
+      rn[q] = TB(3, q);
     }
-    chains::tdma_forward(v, of, dn, lane, t0, cnt);
+    chains::tdma_forward(v, of, rn, lane, t0, cnt);
   }
   {  // back: x_i = g_i for i >= m - 2, else x_i = g_i - x_{i+2} w_i
     double wv[SLICE];
