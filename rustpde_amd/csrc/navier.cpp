@@ -1112,6 +1112,8 @@ class Navier {
       PressureArgs pa{UX.p, UY.p, P0.p, P1.p, stencil_band(by_u).coef.p, stencil_band(by_phi).coef.p, sp.pybp.p,
                       sp.diagp.p, tdma_tables(by_u).tabp.p, mx(), ny, by_u.m, 0, 0, scale[0], 1.0 / scale[1], nu,
                       1.0 / dt};
+      pa.su_kind = by_u.k == CD ? 1 : by_u.k == CN ? 2 : 0;
+      pa.sp_kind = by_phi.k == CD ? 1 : by_phi.k == CN ? 2 : 0;
       const double rows = 2.0 * mx();
       timer.begin(st, 10, 8.0 * rows * (2.0 * by_u.m + ny) + 8.0 * rows * (3.0 * by_u.m + ny) +
                               8.0 * 4.0 * mx() * (double)by_phi.m);

@@ -171,6 +171,9 @@ struct PressureArgs {
   int mx, ny, mu, per, bl;
   double kscale, inv_scy, nu, inv_dt;
   double isc = 0.0;  // 1 / kscale when kscale is a power of two (set by pressure_rows)
+  // stencils of the velocity and pressure bases: 0 from su / sp, 1 Dirichlet, 2 Neumann
+  // (formed in the kernel; the tables' values are the same)
+  int su_kind = 0, sp_kind = 0;
 };
 bool pressure_rows_supported(int ny, int mu);
 void pressure_rows(hipStream_t s, const PressureArgs& a);

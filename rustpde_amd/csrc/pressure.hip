@@ -58,6 +58,21 @@ __device__ __forceinline__ double fdiff_plane(double x, int r, double kk, double
   return v;
 }
 
+// Stencil taps of a composite basis (composite_stencil.rs:117-162) at ortho index j, from
+// the table coef (kind 0: d at coef[j], l at coef[n + j]) or formed in place for the
+// Dirichlet (kind 1: d = 1, l = -1) and Neumann (kind 2: l = -(j-2)^2 / j^2) bases with
+// the host's arithmetic, so the values are bitwise the table's.  m = n - 2 inputs.
+__device__ __forceinline__ double tap_d(int kind, const double* __restrict__ coef, int j, int m) {
+  return kind ? (j < m ? 1.0 : 0.0) : coef[j];
+}
+__device__ __forceinline__ double tap_l(int kind, const double* __restrict__ coef, int j, int n, int m) {
+  if (kind == 0) return coef[n + j];
+  if (j < 2 || j - 2 >= m) return 0.0;
+  if (kind == 1) return -1.0;
+  const long long i = j - 2;
+  return -1.0 * (double)(i * i) / (double)((long long)j * j);
+}
+
 template <int SLICE>
 __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #pragma clang fp contract(off)
@@ -72,10 +87,11 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #define DV(pl) (lds + (4 + (pl)) * bl)
 #define ROWU(pl) ((size_t)((pl) ? mx + k : k) * mu)
 #define ROWN(pl) ((size_t)((pl) ? mx + k : k) * ny)
-  const double* __restrict__ su0 = a.su;
-  const double* __restrict__ su1 = a.su + ny;
-  const double* __restrict__ sp0 = a.sp;
-  const double* __restrict__ sp1 = a.sp + ny;
+  // stencil taps of the velocity (u) and pressure (p) bases
+#define SU0(j) tap_d(a.su_kind, a.su, (j), mu)
+#define SU1(j) tap_l(a.su_kind, a.su, (j), ny, mu)
+#define SP0(j) tap_d(a.sp_kind, a.sp, (j), mu)
+#define SP1(j) tap_l(a.sp_kind, a.sp, (j), ny, mu)
 
   // rows of up to 2 SLICE WAVE + 1 values: unrolled staging bounds (all loads before stores)
   constexpr int QW = (2 * SLICE * WAVE + 8 + WAVE - 1) / WAVE;  // per wavefront
@@ -92,8 +108,8 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int i = tid + q * 256 - 2;
-      s0[q] = (i >= 0 && i < ny) ? su0[i] : 0.0;
-      s1[q] = (i >= 0 && i < ny) ? su1[i] : 0.0;
+      s0[q] = (i >= 0 && i < ny) ? SU0(i) : 0.0;
+      s1[q] = (i >= 0 && i < ny) ? SU1(i) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
@@ -125,7 +141,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       if (q < cntd) {
         const int i = (p ? 1 : 2) + 2 * (t0d + q);
         if (i <= N - 1) {
-          const double s = Y[pad(i + 3)] * su0[i + 1] + Y[pad(i + 1)] * su1[i + 1];
+          const double s = Y[pad(i + 3)] * SU0(i + 1) + Y[pad(i + 1)] * SU1(i + 1);
           v[q] = (2.0 * (double)(i + 1)) * s;
         }
       }
@@ -141,7 +157,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       }
     }
     if (p == 0 && lane == 0) {
-      const double s1 = Y[pad(3)] * su0[1] + Y[pad(1)] * su1[1];
+      const double s1 = Y[pad(3)] * SU0(1) + Y[pad(1)] * SU1(1);
       double d = s1 + v[0] / 2.0;
       if (a.inv_scy != 1.0) d = d * a.inv_scy;
       DV(r)[pad(2)] = d + 1.0 * DV(r)[pad(2)];
@@ -203,8 +219,8 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     for (int q = 0; q < QH; ++q) {
       const int i = th + q * 128;
       pv[q] = i < ny ? prow[i] : 0.0;
-      c0[q] = i < ny ? sp0[i] : 0.0;
-      c1[q] = i < ny ? sp1[i] : 0.0;
+      c0[q] = i < ny ? SP0(i) : 0.0;
+      c1[q] = i < ny ? SP1(i) : 0.0;
     }
     const double* Pp = YB(h);
 #pragma unroll
@@ -220,14 +236,19 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
   // (4b) ux projection: d_i = g_i diag_i + g_{i+2} low2_i with g = S_p(ik pseu / sc), then
   //      the tdma recurrences of from_ortho
 #define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
+  // the from_ortho tables of the velocity basis: diag, low2 and off_{i-2} = diag_i low2_{i-2}
+  // are 1, -1, -1 for the Dirichlet basis (0 for off at the chain start)
+#define TB_DG(q) (a.su_kind == 1 ? 1.0 : TB(0, q))
+#define TB_L2(q) (a.su_kind == 1 ? -1.0 : TB(1, q))
+#define TB_OF(q) (a.su_kind == 1 ? (t0 + (q) >= 1 ? -1.0 : 0.0) : TB(2, q))
   {
     double v[SLICE];
     {
       double dg[SLICE], l2[SLICE];
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
-        dg[q] = TB(0, q);
-        l2[q] = TB(1, q);
+        dg[q] = TB_DG(q);
+        l2[q] = TB_L2(q);
       }
       // plane r of ik pseu / sc at the chain elements i0 - 2 + 2 j (j <= cnt + 1) and the
       // stencil taps at i0 + 2 j (j <= cnt), each formed once for the slice
@@ -239,8 +260,8 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
         cs[j] = (cnt > 0 && j <= cnt + 1) ? fdiff_plane(Xo[pad(i0 + 2 * j)], r, kk, a.kscale, a.isc) : 0.0;
 #pragma unroll
       for (int j = 0; j < SLICE + 1; ++j) {
-        ta[j] = (cnt > 0 && j <= cnt) ? sp0[i0 + 2 * j] : 0.0;
-        tb[j] = (cnt > 0 && j <= cnt) ? sp1[i0 + 2 * j] : 0.0;
+        ta[j] = (cnt > 0 && j <= cnt) ? SP0(i0 + 2 * j) : 0.0;
+        tb[j] = (cnt > 0 && j <= cnt) ? SP1(i0 + 2 * j) : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
@@ -256,7 +277,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       double of[SLICE], rn[SLICE], wv[SLICE];  // off_{i-2}, 1 / den_i, w_i
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
-        of[q] = TB(2, q);
+        of[q] = TB_OF(q);
         rn[q] = TB(3, q);
         wv[q] = TB(4, q);
       }
@@ -280,7 +301,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       if (q < cntd) {
         const int i = (p ? 1 : 2) + 2 * (t0d + q);
         if (i <= N - 1) {
-          const double s = Pp[pad(i + 3)] * sp0[i + 1] + Pp[pad(i + 1)] * sp1[i + 1];
+          const double s = Pp[pad(i + 3)] * SP0(i + 1) + Pp[pad(i + 1)] * SP1(i + 1);
           v[q] = (2.0 * (double)(i + 1)) * s;
         }
       }
@@ -296,7 +317,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       }
     }
     if (p == 0 && lane == 0) {
-      const double s1 = Pp[pad(3)] * sp0[1] + Pp[pad(1)] * sp1[1];
+      const double s1 = Pp[pad(3)] * SP0(1) + Pp[pad(1)] * SP1(1);
       double d = s1 + v[0] / 2.0;
       if (a.inv_scy != 1.0) d = d * a.inv_scy;
       DV(r)[pad(2)] = d;
@@ -310,8 +331,8 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       double dg[SLICE], l2[SLICE];
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
-        dg[q] = TB(0, q);
-        l2[q] = TB(1, q);
+        dg[q] = TB_DG(q);
+        l2[q] = TB_L2(q);
       }
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
@@ -323,7 +344,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       double of[SLICE], rn[SLICE], wv[SLICE];  // off_{i-2}, 1 / den_i, w_i
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
-        of[q] = TB(2, q);
+        of[q] = TB_OF(q);
         rn[q] = TB(3, q);
         wv[q] = TB(4, q);
       }
@@ -335,6 +356,13 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       if (q < cnt) XA(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
   }
 #undef TB
+#undef TB_DG
+#undef TB_L2
+#undef TB_OF
+#undef SU0
+#undef SU1
+#undef SP0
+#undef SP1
   __syncthreads();
   // (7) uy = x (-1) + uy
   chains::unstage_row<QH>(XA(h), a.UY + ROWU(h), true, mu, th, 128, true);
