@@ -1007,8 +1007,12 @@ class Navier {
     // d/dy pres, then the right-hand sides (navier.rs:622-674) with to_ortho of ux, uy, temp,
     // the buoyancy temp + fieldbc (navier.rs:740-742) and d/dx pres formed in the same kernel
     yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
+    // bytes: ux, uy, temp coefficients, pres, d/dy pres, fieldbc, gxx, gyy and three
+    // convective terms read, three right-hand sides written
+    timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 11.0 * ny));
     rhs3_fused(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p, CV.p, GXX.p, GYY.p, RHS.p, mx(), ny,
                scale[0], dt, (long)F);
+    timer.end(st);
     solves_and_pressure();
   }
   // convective terms of ux, uy, temp into CV (FP64): dq/dx ux + dq/dy uy (conv_term.rs:35-41),

@@ -42,7 +42,7 @@ struct RowTables {
 
 // One wavefront per (row, parity chain); the two chains of a row share its LDS buffer.
 template <int SLICE>
-__global__ __launch_bounds__(WAVE* WAVES) void fdma_rows_lds_kernel(const double* __restrict__ R, int ldr, int nr,
+__global__ __launch_bounds__(WAVE* WAVES, SLICE <= 8 ? 4 : 2) void fdma_rows_lds_kernel(const double* __restrict__ R, int ldr, int nr,
                                                                     double* __restrict__ X, int ld, int nrows, int m,
                                                                     int nsets, int per, int bl, RowTables tb, long sr,
                                                                     long sx) {
