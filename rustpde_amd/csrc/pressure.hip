@@ -7,8 +7,8 @@
 // the projection    ux -= FO_u(S_p(ik pseu / sc)),  uy -= FO_u(d/dy S_p(pseu))
 // and the update    pres = (pres - div nu) + S_p(pseu) / dt.
 // One workgroup owns wavenumber k: four wavefronts, one per (plane, parity chain); the rows
-// live in padded LDS buffers between the phases, so the stage reads ux, uy, pres once and
-// writes ux, uy, pres, pseu once.  Every value is formed with the arithmetic of the unfused
+// live in padded LDS buffers between the phases, so the stage reads ux and pres once (uy
+// twice: its staged copy makes room for pseu) and writes ux, uy, pres, pseu once.  Every value is formed with the arithmetic of the unfused
 // kernels it replaces (band stencils, the num-complex Fourier derivative, the derivative
 // suffix sums, the Fdma and tdma recurrences of chains.hpp).
 #include <hip/hip_runtime.h>
@@ -286,11 +286,14 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     }
 #pragma unroll
     for (int q = 0; q < SLICE; ++q)
-      if (q < cnt) XA(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
+      if (q < cnt) {  // XA still holds the staged ux: ux = x (-1) + ux in place
+        double* e = XA(r) + pad(p + 2 * (t0 + q) + 2);
+        *e = v[q] * (-1.0) + *e;
+      }
   }
   __syncthreads();
-  // (5a) ux = x (-1) + ux
-  chains::unstage_row<QH>(XA(h), a.UX + ROWU(h), true, mu, th, 128, true);
+  // (5a) the updated ux rows -> HBM
+  chains::unstage_row<QH>(XA(h), a.UX + ROWU(h), true, mu, th, 128, false);
   // (5b) d/dy S_p(pseu) into DV (the chains of (2b))
   {
     double v[SLICE];
