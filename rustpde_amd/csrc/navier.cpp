@@ -1521,17 +1521,33 @@ class Ensemble {
   struct Group {
     std::unique_ptr<Navier::Slabs> slabs;
     std::vector<std::unique_ptr<Navier>> navs;  // navs[0] is the leader
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;                   // one of the ensemble's streams
     ~Group() {
       if (st) (void)hipStreamSynchronize(st);
       navs.clear();
       slabs.reset();
-      if (st) (void)hipStreamDestroy(st);
     }
   };
+  // Streams the groups are spread over.  The groups are latency-bound chains of small
+  // launches, so they overlap well on separate hardware queues; with more groups than
+  // queues (HIP's default is 4 per process, one of them taken by the null stream) the
+  // groups are packed onto NSTREAMS streams by longest-processing-time-first.
+  static constexpr int NSTREAMS = 3;
+  std::vector<hipStream_t> streams;
   std::vector<std::unique_ptr<Group>> groups;
   std::vector<Navier*> inst;
   std::vector<int> group_of;
+
+  // estimated time of one step of a group of B instances of nx x ny: a launch-latency floor
+  // plus the dense transforms' work (measured on MI355X: ~450 us + 3.6e-6 us per B N^3)
+  static double group_cost(int B, int nx, int ny) {
+    return 450.0 + 3.6e-6 * (double)B * (double)nx * (double)ny * 0.5 * ((double)nx + ny);
+  }
+  ~Ensemble() {
+    for (hipStream_t st : streams) (void)hipStreamSynchronize(st);
+    groups.clear();
+    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+  }
 
   Ensemble(int n, const rbc_desc* d) {
     if (n < 1 || !d) throw std::invalid_argument("ensemble: no instances");
@@ -1542,12 +1558,37 @@ class Ensemble {
     }
     inst.assign(n, nullptr);
     group_of.assign(n, -1);
+    // groups in decreasing cost, each onto the stream with the least work so far
+    std::vector<std::pair<double, const std::vector<int>*>> order;
     for (auto& kv : keys) {
-      const std::vector<int>& mem = kv.second;
+      const rbc_desc& d0 = d[kv.second[0]];
+      order.emplace_back(group_cost((int)kv.second.size(), d0.nx, d0.ny), &kv.second);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const auto& x, const auto& y) { return x.first > y.first; });
+    const int ns = std::min<int>(NSTREAMS, (int)order.size());
+    streams.assign(ns, nullptr);
+    for (int k = 0; k < ns; ++k) HIPCHECK(hipStreamCreate(&streams[k]));
+    try {
+      build_groups(d, order);
+    } catch (...) {
+      for (hipStream_t st : streams) (void)hipStreamSynchronize(st);
+      groups.clear();
+      for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+      streams.clear();
+      throw;
+    }
+  }
+  void build_groups(const rbc_desc* d, const std::vector<std::pair<double, const std::vector<int>*>>& order) {
+    std::vector<double> load(streams.size(), 0.0);
+    for (auto& oc : order) {
+      const std::vector<int>& mem = *oc.second;
       const int B = (int)mem.size();
       const rbc_desc& d0 = d[mem[0]];
       auto g = std::make_unique<Group>();
-      HIPCHECK(hipStreamCreate(&g->st));
+      const int si = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[si] += oc.first;
+      g->st = streams[si];
       g->slabs = std::make_unique<Navier::Slabs>();
       g->slabs->F = (size_t)d0.nx * d0.ny;
       for (int k = 0; k < Navier::Slabs::NK; ++k) g->slabs->slab[k].alloc(g->slabs->F * B);
@@ -1585,14 +1626,14 @@ class Ensemble {
   // every instance advances nsteps steps; the groups run concurrently
   void update(int nsteps) {
     for (auto& g : groups) g->navs[0]->update_n(nsteps);
+    synchronize();
     for (auto& g : groups) {
-      HIPCHECK(hipStreamSynchronize(g->st));
       const double t = g->navs[0]->time;
       for (auto& nv : g->navs) nv->time = t;
     }
   }
   void synchronize() {
-    for (auto& g : groups) HIPCHECK(hipStreamSynchronize(g->st));
+    for (hipStream_t st : streams) HIPCHECK(hipStreamSynchronize(st));
   }
 };
 
