@@ -112,8 +112,11 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *                  power of two (default 1; 0 = dense transform GEMMs)
  *   RBC_OPT_BATCH  periodic step: the transforms of ux, uy and temp and the ux/uy solves
  *                  run as batched launches (default 1; needs the FFT, DCT and band paths)
- *   RBC_OPT_FUSE   periodic batched step: divergence, Poisson rows, projection and
- *                  pressure update as one kernel per x-wavenumber (default 1)
+ *   RBC_OPT_FUSE   periodic batched step: the convective terms in two launches (values and
+ *                  x-derivatives of ux, uy, temp from one x-transform launch, then one
+ *                  y-transform launch forming d/dy q uy + dq/dx ux) and divergence, Poisson
+ *                  rows, projection and pressure update as one kernel per x-wavenumber
+ *                  (default 1)
  *   RBC_OPT_MIXED  mixed precision (periodic batched step): the convective terms (their
  *                  x/y transforms and products) in FP32, everything else in FP64
  *                  (default 0; fields agree with the FP64 step to ~1e-9 after 10 steps) */

@@ -1057,7 +1057,7 @@ class Navier {
     float* CONVf = UXPf + 2 * F;
     float* W2f = CONVf + 3 * F;
     const DBand& band = stencil_band(by_u);
-    const double fx = 4.0 * ((double)nx + mx2);  // bytes per column: FP64 spectrum in, FP32 values out
+    const double fx = 4.0 * nx + 8.0 * mx2;  // bytes per column: FP64 spectrum in, FP32 values out
     auto irfft = [&](int times, double sc, int batch) {
       timer.begin(st, 2, (double)batch * my * fx);
       irfft_cols_f32(st, UX.p, W1f, nx, my, times, sc, batch, F, F);
