@@ -105,9 +105,19 @@ void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int 
   if (M <= 0 || N <= 0 || batch <= 0) return;
   if (K <= 0) throw std::runtime_error("gemm_mfma: K must be positive");
   if (batch > 65535) throw std::runtime_error("gemm_mfma: batch exceeds the grid limit");
-  dim3 grid((N + 63) / 64, (M + 63) / 64, batch);
-  RBC_LAUNCH((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
-                     alpha, beta, U, ldu, sU);
+  // tile per launch shape: 64 x 64 tiles while they give at least four workgroups per CU
+  // (256 CUs), else 32 x 32 tiles (four times the workgroups) so that the small batched
+  // GEMMs of ensembles (N = 64 .. 192) spread over the whole chip
+  const long tiles64 = (long)((N + 63) / 64) * ((M + 63) / 64) * batch;
+  if (tiles64 >= 1024) {
+    dim3 grid((N + 63) / 64, (M + 63) / 64, batch);
+    RBC_LAUNCH((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
+               alpha, beta, U, ldu, sU);
+  } else {
+    dim3 grid((N + 31) / 32, (M + 31) / 32, batch);
+    RBC_LAUNCH((gemm_mfma_kernel<32, 32, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
+               alpha, beta, U, ldu, sU);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("gemm_mfma: ") + hipGetErrorString(e));
 }
