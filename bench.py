@@ -110,7 +110,8 @@ def bench_hholtz_batch(args):
     for _ in range(reps):
         g.solve_device(din.p, dout.p, nb)  # synchronous
     el = (time.perf_counter() - t0) / reps
-    flops = nb * 2 * (2.0 * (n - 2) * n * n + 2.0 * (n - 2) * (n - 2) * (n - 2))
+    # per right-hand side: fwd P_x [n-2, n] x [n, n] and bwd [n-2, n-2] x [n-2, n-2] GEMMs
+    flops = nb * (2.0 * (n - 2) * n * n + 2.0 * (n - 2) * (n - 2) * (n - 2))
     out = {"metric": "Hholtz 512^2 right-hand sides solved per second (FP64)", "value": nb / el, "unit": "rhs/s",
            "n_gpus": 1, "steps": reps, "warmup": max(1, args.warmup // 5), "ms_per_step": 1e3 * el,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

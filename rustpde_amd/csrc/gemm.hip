@@ -7,8 +7,11 @@
 // slabs; the next slab is fetched into registers while the current one is consumed.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 
 #include "ops.hpp"
 
@@ -97,6 +100,105 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
       }
 }
 
+// Tile shapes of the launch (BM x BN); the accumulation order over K is the same for all
+// of them, so the choice never changes a result bit.
+enum { TILE_64x64, TILE_32x32, TILE_64x32, TILE_32x64, NTILES };
+
+template <int BM, int BN>
+void launch_tile(hipStream_t s, bool timed, int M, int N, int K, const double* A, int lda, long sA, const double* B,
+                 int ldb, long sB, double* C, int ldc, long sC, int batch, double alpha, double beta, const double* U,
+                 int ldu, long sU) {
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, batch);
+  if (timed) {
+    RBC_LAUNCH((gemm_mfma_kernel<BM, BN, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
+               alpha, beta, U, ldu, sU);
+  } else {
+    hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc,
+                       sC, alpha, beta, U, ldu, sU);
+  }
+}
+
+void launch_variant(int v, hipStream_t s, bool timed, int M, int N, int K, const double* A, int lda, long sA,
+                    const double* B, int ldb, long sB, double* C, int ldc, long sC, int batch, double alpha,
+                    double beta, const double* U, int ldu, long sU) {
+  switch (v) {
+    case TILE_64x64: launch_tile<64, 64>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
+    case TILE_32x32: launch_tile<32, 32>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
+    case TILE_64x32: launch_tile<64, 32>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
+    default: launch_tile<32, 64>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
+  }
+}
+
+// Autotuner: the first launch of a shape (device, M, N, K, batch) outside stream capture
+// times every tile shape on the caller's operands, writing a scratch output, and keeps the
+// fastest for that shape; under capture, or when timing fails, a shape without an entry
+// gets the rule of thumb (64 x 64 tiles while they give four workgroups per CU, else
+// 32 x 32).
+std::mutex g_tune_mutex;
+std::map<std::tuple<int, int, int, int, int>, int> g_tune;
+
+int tile_rule(int M, int N, int batch) {
+  const long tiles64 = (long)((N + 63) / 64) * ((M + 63) / 64) * batch;
+  return tiles64 >= 1024 ? TILE_64x64 : TILE_32x32;
+}
+
+int tile_for(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb, long sB,
+             int ldc, long sC, int batch) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const auto key = std::make_tuple(dev, M, N, K, batch);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipPeekAtLastError() != hipSuccess) return tile_rule(M, N, batch);  // leave a pending error to the caller
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return tile_rule(M, N, batch);
+  // scratch output covering the launch's output footprint
+  const size_t span = (size_t)(batch - 1) * (size_t)(sC > 0 ? sC : 0) + (size_t)(M - 1) * ldc + N;
+  double* scratch = nullptr;
+  hipEvent_t ev[NTILES + 1] = {};
+  int best = tile_rule(M, N, batch);
+  bool ok = hipMalloc(&scratch, span * sizeof(double)) == hipSuccess;
+  for (int v = 0; ok && v <= NTILES; ++v) ok = hipEventCreate(&ev[v]) == hipSuccess;
+  if (ok) {
+    for (int v = 0; v < NTILES; ++v)  // warm-up of every variant
+      launch_variant(v, s, false, M, N, K, A, lda, sA, B, ldb, sB, scratch, ldc, sC, batch, 1.0, 0.0, nullptr, 0, 0);
+    ok = hipEventRecord(ev[0], s) == hipSuccess;
+    for (int v = 0; ok && v < NTILES; ++v) {
+      for (int r = 0; r < 3; ++r)
+        launch_variant(v, s, false, M, N, K, A, lda, sA, B, ldb, sB, scratch, ldc, sC, batch, 1.0, 0.0, nullptr, 0,
+                       0);
+      ok = hipEventRecord(ev[v + 1], s) == hipSuccess;
+    }
+    ok = ok && hipGetLastError() == hipSuccess && hipEventSynchronize(ev[NTILES]) == hipSuccess;
+    float tbest = 0.0f;
+    for (int v = 0; ok && v < NTILES; ++v) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[v], ev[v + 1]) != hipSuccess) {
+        ok = false;
+        break;
+      }
+      if (v == 0 || ms < tbest) {
+        tbest = ms;
+        best = v;
+      }
+    }
+  }
+  (void)hipGetLastError();
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (scratch) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(scratch);
+  }
+  if (!ok) best = tile_rule(M, N, batch);
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tune[key] = best;
+  return best;
+}
+
 }  // namespace
 
 void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
@@ -105,19 +207,8 @@ void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int 
   if (M <= 0 || N <= 0 || batch <= 0) return;
   if (K <= 0) throw std::runtime_error("gemm_mfma: K must be positive");
   if (batch > 65535) throw std::runtime_error("gemm_mfma: batch exceeds the grid limit");
-  // tile per launch shape: 64 x 64 tiles while they give at least four workgroups per CU
-  // (256 CUs), else 32 x 32 tiles (four times the workgroups) so that the small batched
-  // GEMMs of ensembles (N = 64 .. 192) spread over the whole chip
-  const long tiles64 = (long)((N + 63) / 64) * ((M + 63) / 64) * batch;
-  if (tiles64 >= 1024) {
-    dim3 grid((N + 63) / 64, (M + 63) / 64, batch);
-    RBC_LAUNCH((gemm_mfma_kernel<64, 64, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
-               alpha, beta, U, ldu, sU);
-  } else {
-    dim3 grid((N + 31) / 32, (M + 31) / 32, batch);
-    RBC_LAUNCH((gemm_mfma_kernel<32, 32, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
-               alpha, beta, U, ldu, sU);
-  }
+  const int v = tile_for(s, M, N, K, A, lda, sA, B, ldb, sB, ldc, sC, batch);
+  launch_variant(v, s, true, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("gemm_mfma: ") + hipGetErrorString(e));
 }
