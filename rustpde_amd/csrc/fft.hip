@@ -288,6 +288,16 @@ void rfft_cols_f32(hipStream_t s, const float* X, double* Y, int n, int cols, in
   check("rfft_cols_f32");
 }
 
+void rfft_cols_f32f(hipStream_t s, const float* X, float* Y, int n, int cols, int cut, int batch, long sx, long sy) {
+  check_n(n);
+  if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("rfft_cols_f32f: bad batch or cols");
+  cut = std::min(cut, n / 2 + 1);
+#define RBC_RFFT32F(LL) launch_rfft<LL, float, float, float>(s, X, Y, cols, cut, batch, sx, sy)
+  RBC_FFT_SWITCH(n, RBC_RFFT32F)
+#undef RBC_RFFT32F
+  check("rfft_cols_f32f");
+}
+
 void irfft_cols_f32(hipStream_t s, const double* X, float* Y, int n, int cols, int times, double sc, int batch,
                     long sx, long sy) {
   check_n(n);

@@ -7,6 +7,7 @@
 // slabs; the next slab is fetched into registers while the current one is consumed.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -144,6 +145,11 @@ int tile_rule(int M, int N, int batch) {
 
 int tile_for(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb, long sB,
              int ldc, long sC, int batch) {
+  // RUSTPDE_AMD_GEMM_TILE = 0..3 forces one tile shape (for tests of each shape)
+  if (const char* f = std::getenv("RUSTPDE_AMD_GEMM_TILE")) {
+    const int v = std::atoi(f);
+    if (v >= 0 && v < NTILES) return v;
+  }
   int dev = 0;
   (void)hipGetDevice(&dev);
   const auto key = std::make_tuple(dev, M, N, K, batch);

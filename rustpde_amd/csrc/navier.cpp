@@ -1009,9 +1009,15 @@ class Navier {
     yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
     // bytes: ux, uy, temp coefficients, pres, d/dy pres, fieldbc, gxx, gyy and three
     // convective terms read, three right-hand sides written
-    timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 11.0 * ny));
-    rhs3_fused(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p, CV.p, GXX.p, GYY.p, RHS.p, mx(), ny,
-               scale[0], dt, (long)F);
+    if (mixed) {  // FP32 convective terms (conv_mixed)
+      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 8.0 * ny) + 4.0 * 2.0 * mx() * 3.0 * ny);
+      rhs3_fused_cv32(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p,
+                      reinterpret_cast<const float*>(MIX.p), GXX.p, GYY.p, RHS.p, mx(), ny, scale[0], dt, (long)F);
+    } else {
+      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 11.0 * ny));
+      rhs3_fused(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p, CV.p, GXX.p, GYY.p, RHS.p, mx(), ny,
+                 scale[0], dt, (long)F);
+    }
     timer.end(st);
     solves_and_pressure();
   }
@@ -1046,7 +1052,8 @@ class Navier {
     CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
     RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
   }
-  // the same pipeline with FP32 values and arithmetic (FP64 spectra in, FP64 CV out)
+  // the same pipeline with FP32 values and arithmetic (FP64 spectra in, FP32 convective
+  // terms out in the first three FP32 slots of MIX)
   void conv_mixed() {
     const int my = by_u.m, mx2 = 2 * mx();
     const long F = (long)FS;
@@ -1096,8 +1103,9 @@ class Navier {
     timer.begin(st, 6, 4.0 * 3.0 * nx * 2.0 * ny);
     cheb_y_forward_f32(st, CONVf, ny, W2f, ny, nx, ny, ny * 2 / 3, 3, F, F);
     timer.end(st);
-    timer.begin(st, 5, 3.0 * ny * (4.0 * nx + 8.0 * mx2));
-    rfft_cols_f32(st, W2f, CV.p, nx, ny, bx_f.m * 2 / 3, 3, F, F);
+    // FP32 convective terms into W1f's slots (free after the y transforms)
+    timer.begin(st, 5, 3.0 * ny * (4.0 * nx + 4.0 * mx2));
+    rfft_cols_f32f(st, W2f, W1f, nx, ny, bx_f.m * 2 / 3, 3, F, F);
     timer.end(st);
   }
   void solves_and_pressure() {

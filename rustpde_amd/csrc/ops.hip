@@ -211,10 +211,13 @@ __global__ void ew_rhs3_kernel(double* out, const double* a, const double* g, co
 //   a_q = S U_q (to_ortho stencil, taps i and i - 2), buoy = a_temp + fieldbc,
 //   ux: ((a - dP/dx dt)) - c dt, dP/dx = (i k P / sc) as fourier_diff forms it,
 //   uy: ((a - g1 dt) + buoy dt) - c dt,  temp: ((a + gxx) + gyy) - c dt
+// CV: type of the convective terms (FP64, or FP32 from the mixed-precision pipeline; read
+// as FP64, so the FP32 values enter exactly as the FP64 copy of them would)
+template <typename CV>
 __global__ void rhs3_fused_kernel(const double* __restrict__ U, int my, const double* __restrict__ sc0,
                                   const double* __restrict__ sc1, const double* __restrict__ P0,
                                   const double* __restrict__ g1, const double* __restrict__ thbc,
-                                  const double* __restrict__ cv, const double* __restrict__ gxx,
+                                  const CV* __restrict__ cv, const double* __restrict__ gxx,
                                   const double* __restrict__ gyy, double* __restrict__ out, int mx, int ny, double kscale,
                                   double dt, long F) {
 #pragma clang fp contract(off)
@@ -245,19 +248,27 @@ __global__ void rhs3_fused_kernel(const double* __restrict__ U, int my, const do
   const long j = (long)r * ny + i;
   const double buoy = a[2] + thbc[j];
   double r0 = a[0] - gx * dt;
-  out[j] = r0 - cv[j] * dt;
+  out[j] = r0 - (double)cv[j] * dt;
   double r1 = a[1] - g1[j] * dt;
   r1 = r1 + buoy * dt;
-  out[F + j] = r1 - cv[F + j] * dt;
-  out[2 * F + j] = ((a[2] + gxx[j]) + gyy[j]) - cv[2 * F + j] * dt;
+  out[F + j] = r1 - (double)cv[F + j] * dt;
+  out[2 * F + j] = ((a[2] + gxx[j]) + gyy[j]) - (double)cv[2 * F + j] * dt;
 }
 void rhs3_fused(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                 const double* thbc, const double* cv, const double* gxx, const double* gyy, double* out, int mx, int ny,
                 double kscale, double dt, long F) {
   const long n = (long)2 * mx * ny;
-  RBC_LAUNCH(rhs3_fused_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, U, my, stencil, stencil + ny, P0, g1, thbc,
-             cv, gxx, gyy, out, mx, ny, kscale, dt, F);
+  RBC_LAUNCH(rhs3_fused_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s, U, my, stencil, stencil + ny, P0,
+             g1, thbc, cv, gxx, gyy, out, mx, ny, kscale, dt, F);
   check_launch("rhs3_fused");
+}
+void rhs3_fused_cv32(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
+                     const double* thbc, const float* cv, const double* gxx, const double* gyy, double* out, int mx,
+                     int ny, double kscale, double dt, long F) {
+  const long n = (long)2 * mx * ny;
+  RBC_LAUNCH(rhs3_fused_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, U, my, stencil, stencil + ny, P0,
+             g1, thbc, cv, gxx, gyy, out, mx, ny, kscale, dt, F);
+  check_launch("rhs3_fused_cv32");
 }
 
 EW_KERNEL(ew_update_pres, (double* p, const double* d, const double* phi, double nu, double inv_dt, long n),

@@ -97,6 +97,10 @@ void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const
 void rhs3_fused(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                 const double* thbc, const double* cv, const double* gxx, const double* gyy, double* out, int mx, int ny,
                 double kscale, double dt, long F);
+// the same with FP32 convective terms (the mixed-precision step)
+void rhs3_fused_cv32(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
+                     const double* thbc, const float* cv, const double* gxx, const double* gyy, double* out, int mx,
+                     int ny, double kscale, double dt, long F);
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n);              // f (-c) + u
 void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt,
                     long n);  // (p - div nu) + phi inv_dt
@@ -126,6 +130,7 @@ bool fft_supported(int n);
 // values, rfft from FP32 values to FP64 spectra (arithmetic in FP32).
 void rfft_cols_f32(hipStream_t s, const float* X, double* Y, int n, int cols, int cut, int batch = 1, long sx = 0,
                    long sy = 0);
+void rfft_cols_f32f(hipStream_t s, const float* X, float* Y, int n, int cols, int cut, int batch, long sx, long sy);
 void irfft_cols_f32(hipStream_t s, const double* X, float* Y, int n, int cols, int times, double sc, int batch = 1,
                     long sx = 0, long sy = 0);
 void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch = 1, long sx = 0,

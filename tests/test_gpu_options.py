@@ -60,3 +60,31 @@ def test_graph_replay_matches_eager(oracle, periodic):
         for a, b in ((ge.temp, gg.temp), (ge.ux, gg.ux), (ge.uy, gg.uy), (ge.pres[0], gg.pres[0])):
             assert np.array_equal(a.vhat, b.vhat), rep
     assert abs(ge.get_time() - gg.get_time()) < 1e-15
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+def test_gemm_tile_shapes_bitwise(oracle, monkeypatch, tile):
+    """Every MFMA GEMM tile shape (64x64, 32x32, 64x32, 32x64; normally picked per launch
+    shape by the autotuner) gives bit-identical steps: the accumulation order over K does
+    not depend on the tile.  Confined 48x65 (dense x transforms and ADI operators) and an
+    ensemble group of three instances (batched GEMMs)."""
+    from rustpde_amd import Ensemble
+    print("case gemm tile %d" % tile, flush=True)
+    results = []
+    for forced in (None, tile):
+        if forced is None:
+            monkeypatch.delenv("RUSTPDE_AMD_GEMM_TILE", raising=False)
+        else:
+            monkeypatch.setenv("RUSTPDE_AMD_GEMM_TILE", str(forced))
+        g, o = _pair(oracle, False)
+        g.update(5)
+        ens = Ensemble([(32, 33, 1e4 * (1 + i), 1.0, 0.01, 1.0, True) for i in range(3)])
+        for i in range(3):
+            ens[i].set_velocity(0.2, 1.0, 1.0)
+        ens.update(3)
+        results.append([g.temp.vhat.copy(), g.pres[0].vhat.copy()] + [ens[i].uy.vhat.copy() for i in range(3)])
+        ens.close()
+        g.close()
+    monkeypatch.delenv("RUSTPDE_AMD_GEMM_TILE", raising=False)
+    for a, b in zip(results[0], results[1]):
+        assert np.array_equal(a, b)
