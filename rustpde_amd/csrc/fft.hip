@@ -114,6 +114,9 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __res
     sc_d = sc1;
   }
   const V sc = (V)sc_d;
+  // sc a power of two (and normal): the complex division by the real sc is a product
+  const bool pow2 = sc_d > 0.0 && (__double_as_longlong(sc_d) & 0xFFFFFFFFFFFFFLL) == 0;
+  const V isc = (V)(1.0 / sc_d);
   X += sx * b;
   Y += sy * b;
   const int tid = threadIdx.x;
@@ -141,10 +144,15 @@ __global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __res
       b = nb;
     }
     if (sc != (V)1) {
-      const V ns = sc * sc + zero * zero;
-      const V na = (a * sc + b * zero) / ns, nb = (b * sc - a * zero) / ns;
-      a = na;
-      b = nb;
+      if (pow2) {  // (a sc + 0) / (sc sc) is exactly a / sc = a isc for a power-of-two sc
+        a = a * isc;
+        b = b * isc;
+      } else {
+        const V ns = sc * sc + zero * zero;
+        const V na = (a * sc + b * zero) / ns, nb = (b * sc - a * zero) / ns;
+        a = na;
+        b = nb;
+      }
     }
     V* z = (c & 1) ? im : re;
     const int zb = (c >> 1) * L;

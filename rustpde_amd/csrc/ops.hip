@@ -239,10 +239,17 @@ __global__ void rhs3_fused_kernel(const double* __restrict__ U, int my, const do
   const double pa = P0[(size_t)k * ny + i], pb = P0[(size_t)(mx + k) * ny + i], kk = (double)k;
   double na = pa * 0.0 - pb * kk, nb = pa * kk + pb * 0.0;
   if (kscale != 1.0) {
-    const double ns = kscale * kscale + 0.0 * 0.0;
-    const double qa = (na * kscale + nb * 0.0) / ns, qb = (nb * kscale - na * 0.0) / ns;
-    na = qa;
-    nb = qb;
+    if (kscale > 0.0 && (__double_as_longlong(kscale) & 0xFFFFFFFFFFFFFLL) == 0) {
+      // power-of-two scale: (x kscale + 0) / (kscale kscale) is exactly x / kscale
+      const double isc = 1.0 / kscale;
+      na = na * isc;
+      nb = nb * isc;
+    } else {
+      const double ns = kscale * kscale + 0.0 * 0.0;
+      const double qa = (na * kscale + nb * 0.0) / ns, qb = (nb * kscale - na * 0.0) / ns;
+      na = qa;
+      nb = qb;
+    }
   }
   const double gx = imag ? nb : na;
   const long j = (long)r * ny + i;

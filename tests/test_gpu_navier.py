@@ -77,3 +77,26 @@ def test_diagnostics(oracle, periodic):
         _flush("%s gpu=%.15e oracle=%.15e" % (name, a, b))
         assert abs(a - b) <= 1e-11 * max(abs(b), 1e-3), (name, a, b)
     assert g.exit() is False
+
+
+@pytest.mark.parametrize("aspect", [1.0, 1.5, 3.0])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_periodic_aspect_ratios(oracle, aspect, mixed):
+    """Periodic step at x scales other than 2 (scale[0] = aspect): 1 (no Fourier scaling),
+    1.5 (the num-complex division by a non-power-of-two scale) and 3, in the FP64 and the
+    mixed step (the power-of-two scales take an exact product instead of the division)."""
+    from rustpde_amd import Navier2D
+    nx, ny = 128, 129
+    _flush("case periodic aspect=%g mixed=%s %dx%d" % (aspect, mixed, nx, ny))
+    g = Navier2D.new_periodic(nx, ny, 1e6, 1.0, 0.004, aspect)
+    o = oracle.Navier2D.new_periodic(nx, ny, 1e6, 1.0, 0.004, aspect)
+    init_pair(g, o, nx, ny, True, 1e-3, 17, "gaussian", velocity=(0.05, 2.0, 1.0))
+    if mixed:
+        g.set_option(7, 1)
+    g.update(10)
+    o.update(10)
+    e = compare(g, o)
+    _flush("aspect %g mixed %s errors: %s" % (aspect, mixed, e))
+    tol = 1e-5 if mixed else 1e-12
+    for k in ("temp", "ux", "uy", "pres"):
+        assert e[k] < tol, (k, e)
