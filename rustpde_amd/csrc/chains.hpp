@@ -303,6 +303,136 @@ __device__ __forceinline__ void tdma_backward(double (&v)[SLICE], const double (
   }
 }
 
+// The two tdma passes for two right-hand sides with the same tables at once (the x and y
+// projections of the pressure stage): the maps' multipliers depend on the tables only, so
+// they are composed and scanned once; each right-hand side keeps its own offset.  Every
+// value is formed as by tdma_forward / tdma_backward.
+template <int SLICE>
+__device__ __forceinline__ void tdma_forward2(double (&v)[SLICE], double (&u)[SLICE], const double (&of)[SLICE],
+                                              const double (&rn)[SLICE], int lane, int t0, int cnt) {
+#pragma clang fp contract(off)
+  constexpr int H = SLICE / 2;
+  double A1 = 1.0, B1 = 0.0, C1 = 0.0, A2 = 1.0, B2 = 0.0, C2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j < cnt) {
+      if (t0 + j == 0) {
+        A1 = 0.0;
+        B1 = v[j] * rn[j];
+        C1 = u[j] * rn[j];
+      } else {
+        A1 = -(of[j] * rn[j]) * A1;
+        B1 = (v[j] - B1 * of[j]) * rn[j];
+        C1 = (u[j] - C1 * of[j]) * rn[j];
+      }
+    }
+    const int q = H + j;
+    if (q < cnt) {
+      A2 = -(of[q] * rn[q]) * A2;
+      B2 = (v[q] - B2 * of[q]) * rn[q];
+      C2 = (u[q] - C2 * of[q]) * rn[q];
+    }
+  }
+  double pa = A2 * A1, pb = A2 * B1 + B2, pc = A2 * C1 + C2;
+  for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
+    const double qa = __shfl_up(pa, o2), qb = __shfl_up(pb, o2), qc = __shfl_up(pc, o2);
+    if (lane >= o2) {
+      pb = pa * qb + pb;
+      pc = pa * qc + pc;
+      pa = pa * qa;
+    }
+  }
+  double g1v = __shfl_up(pb, 1), h1v = __shfl_up(pc, 1);
+  if (lane == 0) {
+    g1v = 0.0;
+    h1v = 0.0;
+  }
+  double g2v = A1 * g1v + B1, h2v = A1 * h1v + C1;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j < cnt) {
+      g1v = (t0 + j == 0) ? v[j] * rn[j] : (v[j] - g1v * of[j]) * rn[j];
+      h1v = (t0 + j == 0) ? u[j] * rn[j] : (u[j] - h1v * of[j]) * rn[j];
+      v[j] = g1v;
+      u[j] = h1v;
+    }
+    const int q = H + j;
+    if (q < cnt) {
+      g2v = (v[q] - g2v * of[q]) * rn[q];
+      h2v = (u[q] - h2v * of[q]) * rn[q];
+      v[q] = g2v;
+      u[q] = h2v;
+    }
+  }
+}
+
+template <int SLICE>
+__device__ __forceinline__ void tdma_backward2(double (&v)[SLICE], double (&u)[SLICE], const double (&wv)[SLICE],
+                                               int lane, int t0, int cnt, int p, int m) {
+#pragma clang fp contract(off)
+  constexpr int H = SLICE / 2;
+  double A1 = 1.0, B1 = 0.0, C1 = 0.0, A2 = 1.0, B2 = 0.0, C2 = 0.0;  // L and U, top-down
+#pragma unroll
+  for (int j = H - 1; j >= 0; --j) {
+    if (j < cnt) {
+      if (p + 2 * (t0 + j) >= m - 2) {
+        A1 = 0.0;
+        B1 = v[j];
+        C1 = u[j];
+      } else {
+        A1 = -wv[j] * A1;
+        B1 = v[j] - B1 * wv[j];
+        C1 = u[j] - C1 * wv[j];
+      }
+    }
+    const int q = H + j;
+    if (q < cnt) {
+      if (p + 2 * (t0 + q) >= m - 2) {
+        A2 = 0.0;
+        B2 = v[q];
+        C2 = u[q];
+      } else {
+        A2 = -wv[q] * A2;
+        B2 = v[q] - B2 * wv[q];
+        C2 = u[q] - C2 * wv[q];
+      }
+    }
+  }
+  double pa = A1 * A2, pb = A1 * B2 + B1, pc = A1 * C2 + C1;
+  for (int o2 = 1; o2 < WAVE; o2 <<= 1) {
+    const double qa = __shfl_down(pa, o2), qb = __shfl_down(pb, o2), qc = __shfl_down(pc, o2);
+    if (lane + o2 < WAVE) {
+      pb = pa * qb + pb;
+      pc = pa * qc + pc;
+      pa = pa * qa;
+    }
+  }
+  double x2 = __shfl_down(pb, 1), y2 = __shfl_down(pc, 1);
+  if (lane == WAVE - 1) {
+    x2 = 0.0;
+    y2 = 0.0;
+  }
+  double x1 = A2 * x2 + B2, y1 = A2 * y2 + C2;
+#pragma unroll
+  for (int j = H - 1; j >= 0; --j) {
+    if (j < cnt) {
+      const bool top = p + 2 * (t0 + j) >= m - 2;
+      x1 = top ? v[j] : v[j] - x1 * wv[j];
+      y1 = top ? u[j] : u[j] - y1 * wv[j];
+      v[j] = x1;
+      u[j] = y1;
+    }
+    const int q = H + j;
+    if (q < cnt) {
+      const bool top = p + 2 * (t0 + q) >= m - 2;
+      x2 = top ? v[q] : v[q] - x2 * wv[q];
+      y2 = top ? u[q] : u[q] - y2 * wv[q];
+      v[q] = x2;
+      u[q] = y2;
+    }
+  }
+}
+
 // suffix sums along the chain: v_t <- v_t + v_{t+1} + ... (the derivative recurrence
 // d_i = g_i + d_{i+2} of ortho.rs:107-125, summed from the top as the reference does)
 template <int SLICE>

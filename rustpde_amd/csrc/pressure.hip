@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       if (q < cnt) YB(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
   }
   __syncthreads();
-  // (4a) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt (half block h: plane h)
+  // (4) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt (half block h: plane h)
   {
     double pv[QH], c0[QH], c1[QH];
     double* prow = a.P0 + ROWN(h);
@@ -233,68 +233,8 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       prow[i] = t + w3 * a.inv_dt;
     }
   }
-  // (4b) ux projection: d_i = g_i diag_i + g_{i+2} low2_i with g = S_p(ik pseu / sc), then
-  //      the tdma recurrences of from_ortho
-#define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
-  // the from_ortho tables of the velocity basis: diag, low2 and off_{i-2} = diag_i low2_{i-2}
-  // are 1, -1, -1 for the Dirichlet basis (0 for off at the chain start)
-#define TB_DG(q) (a.su_kind == 1 ? 1.0 : TB(0, q))
-#define TB_L2(q) (a.su_kind == 1 ? -1.0 : TB(1, q))
-#define TB_OF(q) (a.su_kind == 1 ? (t0 + (q) >= 1 ? -1.0 : 0.0) : TB(2, q))
-  {
-    double v[SLICE];
-    {
-      double dg[SLICE], l2[SLICE];
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        dg[q] = TB_DG(q);
-        l2[q] = TB_L2(q);
-      }
-      // plane r of ik pseu / sc at the chain elements i0 - 2 + 2 j (j <= cnt + 1) and the
-      // stencil taps at i0 + 2 j (j <= cnt), each formed once for the slice
-      const int i0 = p + 2 * t0;
-      const double* Xo = YB(1 - r);
-      double cs[SLICE + 2], ta[SLICE + 1], tb[SLICE + 1];
-#pragma unroll
-      for (int j = 0; j < SLICE + 2; ++j)
-        cs[j] = (cnt > 0 && j <= cnt + 1) ? fdiff_plane(Xo[pad(i0 + 2 * j)], r, kk, a.kscale, a.isc) : 0.0;
-#pragma unroll
-      for (int j = 0; j < SLICE + 1; ++j) {
-        ta[j] = (cnt > 0 && j <= cnt) ? SP0(i0 + 2 * j) : 0.0;
-        tb[j] = (cnt > 0 && j <= cnt) ? SP1(i0 + 2 * j) : 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        v[q] = 0.0;
-        if (q < cnt) {
-          const double g0 = cs[q + 1] * ta[q] + cs[q] * tb[q];
-          const double g2 = cs[q + 2] * ta[q + 1] + cs[q + 1] * tb[q + 1];
-          v[q] = g0 * dg[q] + g2 * l2[q];
-        }
-      }
-    }
-    {
-      double of[SLICE], rn[SLICE], wv[SLICE];  // off_{i-2}, 1 / den_i, w_i
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        of[q] = TB_OF(q);
-        rn[q] = TB(3, q);
-        wv[q] = TB(4, q);
-      }
-      chains::tdma_forward(v, of, rn, lane, t0, cnt);
-      chains::tdma_backward(v, wv, lane, t0, cnt, p, mu);
-    }
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q)
-      if (q < cnt) {  // XA still holds the staged ux: ux = x (-1) + ux in place
-        double* e = XA(r) + pad(p + 2 * (t0 + q) + 2);
-        *e = v[q] * (-1.0) + *e;
-      }
-  }
   __syncthreads();
-  // (5a) the updated ux rows -> HBM
-  chains::unstage_row<QH>(XA(h), a.UX + ROWU(h), true, mu, th, 128, false);
-  // (5b) d/dy S_p(pseu) into DV (the chains of (2b))
+  // (5) d/dy S_p(pseu) into DV (the chains of (2b); div is no longer needed)
   {
     double v[SLICE];
     const double* Pp = YB(r);
@@ -327,9 +267,17 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     }
   }
   __syncthreads();
-  // (6) uy projection: tdma of d_i = g_i diag_i + g_{i+2} low2_i with g = DV
+  // (6) both projections, from_ortho of g = S_p(ik pseu / sc) (ux) and of g = DV (uy):
+  //     d_i = g_i diag_i + g_{i+2} low2_i, then the tdma recurrences with the tables shared
+  //     by the two right-hand sides
+#define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
+  // the from_ortho tables of the velocity basis: diag, low2 and off_{i-2} = diag_i low2_{i-2}
+  // are 1, -1, -1 for the Dirichlet basis (0 for off at the chain start)
+#define TB_DG(q) (a.su_kind == 1 ? 1.0 : TB(0, q))
+#define TB_L2(q) (a.su_kind == 1 ? -1.0 : TB(1, q))
+#define TB_OF(q) (a.su_kind == 1 ? (t0 + (q) >= 1 ? -1.0 : 0.0) : TB(2, q))
   {
-    double v[SLICE];
+    double vx[SLICE], vy[SLICE];
     {
       double dg[SLICE], l2[SLICE];
 #pragma unroll
@@ -337,10 +285,30 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
         dg[q] = TB_DG(q);
         l2[q] = TB_L2(q);
       }
+      // plane r of ik pseu / sc at the chain elements i0 - 2 + 2 j (j <= cnt + 1) and the
+      // stencil taps at i0 + 2 j (j <= cnt), each formed once for the slice
+      const int i0 = p + 2 * t0;
+      const double* Xo = YB(1 - r);
+      double cs[SLICE + 2], ta[SLICE + 1], tb[SLICE + 1];
+#pragma unroll
+      for (int j = 0; j < SLICE + 2; ++j)
+        cs[j] = (cnt > 0 && j <= cnt + 1) ? fdiff_plane(Xo[pad(i0 + 2 * j)], r, kk, a.kscale, a.isc) : 0.0;
+#pragma unroll
+      for (int j = 0; j < SLICE + 1; ++j) {
+        ta[j] = (cnt > 0 && j <= cnt) ? SP0(i0 + 2 * j) : 0.0;
+        tb[j] = (cnt > 0 && j <= cnt) ? SP1(i0 + 2 * j) : 0.0;
+      }
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
-        const int j = p + 2 * (t0 + q) + 2;
-        v[q] = q < cnt ? DV(r)[pad(j)] * dg[q] + DV(r)[pad(j + 2)] * l2[q] : 0.0;
+        vx[q] = 0.0;
+        vy[q] = 0.0;
+        if (q < cnt) {
+          const double g0 = cs[q + 1] * ta[q] + cs[q] * tb[q];
+          const double g2 = cs[q + 2] * ta[q + 1] + cs[q + 1] * tb[q + 1];
+          vx[q] = g0 * dg[q] + g2 * l2[q];
+          const int j = p + 2 * (t0 + q) + 2;
+          vy[q] = DV(r)[pad(j)] * dg[q] + DV(r)[pad(j + 2)] * l2[q];
+        }
       }
     }
     {
@@ -351,12 +319,18 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
         rn[q] = TB(3, q);
         wv[q] = TB(4, q);
       }
-      chains::tdma_forward(v, of, rn, lane, t0, cnt);
-      chains::tdma_backward(v, wv, lane, t0, cnt, p, mu);
+      chains::tdma_forward2(vx, vy, of, rn, lane, t0, cnt);
+      chains::tdma_backward2(vx, vy, wv, lane, t0, cnt, p, mu);
     }
+    // ux = x (-1) + ux in place (XA still holds the staged ux); the uy projection replaces
+    // the entries of DV that only this wavefront reads
 #pragma unroll
     for (int q = 0; q < SLICE; ++q)
-      if (q < cnt) XA(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
+      if (q < cnt) {
+        const int j = pad(p + 2 * (t0 + q) + 2);
+        XA(r)[j] = vx[q] * (-1.0) + XA(r)[j];
+        DV(r)[j] = vy[q];
+      }
   }
 #undef TB
 #undef TB_DG
@@ -367,8 +341,9 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #undef SP0
 #undef SP1
   __syncthreads();
-  // (7) uy = x (-1) + uy
-  chains::unstage_row<QH>(XA(h), a.UY + ROWU(h), true, mu, th, 128, true);
+  // (7) the updated ux rows -> HBM; uy = x (-1) + uy (uy reloaded: its staged copy became pseu)
+  chains::unstage_row<QH>(XA(h), a.UX + ROWU(h), true, mu, th, 128, false);
+  chains::unstage_row<QH>(DV(h), a.UY + ROWU(h), true, mu, th, 128, true);
 #undef XA
 #undef YB
 #undef DV
