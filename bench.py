@@ -164,6 +164,26 @@ def bench_ensemble(args):
     el = time.perf_counter() - t0
     n = len(descs)
     points = sum(d[0] * d[1] for d in descs)
+    # per-launch timing of every group over the same number of steps (eager launches)
+    ens.timing(True)
+    t1 = time.perf_counter()
+    ens.update(args.steps)
+    el_timed = time.perf_counter() - t1
+    stats = {k: ens.timing_read(k) for k in KERNEL_CLASSES}
+    ens.timing(False)
+    dom = max((k for k in stats if k != MIXED_CLASS), key=lambda k: stats[k][1])
+    ln, lms, lw = stats[dom]
+    if dom == 0:
+        achieved = lw / (lms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
+    else:
+        achieved = lw / (lms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None}
+    roof.update({"kernel": KERNEL_CLASSES[dom], "launches": ln, "avg_launch_us": 1e3 * lms / max(ln, 1),
+                 "work_per_launch": lw / max(ln, 1), "share_of_kernel_time": lms / max(1e-30, sum(v[1] for v in stats.values())),
+                 "note": "summed over the groups, which run concurrently on three streams"})
     out = {"metric": "ensemble instance-steps/s (96 confined RBC instances, FP64)", "value": n * args.steps / el,
            "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -171,7 +191,9 @@ def bench_ensemble(args):
            "config": {"workload": "ensemble of 96 confined RBC instances, N in {64,96,128,192}, Ra log-uniform "
                                   "[1e4,1e7], dt = 0.01*64/N (FP64)", "instances": n, "groups": ens.groups,
                       "sizes": {str(k): sum(1 for d in descs if d[0] == k) for k in (64, 96, 128, 192)}},
-           "grid_point_updates_per_s": points * args.steps / el}
+           "grid_point_updates_per_s": points * args.steps / el, "roofline": roof,
+           "ms_per_step_with_kernel_events": 1e3 * el_timed / args.steps,
+           "kernel_classes": {KERNEL_CLASSES[k]: {"launches": v[0], "ms": v[1]} for k, v in stats.items()}}
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         threads = max(1, min(16, os.cpu_count() or 1))

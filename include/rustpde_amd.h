@@ -223,6 +223,10 @@ int rbc_ensemble_instance(rbc_ensemble* h, int inst, rbc_navier** out);
 /* Integrate::update of every instance, nsteps times (blocks until done). */
 int rbc_ensemble_update(rbc_ensemble* h, int nsteps);
 int rbc_ensemble_synchronize(rbc_ensemble* h);
+/* Kernel timing of all groups (as rbc_timing_enable / rbc_timing_read of a handle, summed
+ * over the groups; no counterpart in the reference). */
+int rbc_ensemble_timing_enable(rbc_ensemble* h, int on);
+int rbc_ensemble_timing_read(rbc_ensemble* h, int kernel_class, long long* launches, double* total_ms, double* work);
 
 /* ------------------------------------------------------------ device memory helpers
  * For callers without their own HIP runtime binding (e.g. ctypes). */

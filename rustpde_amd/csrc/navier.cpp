@@ -2083,6 +2083,38 @@ int rbc_ensemble_synchronize(rbc_ensemble* h) {
     h->e->synchronize();
   });
 }
+int rbc_ensemble_timing_enable(rbc_ensemble* h, int on) {
+  return guarded([&] {
+    if (!h || !h->e) throw std::invalid_argument("null handle");
+    h->e->synchronize();
+    for (auto& g : h->e->groups) {
+      Navier& L = *g->navs[0];  // the leader launches the group's kernels
+      L.drop_graph();
+      L.timer.reset();
+      L.timer.on = on != 0;
+    }
+  });
+}
+int rbc_ensemble_timing_read(rbc_ensemble* h, int kernel_class, long long* launches, double* total_ms, double* work) {
+  return guarded([&] {
+    if (!h || !h->e) throw std::invalid_argument("null handle");
+    if (kernel_class < 0 || kernel_class >= rbc::KernelTimer::NCAT) throw std::invalid_argument("kernel class");
+    h->e->synchronize();
+    long long nt = 0;
+    double mt = 0.0, wt = 0.0;
+    for (auto& g : h->e->groups) {
+      long long n;
+      double ms, w;
+      g->navs[0]->timer.read(kernel_class, n, ms, w);
+      nt += n;
+      mt += ms;
+      wt += w;
+    }
+    if (launches) *launches = nt;
+    if (total_ms) *total_ms = mt;
+    if (work) *work = wt;
+  });
+}
 
 int rbc_dev_malloc(void** out, unsigned long long bytes) {
   return guarded([&] { HIPCHECK(hipMalloc(out, bytes)); });

@@ -49,6 +49,17 @@ class Ensemble:
     def synchronize(self):
         check(_lib.load().rbc_ensemble_synchronize(self._h))
 
+    def timing(self, on):
+        """Per-launch kernel timing of every group (steps run without graph replay)."""
+        check(_lib.load().rbc_ensemble_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self, kernel_class):
+        """(launches, summed ms, summed work) of one kernel class over all groups."""
+        n, ms, w = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+        check(_lib.load().rbc_ensemble_timing_read(self._h, int(kernel_class), ctypes.byref(n), ctypes.byref(ms),
+                                                   ctypes.byref(w)))
+        return n.value, ms.value, w.value
+
     def close(self):
         if getattr(self, "_h", None):
             for nav in self.instances:
