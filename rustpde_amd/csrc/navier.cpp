@@ -1697,7 +1697,7 @@ static int guarded(F&& f) {
 extern "C" {
 
 const char* rbc_last_error(void) { return g_err.c_str(); }
-const char* rbc_version(void) { return "rustpde_amd 0.1 (gfx950, fp64)"; }
+const char* rbc_version(void) { return "rustpde_amd 0.2 (gfx950; FP64 and mixed FP32/FP64 steps)"; }
 
 int rbc_create_confined(int nx, int ny, double ra, double pr, double dt, double aspect, int adiabatic,
                         rbc_navier** out) {
