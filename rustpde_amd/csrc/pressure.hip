@@ -28,29 +28,10 @@ using chains::buf_len;
 using chains::pad;
 using chains::WAVE;
 
-// (a + ib)(0 + ik) then / (sc + 0i), as num-complex forms them (fourier_diff, times = 1).
-// isc = 1 / sc when sc is a power of two (else 0): the division (x sc) / (sc sc) is then
-// exactly the product x isc.
-__device__ __forceinline__ void fdiff(double a, double b, double kk, double sc, double isc, double& ra, double& rb) {
-#pragma clang fp contract(off)
-  double na = a * 0.0 - b * kk, nb = a * kk + b * 0.0;
-  if (sc != 1.0) {
-    if (isc != 0.0) {
-      na = na * isc;
-      nb = nb * isc;
-    } else {
-      const double ns = sc * sc + 0.0 * 0.0;
-      const double qa = (na * sc + nb * 0.0) / ns, qb = (nb * sc - na * 0.0) / ns;
-      na = qa;
-      nb = qb;
-    }
-  }
-  ra = na;
-  rb = nb;
-}
-// plane r of the same from the other plane's value x: r = 0 gives (0 - x k) / sc (x the
-// imaginary part), r = 1 gives (x k + 0) / sc (x the real part); equal to fdiff up to the
-// sign of a zero result
+// Plane r of (a + ib)(0 + ik) / (sc + 0i) as num-complex forms it (fourier_diff, times = 1),
+// from the other plane's value x: r = 0 gives (a 0 - x k) / sc (x = b), r = 1 gives
+// (x k + b 0) / sc (x = a); the products with 0 only decide the sign of a zero result.
+// isc = 1 / sc when sc is a power of two (else 0): (y sc + 0) / (sc sc) is then exactly y isc.
 __device__ __forceinline__ double fdiff_plane(double x, int r, double kk, double sc, double isc) {
 #pragma clang fp contract(off)
   double v = r ? x * kk : 0.0 - x * kk;
@@ -96,45 +77,32 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
   // rows of up to 2 SLICE WAVE + 1 values: unrolled staging bounds (all loads before stores)
   constexpr int QW = (2 * SLICE * WAVE + 8 + WAVE - 1) / WAVE;  // per wavefront
   constexpr int QH = (2 * SLICE * WAVE + 8 + 127) / 128;       // per half block
-  constexpr int QB = (2 * SLICE * WAVE + 8 + 255) / 256;       // per block
   const int h = tid >> 7, th = tid & 127;                      // half block: plane h
   // (1) ux, uy rows -> LDS (element i at pad(i + 2), zero guards at i in [-2, 0), [mu, mu + 4)):
-  //     wavefront w stages ux re, ux im, uy re, uy im
+  //     wavefront w stages ux re, ux im, uy re, uy im; the zero guards of div around
+  //     i in [0, ny) (read by the Poisson band) are set here
   chains::stage_row<QW>(lds + w * bl, (w < 2 ? a.UX : a.UY) + ROWU(w & 1), true, mu, lane, WAVE);
-  __syncthreads();
-  // (2a) div = S_u(ik ux / sc) for i < ny (taps i, i - 2), zero guards around it
-  {
-    double s0[QB], s1[QB];
-#pragma unroll
-    for (int q = 0; q < QB; ++q) {
-      const int i = tid + q * 256 - 2;
-      s0[q] = (i >= 0 && i < ny) ? SU0(i) : 0.0;
-      s1[q] = (i >= 0 && i < ny) ? SU1(i) : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < QB; ++q) {
-      const int j = tid + q * 256, i = j - 2;
-      if (j >= ny + 6) continue;
-      double d0 = 0.0, d1 = 0.0;
-      if (i >= 0 && i < ny) {
-        double c0r, c0i, c2r, c2i;
-        fdiff(XA(0)[pad(j)], XA(1)[pad(j)], kk, a.kscale, a.isc, c0r, c0i);
-        fdiff(XA(0)[pad(j - 2)], XA(1)[pad(j - 2)], kk, a.kscale, a.isc, c2r, c2i);
-        d0 = c0r * s0[q] + c2r * s1[q];
-        d1 = c0i * s0[q] + c2i * s1[q];
-      }
-      DV(0)[pad(j)] = d0;
-      DV(1)[pad(j)] = d1;
-    }
+  if (tid < 12) {  // plane tid & 1; i = j - 2 in {-2, -1} (tid < 4) or [ny, ny + 4)
+    const int j = tid < 4 ? (tid >> 1) : ny + 2 + ((tid - 4) >> 1);
+    DV(tid & 1)[pad(j)] = 0.0;
   }
   __syncthreads();
-  // (2b) div += d/dy S_u(uy): g_i = 2 (i + 1) s_{i+1} summed from the top along the chains
-  //      1, 3, .., N - 1 and 2, 4, .., N; d_0 = s_1 + d_2 / 2 (ortho.rs:107-125)
+  // (2) div = S_u(ik ux / sc) + d/dy S_u(uy) for i < ny.  The derivative: g_i = 2 (i + 1) s_{i+1}
+  //     summed from the top along the chains 1, 3, .., N - 1 and 2, 4, .., N, then
+  //     d_0 = s_1 + d_2 / 2 (ortho.rs:107-125); each entry then adds the x part (taps i,
+  //     i - 2 of plane r of ik ux / sc) as the unfused kernels add the two terms
   const int Cd = N / 2, perd = (Cd + WAVE - 1) / WAVE, t0d = lane * perd;
   const int cntd = max(0, min(perd, Cd - t0d));
   {
     double v[SLICE];
     const double* Y = YB(r);
+    const double* Xo = XA(1 - r);
+    // x part of entry i: (ik ux / sc)_r at i and i - 2 with the stencil taps
+    auto divx = [&](int i) {
+      const double c0 = fdiff_plane(Xo[pad(i + 2)], r, kk, a.kscale, a.isc);
+      const double c2 = fdiff_plane(Xo[pad(i)], r, kk, a.kscale, a.isc);
+      return c0 * SU0(i) + c2 * SU1(i);
+    };
 #pragma unroll
     for (int q = 0; q < SLICE; ++q) {
       v[q] = 0.0;
@@ -153,14 +121,14 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
         const int i = (p ? 1 : 2) + 2 * (t0d + q);
         double d = v[q];
         if (a.inv_scy != 1.0) d = d * a.inv_scy;
-        DV(r)[pad(i + 2)] = d + 1.0 * DV(r)[pad(i + 2)];
+        DV(r)[pad(i + 2)] = d + 1.0 * divx(i);
       }
     }
     if (p == 0 && lane == 0) {
       const double s1 = Y[pad(3)] * SU0(1) + Y[pad(1)] * SU1(1);
       double d = s1 + v[0] / 2.0;
       if (a.inv_scy != 1.0) d = d * a.inv_scy;
-      DV(r)[pad(2)] = d + 1.0 * DV(r)[pad(2)];
+      DV(r)[pad(2)] = d + 1.0 * divx(0);
     }
   }
   __syncthreads();
