@@ -37,3 +37,24 @@ def test_null_handles_are_rejected_without_device():
     out = ctypes.c_double()
     assert lib.rbc_eval(None, 0, ctypes.byref(out)) != 0
     assert lib.fs_forward(None, None, None, 1) != 0
+
+
+def test_invalid_arguments_are_rejected_before_any_device_call():
+    """Shape and kind errors come back as RBC_ERR_ARG with a message (the reference panics on
+    them); the checks run before the library touches the device."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.rbc_create_periodic(17, 33, 1e5, 1.0, 0.01, 2.0, ctypes.byref(h)) != 0
+    assert b"even" in lib.rbc_last_error()
+    assert lib.rbc_create_confined(4, 4, 1e5, 1.0, 0.01, 1.0, 1, ctypes.byref(h)) != 0
+    assert b">= 8" in lib.rbc_last_error()
+    assert lib.rbc_create_ensemble(0, None, ctypes.byref(h)) != 0
+    desc = (_lib.RbcDesc * 1)(_lib.RbcDesc(4, 4, 1e5, 1.0, 0.01, 1.0, 1))
+    assert lib.rbc_create_ensemble(1, ctypes.cast(desc, ctypes.c_void_p), ctypes.byref(h)) != 0
+    assert b">= 8" in lib.rbc_last_error()
+    assert lib.fs_space2_create(1, 16, 4, 16, ctypes.byref(h)) != 0  # Fourier on axis 1
+    assert b"axis 1" in lib.rbc_last_error()
+    assert lib.fs_space2_create(9, 16, 1, 16, ctypes.byref(h)) != 0
+    assert b"kind" in lib.rbc_last_error()
+    assert lib.rbc_solver_create(7, 1, 16, 1, 16, 1e-3, 1e-3, ctypes.byref(h)) != 0
+    assert b"solver" in lib.rbc_last_error()
