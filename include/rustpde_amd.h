@@ -111,7 +111,9 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *   RBC_OPT_DCT    Chebyshev y transforms by FFT-based DCT-I kernels when ny - 1 is a
  *                  power of two (default 1; 0 = dense transform GEMMs)
  *   RBC_OPT_BATCH  periodic step: the transforms of ux, uy and temp and the ux/uy solves
- *                  run as batched launches (default 1; needs the FFT, DCT and band paths)
+ *                  run as batched launches (default 1; needs the FFT, DCT and band paths);
+ *                  confined step (and ensemble groups): ux and uy's transforms, products,
+ *                  ADI solves and projection as launches of twice the batch
  *   RBC_OPT_FUSE   periodic batched step: the convective terms in two launches (values and
  *                  x-derivatives of ux, uy, temp from one x-transform launch, then one
  *                  y-transform launch forming d/dy q uy + dq/dx ux) and divergence, Poisson

@@ -22,11 +22,15 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Batch entry z reads A at sA (z % amod), B at sB (z % bmod), U at sU (z % umod) (a modulus
+// of 0: z itself) and writes C at sC z: operands shared by groups of entries (an operator
+// per instance, used by several fields of it) need no copies.
 template <int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, const double* __restrict__ A, int lda,
                                                         long sA, const double* __restrict__ B, int ldb, long sB,
                                                         double* __restrict__ C, int ldc, long sC, double alpha,
-                                                        double beta, const double* __restrict__ U, int ldu, long sU) {
+                                                        double beta, const double* __restrict__ U, int ldu, long sU,
+                                                        int amod, int bmod, int umod) {
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
   constexpr int NA = BM * BK / 256, NB = BK * BN / 256;
   __shared__ double As[BK][BM + 1];
@@ -35,10 +39,11 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
   const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int lr = lane & 15, lk = lane >> 4;
-  A += sA * blockIdx.z;  // batch
-  B += sB * blockIdx.z;
-  C += sC * blockIdx.z;
-  if (U) U += sU * blockIdx.z;
+  const int z = blockIdx.z;  // batch entry
+  A += sA * (amod ? z % amod : z);
+  B += sB * (bmod ? z % bmod : z);
+  C += sC * z;
+  if (U) U += sU * (umod ? z % umod : z);
   d4 acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -108,26 +113,29 @@ enum { TILE_64x64, TILE_32x32, TILE_64x32, TILE_32x64, NTILES };
 template <int BM, int BN>
 void launch_tile(hipStream_t s, bool timed, int M, int N, int K, const double* A, int lda, long sA, const double* B,
                  int ldb, long sB, double* C, int ldc, long sC, int batch, double alpha, double beta, const double* U,
-                 int ldu, long sU) {
+                 int ldu, long sU, const BatchMod& bm) {
   dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, batch);
   if (timed) {
     RBC_LAUNCH((gemm_mfma_kernel<BM, BN, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC,
-               alpha, beta, U, ldu, sU);
+               alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u);
   } else {
     hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, 32>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C, ldc,
-                       sC, alpha, beta, U, ldu, sU);
+                       sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u);
   }
 }
 
 void launch_variant(int v, hipStream_t s, bool timed, int M, int N, int K, const double* A, int lda, long sA,
                     const double* B, int ldb, long sB, double* C, int ldc, long sC, int batch, double alpha,
-                    double beta, const double* U, int ldu, long sU) {
+                    double beta, const double* U, int ldu, long sU, const BatchMod& bm) {
+#define RBC_TILE(BM, BN) \
+  launch_tile<BM, BN>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU, bm)
   switch (v) {
-    case TILE_64x64: launch_tile<64, 64>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
-    case TILE_32x32: launch_tile<32, 32>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
-    case TILE_64x32: launch_tile<64, 32>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
-    default: launch_tile<32, 64>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU); break;
+    case TILE_64x64: RBC_TILE(64, 64); break;
+    case TILE_32x32: RBC_TILE(32, 32); break;
+    case TILE_64x32: RBC_TILE(64, 32); break;
+    default: RBC_TILE(32, 64); break;
   }
+#undef RBC_TILE
 }
 
 // Autotuner: the first launch of a shape (device, M, N, K, batch) outside stream capture
@@ -144,7 +152,7 @@ int tile_rule(int M, int N, int batch) {
 }
 
 int tile_for(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb, long sB,
-             int ldc, long sC, int batch) {
+             int ldc, long sC, int batch, const BatchMod& bm) {
   // RUSTPDE_AMD_GEMM_TILE = 0..3 forces one tile shape (for tests of each shape)
   if (const char* f = std::getenv("RUSTPDE_AMD_GEMM_TILE")) {
     const int v = std::atoi(f);
@@ -170,12 +178,12 @@ int tile_for(hipStream_t s, int M, int N, int K, const double* A, int lda, long 
   for (int v = 0; ok && v <= NTILES; ++v) ok = hipEventCreate(&ev[v]) == hipSuccess;
   if (ok) {
     for (int v = 0; v < NTILES; ++v)  // warm-up of every variant
-      launch_variant(v, s, false, M, N, K, A, lda, sA, B, ldb, sB, scratch, ldc, sC, batch, 1.0, 0.0, nullptr, 0, 0);
+      launch_variant(v, s, false, M, N, K, A, lda, sA, B, ldb, sB, scratch, ldc, sC, batch, 1.0, 0.0, nullptr, 0, 0, bm);
     ok = hipEventRecord(ev[0], s) == hipSuccess;
     for (int v = 0; ok && v < NTILES; ++v) {
       for (int r = 0; r < 3; ++r)
         launch_variant(v, s, false, M, N, K, A, lda, sA, B, ldb, sB, scratch, ldc, sC, batch, 1.0, 0.0, nullptr, 0,
-                       0);
+                       0, bm);
       ok = hipEventRecord(ev[v + 1], s) == hipSuccess;
     }
     ok = ok && hipGetLastError() == hipSuccess && hipEventSynchronize(ev[NTILES]) == hipSuccess;
@@ -209,12 +217,13 @@ int tile_for(hipStream_t s, int M, int N, int K, const double* A, int lda, long 
 
 void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
                        long sB, double* C, int ldc, long sC, int batch, double alpha, double beta, const double* U,
-                       int ldu, long sU) {
+                       int ldu, long sU, BatchMod bm) {
+  if (bm.a < 0 || bm.b < 0 || bm.u < 0) throw std::runtime_error("gemm_mfma: negative batch modulus");
   if (M <= 0 || N <= 0 || batch <= 0) return;
   if (K <= 0) throw std::runtime_error("gemm_mfma: K must be positive");
   if (batch > 65535) throw std::runtime_error("gemm_mfma: batch exceeds the grid limit");
-  const int v = tile_for(s, M, N, K, A, lda, sA, B, ldb, sB, ldc, sC, batch);
-  launch_variant(v, s, true, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU);
+  const int v = tile_for(s, M, N, K, A, lda, sA, B, ldb, sB, ldc, sC, batch, bm);
+  launch_variant(v, s, true, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, sU, bm);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string("gemm_mfma: ") + hipGetErrorString(e));
 }

@@ -483,7 +483,20 @@ class Navier {
   DBuf NUARR;
   DBuf ADIH[3][2];
   long adi_s[3][2] = {};
-  long nbn(long n) const { return (long)(nb - 1) * bs + n; }  // element-wise extent over the group
+  // Field batching (confined step): inside a FieldPair scope fb = 2 and every launch covers
+  // ux and uy of every instance, batch entry z = f nb + b (field f, instance b).  Each work
+  // buffer of the velocity part then holds a partner slot pair_off() after it: the slots of
+  // a standalone handle (FS apart), or the second half of the group's slabs.
+  int fb = 1;
+  int nbt() const { return nb * fb; }                            // batch entries of a launch
+  long dstr() const { return nb > 1 ? bs : (long)FS; }           // data stride between entries
+  long pair_off() const { return nb > 1 ? (long)nb * bs : (long)FS; }
+  long nbn(long n) const { return (long)(nbt() - 1) * dstr() + n; }  // element-wise extent
+  struct FieldPair {
+    Navier& n;
+    explicit FieldPair(Navier& nv) : n(nv) { n.fb = 2; }
+    ~FieldPair() { n.fb = 1; }
+  };
   bool mixed = false;     // periodic batched step: convective terms in FP32, the rest FP64
   DBuf MIX;               // FP32 slots of the mixed convective pipeline (11 slots of FS floats)
   bool vel_same = true;   // the ux and uy solvers hold identical tables
@@ -528,12 +541,28 @@ class Navier {
   }  // Fourier x transforms by the FFT kernels (dense DFT GEMMs otherwise)
 
   // instrumented launches
-  // sA / sB: operand strides between the instances of a group (0: shared operator)
+  // sA / sB: DATA for an operand that is a field (one per batch entry), 0 for an operator
+  // shared by the batch, or the stride of the group's per-instance operators (indexed by
+  // the instance, so ux and uy share them).  U is per instance.
+  static constexpr long DATA = -1;
   void GEMM(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc, double beta = 0.0,
             const double* U = nullptr, int ldu = 0, long sA = 0, long sB = 0) {
-    timer.begin(st, 0, 2.0 * M * N * K * nb);
-    if (nb > 1) {
-      gemm_mfma_batched(st, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, bs, nb, 1.0, beta, U, ldu, U ? bs : 0);
+    const int nz = nbt();
+    timer.begin(st, 0, 2.0 * M * N * K * nz);
+    if (nz > 1) {
+      const long d = dstr();
+      BatchMod bm;
+      bm.a = sA > 0 ? nb : 0;
+      bm.b = sB > 0 ? nb : 0;
+      bm.u = nb;
+      if (use_mfma || nb > 1) {
+        gemm_mfma_batched(st, M, N, K, A, lda, sA == DATA ? d : sA, B, ldb, sB == DATA ? d : sB, C, ldc, d, nz, 1.0,
+                          beta, U, ldu, U ? bs : 0, bm);
+      } else {  // one instance, a field pair, FMA GEMMs
+        for (int z = 0; z < nz; ++z)
+          gemm(st, M, N, K, A + (sA == DATA ? z * d : 0), lda, B + (sB == DATA ? z * d : 0), ldb, C + z * d, ldc, 1.0,
+               beta, U, ldu);
+      }
     } else if (use_mfma) {
       gemm_mfma(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
     } else {
@@ -543,12 +572,12 @@ class Navier {
   }
   void ROWS(double* X, int ld, int nrows, int m, int nsets, const Solver& s) {
     // bytes: the rows read and written once, the 4 diagonals of every set read once
-    timer.begin(st, 3, 8.0 * nb * (2.0 * nrows * m + 4.0 * (double)nsets * m));
+    timer.begin(st, 3, 8.0 * nbt() * (2.0 * nrows * m + 4.0 * (double)nsets * m));
     if (fdma_rows_wave_supported(m)) {
-      fdma_rows_wave(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p, nb, bs);
+      fdma_rows_wave(st, X, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p, nbt(), dstr());
     } else {
-      for (int b = 0; b < nb; ++b)
-        fdma_rows(st, X + b * bs, ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
+      for (int b = 0; b < nbt(); ++b)
+        fdma_rows(st, X + b * dstr(), ld, nrows, m, nsets, s.low.p, s.dia.p, s.up1.p, s.up2.p);
     }
     timer.end(st);
   }
@@ -556,8 +585,8 @@ class Navier {
   void ROWSF(const double* R, int ldr, double* X, int ld, int nrows, const Solver& s) {
     const int m = s.m, nr = s.pyb.mi;
     if (use_bands && fdma_rows_lds_supported(m, nr)) {
-      timer.begin(st, 1, 8.0 * nb * ((double)nrows * (nr + m) + 4.0 * (double)s.nsets * m));
-      fdma_rows_lds(st, R, ldr, nr, X, ld, nrows, m, s.nsets, s.pybp.p, s.diagp.p, nb, bs, bs);
+      timer.begin(st, 1, 8.0 * nbt() * ((double)nrows * (nr + m) + 4.0 * (double)s.nsets * m));
+      fdma_rows_lds(st, R, ldr, nr, X, ld, nrows, m, s.nsets, s.pybp.p, s.diagp.p, nbt(), dstr(), dstr());
       timer.end(st);
       return;
     }
@@ -565,13 +594,13 @@ class Navier {
     ROWS(X, ld, nrows, m, s.nsets, s);
   }
   void BANDY(const DBand& b, const double* in, int ldi, double* out, int ldo, int rows, double beta) {
-    timer.begin(st, 9, 8.0 * nb * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
-    band_y(st, in, ldi, out, ldo, rows, b.mo, b.coef.p, b.mask.p, beta, nb, bs, bs);
+    timer.begin(st, 9, 8.0 * nbt() * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
+    band_y(st, in, ldi, out, ldo, rows, b.mo, b.coef.p, b.mask.p, beta, nbt(), dstr(), dstr());
     timer.end(st);
   }
   void BANDX(const DBand& b, const double* in, double* out, int cols, double beta) {
-    timer.begin(st, 3, 8.0 * nb * (double)cols * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
-    band_x(st, in, out, cols, b.mo, b.coef.p, b.mask.p, beta, nb, bs, bs);
+    timer.begin(st, 3, 8.0 * nbt() * (double)cols * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
+    band_x(st, in, out, cols, b.mo, b.coef.p, b.mask.p, beta, nbt(), dstr(), dstr());
     timer.end(st);
   }
   // cached stencil bands per basis
@@ -599,8 +628,8 @@ class Navier {
     return ref;
   }
   void FDIFF(const double* X, double* Y, int m, int cols, int ld, int times, double sc) {
-    timer.begin(st, 3, 8.0 * 4.0 * m * cols * nb);
-    for (int b = 0; b < nb; ++b) fourier_diff(st, X + b * bs, Y + b * bs, m, cols, ld, times, sc);
+    timer.begin(st, 3, 8.0 * 4.0 * m * cols * nbt());
+    for (int b = 0; b < nbt(); ++b) fourier_diff(st, X + b * dstr(), Y + b * dstr(), m, cols, ld, times, sc);
     timer.end(st);
   }
 
@@ -611,11 +640,26 @@ class Navier {
   // Ensemble groups: `slabs` places every field and work buffer of this instance in slot `slot`
   // of group-owned slabs (slot stride = the buffer size), `ext` is the group's stream and
   // `share` an instance of the same shape whose Poisson solver (Ra-independent) is reused.
+  // The velocity part's buffers (ux/uy, their physical values, A, CONV, CV, RHS, W1, W2, G)
+  // have 2 B slots: uy and uy's values are the second half of the ux slabs, and the other
+  // work buffers keep their field-pair partners there.
   struct Slabs {
     static constexpr int NK = 29;
     size_t F = 0;
+    int B = 0;
     DBuf slab[NK];
-    double* at(int k, int slot) { return slab[k].p + (size_t)slot * F; }
+    static bool paired(int k) { return k == 0 || k == 3 || (k >= 5 && k <= 11); }
+    static bool second(int k) { return k == 1 || k == 4; }  // uy, uy values
+    void alloc(size_t f, int b) {
+      F = f;
+      B = b;
+      for (int k = 0; k < NK; ++k)
+        if (!second(k)) slab[k].alloc(F * (size_t)b * (paired(k) ? 2 : 1));
+    }
+    double* at(int k, int slot) {
+      if (second(k)) return slab[k - 1].p + ((size_t)B + slot) * F;
+      return slab[k].p + (size_t)slot * F;
+    }
   };
   bool owns_stream = true;
 
@@ -729,7 +773,7 @@ class Navier {
       if (name == "BDS") {
         FDIFF(in, tmp, m, cols, cols, 1, scale[0]);
         const DOp& op = X.get(b, "BS");
-        GEMM(op.rows, cols, op.cols, op.b.p, op.cols, tmp, cols, out, cols, 0.0, nullptr, 0, 0, bs);
+        GEMM(op.rows, cols, op.cols, op.b.p, op.cols, tmp, cols, out, cols, 0.0, nullptr, 0, 0, DATA);
         return out;
       }
     } else if (b.k == CHEB && (name == "S" || name == "FO")) {
@@ -740,13 +784,13 @@ class Navier {
       return out;
     }
     const DOp& op = X.get(b, name);
-    GEMM(op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 0.0, nullptr, 0, 0, bs);
+    GEMM(op.rows, cols, op.cols, op.b.p, op.cols, in, cols, out, cols, 0.0, nullptr, 0, 0, DATA);
     return out;
   }
   void xmat(const DOp& op, const double* in, int cols, double* out, const double* per_inst = nullptr,
             long sop = 0) {  // per_inst: the group's operators, sop apart
     GEMM(op.rows, cols, op.cols, per_inst ? per_inst : op.b.p, op.cols, in, cols, out, cols, 0.0, nullptr, 0,
-         per_inst ? sop : 0, bs);
+         per_inst ? sop : 0, DATA);
   }
   // y (axis 1): in [rows, cin] -> out [rows, cout]; out = (in Y^T) .* U + beta out
   const double* yop(const Basis& b, const std::string& name, const double* in, int rows, double* out,
@@ -759,18 +803,18 @@ class Navier {
     if (use_dct && use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) &&
         cheb_y_supported(b.n)) {
       const DBand* band = b.k != CHEB ? &stencil_band(b) : nullptr;
-      timer.begin(st, 7, 8.0 * nb * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
-      cheb_y_diff(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr, 1.0 / scale[1], beta, nb, bs,
-                  bs);
+      timer.begin(st, 7, 8.0 * nbt() * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
+      cheb_y_diff(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr, 1.0 / scale[1], beta, nbt(),
+                  dstr(), dstr());
       timer.end(st);
       return out;
     }
     if (use_bands && name == "DS" && !U && (b.k == CD || b.k == CN || b.k == CHEB) && cheb_diff_y_supported(b.n)) {
       const bool comp = b.k != CHEB;
       const DBand* band = comp ? &stencil_band(b) : nullptr;
-      timer.begin(st, 3, 8.0 * nb * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
+      timer.begin(st, 3, 8.0 * nbt() * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
       cheb_diff_y(st, in, b.m, out, b.n, rows, b.n, comp ? band->coef.p : nullptr, comp ? band->mask.p : nullptr, 1.0,
-                  beta, nb, bs, bs);
+                  beta, nbt(), dstr(), dstr());
       timer.end(st);
       return out;
     }
@@ -778,40 +822,45 @@ class Navier {
       if (name == "BS" || name == "BDS") {
         const DBand* band = b.k == CHEB ? nullptr : &stencil_band(b);
         const int times = name == "BDS" ? 1 : 0;
-        timer.begin(st, 4,
-                    8.0 * nb * (double)rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
-        cheb_y_backward(st, in, b.m, b.m, out, b.n, rows, b.n, band ? band->coef.p : nullptr,
-                        band ? band->mask.p : nullptr, times, 1.0 / scale[1], U, b.n, beta, nb, bs, bs, U ? bs : 0);
-        timer.end(st);
+        // the multiplier U is per instance: one launch per field of a field pair
+        const int nf = U ? fb : 1, nz = nbt() / nf;
+        const long d = dstr(), po = pair_off();
+        for (int f = 0; f < nf; ++f) {
+          timer.begin(st, 4,
+                      8.0 * nz * (double)rows * ((double)b.m + b.n * (1.0 + (U ? 1.0 : 0.0) + (beta != 0.0 ? 1.0 : 0.0))));
+          cheb_y_backward(st, in + f * po, b.m, b.m, out + f * po, b.n, rows, b.n, band ? band->coef.p : nullptr,
+                          band ? band->mask.p : nullptr, times, 1.0 / scale[1], U, b.n, beta, nz, d, d, U ? bs : 0);
+          timer.end(st);
+        }
         return out;
       }
       if ((name == "Fdl" || name == "FS") && b.k == CHEB && !U && beta == 0.0) {
-        timer.begin(st, 6, 8.0 * nb * (double)rows * 2.0 * b.n);
-        cheb_y_forward(st, in, b.n, out, b.n, rows, b.n, name == "Fdl" ? b.n * 2 / 3 : b.n, nb, bs, bs);
+        timer.begin(st, 6, 8.0 * nbt() * (double)rows * 2.0 * b.n);
+        cheb_y_forward(st, in, b.n, out, b.n, rows, b.n, name == "Fdl" ? b.n * 2 / 3 : b.n, nbt(), dstr(), dstr());
         timer.end(st);
         return out;
       }
     }
     if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_rows_lds_supported(b.m)) {
       const DTdma& t = tdma_tables(b);
-      timer.begin(st, 8, 8.0 * nb * (double)rows * ((double)b.n + b.m));
-      tdma_rows_lds(st, in, b.n, out, b.m, rows, b.m, t.tabp.p, nb, bs, bs);
+      timer.begin(st, 8, 8.0 * nbt() * (double)rows * ((double)b.n + b.m));
+      tdma_rows_lds(st, in, b.n, out, b.m, rows, b.m, t.tabp.p, nbt(), dstr(), dstr());
       timer.end(st);
       return out;
     }
     if (use_bands && name == "FO" && !U && beta == 0.0 && (b.k == CD || b.k == CN) && tdma_y_supported(b.m)) {
       const DTdma& t = tdma_tables(b);
-      timer.begin(st, 3, 8.0 * nb * (double)rows * ((double)b.n + b.m));
-      tdma_y(st, in, b.n, out, b.m, rows, b.m, t.diag.p, t.low2.p, t.off.p, t.w.p, t.den.p, nb, bs, bs);
+      timer.begin(st, 3, 8.0 * nbt() * (double)rows * ((double)b.n + b.m));
+      tdma_y(st, in, b.n, out, b.m, rows, b.m, t.diag.p, t.low2.p, t.off.p, t.w.p, t.den.p, nbt(), dstr(), dstr());
       timer.end(st);
       return out;
     }
     const DOp& op = Y.get(b, name);
-    GEMM(rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, beta, U, op.cols, bs, 0);
+    GEMM(rows, op.cols, op.rows, in, op.rows, op.b.p, op.cols, out, op.cols, beta, U, op.cols, DATA, 0);
     return out;
   }
   void ymat(const DOp& op, const double* in, int rows, double* out, const double* per_inst = nullptr, long sop = 0) {
-    GEMM(rows, op.cols, op.rows, in, op.rows, per_inst ? per_inst : op.b.p, op.cols, out, op.cols, 0.0, nullptr, 0, bs,
+    GEMM(rows, op.cols, op.rows, in, op.rows, per_inst ? per_inst : op.b.p, op.cols, out, op.cols, 0.0, nullptr, 0, DATA,
          per_inst ? sop : 0);
   }
 
@@ -911,6 +960,45 @@ class Navier {
     forward_dealias_conv();
     ew_rhs_vel(st, RHS.p, A.p, G.p, buoy, CV.p, dt, nbn(no));
     solve(which, RHS.p, U);
+  }
+  // solve_vel of ux (no buoyancy) and uy (buoyancy THAT) with their transforms, products
+  // and implicit solves batched (the two use the same operators; each reads only the
+  // physical ux, uy and pres of the step start, so the result equals the two calls in turn)
+  bool pair_ok() const {
+    return use_batch && !periodic && vel_same && solver[0]->kind == ADI && solver[1]->kind == ADI;
+  }
+  void solve_vel_pair() {
+    const long no = (long)rows_ortho(bx_f) * ny, po = pair_off();
+    {
+      FieldPair fp(*this);
+      to_ortho(bx_u, by_u, UX.p, A.p);
+    }
+    xop(bx_p, "DS", P0.p, ny, G.p, W9.p);                    // d/dx pres
+    const double* gy = yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + po);  // d/dy pres
+    if (gy != G.p + po) throw std::logic_error("pair: d/dy pres expected in its slot");
+    {
+      FieldPair fp(*this);
+      conv_pair(bx_u, UX.p);
+      forward_dealias_conv();
+    }
+    ew_rhs_vel(st, RHS.p, A.p, G.p, nullptr, CV.p, dt, nbn(no));
+    ew_rhs_vel(st, RHS.p + po, A.p + po, G.p + po, THAT.p, CV.p + po, dt, nbn(no));
+    FieldPair fp(*this);
+    solve(0, RHS.p, UX.p);
+  }
+  // project_velocity with the from_ortho transforms and the subtraction of both fields batched
+  void project_velocity_pair() {
+    const int myp = by_phi.m, ro = rows_ortho(bx_phi);
+    const long ns = (long)rows_spec(bx_u) * by_u.m, po = pair_off();
+    const double* a = xop(bx_phi, "DS", P1.p, myp, W1.p, W9.p);
+    const double* g = yop(by_phi, "S", a, ro, W2.p);
+    a = xop(bx_phi, "S", P1.p, myp, W1.p, W9.p);
+    const double* g2 = yop(by_phi, "DS", a, ro, W2.p + po);
+    if (g != W2.p || g2 != W2.p + po) throw std::logic_error("pair: projection gradients expected in their slots");
+    FieldPair fp(*this);
+    const double* f = xop(bx_u, "FO", W2.p, ny, W1.p, W9.p);
+    yop(by_u, "FO", f, rows_spec(bx_u), RHS.p);
+    ew_project(st, UX.p, RHS.p, 1.0, nbn(ns));
   }
   void divergence(double* out) {  // navier.rs:698-703
     const int my = by_u.m, ro = rows_ortho(bx_u);
@@ -1166,13 +1254,24 @@ class Navier {
     const long no = (long)rows_ortho(bx_f) * ny;
     to_ortho(bx_t, by_u, T.p, W3.p);
     ew_add(st, THAT.p, W3.p, THAT_BC.p, nbn(no));
-    backward_vel(UX.p, UXP.p);
-    backward_vel(UY.p, UYP.p);
-    solve_vel(0, UX.p, nullptr);
-    solve_vel(1, UY.p, THAT.p);
-    divergence(DIV.p);
-    solve_pres();
-    project_velocity();
+    if (pair_ok()) {  // ux and uy as one batch of twice the size
+      {
+        FieldPair fp(*this);
+        backward_vel(UX.p, UXP.p);
+      }
+      solve_vel_pair();
+      divergence(DIV.p);
+      solve_pres();
+      project_velocity_pair();
+    } else {
+      backward_vel(UX.p, UXP.p);
+      backward_vel(UY.p, UYP.p);
+      solve_vel(0, UX.p, nullptr);
+      solve_vel(1, UY.p, THAT.p);
+      divergence(DIV.p);
+      solve_pres();
+      project_velocity();
+    }
     update_pres();
     solve_temp();
     time += dt;
@@ -1598,8 +1697,7 @@ class Ensemble {
       load[si] += oc.first;
       g->st = streams[si];
       g->slabs = std::make_unique<Navier::Slabs>();
-      g->slabs->F = (size_t)d0.nx * d0.ny;
-      for (int k = 0; k < Navier::Slabs::NK; ++k) g->slabs->slab[k].alloc(g->slabs->F * B);
+      g->slabs->alloc((size_t)d0.nx * d0.ny, B);
       for (int b = 0; b < B; ++b) {
         const rbc_desc& di = d[mem[b]];
         g->navs.push_back(std::make_unique<Navier>(false, di.nx, di.ny, di.ra, di.pr, di.dt, di.aspect,

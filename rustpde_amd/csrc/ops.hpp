@@ -31,14 +31,20 @@ LaunchTiming& launch_timing();
 void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
           int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
 
-// Same product on the FP64 matrix cores (v_mfma_f64_16x16x4f64), 64 x 64 tiles.
+// Same product on the FP64 matrix cores (v_mfma_f64_16x16x4f64), tile shape per launch shape
+// picked by an autotuner.
 void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
                int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
 
-// Batched: operand b of batch index z starts at A + z sA, B + z sB, C + z sC, U + z sU.
+// Batch-index moduli of A, B, U (0: none): entry z reads A + sA (z % a) etc.
+struct BatchMod {
+  int a = 0, b = 0, u = 0;
+};
+// Batched: operand b of batch index z starts at A + z sA, B + z sB, C + z sC, U + z sU
+// (with the moduli of bm applied to the index of A, B, U).
 void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
                        long sB, double* C, int ldc, long sC, int batch, double alpha, double beta,
-                       const double* U = nullptr, int ldu = 0, long sU = 0);
+                       const double* U = nullptr, int ldu = 0, long sU = 0, BatchMod bm = BatchMod{});
 
 // Banded 4-diagonal solves (offsets -2, 0, 2, 4; pre-swept as fdma.rs:73-82) of every
 // row r of X[nrows, m] with diagonal set (r % nsets); diag arrays are [nsets, m].

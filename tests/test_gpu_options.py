@@ -88,3 +88,20 @@ def test_gemm_tile_shapes_bitwise(oracle, monkeypatch, tile):
     monkeypatch.delenv("RUSTPDE_AMD_GEMM_TILE", raising=False)
     for a, b in zip(results[0], results[1]):
         assert np.array_equal(a, b)
+
+
+def test_confined_field_pair_bitwise(oracle):
+    """The confined step's ux/uy pair (their transforms, products, implicit solves and
+    projection as launches of twice the batch; option batch = 1) reproduces the step that
+    treats the two fields one after the other (batch = 0) bit for bit: every batch entry
+    runs the same arithmetic as its own launch would."""
+    print("case confined field pair bitwise", flush=True)
+    out = []
+    for batch in (1, 0):
+        g, _ = _pair(oracle, False)
+        g.set_option(OPTS["batch"], batch)
+        g.update(6)
+        out.append([f.vhat.copy() for f in (g.temp, g.ux, g.uy, g.pres[0], g.pres[1])])
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
