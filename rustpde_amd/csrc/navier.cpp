@@ -981,8 +981,7 @@ class Navier {
       conv_pair(bx_u, UX.p);
       forward_dealias_conv();
     }
-    ew_rhs_vel(st, RHS.p, A.p, G.p, nullptr, CV.p, dt, nbn(no));
-    ew_rhs_vel(st, RHS.p + po, A.p + po, G.p + po, THAT.p, CV.p + po, dt, nbn(no));
+    ew_rhs_vel_pair(st, RHS.p, A.p, G.p, THAT.p, CV.p, dt, nbn(no), po);
     FieldPair fp(*this);
     solve(0, RHS.p, UX.p);
   }
@@ -1039,8 +1038,7 @@ class Navier {
     const long np = (long)nx * ny;
     to_ortho(bx_t, by_u, T.p, A.p);
     conv_pair(bx_t, T.p);
-    ew_acc_prod(st, CONV.p, UXP.p, DTBCX.p, nbn(np));
-    ew_acc_prod(st, CONV.p, UYP.p, DTBCY.p, nbn(np));
+    ew_acc_prod2(st, CONV.p, UXP.p, DTBCX.p, UYP.p, DTBCY.p, nbn(np));
     forward_dealias_conv();
     ew_rhs_temp(st, RHS.p, A.p, GXX.p, GYY.p, CV.p, dt, nbn(no));
     solve(2, RHS.p, T.p);

@@ -305,6 +305,23 @@ void ew_rhs_vel(hipStream_t s, double* out, const double* a, const double* g, co
                 double dt, long n) {
   EW_LAUNCH(ew_rhs_vel, n, out, a, g, buoy, c, dt);
 }
+// ux (entry 0, no buoyancy) and uy (entry 1, + buoy dt) of a field pair, entry 1 at offset po
+__global__ void ew_rhs_vel_pair_kernel(double* out, const double* a, const double* g, const double* buoy,
+                                       const double* c, double dt, long n, long po) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long j = i + (blockIdx.y ? po : 0);
+  double r = a[j] - g[j] * dt;
+  if (blockIdx.y) r = r + buoy[i] * dt;
+  out[j] = r - c[j] * dt;
+}
+void ew_rhs_vel_pair(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
+                     double dt, long n, long po) {
+  if (n <= 0) return;
+  RBC_LAUNCH(ew_rhs_vel_pair_kernel, dim3(blocks_for(n, 256), 2), dim3(256), 0, s, out, a, g, buoy, c, dt, n, po);
+  check_launch("ew_rhs_vel_pair");
+}
 void ew_rhs_temp(hipStream_t s, double* out, const double* a, const double* bx, const double* by, const double* c,
                  double dt, long n) {
   EW_LAUNCH(ew_rhs_temp, n, out, a, bx, by, c, dt);

@@ -85,7 +85,10 @@ void fourier_diff(hipStream_t s, const double* X, double* Y, int m, int cols, in
 // Element-wise combinations of the step (n = number of doubles, all contiguous)
 void ew_add(hipStream_t s, double* out, const double* a, const double* b, long n);          // a + b
 void ew_rhs_vel(hipStream_t s, double* out, const double* a, const double* g, const double* buoy,
-                const double* c, double dt, long n);  // ((a - g dt) [+ buoy dt]) - c dt
+                const double* c, double dt, long n);
+// the same for a field pair: entry 0 (no buoyancy) at offset 0, entry 1 (+ buoy dt) at po
+void ew_rhs_vel_pair(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
+                     double dt, long n, long po);  // ((a - g dt) [+ buoy dt]) - c dt
 void ew_rhs_temp(hipStream_t s, double* out, const double* a, const double* bx, const double* by,
                  const double* c, double dt, long n);  // ((a + bx) + by) - c dt
 void ew_acc_prod(hipStream_t s, double* out, const double* a, const double* b, long n);     // out + a b
