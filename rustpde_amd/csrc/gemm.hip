@@ -14,6 +14,7 @@
 #include <string>
 #include <tuple>
 
+#include "fft_lds.hpp"
 #include "ops.hpp"
 
 namespace rbc {
@@ -37,9 +38,15 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
   __shared__ double Bs[BK][BN + 1];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  // Tile (bx, by) of entry z from the remapped block index: consecutive logical blocks run
+  // on one XCD, so the tiles that share an operand panel (the M tiles of one N panel and
+  // the N tiles of one M panel of an entry) read it through the same L2.
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int lg = fftlds::xcd_remap(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+  const int bx = lg % gx, by = (lg / gx) % gy;
+  const int m0 = by * BM, n0 = bx * BN;
   const int lr = lane & 15, lk = lane >> 4;
-  const int z = blockIdx.z;  // batch entry
+  const int z = lg / (gx * gy);  // batch entry
   A += sA * (amod ? z % amod : z);
   B += sB * (bmod ? z % bmod : z);
   C += sC * z;
