@@ -31,10 +31,12 @@ using fftlds::lp;
 using fftlds::lp_size;
 using fftlds::xcd_remap;
 
-// geometry for length L: PP column pairs per block, E = PP L complex values, T = E / 8
-template <int L>
+// geometry for length L and LDS value size VB: PP column pairs per block, E = PP L complex
+// values, T = E / 8.  FP32 blocks of L >= 512 take twice the columns (row segments of 32
+// bytes, as FP64 blocks of 4 columns have).
+template <int L, int VB = 8>
 struct CGeo {
-  static constexpr int PP = L >= 2048 ? 1 : 2048 / L;
+  static constexpr int PP = L >= 2048 ? 1 : (VB == 4 && L >= 512 ? 4096 : 2048) / L;
   static constexpr int E = PP * L;
   static constexpr int T = E / 8;
   static constexpr int C = 2 * PP;  // columns per block
@@ -44,10 +46,10 @@ struct CGeo {
 
 // Tin: input type, V: arithmetic (and LDS) type, Tout: output type
 template <int L, typename Tin, typename V, typename Tout>
-__global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
+__global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
                                                                int cols, int cut, const double* __restrict__ tw,
                                                                long sx, long sy) {
-  using G = CGeo<L>;
+  using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, m = G::m;
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   X += sx * blockIdx.y;
@@ -97,13 +99,13 @@ __global__ __launch_bounds__(CGeo<L>::T) void rfft_cols_kernel(const Tin* __rest
 // entries b >= split read input b - split again, apply (i k)^times1 / sc1 and write Y1 (the
 // values and the x-derivatives of the same fields in one launch).
 template <int L, typename Tin, typename V, typename Tout>
-__global__ __launch_bounds__(CGeo<L>::T) void irfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
+__global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
                                                                 int cols, int times, double sc_d,
                                                                 const double* __restrict__ tw, long sx, long sy,
                                                                 Tout* __restrict__ Y1, int split, int times1,
                                                                 double sc1) {
 #pragma clang fp contract(off)
-  using G = CGeo<L>;
+  using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   int b = blockIdx.y;
@@ -212,7 +214,7 @@ void check_n(int n) {
 
 template <int L, typename Tin, typename V, typename Tout>
 void launch_rfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int cut, int batch, long sx, long sy) {
-  using G = CGeo<L>;
+  using G = CGeo<L, sizeof(V)>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
   RBC_LAUNCH((rfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut,
              fft_table(L), sx, sy);
@@ -220,7 +222,7 @@ void launch_rfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int cut, int ba
 template <int L, typename Tin, typename V, typename Tout>
 void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, double sc, int batch, long sx, long sy,
                   Tout* Y1 = nullptr, int split = 0, int times1 = 0, double sc1 = 1.0) {
-  using G = CGeo<L>;
+  using G = CGeo<L, sizeof(V)>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
   RBC_LAUNCH((irfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
              fft_table(L), sx, sy, Y1, split, times1, sc1);
