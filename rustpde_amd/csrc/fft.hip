@@ -32,11 +32,11 @@ using fftlds::lp_size;
 using fftlds::xcd_remap;
 
 // geometry for length L and LDS value size VB: PP column pairs per block, E = PP L complex
-// values, T = E / 8.  FP32 blocks of L >= 512 take twice the columns (row segments of 32
-// bytes, as FP64 blocks of 4 columns have).
+// values, T = E / 8.  Blocks of L >= 512 take 8 columns (row segments of 64 bytes in FP64,
+// 32 in FP32); shorter lengths take 2048 / L column pairs.
 template <int L, int VB = 8>
 struct CGeo {
-  static constexpr int PP = L >= 2048 ? 1 : (VB == 4 && L >= 512 ? 4096 : 2048) / L;
+  static constexpr int PP = L >= 2048 ? 1 : (L >= 512 ? 4096 : 2048) / L;
   static constexpr int E = PP * L;
   static constexpr int T = E / 8;
   static constexpr int C = 2 * PP;  // columns per block
