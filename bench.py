@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Navier2D steps/sec on 1 MI355X: mixed vs FP64 GPU at stated field/Nu error"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 CONFIGS = {
@@ -42,7 +43,7 @@ WORKLOAD_NAMES = {
 KERNEL_CLASSES = {0: "gemm_mfma_kernel", 1: "fdma_rows_lds_kernel", 2: "irfft_cols_kernel", 3: "other kernels",
                   4: "cheb_y_bwd_kernel", 5: "rfft_cols_kernel", 6: "cheb_y_fwd_kernel", 7: "cheb_y_diff_kernel",
                   8: "tdma_rows_lds_kernel", 9: "band_y_kernel", 10: "pressure_rows_kernel",
-                  11: "cheb_y_conv_kernel"}
+                  11: "cheb_y_conv_kernel", 12: "gemm_mfma_kernel (FP32)"}
 MIXED_CLASS = 3
 
 
@@ -52,12 +53,17 @@ def step_roofline(stats, steps, ms_per_step):
     the measured step time."""
     t_roof = 0.0
     for k, (n, ms, w) in stats.items():
-        t_roof += w / (PEAK_FP64_TFLOPS * 1e12) if k == 0 else w / (PEAK_HBM_GBS * 1e9)
+        if k == 0:
+            t_roof += w / (PEAK_FP64_TFLOPS * 1e12)
+        elif k == 12:
+            t_roof += w / (PEAK_FP32_TFLOPS * 1e12)
+        else:
+            t_roof += w / (PEAK_HBM_GBS * 1e9)
     t_roof_us = 1e6 * t_roof / steps
     t_kernels_us = 1e3 * sum(v[1] for v in stats.values()) / steps
     return {"t_roof_us": t_roof_us, "t_step_us": 1e3 * ms_per_step, "frac": t_roof_us / (1e3 * ms_per_step),
             "t_kernels_us": t_kernels_us,
-            "note": "sum over kernel classes of algorithmic work / peak (FP64 MFMA for GEMM flops, HBM for "
+            "note": "sum over kernel classes of algorithmic work / peak (FP64 or FP32 MFMA for GEMM flops, HBM for "
                     "bytes) per step, over the graph-replayed step time; t_kernels_us is the summed "
                     "event-timed kernel time per step"}
 
@@ -161,23 +167,64 @@ def c4_descs():
     return [(int(nn), int(nn), float(ra), 1.0, 0.01 * 64.0 / nn, 1.0, True) for nn, ra in zip(sizes, ras)]
 
 
-def bench_ensemble(args):
-    """Ensemble of 96 confined instances advanced together (rbc_create_ensemble)."""
-    import ctypes
-
-    from rustpde_amd import Ensemble, _lib, seeded
-    L = _lib.load()
-    descs = c4_descs()
+def make_ensemble(descs, mixed):
+    from rustpde_amd import Ensemble, Navier2D, seeded
     ens = Ensemble(descs)
     for i, d in enumerate(descs):
         nav = ens[i]
         nav.set_velocity(0.2, 1.0, 1.0)
         nav.temp.v = seeded.interior_perturbation(d[0], d[1], 1e-2, 1000 + i, kind="uniform", periodic=False)
         nav.temp.forward()
+    if mixed:
+        ens.set_option(Navier2D.OPT_MIXED, 1)
+    return ens
+
+
+def timed_ensemble(ens, args):
     ens.update(args.warmup)
     t0 = time.perf_counter()
     ens.update(args.steps)
-    el = time.perf_counter() - t0
+    return time.perf_counter() - t0
+
+
+def ensemble_mixed_errors(descs):
+    """Largest relative max-norm difference per field over the instances, mixed against FP64
+    ensembles after 10 steps from the same seeded fields."""
+    import numpy as np
+    a = make_ensemble(descs, False)
+    b = make_ensemble(descs, True)
+    a.update(10)
+    b.update(10)
+    err = {"temp": 0.0, "ux": 0.0, "uy": 0.0, "pres": 0.0, "nu": 0.0}
+    for i in range(len(descs)):
+        for name in ("temp", "ux", "uy"):
+            x, y = getattr(a[i], name).vhat, getattr(b[i], name).vhat
+            err[name] = max(err[name], float(np.abs(x - y).max() / np.abs(x).max()))
+        x, y = a[i].pres[0].vhat, b[i].pres[0].vhat
+        err["pres"] = max(err["pres"], float(np.abs(x - y).max() / np.abs(x).max()))
+        nua, nub = a[i].eval_nu(), b[i].eval_nu()
+        err["nu"] = max(err["nu"], abs(nua - nub) / abs(nua))
+    a.close()
+    b.close()
+    return err
+
+
+def bench_ensemble(args):
+    """Ensemble of 96 confined instances advanced together (rbc_create_ensemble); with
+    --precision mixed the convective terms' dense transforms run as FP32 matrix-core GEMMs
+    (the FP64 ensemble is timed beside it)."""
+    descs = c4_descs()
+    mixed = args.precision == "mixed"
+    extra = {}
+    if mixed:
+        ens64 = make_ensemble(descs, False)
+        el64 = timed_ensemble(ens64, args)
+        ens64.close()
+        extra["fp64_instance_steps_per_s"] = len(descs) * args.steps / el64
+    ens = make_ensemble(descs, mixed)
+    el = timed_ensemble(ens, args)
+    if mixed:
+        extra["mixed_speedup_vs_fp64"] = el64 / el
     n = len(descs)
     points = sum(d[0] * d[1] for d in descs)
     # per-launch timing of every group over the same number of steps (eager launches)
@@ -187,14 +234,19 @@ def bench_ensemble(args):
     el_timed = time.perf_counter() - t1
     stats = {k: ens.timing_read(k) for k in KERNEL_CLASSES}
     ens.timing(False)
+    groups = ens.groups
+    ens.close()
+    if mixed:
+        extra["mixed_error_vs_fp64_after_10_steps"] = ensemble_mixed_errors(descs)
     sroof = step_roofline(stats, args.steps, 1e3 * el / args.steps)
     sroof["note"] += "; summed over the groups, which run concurrently on three streams"
     dom = max((k for k in stats if k != MIXED_CLASS), key=lambda k: stats[k][1])
     ln, lms, lw = stats[dom]
-    if dom == 0:
+    if dom in (0, 12):
+        peak = PEAK_FP64_TFLOPS if dom == 0 else PEAK_FP32_TFLOPS
         achieved = lw / (lms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
     else:
         achieved = lw / (lms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -202,16 +254,19 @@ def bench_ensemble(args):
     roof.update({"kernel": KERNEL_CLASSES[dom], "launches": ln, "avg_launch_us": 1e3 * lms / max(ln, 1),
                  "work_per_launch": lw / max(ln, 1), "share_of_kernel_time": lms / max(1e-30, sum(v[1] for v in stats.values())),
                  "note": "summed over the groups, which run concurrently on three streams"})
-    out = {"metric": "ensemble instance-steps/s (96 confined RBC instances, FP64)", "value": n * args.steps / el,
+    prec = "mixed FP32/FP64" if mixed else "FP64"
+    out = {"metric": "ensemble instance-steps/s (96 confined RBC instances, %s)" % prec, "value": n * args.steps / el,
            "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic",
+           "dtype": "f32+f64 mixed" if mixed else "f64", "data": "synthetic",
            "config": {"workload": "ensemble of 96 confined RBC instances, N in {64,96,128,192}, Ra log-uniform "
-                                  "[1e4,1e7], dt = 0.01*64/N (FP64)", "instances": n, "groups": ens.groups,
+                                  "[1e4,1e7], dt = 0.01*64/N (%s)" % prec, "instances": n, "groups": groups,
+                      "precision": "mixed" if mixed else "fp64",
                       "sizes": {str(k): sum(1 for d in descs if d[0] == k) for k in (64, 96, 128, 192)}},
            "grid_point_updates_per_s": points * args.steps / el, "roofline": roof, "step_roofline": sroof,
            "ms_per_step_with_kernel_events": 1e3 * el_timed / args.steps,
            "kernel_classes": {KERNEL_CLASSES[k]: {"launches": v[0], "ms": v[1]} for k, v in stats.items()}}
+    out.update(extra)
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -239,7 +294,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS) + ["c2", "c4"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
-                    help="mixed: convective terms in FP32, the rest FP64 (periodic configs); the FP64 step is "
+                    help="mixed: convective terms in FP32, the rest FP64 (periodic configs and the c4 ensemble); "
+                         "the FP64 step is "
                          "timed beside it and the field / Nu differences after 10 steps are reported")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -309,10 +365,11 @@ def main():
     dom = max((k for k in stats if k != MIXED_CLASS), key=lambda k: stats[k][1])
     n, ms, w = stats[dom]
     avg_ms = ms / max(n, 1)
-    if dom == 0:
+    if dom in (0, 12):
+        peak = PEAK_FP64_TFLOPS if dom == 0 else PEAK_FP32_TFLOPS
         achieved = (w / n) / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
     else:
         achieved = (w / n) / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -119,9 +119,12 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *                  y-transform launch forming d/dy q uy + dq/dx ux) and divergence, Poisson
  *                  rows, projection and pressure update as one kernel per x-wavenumber
  *                  (default 1)
- *   RBC_OPT_MIXED  mixed precision (periodic batched step): the convective terms (their
- *                  x/y transforms and products) in FP32, everything else in FP64
- *                  (default 0; fields agree with the FP64 step to ~1e-9 after 10 steps) */
+ *   RBC_OPT_MIXED  mixed precision: the convective terms (their x/y transforms and
+ *                  products) in FP32, everything else in FP64 (default 0).  Periodic
+ *                  batched step: FFT/DCT kernels in FP32; confined step with field pairs
+ *                  (and ensemble groups, rbc_ensemble_set_option): the dense transforms
+ *                  as FP32 matrix-core GEMMs.  Fields agree with the FP64 step to ~1e-9
+ *                  after 10 steps. */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2
@@ -144,6 +147,7 @@ int rbc_set_option(rbc_navier* h, int option, int value);
  *   cheb_y_diff_kernel, RBC_K_TDMA tdma_rows_lds_kernel, RBC_K_BAND band_y_kernel;
  *   RBC_K_PRESSURE pressure_rows_kernel (the fused pressure stage); RBC_K_CONV
  *   cheb_y_conv_kernel (the convective term's y transforms and products, one launch);
+ *   RBC_K_GEMM32 gemm_mfma_kernel in FP32 (the confined mixed step's convective terms);
  *   RBC_K_ELEMENTWISE the remaining timed kernels (Fourier derivative, fallbacks). */
 #define RBC_K_GEMM 0
 #define RBC_K_ROWSOLVE 1
@@ -157,6 +161,7 @@ int rbc_set_option(rbc_navier* h, int option, int value);
 #define RBC_K_BAND 9
 #define RBC_K_PRESSURE 10
 #define RBC_K_CONV 11
+#define RBC_K_GEMM32 12
 int rbc_timing_enable(rbc_navier* h, int on);
 int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work);
 
@@ -224,6 +229,9 @@ int rbc_ensemble_size(rbc_ensemble* h, int* n, int* groups);
 int rbc_ensemble_instance(rbc_ensemble* h, int inst, rbc_navier** out);
 /* Integrate::update of every instance, nsteps times (blocks until done). */
 int rbc_ensemble_update(rbc_ensemble* h, int nsteps);
+/* rbc_set_option for every instance of the ensemble (rbc_set_option refuses ensemble
+ * instances); RBC_OPT_MIXED selects the mixed-precision confined step for all groups. */
+int rbc_ensemble_set_option(rbc_ensemble* h, int option, int value);
 int rbc_ensemble_synchronize(rbc_ensemble* h);
 /* Kernel timing of all groups (as rbc_timing_enable / rbc_timing_read of a handle, summed
  * over the groups; no counterpart in the reference). */

@@ -69,6 +69,7 @@ SIGNATURES = {
     "rbc_ensemble_instance": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
     "rbc_ensemble_update": (_i, [_vp, _i]),
     "rbc_ensemble_synchronize": (_i, [_vp]),
+    "rbc_ensemble_set_option": (_i, [_vp, _i, _i]),
     "rbc_ensemble_timing_enable": (_i, [_vp, _i]),
     "rbc_ensemble_timing_read": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong), _dp, _dp]),
 }

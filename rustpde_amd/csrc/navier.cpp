@@ -78,6 +78,7 @@ struct DBuf {
 // Device matrix: x-operators are stored as [out, in], y-operators transposed [in, out].
 struct DOp {
   DBuf b;
+  DBuf f32;  // FP32 copy for the mixed-precision products (filled on first use)
   int rows = 0, cols = 0;
 };
 
@@ -391,7 +392,7 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
 // the launch's algorithmic work (flops for GEMMs, bytes for memory-bound kernels).
 class KernelTimer {
  public:
-  static constexpr int NCAT = 12;  // the RBC_K_* classes of the C ABI
+  static constexpr int NCAT = 13;  // the RBC_K_* classes of the C ABI
   bool on = false;
   struct Rec { int cat; double work; hipEvent_t a, b; };
   ~KernelTimer() { clear(); for (auto e : pool_) (void)hipEventDestroy(e); }
@@ -499,6 +500,16 @@ class Navier {
   };
   bool mixed = false;     // periodic batched step: convective terms in FP32, the rest FP64
   DBuf MIX;               // FP32 slots of the mixed convective pipeline (11 slots of FS floats)
+  // confined mixed step: FP32 work buffers of the convective terms, kind k (0: x-transform
+  // output, 1: physical ux and uy, 2: convective products, 3: dealiased y-transform) at
+  // k * mixc_kind(), each with the partner slots of a field pair
+  DBuf MIXC;
+  long mixc_kind() const { return 2 * pair_off(); }
+  float* mixc(int k) { return reinterpret_cast<float*>(MIXC.p) + (size_t)k * mixc_kind(); }
+  void alloc_mixc() {
+    const size_t floats = 4 * (size_t)mixc_kind();
+    if (MIXC.n < (floats + 1) / 2) MIXC.alloc((floats + 1) / 2);
+  }
   bool vel_same = true;   // the ux and uy solvers hold identical tables
   size_t FS = 0;          // slot stride of the batched buffers
   DBuf STATE, PHYS2;      // [ux, uy, temp] spectral and [ux, uy] physical, one slot each
@@ -569,6 +580,33 @@ class Navier {
       gemm(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
     }
     timer.end(st);
+  }
+  // the same in FP32 (A and U FP32; B and C FP32 or FP64)
+  void GEMM32(int M, int N, int K, const float* A, int lda, const void* B, bool b64, int ldb, void* C, bool c64,
+              int ldc, double beta, const float* U, int ldu, long sA, long sB) {
+    const int nz = nbt();
+    const long d = dstr();
+    BatchMod bm;
+    bm.a = sA > 0 ? nb : 0;
+    bm.b = sB > 0 ? nb : 0;
+    bm.u = nb;
+    timer.begin(st, 12, 2.0 * M * N * K * nz);
+    gemm_mfma_f32_batched(st, M, N, K, A, lda, sA == DATA ? d : sA, B, b64, ldb, sB == DATA ? d : sB, C, c64, ldc, d,
+                          nz, 1.0, beta, U, ldu, U ? bs : 0, bm);
+    timer.end(st);
+  }
+  // FP32 copy of an operator (made on first use, outside graph capture: the first step of
+  // update_n runs eagerly)
+  const float* op32(const DOp& op) {
+    DOp& o = const_cast<DOp&>(op);
+    if (!o.f32.p) {
+      const size_t n = (size_t)o.rows * o.cols;
+      std::vector<double> h = o.b.download(n);
+      std::vector<float> f(h.begin(), h.end());
+      o.f32.alloc((n + 1) / 2);
+      HIPCHECK(hipMemcpy(o.f32.p, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return reinterpret_cast<const float*>(o.f32.p);
   }
   void ROWS(double* X, int ld, int nrows, int m, int nsets, const Solver& s) {
     // bytes: the rows read and written once, the 4 diagonals of every set read once
@@ -864,6 +902,40 @@ class Navier {
          per_inst ? sop : 0);
   }
 
+  // FP32 dense transforms of the confined mixed step: x: out [rows_out, cols] = op in; y:
+  // out [rows, cout] = (in Y^T) .* U + beta out
+  void xop32(const Basis& b, const std::string& name, const void* in, bool in64, int cols, void* out, bool out64) {
+    const DOp& op = X.get(b, name);
+    GEMM32(op.rows, cols, op.cols, op32(op), op.cols, in, in64, cols, out, out64, cols, 0.0, nullptr, 0, 0, DATA);
+  }
+  void yop32(const Basis& b, const std::string& name, const float* in, int rows, void* out, bool out64,
+             const float* U = nullptr, double beta = 0.0) {
+    const DOp& op = Y.get(b, name);
+    GEMM32(rows, op.cols, op.rows, in, op.rows, op32(op), false, op.cols, out, out64, op.cols, beta, U, op.cols, DATA,
+           0);
+  }
+  // ux, uy (a field pair) in physical space, FP32
+  void backward_vel32() {
+    xop32(bx_u, "BS", UX.p, true, by_u.m, mixc(0), false);
+    yop32(by_u, "BS", mixc(0), nx, mixc(1), false);
+  }
+  // convective term of q (FP64 spectral) into the FP32 products: dq/dx ux + dq/dy uy
+  // (conv_term.rs:35-41) with the FP32 physical velocity
+  void conv32(const Basis& bxq, const double* q) {
+    const int my = by_u.m;
+    float* ux = mixc(1);
+    float* uy = ux + pair_off();
+    xop32(bxq, "BDS", q, true, my, mixc(0), false);
+    yop32(by_u, "BS", mixc(0), nx, mixc(2), false, ux, 0.0);
+    xop32(bxq, "BS", q, true, my, mixc(0), false);
+    yop32(by_u, "BDS", mixc(0), nx, mixc(2), false, uy, 1.0);
+  }
+  // dealiased forward transforms of the FP32 products into CV (FP64)
+  void forward_dealias32() {
+    yop32(by_p, "Fdl", mixc(2), nx, mixc(3), false);
+    xop32(bx_f, "Fdl", mixc(3), false, ny, CV.p, true);
+  }
+
   // ------------------------------------------------------------------ set-up pieces
   Basis xb_field() const { return bx_f; }
 
@@ -967,6 +1039,9 @@ class Navier {
   bool pair_ok() const {
     return use_batch && !periodic && vel_same && solver[0]->kind == ADI && solver[1]->kind == ADI;
   }
+  // confined mixed step: the convective terms (their dense transforms and products) in FP32,
+  // everything else FP64; it runs with the field pairs
+  bool mixed_confined() const { return mixed && !periodic && pair_ok(); }
   void solve_vel_pair() {
     const long no = (long)rows_ortho(bx_f) * ny, po = pair_off();
     {
@@ -978,8 +1053,13 @@ class Navier {
     if (gy != G.p + po) throw std::logic_error("pair: d/dy pres expected in its slot");
     {
       FieldPair fp(*this);
-      conv_pair(bx_u, UX.p);
-      forward_dealias_conv();
+      if (mixed) {
+        conv32(bx_u, UX.p);
+        forward_dealias32();
+      } else {
+        conv_pair(bx_u, UX.p);
+        forward_dealias_conv();
+      }
     }
     ew_rhs_vel_pair(st, RHS.p, A.p, G.p, THAT.p, CV.p, dt, nbn(no), po);
     FieldPair fp(*this);
@@ -1037,9 +1117,16 @@ class Navier {
     const long no = (long)rows_ortho(bx_f) * ny;
     const long np = (long)nx * ny;
     to_ortho(bx_t, by_u, T.p, A.p);
-    conv_pair(bx_t, T.p);
-    ew_acc_prod2(st, CONV.p, UXP.p, DTBCX.p, UYP.p, DTBCY.p, nbn(np));
-    forward_dealias_conv();
+    if (mixed_confined()) {
+      conv32(bx_t, T.p);
+      float* ux = mixc(1);
+      ew_acc_prod2_f32(st, mixc(2), ux, DTBCX.p, ux + pair_off(), DTBCY.p, nbn(np));
+      forward_dealias32();
+    } else {
+      conv_pair(bx_t, T.p);
+      ew_acc_prod2(st, CONV.p, UXP.p, DTBCX.p, UYP.p, DTBCY.p, nbn(np));
+      forward_dealias_conv();
+    }
     ew_rhs_temp(st, RHS.p, A.p, GXX.p, GYY.p, CV.p, dt, nbn(no));
     solve(2, RHS.p, T.p);
   }
@@ -1255,7 +1342,8 @@ class Navier {
     if (pair_ok()) {  // ux and uy as one batch of twice the size
       {
         FieldPair fp(*this);
-        backward_vel(UX.p, UXP.p);
+        if (mixed) backward_vel32();
+        else backward_vel(UX.p, UXP.p);
       }
       solve_vel_pair();
       divergence(DIV.p);
@@ -1976,26 +2064,34 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
   });
 }
 
+namespace {
+// alloc = false: an ensemble member (its group's leader holds the work buffers)
+void apply_option(Navier& n, int option, int value, bool alloc = true) {
+  HIPCHECK(hipStreamSynchronize(n.st));
+  n.drop_graph();
+  switch (option) {
+    case RBC_OPT_GRAPH: n.use_graph = value != 0; break;
+    case RBC_OPT_MFMA: n.use_mfma = value != 0; break;
+    case RBC_OPT_FFT: n.use_fft = value != 0; break;
+    case RBC_OPT_BANDS: n.use_bands = value != 0; break;
+    case RBC_OPT_DCT: n.use_dct = value != 0; break;
+    case RBC_OPT_BATCH: n.use_batch = value != 0; break;
+    case RBC_OPT_FUSE: n.use_fuse = value != 0; break;
+    case RBC_OPT_MIXED:
+      n.mixed = value != 0;
+      if (alloc && n.mixed && n.periodic && n.MIX.n < (11 * n.FS + 1) / 2) n.MIX.alloc((11 * n.FS + 1) / 2);
+      if (alloc && n.mixed && !n.periodic) n.alloc_mixc();
+      break;
+    default: throw std::invalid_argument("unknown option");
+  }
+}
+}  // namespace
+
 int rbc_set_option(rbc_navier* h, int option, int value) {
   return guarded([&] {
     NEED(h);
-    Navier& n = *h->n;
-    HIPCHECK(hipStreamSynchronize(n.st));
-    n.drop_graph();
-    switch (option) {
-      case RBC_OPT_GRAPH: n.use_graph = value != 0; break;
-      case RBC_OPT_MFMA: n.use_mfma = value != 0; break;
-      case RBC_OPT_FFT: n.use_fft = value != 0; break;
-      case RBC_OPT_BANDS: n.use_bands = value != 0; break;
-      case RBC_OPT_DCT: n.use_dct = value != 0; break;
-      case RBC_OPT_BATCH: n.use_batch = value != 0; break;
-      case RBC_OPT_FUSE: n.use_fuse = value != 0; break;
-      case RBC_OPT_MIXED:
-        n.mixed = value != 0;
-        if (n.mixed && n.MIX.n < (11 * n.FS + 1) / 2) n.MIX.alloc((11 * n.FS + 1) / 2);
-        break;
-      default: throw std::invalid_argument("unknown option");
-    }
+    if (h->member) throw std::invalid_argument("options of ensemble instances are set with rbc_ensemble_set_option");
+    apply_option(*h->n, option, value);
   });
 }
 
@@ -2177,6 +2273,14 @@ int rbc_ensemble_synchronize(rbc_ensemble* h) {
   return guarded([&] {
     if (!h || !h->e) throw std::invalid_argument("null handle");
     h->e->synchronize();
+  });
+}
+int rbc_ensemble_set_option(rbc_ensemble* h, int option, int value) {
+  return guarded([&] {
+    if (!h || !h->e) throw std::invalid_argument("null handle");
+    h->e->synchronize();
+    for (auto& g : h->e->groups)  // the leader launches the group's kernels
+      for (size_t k = 0; k < g->navs.size(); ++k) apply_option(*g->navs[k], option, value, k == 0);
   });
 }
 int rbc_ensemble_timing_enable(rbc_ensemble* h, int on) {

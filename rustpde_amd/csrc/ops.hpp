@@ -45,6 +45,11 @@ struct BatchMod {
 void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
                        long sB, double* C, int ldc, long sC, int batch, double alpha, double beta,
                        const double* U = nullptr, int ldu = 0, long sU = 0, BatchMod bm = BatchMod{});
+// The same product in FP32 on the matrix cores (v_mfma_f32_16x16x4f32): A and U FP32, B FP32
+// or FP64 (b_f64, converted as it is staged), C FP32 or FP64 (c_f64).
+void gemm_mfma_f32_batched(hipStream_t s, int M, int N, int K, const float* A, int lda, long sA, const void* B,
+                           bool b_f64, int ldb, long sB, void* C, bool c_f64, int ldc, long sC, int batch, double alpha,
+                           double beta, const float* U, int ldu, long sU, BatchMod bm = BatchMod{});
 
 // Banded 4-diagonal solves (offsets -2, 0, 2, 4; pre-swept as fdma.rs:73-82) of every
 // row r of X[nrows, m] with diagonal set (r % nsets); diag arrays are [nsets, m].

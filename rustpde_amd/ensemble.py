@@ -49,6 +49,11 @@ class Ensemble:
     def synchronize(self):
         check(_lib.load().rbc_ensemble_synchronize(self._h))
 
+    def set_option(self, option, value):
+        """rbc_ensemble_set_option: an execution option (Navier2D.OPT_*) of every instance,
+        e.g. Navier2D.OPT_MIXED for the mixed-precision step."""
+        check(_lib.load().rbc_ensemble_set_option(self._h, int(option), int(value)))
+
     def timing(self, on):
         """Per-launch kernel timing of every group (steps run without graph replay)."""
         check(_lib.load().rbc_ensemble_timing_enable(self._h, 1 if on else 0))
