@@ -495,8 +495,9 @@ class Navier {
   long nbn(long n) const { return (long)(nbt() - 1) * dstr() + n; }  // element-wise extent
   struct FieldPair {
     Navier& n;
-    explicit FieldPair(Navier& nv) : n(nv) { n.fb = 2; }
-    ~FieldPair() { n.fb = 1; }
+    int prev;
+    explicit FieldPair(Navier& nv) : n(nv), prev(nv.fb) { n.fb = 2; }
+    ~FieldPair() { n.fb = prev; }
   };
   bool mixed = false;     // periodic batched step: convective terms in FP32, the rest FP64
   DBuf MIX;               // FP32 slots of the mixed convective pipeline (11 slots of FS floats)
