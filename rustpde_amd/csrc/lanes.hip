@@ -192,6 +192,45 @@ __global__ void band_x_kernel(const double* __restrict__ in, double* __restrict_
   *o = (beta != 0.0) ? acc + beta * (*o) : acc;
 }
 
+// band_x then band_y in one pass (the 2-D composite stencil to_ortho of a field): each
+// output forms the band_x values it needs with band_x's arithmetic and combines them as
+// band_y does; optionally + add (out = acc + add, as ew_add).  in [*, ldi] (ldi = the
+// y-input length), out [mox, moy].
+__global__ void band_xy_kernel(const double* __restrict__ in, int ldi, double* __restrict__ out, int mox, int moy,
+                               const double* __restrict__ cx, const unsigned char* __restrict__ mkx,
+                               const double* __restrict__ cy, const unsigned char* __restrict__ mky,
+                               const double* __restrict__ add, long si, long so) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)mox * moy) return;
+  in += si * blockIdx.y;
+  out += so * blockIdx.y;
+  if (add) add += so * blockIdx.y;
+  const int i = (int)(t / moy), j = (int)(t % moy);
+  const unsigned bx = mkx[i], by = mky[j];
+  const int off[4] = {0, -2, 2, 4};
+  auto xval = [&](int jj) {
+    double acc = 0.0;
+    bool first = true;
+    for (int k = 0; k < 4; ++k) {
+      if (!(bx & (1u << k))) continue;
+      const double term = in[(size_t)(i + off[k]) * ldi + jj] * cx[k * mox + i];
+      acc = first ? term : acc + term;
+      first = false;
+    }
+    return acc;
+  };
+  double acc = 0.0;
+  bool first = true;
+  for (int k = 0; k < 4; ++k) {
+    if (!(by & (1u << k))) continue;
+    const double term = xval(j + off[k]) * cy[k * moy + j];
+    acc = first ? term : acc + term;
+    first = false;
+  }
+  const size_t o = (size_t)i * moy + j;
+  out[o] = add ? acc + add[o] : acc;
+}
 
 // ---------------------------------------------------------------------------------
 // cheb_diff_y: d/dy of every row (ortho.rs:107-125, once), the composite stencil
@@ -421,6 +460,17 @@ void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int 
   RBC_LAUNCH(band_y_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, ldo, rows,
                      mo, coef, mask, beta, si, so);
   check("band_y");
+}
+
+void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int moy, const double* cx,
+             const unsigned char* mkx, const double* cy, const unsigned char* mky, const double* add, int batch,
+             long si, long so) {
+  const long n = (long)mox * moy;
+  if (n <= 0) return;
+  if (batch < 1 || batch > 65535) throw std::invalid_argument("band_xy: bad batch");
+  RBC_LAUNCH(band_xy_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, mox, moy, cx,
+             mkx, cy, mky, add, si, so);
+  check("band_xy");
 }
 
 void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,

@@ -1017,11 +1017,21 @@ class Navier {
     yop(by_p, "Fdl", CONV.p, nx, W2.p);
     xop(bx_f, "Fdl", W2.p, ny, CV.p, W9.p);
   }
-  void to_ortho(const Basis& bx, const Basis& by, const double* q, double* out) {
+  // out = to_ortho(q) (+ add, same layout, when given)
+  void to_ortho(const Basis& bx, const Basis& by, const double* q, double* out, const double* add = nullptr) {
+    const long n = (long)rows_ortho(bx) * by.n;
+    if (use_bands && (bx.k == CD || bx.k == CN) && (by.k == CD || by.k == CN)) {  // both stencils in one pass
+      const DBand &sx = stencil_band(bx), &sy = stencil_band(by);
+      timer.begin(st, 3, 8.0 * nbt() * ((double)bx.m * by.m + n * (add ? 2.0 : 1.0)));
+      band_xy(st, q, by.m, out, bx.n, by.n, sx.coef.p, sx.mask.p, sy.coef.p, sy.mask.p, add, nbt(), dstr(), dstr());
+      timer.end(st);
+      return;
+    }
     const double* a = xop(bx, "S", q, by.m, W1.p, W9.p);
     const double* r = yop(by, "S", a, rows_ortho(bx), out);
     if (r != out)
-      HIPCHECK(hipMemcpyAsync(out, r, sizeof(double) * nbn((long)rows_ortho(bx) * by.n), hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipMemcpyAsync(out, r, sizeof(double) * nbn(n), hipMemcpyDeviceToDevice, st));
+    if (add) ew_add(st, out, out, add, nbn(n));
   }
 
   void solve_vel(int which, double* U, const double* buoy) {  // navier.rs:622-654
@@ -1337,9 +1347,7 @@ class Navier {
       update_batched();
       return;
     }
-    const long no = (long)rows_ortho(bx_f) * ny;
-    to_ortho(bx_t, by_u, T.p, W3.p);
-    ew_add(st, THAT.p, W3.p, THAT_BC.p, nbn(no));
+    to_ortho(bx_t, by_u, T.p, THAT.p, THAT_BC.p);  // buoyancy temp + fieldbc (navier.rs:740-742)
     if (pair_ok()) {  // ux and uy as one batch of twice the size
       {
         FieldPair fp(*this);

@@ -69,6 +69,11 @@ void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int 
             const unsigned char* mask, double beta, int batch = 1, long si = 0, long so = 0);
 void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
             const unsigned char* mask, double beta, int batch = 1, long si = 0, long so = 0);
+// band_x (x stencil cx/mkx, mox outputs) then band_y (cy/mky, moy outputs, input length ldi) in
+// one launch with the two passes' arithmetic; out [mox, moy] (+ add, same layout, when given)
+void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int moy, const double* cx,
+             const unsigned char* mkx, const double* cy, const unsigned char* mky, const double* add, int batch,
+             long si, long so);
 
 // Chebyshev d/dy of every row of in[rows, *] (ortho length n), with the composite
 // stencil of band coef/mask fused on load (coef == nullptr: input already ortho);
