@@ -194,18 +194,17 @@ __global__ void band_x_kernel(const double* __restrict__ in, double* __restrict_
 
 // band_x then band_y in one pass (the 2-D composite stencil to_ortho of a field): each
 // output forms the band_x values it needs with band_x's arithmetic and combines them as
-// band_y does; optionally + add (out = acc + add, as ew_add).  in [*, ldi] (ldi = the
-// y-input length), out [mox, moy].
+// band_y does.  in [*, ldi] (ldi = the y-input length), out [mox, moy].  Epilogue: out = acc,
+// out = acc + add (as ew_add), or with p0 the pressure update of ew_update_pres:
+// p0 = (p0 - div nu) + acc inv_dt, nu = nuarr[entry] when given.
 __global__ void band_xy_kernel(const double* __restrict__ in, int ldi, double* __restrict__ out, int mox, int moy,
                                const double* __restrict__ cx, const unsigned char* __restrict__ mkx,
-                               const double* __restrict__ cy, const unsigned char* __restrict__ mky,
-                               const double* __restrict__ add, long si, long so) {
+                               const double* __restrict__ cy, const unsigned char* __restrict__ mky, BandXYEpi ep,
+                               long si, long so) {
 #pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)mox * moy) return;
   in += si * blockIdx.y;
-  out += so * blockIdx.y;
-  if (add) add += so * blockIdx.y;
   const int i = (int)(t / moy), j = (int)(t % moy);
   const unsigned bx = mkx[i], by = mky[j];
   const int off[4] = {0, -2, 2, 4};
@@ -228,8 +227,14 @@ __global__ void band_xy_kernel(const double* __restrict__ in, int ldi, double* _
     acc = first ? term : acc + term;
     first = false;
   }
-  const size_t o = (size_t)i * moy + j;
-  out[o] = add ? acc + add[o] : acc;
+  const size_t o = so * blockIdx.y + (size_t)i * moy + j;
+  if (ep.p0) {
+    const double nu = ep.nuarr ? ep.nuarr[blockIdx.y] : ep.nu;
+    const double q = ep.p0[o] - ep.div[o] * nu;
+    ep.p0[o] = q + acc * ep.inv_dt;
+  } else {
+    out[o] = ep.add ? acc + ep.add[o] : acc;
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -463,13 +468,14 @@ void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int 
 }
 
 void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int moy, const double* cx,
-             const unsigned char* mkx, const double* cy, const unsigned char* mky, const double* add, int batch,
+             const unsigned char* mkx, const double* cy, const unsigned char* mky, const BandXYEpi& ep, int batch,
              long si, long so) {
   const long n = (long)mox * moy;
   if (n <= 0) return;
   if (batch < 1 || batch > 65535) throw std::invalid_argument("band_xy: bad batch");
+  if (ep.p0 && !ep.div) throw std::invalid_argument("band_xy: pressure update without div");
   RBC_LAUNCH(band_xy_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, mox, moy, cx,
-             mkx, cy, mky, add, si, so);
+             mkx, cy, mky, ep, si, so);
   check("band_xy");
 }
 

@@ -1017,14 +1017,24 @@ class Navier {
     yop(by_p, "Fdl", CONV.p, nx, W2.p);
     xop(bx_f, "Fdl", W2.p, ny, CV.p, W9.p);
   }
+  bool stencil2d_ok(const Basis& bx, const Basis& by) const {
+    return use_bands && (bx.k == CD || bx.k == CN) && (by.k == CD || by.k == CN);
+  }
+  // both stencils in one pass with the epilogue ep
+  void STENCIL2D(const Basis& bx, const Basis& by, const double* q, double* out, const BandXYEpi& ep) {
+    const DBand &sx = stencil_band(bx), &sy = stencil_band(by);
+    const double n = (double)bx.n * by.n, extra = ep.p0 ? 3.0 : ep.add ? 2.0 : 1.0;
+    timer.begin(st, 3, 8.0 * nbt() * ((double)bx.m * by.m + n * extra));
+    band_xy(st, q, by.m, out, bx.n, by.n, sx.coef.p, sx.mask.p, sy.coef.p, sy.mask.p, ep, nbt(), dstr(), dstr());
+    timer.end(st);
+  }
   // out = to_ortho(q) (+ add, same layout, when given)
   void to_ortho(const Basis& bx, const Basis& by, const double* q, double* out, const double* add = nullptr) {
     const long n = (long)rows_ortho(bx) * by.n;
-    if (use_bands && (bx.k == CD || bx.k == CN) && (by.k == CD || by.k == CN)) {  // both stencils in one pass
-      const DBand &sx = stencil_band(bx), &sy = stencil_band(by);
-      timer.begin(st, 3, 8.0 * nbt() * ((double)bx.m * by.m + n * (add ? 2.0 : 1.0)));
-      band_xy(st, q, by.m, out, bx.n, by.n, sx.coef.p, sx.mask.p, sy.coef.p, sy.mask.p, add, nbt(), dstr(), dstr());
-      timer.end(st);
+    if (stencil2d_ok(bx, by)) {
+      BandXYEpi ep;
+      ep.add = add;
+      STENCIL2D(bx, by, q, out, ep);
       return;
     }
     const double* a = xop(bx, "S", q, by.m, W1.p, W9.p);
@@ -1087,6 +1097,15 @@ class Navier {
     if (g != W2.p || g2 != W2.p + po) throw std::logic_error("pair: projection gradients expected in their slots");
     FieldPair fp(*this);
     const double* f = xop(bx_u, "FO", W2.p, ny, W1.p, W9.p);
+    if (use_bands && (by_u.k == CD || by_u.k == CN) && tdma_rows_lds_supported(by_u.m)) {
+      // from_ortho along y with the subtraction u = x (-1) + u of ew_project in its store
+      const DTdma& t = tdma_tables(by_u);
+      const int rows = rows_spec(bx_u);
+      timer.begin(st, 8, 8.0 * nbt() * (double)rows * ((double)by_u.n + 2.0 * by_u.m));
+      tdma_rows_lds(st, f, by_u.n, UX.p, by_u.m, rows, by_u.m, t.tabp.p, nbt(), dstr(), dstr(), true);
+      timer.end(st);
+      return;
+    }
     yop(by_u, "FO", f, rows_spec(bx_u), RHS.p);
     ew_project(st, UX.p, RHS.p, 1.0, nbn(ns));
   }
@@ -1120,6 +1139,16 @@ class Navier {
   }
   void update_pres() {  // navier.rs:717-721
     const long no = (long)rows_ortho(bx_p) * ny;
+    if (!periodic && stencil2d_ok(bx_phi, by_phi) && bx_phi.n == rows_ortho(bx_p) && by_phi.n == ny) {
+      BandXYEpi ep;  // to_ortho(pseu) and the update in one pass
+      ep.p0 = P0.p;
+      ep.div = DIV.p;
+      ep.nuarr = nb > 1 ? NUARR.p : nullptr;
+      ep.nu = nu;
+      ep.inv_dt = 1.0 / dt;
+      STENCIL2D(bx_phi, by_phi, P1.p, W3.p, ep);
+      return;
+    }
     to_ortho(bx_phi, by_phi, P1.p, W3.p);
     if (nb > 1) ew_update_pres_group(st, P0.p, DIV.p, W3.p, NUARR.p, 1.0 / dt, nbn(no), bs);
     else ew_update_pres(st, P0.p, DIV.p, W3.p, nu, 1.0 / dt, no);

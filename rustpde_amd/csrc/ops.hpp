@@ -70,9 +70,18 @@ void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int 
 void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
             const unsigned char* mask, double beta, int batch = 1, long si = 0, long so = 0);
 // band_x (x stencil cx/mkx, mox outputs) then band_y (cy/mky, moy outputs, input length ldi) in
-// one launch with the two passes' arithmetic; out [mox, moy] (+ add, same layout, when given)
+// one launch with the two passes' arithmetic; out [mox, moy].  Epilogue (same layout as out):
+// + add when given, or with p0 the pressure update p0 = (p0 - div nu) + result inv_dt
+// (nu = nuarr[batch entry] when given; out unused).
+struct BandXYEpi {
+  const double* add = nullptr;
+  double* p0 = nullptr;
+  const double* div = nullptr;
+  const double* nuarr = nullptr;
+  double nu = 0.0, inv_dt = 0.0;
+};
 void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int moy, const double* cx,
-             const unsigned char* mkx, const double* cy, const unsigned char* mky, const double* add, int batch,
+             const unsigned char* mkx, const double* cy, const unsigned char* mky, const BandXYEpi& ep, int batch,
              long si, long so);
 
 // Chebyshev d/dy of every row of in[rows, *] (ortho length n), with the composite
