@@ -45,13 +45,16 @@ struct Mfma<float> {
 // (converted as it is staged); C is V or FP64.
 // Batch entry z reads A at sA (z % amod), B at sB (z % bmod), U at sU (z % umod) (a modulus
 // of 0: z itself) and writes C at sC z: operands shared by groups of entries (an operator
-// per instance, used by several fields of it) need no copies.
+// per instance, used by several fields of it) need no copies.  Entries z >= split (when
+// split > 0) take A2 / B2 (when given) in place of A / B: a second field with its own
+// operators in the same launch.
 template <int BM, int BN, int BK, class V, class TB, class TC>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, const V* __restrict__ A, int lda, long sA,
                                                         const TB* __restrict__ B, int ldb, long sB, TC* __restrict__ C,
                                                         int ldc, long sC, double alpha, double beta,
                                                         const V* __restrict__ U, int ldu, long sU, int amod, int bmod,
-                                                        int umod) {
+                                                        int umod, const V* __restrict__ A2, const TB* __restrict__ B2,
+                                                        int split) {
   using MF = Mfma<V>;
   typedef typename MF::v4 v4;
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
@@ -69,6 +72,10 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
   const int m0 = by * BM, n0 = bx * BN;
   const int lr = lane & 15, lk = lane >> 4;
   const int z = lg / (gx * gy);  // batch entry
+  if (split > 0 && z >= split) {  // the second operator pair
+    if (A2) A = A2;
+    if (B2) B = B2;
+  }
   A += sA * (amod ? z % amod : z);
   B += sB * (bmod ? z % bmod : z);
   C += sC * z;
@@ -147,10 +154,12 @@ void launch_tile(hipStream_t s, bool timed, int M, int N, int K, const V* A, int
   dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, batch);
   if (timed) {
     RBC_LAUNCH((gemm_mfma_kernel<BM, BN, 32, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C,
-               ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u);
+               ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u, static_cast<const V*>(bm.a2),
+               static_cast<const TB*>(bm.b2), bm.split);
   } else {
     hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, 32, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb,
-                       sB, C, ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u);
+                       sB, C, ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u, static_cast<const V*>(bm.a2),
+                       static_cast<const TB*>(bm.b2), bm.split);
   }
 }
 

@@ -205,6 +205,10 @@ __global__ void band_xy_kernel(const double* __restrict__ in, int ldi, double* _
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)mox * moy) return;
   in += si * blockIdx.y;
+  if (ep.split > 0 && (int)blockIdx.y >= ep.split) {  // the second field's x stencil
+    cx = ep.cx2;
+    mkx = ep.mkx2;
+  }
   const int i = (int)(t / moy), j = (int)(t % moy);
   const unsigned bx = mkx[i], by = mky[j];
   const int off[4] = {0, -2, 2, 4};
@@ -474,6 +478,7 @@ void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int
   if (n <= 0) return;
   if (batch < 1 || batch > 65535) throw std::invalid_argument("band_xy: bad batch");
   if (ep.p0 && !ep.div) throw std::invalid_argument("band_xy: pressure update without div");
+  if (ep.split > 0 && (!ep.cx2 || !ep.mkx2)) throw std::invalid_argument("band_xy: second stencil missing");
   RBC_LAUNCH(band_xy_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, mox, moy, cx,
              mkx, cy, mky, ep, si, so);
   check("band_xy");

@@ -484,28 +484,29 @@ class Navier {
   DBuf NUARR;
   DBuf ADIH[3][2];
   long adi_s[3][2] = {};
-  // Field batching (confined step): inside a FieldPair scope fb = 2 and every launch covers
-  // ux and uy of every instance, batch entry z = f nb + b (field f, instance b).  Each work
-  // buffer of the velocity part then holds a partner slot pair_off() after it: the slots of
-  // a standalone handle (FS apart), or the second half of the group's slabs.
+  // Field batching (confined step): inside a FieldPair scope fb = 2 (or 3) and every launch
+  // covers ux and uy (and temp) of every instance, batch entry z = f nb + b (field f,
+  // instance b).  Each work buffer of the velocity part then holds partner slots pair_off()
+  // apart: the slots of a standalone handle (FS apart), or the parts of the group's slabs.
   int fb = 1;
   int nbt() const { return nb * fb; }                            // batch entries of a launch
   long dstr() const { return nb > 1 ? bs : (long)FS; }           // data stride between entries
   long pair_off() const { return nb > 1 ? (long)nb * bs : (long)FS; }
   long nbn(long n) const { return (long)(nbt() - 1) * dstr() + n; }  // element-wise extent
+  // fields = 3: ux, uy and temp (entries 2 nb .. 3 nb - 1 take temp's x operators)
   struct FieldPair {
     Navier& n;
     int prev;
-    explicit FieldPair(Navier& nv) : n(nv), prev(nv.fb) { n.fb = 2; }
+    explicit FieldPair(Navier& nv, int fields = 2) : n(nv), prev(nv.fb) { n.fb = fields; }
     ~FieldPair() { n.fb = prev; }
   };
   bool mixed = false;     // periodic batched step: convective terms in FP32, the rest FP64
   DBuf MIX;               // FP32 slots of the mixed convective pipeline (11 slots of FS floats)
   // confined mixed step: FP32 work buffers of the convective terms, kind k (0: x-transform
   // output, 1: physical ux and uy, 2: convective products, 3: dealiased y-transform) at
-  // k * mixc_kind(), each with the partner slots of a field pair
+  // k * mixc_kind(), each with the partner slots of a field triple
   DBuf MIXC;
-  long mixc_kind() const { return 2 * pair_off(); }
+  long mixc_kind() const { return 3 * pair_off(); }
   float* mixc(int k) { return reinterpret_cast<float*>(MIXC.p) + (size_t)k * mixc_kind(); }
   void alloc_mixc() {
     const size_t floats = 4 * (size_t)mixc_kind();
@@ -557,9 +558,12 @@ class Navier {
   // shared by the batch, or the stride of the group's per-instance operators (indexed by
   // the instance, so ux and uy share them).  U is per instance.
   static constexpr long DATA = -1;
+  // A2 / B2: temp's operators for the entries of the third field (fb = 3)
   void GEMM(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc, double beta = 0.0,
-            const double* U = nullptr, int ldu = 0, long sA = 0, long sB = 0) {
+            const double* U = nullptr, int ldu = 0, long sA = 0, long sB = 0, const double* A2 = nullptr,
+            const double* B2 = nullptr) {
     const int nz = nbt();
+    if ((A2 || B2) && fb != 3) throw std::logic_error("GEMM: second operators need fb = 3");
     timer.begin(st, 0, 2.0 * M * N * K * nz);
     if (nz > 1) {
       const long d = dstr();
@@ -567,13 +571,20 @@ class Navier {
       bm.a = sA > 0 ? nb : 0;
       bm.b = sB > 0 ? nb : 0;
       bm.u = nb;
+      bm.split = (A2 || B2) ? 2 * nb : 0;
+      bm.a2 = A2;
+      bm.b2 = B2;
       if (use_mfma || nb > 1) {
         gemm_mfma_batched(st, M, N, K, A, lda, sA == DATA ? d : sA, B, ldb, sB == DATA ? d : sB, C, ldc, d, nz, 1.0,
                           beta, U, ldu, U ? bs : 0, bm);
-      } else {  // one instance, a field pair, FMA GEMMs
-        for (int z = 0; z < nz; ++z)
-          gemm(st, M, N, K, A + (sA == DATA ? z * d : 0), lda, B + (sB == DATA ? z * d : 0), ldb, C + z * d, ldc, 1.0,
-               beta, U, ldu);
+      } else {  // one instance, a field pair or triple, FMA GEMMs
+        for (int z = 0; z < nz; ++z) {
+          const bool second = bm.split > 0 && z >= bm.split;
+          const double* Az = second && A2 ? A2 : A;
+          const double* Bz = second && B2 ? B2 : B;
+          gemm(st, M, N, K, Az + (sA == DATA ? z * d : 0), lda, Bz + (sB == DATA ? z * d : 0), ldb, C + z * d, ldc,
+               1.0, beta, U, ldu);
+        }
       }
     } else if (use_mfma) {
       gemm_mfma(st, M, N, K, A, lda, B, ldb, C, ldc, 1.0, beta, U, ldu);
@@ -584,13 +595,16 @@ class Navier {
   }
   // the same in FP32 (A and U FP32; B and C FP32 or FP64)
   void GEMM32(int M, int N, int K, const float* A, int lda, const void* B, bool b64, int ldb, void* C, bool c64,
-              int ldc, double beta, const float* U, int ldu, long sA, long sB) {
+              int ldc, double beta, const float* U, int ldu, long sA, long sB, const float* A2 = nullptr) {
     const int nz = nbt();
     const long d = dstr();
+    if (A2 && fb != 3) throw std::logic_error("GEMM32: second operators need fb = 3");
     BatchMod bm;
     bm.a = sA > 0 ? nb : 0;
     bm.b = sB > 0 ? nb : 0;
     bm.u = nb;
+    bm.split = A2 ? 2 * nb : 0;
+    bm.a2 = A2;
     timer.begin(st, 12, 2.0 * M * N * K * nz);
     gemm_mfma_f32_batched(st, M, N, K, A, lda, sA == DATA ? d : sA, B, b64, ldb, sB == DATA ? d : sB, C, c64, ldc, d,
                           nz, 1.0, beta, U, ldu, U ? bs : 0, bm);
@@ -679,25 +693,33 @@ class Navier {
   // Ensemble groups: `slabs` places every field and work buffer of this instance in slot `slot`
   // of group-owned slabs (slot stride = the buffer size), `ext` is the group's stream and
   // `share` an instance of the same shape whose Poisson solver (Ra-independent) is reused.
-  // The velocity part's buffers (ux/uy, their physical values, A, CONV, CV, RHS, W1, W2, G)
-  // have 2 B slots: uy and uy's values are the second half of the ux slabs, and the other
-  // work buffers keep their field-pair partners there.
+  // The field batches' buffers have partner slots B apart: the ux slab holds ux, uy and temp
+  // (3 B slots), the physical values ux and uy (2 B), A, CONV, CV, RHS, W1 and W2 three
+  // fields' partners (3 B) and G the two pressure gradients (2 B).
   struct Slabs {
     static constexpr int NK = 29;
     size_t F = 0;
     int B = 0;
     DBuf slab[NK];
-    static bool paired(int k) { return k == 0 || k == 3 || (k >= 5 && k <= 11); }
-    static bool second(int k) { return k == 1 || k == 4; }  // uy, uy values
+    static int parts(int k) { return k == 0 || (k >= 5 && k <= 10) ? 3 : (k == 3 || k == 11) ? 2 : 1; }
+    static int part_of(int k, int& base) {  // uy, temp: parts 1, 2 of the ux slab; uy values: part 1
+      base = k;
+      if (k == 1 || k == 2) { base = 0; return k; }
+      if (k == 4) { base = 3; return 1; }
+      return 0;
+    }
     void alloc(size_t f, int b) {
       F = f;
       B = b;
-      for (int k = 0; k < NK; ++k)
-        if (!second(k)) slab[k].alloc(F * (size_t)b * (paired(k) ? 2 : 1));
+      for (int k = 0; k < NK; ++k) {
+        int base = k;
+        if (part_of(k, base) == 0) slab[k].alloc(F * (size_t)b * parts(k));
+      }
     }
     double* at(int k, int slot) {
-      if (second(k)) return slab[k - 1].p + ((size_t)B + slot) * F;
-      return slab[k].p + (size_t)slot * F;
+      int base = k;
+      const int part = part_of(k, base);
+      return slab[base].p + ((size_t)part * B + slot) * F;
     }
   };
   bool owns_stream = true;
@@ -935,6 +957,109 @@ class Navier {
   void forward_dealias32() {
     yop32(by_p, "Fdl", mixc(2), nx, mixc(3), false);
     xop32(bx_f, "Fdl", mixc(3), false, ny, CV.p, true);
+  }
+
+  // ------------------------------------------- the field triple (ux, uy, temp; confined)
+  // The three fields share the y operators and the dealiasing; temp's x operators (bx_t)
+  // and ADI operators enter as the second operators of the same launches.
+  bool triple_ok() {
+    if (!pair_ok()) return false;
+    const Solver &s0 = *solver[0], &s2 = *solver[2];
+    if (s2.kind != ADI || bx_t.m != bx_u.m || bx_t.n != bx_u.n) return false;
+    if (s0.hx.rows != s2.hx.rows || s0.hx.cols != s2.hx.cols || s0.hy.rows != s2.hy.rows || s0.hy.cols != s2.hy.cols)
+      return false;
+    return nb == 1 || (adi_s[0][0] == adi_s[2][0] && adi_s[0][1] == adi_s[2][1]);
+  }
+  // x-transform `name` of the triple [m_x, cols] -> [rows, cols] (FP64 GEMM, fb = 3)
+  void xop3(const std::string& name, const double* in, int cols, double* out) {
+    const DOp &ou = X.get(bx_u, name), &ot = X.get(bx_t, name);
+    if (ou.rows != ot.rows || ou.cols != ot.cols) throw std::logic_error("xop3: operator shapes differ");
+    GEMM(ou.rows, cols, ou.cols, ou.b.p, ou.cols, in, cols, out, cols, 0.0, nullptr, 0, 0, DATA, ot.b.p);
+  }
+  void xop3_32(const std::string& name, const double* in, int cols, float* out) {
+    const DOp &ou = X.get(bx_u, name), &ot = X.get(bx_t, name);
+    if (ou.rows != ot.rows || ou.cols != ot.cols) throw std::logic_error("xop3: operator shapes differ");
+    GEMM32(ou.rows, cols, ou.cols, op32(ou), ou.cols, in, true, cols, out, false, cols, 0.0, nullptr, 0, 0, DATA,
+           op32(ot));
+  }
+  // convective terms of ux, uy, temp into CV: dq/dx ux + dq/dy uy (conv_term.rs:35-41), temp
+  // + u . grad(fieldbc) (navier.rs:547-550), then the dealiased forward transforms
+  void conv_triple() {
+    const int my = by_u.m;
+    const long np = (long)nx * ny, po = pair_off();
+    if (mixed) {
+      float* ux = mixc(1);
+      float* uy = ux + po;
+      {
+        FieldPair ft(*this, 3);
+        xop3_32("BDS", UX.p, my, mixc(0));
+        yop32(by_u, "BS", mixc(0), nx, mixc(2), false, ux, 0.0);
+        xop3_32("BS", UX.p, my, mixc(0));
+        yop32(by_u, "BDS", mixc(0), nx, mixc(2), false, uy, 1.0);
+      }
+      ew_acc_prod2_f32(st, mixc(2) + 2 * po, ux, DTBCX.p, uy, DTBCY.p, nbn(np));
+      FieldPair ft(*this, 3);
+      forward_dealias32();
+      return;
+    }
+    {
+      FieldPair ft(*this, 3);
+      xop3("BDS", UX.p, my, W1.p);
+      yop(by_u, "BS", W1.p, nx, CONV.p, UXP.p, 0.0);
+      xop3("BS", UX.p, my, W1.p);
+      yop(by_u, "BDS", W1.p, nx, CONV.p, UYP.p, 1.0);
+    }
+    ew_acc_prod2(st, CONV.p + 2 * po, UXP.p, DTBCX.p, UYP.p, DTBCY.p, nbn(np));
+    FieldPair ft(*this, 3);
+    forward_dealias_conv();
+  }
+  // the implicit ADI solves of ux, uy (solver 0) and temp (solver 2) in one pair of launches
+  void solve_triple() {
+    FieldPair ft(*this, 3);
+    const Solver &s0 = *solver[0], &s2 = *solver[2];
+    const bool grp = nb > 1;
+    const double* ax = grp ? ADIH[0][0].p : s0.hx.b.p;
+    const double* ax2 = grp ? ADIH[2][0].p : s2.hx.b.p;
+    const double* ay = grp ? ADIH[0][1].p : s0.hy.b.p;
+    const double* ay2 = grp ? ADIH[2][1].p : s2.hy.b.p;
+    const DOp &hx = s0.hx, &hy = s0.hy;
+    GEMM(hx.rows, ny, hx.cols, ax, hx.cols, RHS.p, ny, W1.p, ny, 0.0, nullptr, 0, grp ? adi_s[0][0] : 0, DATA, ax2);
+    GEMM(hx.rows, hy.cols, hy.rows, W1.p, hy.rows, ay, hy.cols, UX.p, hy.cols, 0.0, nullptr, 0, DATA,
+         grp ? adi_s[0][1] : 0, nullptr, ay2);
+  }
+  // one confined step with the field triple (navier.rs:737-765; temp's solve moves ahead of
+  // the projection, which does not read temp)
+  void update_triple() {
+    to_ortho(bx_t, by_u, T.p, THAT.p, THAT_BC.p);  // buoyancy temp + fieldbc (navier.rs:740-742)
+    {
+      FieldPair fp(*this);
+      if (mixed) backward_vel32();
+      else backward_vel(UX.p, UXP.p);
+    }
+    const long no = (long)rows_ortho(bx_f) * ny, po = pair_off();
+    {
+      FieldPair ft(*this, 3);
+      const DBand &su = stencil_band(bx_u), &stt = stencil_band(bx_t), &sy = stencil_band(by_u);
+      BandXYEpi ep;
+      ep.cx2 = stt.coef.p;
+      ep.mkx2 = stt.mask.p;
+      ep.split = 2 * nb;
+      timer.begin(st, 3, 8.0 * nbt() * ((double)bx_u.m * by_u.m + (double)bx_u.n * by_u.n));
+      band_xy(st, UX.p, by_u.m, A.p, bx_u.n, by_u.n, su.coef.p, su.mask.p, sy.coef.p, sy.mask.p, ep, nbt(), dstr(),
+              dstr());
+      timer.end(st);
+    }
+    xop(bx_p, "DS", P0.p, ny, G.p, W9.p);                                  // d/dx pres
+    const double* gy = yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + po);  // d/dy pres
+    if (gy != G.p + po) throw std::logic_error("triple: d/dy pres expected in its slot");
+    conv_triple();
+    ew_rhs3(st, RHS.p, A.p, G.p, THAT.p, CV.p, GXX.p, GYY.p, dt, nbn(no), po);
+    solve_triple();
+    divergence(DIV.p);
+    solve_pres();
+    project_velocity_pair();
+    update_pres();
+    time += dt;
   }
 
   // ------------------------------------------------------------------ set-up pieces
@@ -1374,6 +1499,10 @@ class Navier {
   void update() {  // navier.rs:737-765
     if (batch_ok()) {
       update_batched();
+      return;
+    }
+    if (!periodic && triple_ok() && stencil2d_ok(bx_t, by_u) && stencil2d_ok(bx_u, by_u)) {
+      update_triple();
       return;
     }
     to_ortho(bx_t, by_u, T.p, THAT.p, THAT_BC.p);  // buoyancy temp + fieldbc (navier.rs:740-742)

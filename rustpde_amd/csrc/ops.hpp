@@ -37,11 +37,16 @@ void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, con
                int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
 
 // Batch-index moduli of A, B, U (0: none): entry z reads A + sA (z % a) etc.
+// Entries z >= split read A from a2 and B from b2 instead (when given; same types, strides
+// and moduli): two operators in one launch.
 struct BatchMod {
   int a = 0, b = 0, u = 0;
+  int split = 0;
+  const void* a2 = nullptr;
+  const void* b2 = nullptr;
 };
 // Batched: operand b of batch index z starts at A + z sA, B + z sB, C + z sC, U + z sU
-// (with the moduli of bm applied to the index of A, B, U).
+// (with the moduli and the operator switch of bm applied to A, B, U).
 void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
                        long sB, double* C, int ldc, long sC, int batch, double alpha, double beta,
                        const double* U = nullptr, int ldu = 0, long sU = 0, BatchMod bm = BatchMod{});
@@ -73,12 +78,16 @@ void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, cons
 // one launch with the two passes' arithmetic; out [mox, moy].  Epilogue (same layout as out):
 // + add when given, or with p0 the pressure update p0 = (p0 - div nu) + result inv_dt
 // (nu = nuarr[batch entry] when given; out unused).
+// Batch entries >= split (when split > 0) use the second x stencil cx2/mkx2.
 struct BandXYEpi {
   const double* add = nullptr;
   double* p0 = nullptr;
   const double* div = nullptr;
   const double* nuarr = nullptr;
   double nu = 0.0, inv_dt = 0.0;
+  const double* cx2 = nullptr;
+  const unsigned char* mkx2 = nullptr;
+  int split = 0;
 };
 void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int moy, const double* cx,
              const unsigned char* mkx, const double* cy, const unsigned char* mky, const BandXYEpi& ep, int batch,
