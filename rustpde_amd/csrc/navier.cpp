@@ -1030,7 +1030,6 @@ class Navier {
   // one confined step with the field triple (navier.rs:737-765; temp's solve moves ahead of
   // the projection, which does not read temp)
   void update_triple() {
-    to_ortho(bx_t, by_u, T.p, THAT.p, THAT_BC.p);  // buoyancy temp + fieldbc (navier.rs:740-742)
     {
       FieldPair fp(*this);
       if (mixed) backward_vel32();
@@ -1053,7 +1052,8 @@ class Navier {
     const double* gy = yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + po);  // d/dy pres
     if (gy != G.p + po) throw std::logic_error("triple: d/dy pres expected in its slot");
     conv_triple();
-    ew_rhs3(st, RHS.p, A.p, G.p, THAT.p, CV.p, GXX.p, GYY.p, dt, nbn(no), po);
+    // buoyancy to_ortho(temp) + fieldbc (navier.rs:740-742) from temp's slot of A
+    ew_rhs3(st, RHS.p, A.p, G.p, THAT_BC.p, CV.p, GXX.p, GYY.p, dt, nbn(no), po, true);
     solve_triple();
     divergence(DIV.p);
     solve_pres();

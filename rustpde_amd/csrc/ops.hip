@@ -192,8 +192,10 @@ EW_KERNEL(ew_acc_prod2, (double* out, const double* a, const double* b, const do
             out[i] = t + c[i] * d[i];
           })
 
+// buoy_bc: the buoyancy is temp's entry of a plus buoy (to_ortho(temp) + fieldbc, formed
+// here as the separate addition would form it)
 __global__ void ew_rhs3_kernel(double* out, const double* a, const double* g, const double* buoy, const double* c,
-                               const double* bx, const double* by, double dt, long n, long F) {
+                               const double* bx, const double* by, double dt, long n, long F, int buoy_bc) {
 #pragma clang fp contract(off)
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -203,7 +205,10 @@ __global__ void ew_rhs3_kernel(double* out, const double* a, const double* g, co
     out[j] = ((a[j] + bx[i]) + by[i]) - c[j] * dt;
   } else {
     double r = a[j] - g[j] * dt;
-    if (k == 1) r = r + buoy[i] * dt;
+    if (k == 1) {
+      const double b = buoy_bc ? a[2 * F + i] + buoy[i] : buoy[i];
+      r = r + b * dt;
+    }
     out[j] = r - c[j] * dt;
   }
 }
@@ -346,10 +351,10 @@ void ew_acc_prod2_f32(hipStream_t s, float* out, const float* a, const double* b
   EW_LAUNCH(ew_acc_prod2_f32, n, out, a, b, c, d);
 }
 void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
-             const double* bx, const double* by, double dt, long n, long F) {
+             const double* bx, const double* by, double dt, long n, long F, bool buoy_bc) {
   if (n <= 0) return;
   RBC_LAUNCH(ew_rhs3_kernel, dim3(blocks_for(n, 256), 3), dim3(256), 0, s, out, a, g, buoy, c, bx, by, dt, n,
-                     F);
+                     F, buoy_bc ? 1 : 0);
   check_launch("ew_rhs3");
 }
 void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt, long n) {

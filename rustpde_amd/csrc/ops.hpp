@@ -125,9 +125,10 @@ void ew_acc_prod2(hipStream_t s, double* out, const double* a, const double* b, 
 void ew_acc_prod2_f32(hipStream_t s, float* out, const float* a, const double* b, const float* c, const double* d,
                       long n);  // the same in FP32 (b, d rounded to FP32)
 // The right-hand sides of ux, uy, temp in slots 0, 1, 2 (slot stride F) in one launch:
-// slot 0 ((a - g dt)) - c dt, slot 1 ((a - g dt) + buoy dt) - c dt, slot 2 ((a + bx) + by) - c dt
+// slot 0 ((a - g dt)) - c dt, slot 1 ((a - g dt) + buoy dt) - c dt, slot 2 ((a + bx) + by) - c dt;
+// buoy_bc: buoy holds to_ortho(fieldbc) and the buoyancy is a's slot 2 + buoy
 void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
-             const double* bx, const double* by, double dt, long n, long F);
+             const double* bx, const double* by, double dt, long n, long F, bool buoy_bc = false);
 // The same with the to_ortho stencil of ux, uy, temp (U: slots of [2 mx, my], taps
 // stencil[0, ny) and [ny, 2 ny)), the buoyancy temp + fieldbc (thbc) and d/dx pres (from P0,
 // Fourier x) formed on the fly; g1 = d/dy pres; all other arrays [2 mx, ny], slot stride F.
