@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
                                                         int ldc, long sC, double alpha, double beta,
                                                         const V* __restrict__ U, int ldu, long sU, int amod, int bmod,
                                                         int umod, const V* __restrict__ A2, const TB* __restrict__ B2,
-                                                        int split) {
+                                                        int split, int zero00) {
   using MF = Mfma<V>;
   typedef typename MF::v4 v4;
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
@@ -138,6 +138,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
           V v = va * acc[i][j][r];
           if (U) v = v * U[(size_t)gm * ldu + gn];
           if (beta != 0.0) v = v + vb * (V)C[(size_t)gm * ldc + gn];
+          if (zero00 && gm == 0 && gn == 0) v = (V)0;
           C[(size_t)gm * ldc + gn] = (TC)v;
         }
       }
@@ -155,11 +156,11 @@ void launch_tile(hipStream_t s, bool timed, int M, int N, int K, const V* A, int
   if (timed) {
     RBC_LAUNCH((gemm_mfma_kernel<BM, BN, 32, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C,
                ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u, static_cast<const V*>(bm.a2),
-               static_cast<const TB*>(bm.b2), bm.split);
+               static_cast<const TB*>(bm.b2), bm.split, bm.zero00);
   } else {
     hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, 32, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb,
                        sB, C, ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u, static_cast<const V*>(bm.a2),
-                       static_cast<const TB*>(bm.b2), bm.split);
+                       static_cast<const TB*>(bm.b2), bm.split, bm.zero00);
   }
 }
 

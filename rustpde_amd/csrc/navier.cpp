@@ -565,6 +565,19 @@ class Navier {
     const int nz = nbt();
     if ((A2 || B2) && fb != 3) throw std::logic_error("GEMM: second operators need fb = 3");
     timer.begin(st, 0, 2.0 * M * N * K * nz);
+    if (gemm_zero00 && !A2 && !B2 && (use_mfma || nz > 1)) {  // the product and the pin of C(0, 0)
+      const long d = dstr();
+      BatchMod bm;
+      bm.a = sA > 0 ? nb : 0;
+      bm.b = sB > 0 ? nb : 0;
+      bm.u = nb;
+      bm.zero00 = 1;
+      gemm_mfma_batched(st, M, N, K, A, lda, sA == DATA ? d : sA, B, ldb, sB == DATA ? d : sB, C, ldc, d, nz, 1.0,
+                        beta, U, ldu, U ? bs : 0, bm);
+      timer.end(st);
+      return;
+    }
+    if (gemm_zero00) throw std::logic_error("GEMM: pinned output needs the matrix-core path");
     if (nz > 1) {
       const long d = dstr();
       BatchMod bm;
@@ -1112,7 +1125,9 @@ class Navier {
   }
 
   // ------------------------------------------------------------------------ step
-  void solve(int which, double* rhs, double* out) {
+  bool gemm_zero00 = false;  // the next GEMM pins C(0, 0) of every entry to 0
+  // zero00: out(0, 0) = 0 of every entry after a confined Poisson solve (formed in its last GEMM)
+  void solve(int which, double* rhs, double* out, bool zero00 = false) {
     Solver& s = *solver[which];
     const int my = s.by.m;
     if (s.kind == ADI) {
@@ -1127,7 +1142,14 @@ class Navier {
     }
     xmat(s.pf, rhs, ny, W1.p);              // fwd * P_x : [m0, ny]
     ROWSF(W1.p, ny, W2.p, my, s.pf.rows, s);  // P_y and the row systems : [m0, my]
-    xmat(s.pb, W2.p, my, out);              // bwd       : [m0, my]
+    gemm_zero00 = zero00;
+    try {
+      xmat(s.pb, W2.p, my, out);  // bwd : [m0, my]
+    } catch (...) {
+      gemm_zero00 = false;
+      throw;
+    }
+    gemm_zero00 = false;
   }
 
   // convective term of q with x-basis bxq (y basis cheb_dirichlet), into CONV (physical)
@@ -1258,6 +1280,10 @@ class Navier {
     ew_project(st, UY.p, W4.p, 1.0, nbn(ns));
   }
   void solve_pres() {  // navier.rs:710-715
+    if (!periodic && solver[3]->has_fb && (use_mfma || nb > 1)) {  // pseu(0, 0) = 0 in the solve's last GEMM
+      solve(3, DIV.p, P1.p, true);
+      return;
+    }
     solve(3, DIV.p, P1.p);
     if (periodic) set_zero_entries(st, P1.p, 2, 0, (long)bx_phi.m * by_phi.m, nb, bs);
     else set_zero_entries(st, P1.p, 1, 0, 0, nb, bs);

@@ -39,11 +39,13 @@ void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, con
 // Batch-index moduli of A, B, U (0: none): entry z reads A + sA (z % a) etc.
 // Entries z >= split read A from a2 and B from b2 instead (when given; same types, strides
 // and moduli): two operators in one launch.
+// zero00: every entry's C(0, 0) is set to 0 after the product (the pseudo-pressure pin).
 struct BatchMod {
   int a = 0, b = 0, u = 0;
   int split = 0;
   const void* a2 = nullptr;
   const void* b2 = nullptr;
+  int zero00 = 0;
 };
 // Batched: operand b of batch index z starts at A + z sA, B + z sB, C + z sC, U + z sU
 // (with the moduli and the operator switch of bm applied to A, B, U).
