@@ -58,3 +58,14 @@ def test_invalid_arguments_are_rejected_before_any_device_call():
     assert b"kind" in lib.rbc_last_error()
     assert lib.rbc_solver_create(7, 1, 16, 1, 16, 1e-3, 1e-3, ctypes.byref(h)) != 0
     assert b"solver" in lib.rbc_last_error()
+
+
+def test_null_handles_are_rejected():
+    """Entry points given a null handle return an error code with a message instead of
+    dereferencing it."""
+    lib = _lib.load()
+    assert lib.rbc_set_option(None, 7, 1) != 0
+    assert lib.rbc_ensemble_set_option(None, 7, 1) != 0
+    assert b"null" in lib.rbc_last_error()
+    assert lib.rbc_ensemble_update(None, 1) != 0
+    assert lib.rbc_update(None, 1) != 0
