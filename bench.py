@@ -86,6 +86,50 @@ def make(cfg, lib_mod):
     return n
 
 
+def nu_run(cfg, t_end, lib_mod):
+    """Time-averaged Nu of the mixed and the FP64 step over t_end time units from the same
+    seeded fields (Nu sampled once per time unit, as integrate() with save_intervall = 1);
+    prints one JSON line and a progress line per 10 time units.  A run whose Nu turns NaN
+    (the explicit convective term past its stability limit) stops there and reports the
+    time; the averages then cover the time units before it."""
+    import numpy as np
+    dt = CONFIGS[cfg][5]
+    per = int(round(1.0 / dt))
+    out = {"workload": WORKLOAD_NAMES[cfg], "time_units": t_end, "steps_per_unit": per}
+    series = {}
+    for prec in ("fp64", "mixed"):
+        nav = make(cfg, lib_mod)
+        if prec == "mixed":
+            nav.set_option(nav.OPT_MIXED, 1)
+        nu = []
+        t0 = time.perf_counter()
+        for u in range(int(t_end)):
+            nav.update(per)
+            v = nav.eval_nu()
+            if not np.isfinite(v):
+                out["%s_diverged_at" % prec] = u + 1
+                break
+            nu.append(v)
+            if (u + 1) % 10 == 0:
+                print("nu-run %s t=%d Nu=%.6f (%.0f s)" % (prec, u + 1, nu[-1], time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+        series[prec] = np.array(nu)
+        out["%s_nu_mean" % prec] = float(series[prec].mean()) if len(nu) else float("nan")
+        out["%s_nu_final" % prec] = float(series[prec][-1]) if len(nu) else float("nan")
+        del nav
+    common = min(len(series["fp64"]), len(series["mixed"]))
+    if common < len(series["fp64"]) or common < len(series["mixed"]):
+        for prec in ("fp64", "mixed"):
+            out["%s_nu_mean" % prec] = float(series[prec][:common].mean()) if common else float("nan")
+    out["time_units_compared"] = common
+    out["nu_mean_rel_diff"] = abs(out["mixed_nu_mean"] - out["fp64_nu_mean"]) / abs(out["fp64_nu_mean"])
+    for t in (10, 20, 50, 100):  # the early (deterministic) part of the run
+        if t <= common:
+            a, b = series["fp64"][:t], series["mixed"][:t]
+            out["nu_mean_rel_diff_first_%d" % t] = float(abs(b.mean() - a.mean()) / abs(a.mean()))
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(cfg, budget_s):
     from oracle import oracle as O
     if not os.path.exists(O._LIB_PATH):
@@ -299,6 +343,9 @@ def main():
                          "timed beside it and the field / Nu differences after 10 steps are reported")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nu-time", type=float, default=0.0, metavar="T",
+                    help="instead of timing: the mixed and the FP64 step over T time units, Nu averaged "
+                         "(sampled once per time unit)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device to device 3 times: the known-size copy that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -312,6 +359,10 @@ def main():
         return
     if args.config == "c4":
         bench_ensemble(args)
+        return
+    if args.nu_time > 0:
+        import rustpde_amd
+        nu_run(args.config, args.nu_time, rustpde_amd)
         return
 
     import ctypes
