@@ -123,8 +123,10 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
   Y += sy * b;
   const int tid = threadIdx.x;
   const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
-  // (1) half spectra, (i k)^times / sc applied, packed per pair: re[k] = Re A_k (k <= N),
-  //     re[N + k] = Im A_k (0 < k < N), the same for B in im
+  // (1) half spectra with (i k)^times / sc applied; the two columns of a pair (lanes e and
+  //     e ^ 1: C is even) exchange theirs by a lane swap and write conj Z of the pair:
+  //     conj Z_k = (Ar - Bi) - i (Ai + Br), conj Z_{L-k} = (Ar + Bi) + i (Ai - Br)
+  //     (Im A_0, Im A_N ignored: the half-spectrum convention)
   constexpr int QS = (m * C + T - 1) / T;
   V va[QS], vb[QS];
 #pragma unroll
@@ -137,7 +139,6 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
 #pragma unroll
   for (int q = 0; q < QS; ++q) {
     const int e = tid + q * T, c = e % C, k = e / C;
-    if (e >= m * C) continue;
     V a = va[q], b = vb[q];
     const V kk = (V)k, zero = 0;
     for (int r = 0; r < times; ++r) {
@@ -156,38 +157,18 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
         b = nb;
       }
     }
-    V* z = (c & 1) ? im : re;
+    if (k == 0 || k == N) b = 0;
+    const V pa = __shfl_xor(a, 1), pb = __shfl_xor(b, 1);  // every lane takes part
+    if (e >= m * C) continue;
+    const bool odd = (c & 1) != 0;
+    const V ar = odd ? pa : a, ai = odd ? pb : b, br = odd ? a : pa, bi = odd ? b : pb;
     const int zb = (c >> 1) * L;
-    z[lp(zb + k)] = a;
-    if (k > 0 && k < N) z[lp(zb + N + k)] = b;
-  }
-  __syncthreads();
-  // (2) conj Z_k = (Ar - Bi) - i (Ai + Br),  conj Z_{L-k} = (Ar + Bi) + i (Ai - Br)
-  constexpr int QZ = (m * G::PP + T - 1) / T;
-  V ar[QZ], ai[QZ], br[QZ], bi[QZ];
-#pragma unroll
-  for (int q = 0; q < QZ; ++q) {
-    const int e = tid + q * T, p = e / m, k = e - p * m;
-    ar[q] = ai[q] = br[q] = bi[q] = 0;
-    if (e < m * G::PP) {
-      ar[q] = re[lp(p * L + k)];
-      br[q] = im[lp(p * L + k)];
-      if (k > 0 && k < N) {
-        ai[q] = re[lp(p * L + N + k)];
-        bi[q] = im[lp(p * L + N + k)];
-      }
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < QZ; ++q) {
-    const int e = tid + q * T, p = e / m, k = e - p * m;
-    if (e >= m * G::PP) continue;
-    re[lp(p * L + k)] = ar[q] - bi[q];
-    im[lp(p * L + k)] = -(ai[q] + br[q]);
-    if (k > 0 && k < N) {
-      re[lp(p * L + L - k)] = ar[q] + bi[q];
-      im[lp(p * L + L - k)] = ai[q] - br[q];
+    if (!odd) {
+      re[lp(zb + k)] = ar - bi;
+      im[lp(zb + k)] = -(ai + br);
+    } else if (k > 0 && k < N) {
+      re[lp(zb + L - k)] = ar + bi;
+      im[lp(zb + L - k)] = ai - br;
     }
   }
   __syncthreads();
