@@ -513,6 +513,10 @@ class Navier {
     if (MIXC.n < (floats + 1) / 2) MIXC.alloc((floats + 1) / 2);
   }
   bool vel_same = true;   // the ux and uy solvers hold identical tables
+  // G's second slot holds d/dy of the current pres: formed by the fused pressure stage of the
+  // previous batched step (or zero with pres at the start); a host write of a field or a step
+  // on another path clears it, and the next batched step forms it first
+  bool gpres_valid = true;
   size_t FS = 0;          // slot stride of the batched buffers
   DBuf STATE, PHYS2;      // [ux, uy, temp] spectral and [ux, uy] physical, one slot each
   hipGraphExec_t gexec = nullptr;
@@ -523,6 +527,7 @@ class Navier {
   // Integrate::update nsteps times: one eager step, then a captured step replayed
   void update_n(int nsteps) {
     int done = 0;
+    if (!gpres_valid) drop_graph();  // the captured step relies on d/dy pres from the last step
     if (use_graph && !timer.on && nsteps > 1 && !gexec) {
       update();
       ++done;
@@ -1370,7 +1375,7 @@ class Navier {
     }
     // d/dy pres, then the right-hand sides (navier.rs:622-674) with to_ortho of ux, uy, temp,
     // the buoyancy temp + fieldbc (navier.rs:740-742) and d/dx pres formed in the same kernel
-    yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
+    if (!gpres_valid || !pressure_fused_ok()) yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
     // bytes: ux, uy, temp coefficients, pres, d/dy pres, fieldbc, gxx, gyy and three
     // convective terms read, three right-hand sides written
     if (mixed) {  // FP32 convective terms (conv_mixed)
@@ -1490,12 +1495,16 @@ class Navier {
                       1.0 / dt};
       pa.su_kind = by_u.k == CD ? 1 : by_u.k == CN ? 2 : 0;
       pa.sp_kind = by_phi.k == CD ? 1 : by_phi.k == CN ? 2 : 0;
+      pa.dP = G.p + FS;  // d/dy of the new pres for the next step
       const double rows = 2.0 * mx();
+      // bytes: + the new pres rows read back and d/dy pres written (2 n per row)
       timer.begin(st, 10, 8.0 * rows * (2.0 * by_u.m + ny) + 8.0 * rows * (3.0 * by_u.m + ny) +
-                              8.0 * 4.0 * mx() * (double)by_phi.m);
+                              8.0 * 4.0 * mx() * (double)by_phi.m + 8.0 * rows * 2.0 * ny);
       pressure_rows(st, pa);
       timer.end(st);
+      gpres_valid = true;
     } else {
+      gpres_valid = false;
       divergence(DIV.p);
       solve_pres();
       project_velocity_batched();
@@ -1527,6 +1536,7 @@ class Navier {
       update_batched();
       return;
     }
+    gpres_valid = false;
     if (!periodic && triple_ok() && stencil2d_ok(bx_t, by_u) && stencil2d_ok(bx_u, by_u)) {
       update_triple();
       return;
@@ -1581,6 +1591,7 @@ class Navier {
     cplx = periodic ? 1 : 0;
   }
   void forward(int which) {  // Space2::forward: axis 1 then axis 0
+    gpres_valid = false;
     FieldRef f = field(which);
     const double* a = yop(f.by, "FS", f.phys->p, nx, W1.p);
     const double* r = xop(f.bx, "FS", a, f.by.m, W2.p, W9.p);
@@ -1607,6 +1618,7 @@ class Navier {
     for (size_t i = 0; i < pl; ++i) { host[2 * i] = v[i]; host[2 * i + 1] = v[pl + i]; }
   }
   void set(int which, int space, const double* host) {
+    gpres_valid = false;
     int n0, n1, c;
     shape(which, space, n0, n1, c);
     FieldRef f = field(which);
@@ -2262,6 +2274,7 @@ namespace {
 void apply_option(Navier& n, int option, int value, bool alloc = true) {
   HIPCHECK(hipStreamSynchronize(n.st));
   n.drop_graph();
+  n.gpres_valid = false;
   switch (option) {
     case RBC_OPT_GRAPH: n.use_graph = value != 0; break;
     case RBC_OPT_MFMA: n.use_mfma = value != 0; break;

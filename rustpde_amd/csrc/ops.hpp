@@ -219,6 +219,9 @@ struct PressureArgs {
   // stencils of the velocity and pressure bases: 0 from su / sp, 1 Dirichlet, 2 Neumann
   // (formed in the kernel; the tables' values are the same)
   int su_kind = 0, sp_kind = 0;
+  // when set: d/dy of the updated pres (Chebyshev derivative / scale_y, rows as P0) for the
+  // next step's right-hand sides
+  double* dP = nullptr;
 };
 bool pressure_rows_supported(int ny, int mu);
 void pressure_rows(hipStream_t s, const PressureArgs& a);

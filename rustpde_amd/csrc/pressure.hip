@@ -312,6 +312,43 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
   // (7) the updated ux rows -> HBM; uy = x (-1) + uy (uy reloaded: its staged copy became pseu)
   chains::unstage_row<QH>(XA(h), a.UX + ROWU(h), true, mu, th, 128, false);
   chains::unstage_row<QH>(DV(h), a.UY + ROWU(h), true, mu, th, 128, true);
+  if (a.dP) {
+    // (8) d/dy of the updated pres (ortho.rs:107-125, the chains of (2) on the Chebyshev
+    //     coefficients) for the next step's right-hand sides: pres rows (written in (4)) ->
+    //     YB, derivative -> XA -> dP
+    __syncthreads();
+    chains::stage_row<QH>(YB(h), a.P0 + ROWN(h), true, ny, th, 128);
+    __syncthreads();
+    {
+      double v[SLICE];
+      const double* Y = YB(r);
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        v[q] = 0.0;
+        if (q < cntd) {
+          const int i = (p ? 1 : 2) + 2 * (t0d + q);
+          if (i <= N - 1) v[q] = (2.0 * (double)(i + 1)) * Y[pad(i + 3)];
+        }
+      }
+      chains::suffix_sum(v, lane, cntd);
+#pragma unroll
+      for (int q = 0; q < SLICE; ++q) {
+        if (q < cntd) {
+          const int i = (p ? 1 : 2) + 2 * (t0d + q);
+          double d = v[q];
+          if (a.inv_scy != 1.0) d = d * a.inv_scy;
+          XA(r)[pad(i + 2)] = d;
+        }
+      }
+      if (p == 0 && lane == 0) {
+        double d = Y[pad(3)] + v[0] / 2.0;
+        if (a.inv_scy != 1.0) d = d * a.inv_scy;
+        XA(r)[pad(2)] = d;
+      }
+    }
+    __syncthreads();
+    chains::unstage_row<QH>(XA(h), a.dP + ROWN(h), true, ny, th, 128, false);
+  }
 #undef XA
 #undef YB
 #undef DV
