@@ -112,8 +112,9 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *                  power of two (default 1; 0 = dense transform GEMMs)
  *   RBC_OPT_BATCH  periodic step: the transforms of ux, uy and temp and the ux/uy solves
  *                  run as batched launches (default 1; needs the FFT, DCT and band paths);
- *                  confined step (and ensemble groups): ux and uy's transforms, products,
- *                  ADI solves and projection as launches of twice the batch
+ *                  confined step (and ensemble groups): ux, uy and temp's transforms,
+ *                  products and ADI solves as launches of three times the batch (with
+ *                  RBC_OPT_BANDS off: ux and uy as launches of twice the batch)
  *   RBC_OPT_FUSE   periodic batched step: the convective terms in two launches (values and
  *                  x-derivatives of ux, uy, temp from one x-transform launch, then one
  *                  y-transform launch forming d/dy q uy + dq/dx ux) and divergence, Poisson
@@ -121,10 +122,10 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *                  (default 1)
  *   RBC_OPT_MIXED  mixed precision: the convective terms (their x/y transforms and
  *                  products) in FP32, everything else in FP64 (default 0).  Periodic
- *                  batched step: FFT/DCT kernels in FP32; confined step with field pairs
- *                  (and ensemble groups, rbc_ensemble_set_option): the dense transforms
- *                  as FP32 matrix-core GEMMs.  Fields agree with the FP64 step to ~1e-9
- *                  after 10 steps. */
+ *                  batched step: FFT/DCT kernels in FP32; confined step with RBC_OPT_BATCH
+ *                  on (and ensemble groups, rbc_ensemble_set_option): the dense transforms
+ *                  as FP32 matrix-core GEMMs (with RBC_OPT_BATCH off the confined step
+ *                  stays FP64).  Fields agree with the FP64 step to ~1e-9 after 10 steps. */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2

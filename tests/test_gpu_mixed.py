@@ -91,6 +91,27 @@ def test_confined_mixed_vs_oracle(oracle, shape):
     assert rel_err(g.temp.vhat, a.temp.vhat) > 1e-13  # the FP32 path is taken
 
 
+def test_confined_mixed_pair_form(oracle):
+    """With the banded stencils off (option 3 = 0) the confined step runs the ux/uy pair form
+    instead of the field triple; its mixed variant (the FP32 convective GEMMs of the pair
+    and of temp) stays within 1e-5 of the oracle and of the triple's mixed step."""
+    print("case confined mixed pair form", flush=True)
+    g, o = _confined(oracle, 48, 65, 1e5, 0.02, 5)
+    t, _ = _confined(oracle, 48, 65, 1e5, 0.02, 5)
+    a, _ = _confined(oracle, 48, 65, 1e5, 0.02, 5)
+    g.set_option(MIXED, 1)
+    g.set_option(3, 0)
+    t.set_option(MIXED, 1)
+    for nav in (g, o, t, a):
+        nav.update(10)
+    e = compare(g, o)
+    print("confined mixed pair-form errors", e, flush=True)
+    for k in ("temp", "ux", "uy", "pres"):
+        assert e[k] < 1e-5, (k, e)
+    assert rel_err(g.temp.vhat, t.temp.vhat) < 1e-5
+    assert rel_err(g.temp.vhat, a.temp.vhat) > 1e-13  # the FP32 path is taken
+
+
 def test_confined_mixed_graph_and_switch_back(oracle):
     """Graph replays of the confined mixed step equal eager mixed steps bit for bit; switching
     the option off continues with the FP64 step."""
