@@ -6,13 +6,17 @@ velocity zero.  A step is one Integrate::update (navier.rs:737-765); only update
 timed, as benches/benchmark_navier.rs:34 does.  Inputs are resident on the GPU when the
 timed region starts.
 
-value      : steps/s of the FP64 GPU step over the timed region (K steps, the captured
-             step replayed as a HIP graph)
+value      : steps/s of the GPU step over the timed region (K steps, the captured step
+             replayed as a HIP graph); mixed precision by default (--precision fp64: all
+             FP64), with the FP64 step timed beside it
 roofline   : the kernel class with the largest summed time over the same K steps run
              with HIP events recorded on the handle's stream around every launch;
              achieved = algorithmic work per launch / average launch duration
 cpu_baseline: the CPU oracle (FP64 restatement of the reference algorithm) on the host,
              a bounded sample of the same workload
+statistics_and_snapshot: after the timed steps, the device statistics update
+             (src/navier/statistics.rs:130-153) per call and a snapshot write / read
+             (navier.rs:961-1008), with the oracle's statistics update beside them
 """
 import argparse
 import json
@@ -148,9 +152,45 @@ def cpu_baseline(cfg, budget_s):
     t0 = time.perf_counter()
     o.update(steps)
     el = time.perf_counter() - t0
+    # one statistics update of the oracle at the same state (statistics.rs:130-153)
+    o.stats_enable(1.0, 10.0)
+    t1 = time.perf_counter()
+    o.stats_update()
+    stats_ms = 1e3 * (time.perf_counter() - t1)
     return {"value": steps / el, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": "%d steps of the same workload after 1 warm step (oracle/rbc_oracle.cpp, OpenMP over lanes)"
-                      % steps}
+                      % steps}, stats_ms
+
+
+def stats_and_snapshot(nav, updates=20):
+    """After the timed steps: the device statistics update (statistics.rs:130-153, as
+    callback calls it) per call, and a snapshot write and read (navier.rs:961-1008) through
+    host memory to a temporary file."""
+    import tempfile
+
+    from rustpde_amd import checkpoint
+    st = nav.init_statistics(1.0, 10.0)
+    st.update()
+    nav.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(updates):
+        st.update()
+    nav.synchronize()
+    upd_ms = 1e3 * (time.perf_counter() - t0) / updates
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "snap.npz")
+        t0 = time.perf_counter()
+        checkpoint.write_navier(nav, f)
+        w_ms = 1e3 * (time.perf_counter() - t0)
+        size = os.path.getsize(f)
+        t0 = time.perf_counter()
+        checkpoint.read_navier(nav, f)
+        nav.synchronize()
+        r_ms = 1e3 * (time.perf_counter() - t0)
+    return {"stats_update_ms": upd_ms, "stats_updates_timed": updates, "snapshot_write_ms": w_ms,
+            "snapshot_read_ms": r_ms, "snapshot_bytes": size,
+            "note": "statistics update: wall time per call with the queue drained at the end (FP64 on the "
+                    "device); snapshot: device -> host -> .npz container in a temporary directory and back"}
 
 
 def bench_hholtz_batch(args):
@@ -446,8 +486,10 @@ def main():
     if traffic is not None:
         roof["traffic"] = traffic["bytes_per_launch"]
         roof["traffic_source"] = traffic["source"]
+    out["statistics_and_snapshot"] = stats_and_snapshot(nav)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_budget)
+        out["cpu_baseline"], out["statistics_and_snapshot"]["stats_update_cpu_ms"] = cpu_baseline(
+            args.config, args.cpu_budget)
     if args.copy_probe > 0:
         copy_probe(L, args.copy_probe)
     print(json.dumps(out), flush=True)

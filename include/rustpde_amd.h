@@ -91,6 +91,36 @@ int rbc_set_time(rbc_navier* h, double time);
 #define RBC_RE 2
 #define RBC_DIV 3
 int rbc_eval(rbc_navier* h, int what, double* out);
+/* Statistics (src/navier/statistics.rs:10-153; Navier2D::statistics, navier.rs:193-194),
+ * FP64 on the device.  Four arrays of the `field` space (confined: [nx, ny] real; periodic:
+ * [nx/2 + 1, ny] complex; physical [nx, ny]):
+ *   RBC_STAT_TEMP     t_avg    running average of temp.to_ortho() + fieldbc.to_ortho()
+ *   RBC_STAT_UX       ux_avg   ux.to_ortho() of the latest update (assigned, not averaged,
+ *   RBC_STAT_UY       uy_avg   uy.to_ortho()  as statistics.rs:145-146 do)
+ *   RBC_STAT_NUSSELT  nusselt  the Nusselt field (dT/dy + uy T / ka) 2 of the latest update
+ *                              (statistics.rs:220-247)
+ * rbc_stats_enable is Statistics::new(navier, save_stat, write_stat) (statistics.rs:54-88:
+ * zero arrays, avg_time 0, tot_time = the handle's time, num_save 0; calling it again
+ * restarts the statistics).  rbc_stats_update is the statistics.update call of callback
+ * (navier.rs:796-807, statistics.rs:130-153): it changes nothing and sets *updated = 0 when the
+ * handle's time is below tot_time (statistics.rs:134-140).  The save/write cadence
+ * (navier.rs:798-799, 809-810) is the caller's, as in Integrate::callback.
+ * rbc_stats_get reads .vhat (RBC_SPEC) or .v (RBC_PHYS, the field space's backward
+ * transform that Statistics::write applies, statistics.rs:176-179); rbc_stats_set and
+ * rbc_stats_set_info restore them (Statistics::read, statistics.rs:197-212).  Not available
+ * for ensemble instances. */
+#define RBC_STAT_TEMP 0
+#define RBC_STAT_UX 1
+#define RBC_STAT_UY 2
+#define RBC_STAT_NUSSELT 3
+int rbc_stats_enable(rbc_navier* h, double save_stat, double write_stat);
+int rbc_stats_update(rbc_navier* h, int* updated);
+int rbc_stats_get(rbc_navier* h, int which, int space, double* host_out);
+int rbc_stats_set(rbc_navier* h, int which, const double* host_vhat);
+int rbc_stats_info(rbc_navier* h, double* save_stat, double* write_stat, double* avg_time, double* tot_time,
+                   long long* num_save);
+int rbc_stats_set_info(rbc_navier* h, double avg_time, double tot_time, long long num_save);
+
 /* Integrate::exit  navier.rs:855-862: *out = 1 if |div|_2 is NaN. */
 int rbc_exit(rbc_navier* h, int* out);
 

@@ -63,6 +63,11 @@ def lib():
         L.orc_fdma_solve.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.orc_dct1.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_cheb_diff.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_stats_enable.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_stats_update.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.orc_stats_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_stats_info.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_navier_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
         _lib = L
@@ -250,6 +255,30 @@ class Navier2D:
 
     def get_time(self):
         return lib().orc_navier_time(self._h)
+
+    def set_time(self, t):
+        _check(lib().orc_navier_set_time(self._h, float(t)))
+
+    # Statistics (src/navier/statistics.rs): 0 t_avg, 1 ux_avg, 2 uy_avg, 3 nusselt
+    def stats_enable(self, save_stat, write_stat):
+        _check(lib().orc_stats_enable(self._h, save_stat, write_stat))
+
+    def stats_update(self):
+        u = ctypes.c_int()
+        _check(lib().orc_stats_update(self._h, ctypes.byref(u)))
+        return bool(u.value)
+
+    def stats_get(self, which, spectral=True):
+        shp = self._shape("field", spectral)
+        out = np.zeros(shp, dtype=complex if (spectral and self.periodic) else float)
+        _check(lib().orc_stats_get(self._h, which, int(spectral), _p(out.view(float))))
+        return out
+
+    def stats_info(self):
+        """(avg_time, tot_time, num_save)"""
+        o = np.zeros(3)
+        _check(lib().orc_stats_info(self._h, _p(o)))
+        return o[0], o[1], int(o[2])
 
     def _eval(self, k):
         v = ctypes.c_double()

@@ -1198,6 +1198,56 @@ template <class T> struct Navier2D {  // src/navier/navier.rs:153-195
     }
     return field->average();
   }
+  // Statistics  src/navier/statistics.rs:10-153, :220-247 (arrays of the field space)
+  bool st_on = false;
+  double st_save = 0.0, st_write = 0.0, st_avg_time = 0.0, st_tot_time = 0.0;
+  long long st_num = 0;
+  Arr<T> st_arr[4];  // t_avg, ux_avg, uy_avg, nusselt (.vhat)
+  void stats_new(double save_stat, double write_stat) {  // statistics.rs:54-88
+    for (auto& a : st_arr) a = Arr<T>(field->vhat.n0, field->vhat.n1);
+    st_save = save_stat;
+    st_write = write_stat;
+    st_avg_time = 0.0;
+    st_tot_time = time;
+    st_num = 0;
+    st_on = true;
+  }
+  // callback's update (navier.rs:800-806) -> Statistics::update (statistics.rs:130-153)
+  bool stats_update() {
+    if (time < st_tot_time) return false;
+    Arr<T> that = operator_add(temp->to_ortho(), fieldbc->to_ortho());
+    Arr<T> uxhat = ux->to_ortho(), uyhat = uy->to_ortho();
+    const double w = (double)st_num;
+    for (size_t i = 0; i < that.d.size(); ++i) st_arr[0].d[i] = divr(mulr(st_arr[0].d[i], w) + that.d[i], w + 1.0);
+    st_arr[1] = uxhat;
+    st_arr[2] = uyhat;
+    // nusselt (statistics.rs:220-247)
+    field->vhat = uyhat;
+    field->backward();
+    Arr<double> uy_v = field->v;
+    field->vhat = that;
+    field->backward();
+    Arr<double> uy_temp(uy_v.n0, uy_v.n1);
+    for (size_t i = 0; i < uy_temp.d.size(); ++i) uy_temp.d[i] = field->v.d[i] * uy_v.d[i];
+    Arr<T> g = field->gradient(0, 1, nullptr);
+    for (auto& v : g.d) v = divr(v, scale[1] * -1.0);
+    field->vhat = g;
+    field->backward();
+    for (size_t i = 0; i < uy_temp.d.size(); ++i)
+      field->v.d[i] = (field->v.d[i] + uy_temp.d[i] / ka) * 2.0 * scale[1];
+    field->forward();
+    st_arr[3] = field->vhat;
+    st_num += 1;
+    st_avg_time += time - st_tot_time;
+    st_tot_time = time;
+    return true;
+  }
+  // statistics array `which` in the field space (spec) or after its backward (phys)
+  Arr<double> stats_phys(int which) {
+    field->vhat = st_arr[which];
+    field->backward();
+    return field->v;
+  }
   double div_norm() {
     Arr<T> d = divergence();
     double s = 0.0;
@@ -1389,6 +1439,41 @@ int orc_navier_eval(void* p, int what, double* out) {
     }
     *out = v;
   })
+}
+// statistics: enable (Statistics::new), update (0/1 in *updated), arrays, scalars
+int orc_stats_enable(void* p, double save_stat, double write_stat) {
+  Handle* h = (Handle*)p;
+  GUARD({ if (h->periodic) h->c->stats_new(save_stat, write_stat); else h->r->stats_new(save_stat, write_stat); })
+}
+int orc_stats_update(void* p, int* updated) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    if (h->periodic ? !h->c->st_on : !h->r->st_on) throw std::invalid_argument("statistics are not enabled");
+    *updated = (h->periodic ? h->c->stats_update() : h->r->stats_update()) ? 1 : 0;
+  })
+}
+int orc_stats_get(void* p, int which, int space, double* out) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    if (which < 0 || which > 3) throw std::invalid_argument("unknown statistics field");
+    if (h->periodic) {
+      if (space) copy_out(h->c->st_arr[which], out); else copy_out(h->c->stats_phys(which), out);
+    } else {
+      if (space) copy_out(h->r->st_arr[which], out); else copy_out(h->r->stats_phys(which), out);
+    }
+  })
+}
+// out: avg_time, tot_time, num_save
+int orc_stats_info(void* p, double* out) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    if (h->periodic) { out[0] = h->c->st_avg_time; out[1] = h->c->st_tot_time; out[2] = (double)h->c->st_num; }
+    else { out[0] = h->r->st_avg_time; out[1] = h->r->st_tot_time; out[2] = (double)h->r->st_num; }
+  })
+}
+int orc_navier_set_time(void* p, double t) {
+  Handle* h = (Handle*)p;
+  GUARD({ if (h->periodic) h->c->time = t; else h->r->time = t; })
 }
 // tensor-solver setup of solver s (0 ux, 1 uy, 2 temp, 3 pres): sizes and arrays
 int orc_navier_tensor_info(void* p, int s, int* nlam, int* has_fb) {

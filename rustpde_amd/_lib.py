@@ -37,6 +37,12 @@ SIGNATURES = {
     "rbc_set_time": (_i, [_vp, _d]),
     "rbc_eval": (_i, [_vp, _i, _dp]),
     "rbc_exit": (_i, [_vp, _ip]),
+    "rbc_stats_enable": (_i, [_vp, _d, _d]),
+    "rbc_stats_update": (_i, [_vp, _ip]),
+    "rbc_stats_get": (_i, [_vp, _i, _i, _dp]),
+    "rbc_stats_set": (_i, [_vp, _i, _dp]),
+    "rbc_stats_info": (_i, [_vp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "rbc_stats_set_info": (_i, [_vp, _d, _d, ctypes.c_longlong]),
     "rbc_tensor_info": (_i, [_vp, _i, _ip, _ip]),
     "rbc_get_tensor": (_i, [_vp, _i, _dp, _dp, _dp]),
     "rbc_set_tensor": (_i, [_vp, _i, _dp, _dp, _dp]),

@@ -1702,6 +1702,96 @@ class Navier {
     for (int j = 0; j < ny; ++j) tot += (s[j] / len0) * wy_dx[j] / len1;
     return tot;
   }
+  // -------------------------------------------------------------- statistics
+  // Statistics (src/navier/statistics.rs:10-153): the running average of the total
+  // temperature, the latest ortho velocities and the latest Nusselt field, each an array of
+  // the `field` space (ortho x basis of the field, Chebyshev y), FP64 on the device
+  bool st_on = false;
+  double st_save = 0.0, st_write = 0.0, st_avg_time = 0.0, st_tot_time = 0.0;
+  long long st_num = 0;
+  DBuf STA[4];  // t_avg, ux_avg, uy_avg, nusselt (.vhat)
+  long stats_len() const { return (long)rows_ortho(bx_f) * ny; }
+  void stats_enable(double save_stat, double write_stat) {  // Statistics::new, statistics.rs:54-88
+    const long no = stats_len();
+    for (DBuf& b : STA) {
+      if (b.n < (size_t)no) b.alloc(no);
+      HIPCHECK(hipMemsetAsync(b.p, 0, sizeof(double) * no, st));
+    }
+    st_save = save_stat;
+    st_write = write_stat;
+    st_avg_time = 0.0;
+    st_tot_time = time;
+    st_num = 0;
+    st_on = true;
+  }
+  void stats_need() const {
+    if (!st_on) throw std::invalid_argument("statistics are not enabled (rbc_stats_enable)");
+  }
+  // Statistics::update(that, ux.to_ortho(), uy.to_ortho(), time) as callback calls it
+  // (navier.rs:800-806, statistics.rs:130-153); false when time < tot_time (no update)
+  bool stats_update() {
+    stats_need();
+    if (time < st_tot_time) return false;
+    const long no = stats_len(), np = (long)nx * ny;
+    temp_total_ortho(A.p);  // that = temp.to_ortho() + fieldbc.to_ortho()
+    ew_running_avg(st, STA[0].p, A.p, (double)st_num, no);
+    to_ortho(bx_u, by_u, UX.p, STA[1].p);
+    to_ortho(bx_u, by_u, UY.p, STA[2].p);
+    // nusselt (statistics.rs:220-247): uy and that to physical space through the field
+    // space, dT/dy / (-scale_y) (scale_y = 1: the product with its reciprocal is exact),
+    // (dT/dy + uy T / ka) 2 scale_y, then the field space's forward transform
+    const double* u = field_backward(STA[2].p);
+    HIPCHECK(hipMemcpyAsync(CV.p, u, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    const double* t = field_backward(A.p);
+    HIPCHECK(hipMemcpyAsync(CONV.p, t, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    yop(by_p, "DS", A.p, rows_ortho(bx_f), G.p);
+    ew_scale(st, G.p, G.p, 1.0 / (scale[1] * -1.0), no);
+    const double* dv = field_backward(G.p);
+    ew_nuvol(st, RHS.p, dv, CONV.p, CV.p, ka, scale[1], np);
+    const double* a = yop(by_p, "FS", RHS.p, nx, W1.p);
+    const double* r = xop(bx_f, "FS", a, ny, W2.p, W9.p);
+    HIPCHECK(hipMemcpyAsync(STA[3].p, r, sizeof(double) * no, hipMemcpyDeviceToDevice, st));
+    ++st_num;
+    st_avg_time += time - st_tot_time;
+    st_tot_time = time;
+    return true;
+  }
+  // .vhat (RBC_SPEC) or, after the field space's backward transform, .v (RBC_PHYS) of
+  // statistics array `which` (statistics.rs:176-179)
+  void stats_get(int which, int space, double* host) {
+    stats_need();
+    if (which < 0 || which > 3) throw std::invalid_argument("unknown statistics field");
+    const long no = stats_len(), np = (long)nx * ny;
+    if (space == RBC_PHYS) {
+      const double* v = field_backward(STA[which].p);
+      HIPCHECK(hipStreamSynchronize(st));
+      HIPCHECK(hipMemcpy(host, v, sizeof(double) * np, hipMemcpyDeviceToHost));
+      return;
+    }
+    if (space != RBC_SPEC) throw std::invalid_argument("unknown space");
+    HIPCHECK(hipStreamSynchronize(st));
+    std::vector<double> v = STA[which].download(no);
+    if (!periodic) {
+      std::memcpy(host, v.data(), v.size() * sizeof(double));
+      return;
+    }
+    const size_t pl = (size_t)no / 2;
+    for (size_t i = 0; i < pl; ++i) { host[2 * i] = v[i]; host[2 * i + 1] = v[pl + i]; }
+  }
+  void stats_set(int which, const double* host) {  // Statistics::read, statistics.rs:197-212
+    stats_need();
+    if (which < 0 || which > 3) throw std::invalid_argument("unknown statistics field");
+    const long no = stats_len();
+    std::vector<double> v(no);
+    if (!periodic) {
+      std::memcpy(v.data(), host, v.size() * sizeof(double));
+    } else {
+      const size_t pl = (size_t)no / 2;
+      for (size_t i = 0; i < pl; ++i) { v[i] = host[2 * i]; v[pl + i] = host[2 * i + 1]; }
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipMemcpy(STA[which].p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   double div_norm() {
     divergence(G.p);
     const long no = (long)rows_ortho(bx_u) * ny;
@@ -2217,6 +2307,58 @@ int rbc_eval(rbc_navier* h, int what, double* out) {
       case RBC_DIV: *out = n.div_norm(); break;
       default: throw std::invalid_argument("unknown diagnostic");
     }
+  });
+}
+int rbc_stats_enable(rbc_navier* h, double save_stat, double write_stat) {
+  return guarded([&] {
+    NEED(h);
+    if (h->member) throw std::invalid_argument("statistics of ensemble instances are not supported");
+    h->n->stats_enable(save_stat, write_stat);
+  });
+}
+int rbc_stats_update(rbc_navier* h, int* updated) {
+  return guarded([&] {
+    NEED(h);
+    const bool u = h->n->stats_update();
+    if (updated) *updated = u ? 1 : 0;
+  });
+}
+int rbc_stats_get(rbc_navier* h, int which, int space, double* out) {
+  return guarded([&] {
+    NEED(h);
+    if (!out) throw std::invalid_argument("null out");
+    h->n->stats_get(which, space, out);
+  });
+}
+int rbc_stats_set(rbc_navier* h, int which, const double* vhat) {
+  return guarded([&] {
+    NEED(h);
+    if (!vhat) throw std::invalid_argument("null vhat");
+    h->n->stats_set(which, vhat);
+  });
+}
+int rbc_stats_info(rbc_navier* h, double* save_stat, double* write_stat, double* avg_time, double* tot_time,
+                   long long* num_save) {
+  return guarded([&] {
+    NEED(h);
+    const Navier& n = *h->n;
+    n.stats_need();
+    if (save_stat) *save_stat = n.st_save;
+    if (write_stat) *write_stat = n.st_write;
+    if (avg_time) *avg_time = n.st_avg_time;
+    if (tot_time) *tot_time = n.st_tot_time;
+    if (num_save) *num_save = n.st_num;
+  });
+}
+int rbc_stats_set_info(rbc_navier* h, double avg_time, double tot_time, long long num_save) {
+  return guarded([&] {
+    NEED(h);
+    Navier& n = *h->n;
+    n.stats_need();
+    if (num_save < 0) throw std::invalid_argument("num_save < 0");
+    n.st_avg_time = avg_time;
+    n.st_tot_time = tot_time;
+    n.st_num = num_save;
   });
 }
 int rbc_exit(rbc_navier* h, int* out) {

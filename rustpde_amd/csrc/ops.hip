@@ -289,6 +289,8 @@ EW_KERNEL(ew_update_pres, (double* p, const double* d, const double* phi, double
             p[i] = q + phi[i] * inv_dt;
           })
 EW_KERNEL(ew_scale, (double* out, const double* a, double f, long n), out[i] = a[i] * f)
+// running average of the statistics (statistics.rs:136-138): (avg w + x) / (w + 1)
+EW_KERNEL(ew_running_avg, (double* avg, const double* x, double w, long n), avg[i] = (avg[i] * w + x[i]) / (w + 1.0))
 EW_KERNEL(ew_speed, (double* out, const double* a, const double* b, double f, long n),
           out[i] = sqrt(a[i] * a[i] + b[i] * b[i]) * f)
 EW_KERNEL(ew_nuvol, (double* out, const double* d, const double* t, const double* uy, double ka, double sy, long n),
@@ -306,6 +308,9 @@ EW_KERNEL(ew_nuvol, (double* out, const double* d, const double* t, const double
   } while (0)
 
 void ew_add(hipStream_t s, double* out, const double* a, const double* b, long n) { EW_LAUNCH(ew_add, n, out, a, b); }
+void ew_running_avg(hipStream_t s, double* avg, const double* x, double w, long n) {
+  EW_LAUNCH(ew_running_avg, n, avg, x, w);
+}
 void ew_rhs_vel(hipStream_t s, double* out, const double* a, const double* g, const double* buoy, const double* c,
                 double dt, long n) {
   EW_LAUNCH(ew_rhs_vel, n, out, a, g, buoy, c, dt);

@@ -6,6 +6,8 @@ callback, exit) and the diagnostics; fields are reached through `temp`, `ux`, `u
 in the reference's layouts (numpy, C order; complex128 spectral arrays when periodic).
 """
 import ctypes
+import math
+import os
 
 import numpy as np
 
@@ -83,6 +85,8 @@ class Navier2D:
         self.fieldbc = FieldProxy(self, FIELDBC)
         self.diagnostics = {"time": [], "Nu": [], "Nuvol": [], "Re": []}
         self.write_intervall = None
+        self.statistics = None  # Statistics (navier.rs:193-194); see Navier2D.init_statistics
+        self.data_dir = None  # where callback writes snapshots and statistics (None: no files)
 
     @classmethod
     def new(cls, nx, ny, ra, pr, dt, aspect, adiabatic):
@@ -152,10 +156,30 @@ class Navier2D:
     def div_norm(self):
         return self._eval(DIV)
 
+    @staticmethod
+    def _on_cadence(t, every, dt):
+        """(time % every) < dt / 2 or > every - dt / 2 (navier.rs:784-786, :798-799)."""
+        r = math.fmod(t, every)
+        return r < dt / 2.0 or r > every - dt / 2.0
+
     def callback(self):
-        """navier.rs:775-853 without the HDF5 snapshot: print and record diagnostics."""
+        """navier.rs:775-853: the snapshot (at write_intervall, every call when it is None)
+        and the statistics update and file (at their save/write cadence), then the printed
+        and recorded diagnostics.  Files go to `data_dir` (the reference's data/) and are
+        written only when it is set; snapshots are the container of rustpde_amd.checkpoint."""
+        from . import checkpoint
+        t, dt = self.get_time(), self.get_dt()
+        if self.data_dir is not None:
+            os.makedirs(self.data_dir, exist_ok=True)
+            if self.write_intervall is None or self._on_cadence(t, self.write_intervall, dt):
+                self.write(os.path.join(self.data_dir, checkpoint.flow_filename(t)))
+        if self.statistics is not None:
+            st = self.statistics
+            if self._on_cadence(t, st.save_stat, dt):
+                st.update()
+            if self.data_dir is not None and self._on_cadence(t, st.write_stat, dt):
+                st.write(os.path.join(self.data_dir, "statistics" + checkpoint._SUFFIX))
         div, nu, nuvol, re = self.div_norm(), self.eval_nu(), self.eval_nuvol(), self.eval_re()
-        t = self.get_time()
         print("time = %4.2f      |div| = %4.2e     Nu = %5.3e     Nuv = %5.3e    Re = %5.3e" % (t, div, nu, nuvol, re))
         for k, v in (("time", t), ("Nu", nu), ("Nuvol", nuvol), ("Re", re)):
             self.diagnostics[k].append(v)
@@ -165,6 +189,26 @@ class Navier2D:
         r = ctypes.c_int()
         check(_lib.load().rbc_exit(self._h, ctypes.byref(r)))
         return bool(r.value)
+
+    # ---- checkpoint / restart (navier.rs:961-1008) and statistics (navier.rs:193-194)
+    def write(self, filename):
+        """Navier2D::write: fields (v with the boundary field for temp, vhat), coordinates
+        and scalars, in the container of rustpde_amd.checkpoint."""
+        from . import checkpoint
+        checkpoint.write_navier(self, filename)
+        print(" ==> %r" % filename)
+
+    def read(self, filename):
+        """Navier2D::read: temp, ux, uy, pres[0] (broadcast across resolutions) and time."""
+        from . import checkpoint
+        checkpoint.read_navier(self, filename)
+        print(" <== %r" % filename)
+
+    def init_statistics(self, save_stat, write_stat):
+        """navier.statistics = Some(Statistics::new(&navier, save_stat, write_stat))."""
+        from .statistics import Statistics
+        self.statistics = Statistics.new(self, save_stat, write_stat)
+        return self.statistics
 
     # ---- initial conditions (navier.rs:927-948)
     def set_velocity(self, amp, m, n):
