@@ -162,6 +162,28 @@ def cpu_baseline(cfg, budget_s):
                       % steps}, stats_ms
 
 
+def solid_step(cfg, lib_mod, steps=20):
+    """The step with a solid obstacle (navier.rs:551-608; roughness elements of
+    solid_masks.rs:61-101 on both plates) on the same workload: the field-by-field FP64
+    step with the penalisation in its convective terms."""
+    from rustpde_amd.checkpoint import coords
+    from rustpde_amd.solid_masks import solid_roughness_sinusoid
+    nav = make(cfg, lib_mod)
+    x = coords(nav.nx, nav.periodic, nav.scale[0])
+    nav.solid = solid_roughness_sinusoid(x, coords(nav.ny, False, 1.0), 0.1, 3.0)
+    nav.update(3)
+    nav.synchronize()
+    t0 = time.perf_counter()
+    nav.update(steps)
+    nav.synchronize()
+    el = time.perf_counter() - t0
+    ok = not nav.exit()
+    nav.close()
+    return {"ms_per_step": 1e3 * el / steps, "steps": steps, "finite": ok,
+            "mask": "solid_roughness_sinusoid(height 0.1, wavenumber 3)",
+            "note": "field-by-field FP64 step (the batched, fused and mixed paths do not take the terms)"}
+
+
 def stats_and_snapshot(nav, updates=20):
     """After the timed steps: the device statistics update (statistics.rs:130-153, as
     callback calls it) per call, and a snapshot write and read (navier.rs:961-1008) through
@@ -487,6 +509,7 @@ def main():
         roof["traffic"] = traffic["bytes_per_launch"]
         roof["traffic_source"] = traffic["source"]
     out["statistics_and_snapshot"] = stats_and_snapshot(nav)
+    out["solid_step"] = solid_step(args.config, rustpde_amd)
     if not args.no_cpu_baseline:
         out["cpu_baseline"], out["statistics_and_snapshot"]["stats_update_cpu_ms"] = cpu_baseline(
             args.config, args.cpu_budget)

@@ -68,6 +68,7 @@ def lib():
         L.orc_stats_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_stats_info.argtypes = [ctypes.c_void_p, _dp]
         L.orc_navier_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_navier_set_solid.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
         _lib = L
@@ -258,6 +259,15 @@ class Navier2D:
 
     def set_time(self, t):
         _check(lib().orc_navier_set_time(self._h, float(t)))
+
+    def set_solid(self, solid):
+        """navier.solid = Some([mask, value]) or None (physical [nx, ny] arrays)."""
+        if solid is None:
+            _check(lib().orc_navier_set_solid(self._h, None, None))
+            return
+        m = np.ascontiguousarray(solid[0], dtype=float)
+        v = np.ascontiguousarray(solid[1], dtype=float)
+        _check(lib().orc_navier_set_solid(self._h, _p(m), _p(v)))
 
     # Statistics (src/navier/statistics.rs): 0 t_avg, 1 ux_avg, 2 uy_avg, 3 nusselt
     def stats_enable(self, save_stat, write_stat):

@@ -1090,23 +1090,39 @@ template <class T> struct Navier2D {  // src/navier/navier.rs:153-195
     }
     return field->vhat;
   }
+  // solid obstacle: navier.solid = Some([mask, value]) (navier.rs:189-190), physical [nx, ny]
+  bool solid_on = false;
+  Arr<double> solid_m, solid_v;
   Arr<T> conv_temp(const Arr<double>& u, const Arr<double>& v) {
     Arr<double> c = conv_term(*temp, u, 1, 0);
     addto(c, conv_term(*temp, v, 0, 1));
     addto(c, conv_term(*fieldbc, u, 1, 0));
     addto(c, conv_term(*fieldbc, v, 0, 1));
+    if (solid_on) {  // navier.rs:551-560 (eta = 1e-2)
+      const double eta = 1e-2;
+      temp->backward();
+      for (size_t i = 0; i < c.d.size(); ++i) {
+        const double damp = (-1.0 / eta * solid_m.d[i]) * ((temp->v.d[i] + fieldbc->v.d[i]) - solid_v.d[i]);
+        c.d[i] = c.d[i] - damp;
+      }
+    }
     return finish_conv(c);
   }
-  Arr<T> conv_vel(const Field2<T>& f, const Arr<double>& u, const Arr<double>& v) {
+  // conv_ux / conv_uy (navier.rs:564-616); w is the velocity component of the field f
+  Arr<T> conv_vel(const Field2<T>& f, const Arr<double>& u, const Arr<double>& v, const Arr<double>& w) {
     Arr<double> c = conv_term(f, u, 1, 0);
     addto(c, conv_term(f, v, 0, 1));
+    if (solid_on) {  // navier.rs:579-584, :603-608
+      const double eta = 1e-2;
+      for (size_t i = 0; i < c.d.size(); ++i) c.d[i] = c.d[i] - (-1.0 / eta * solid_m.d[i]) * w.d[i];
+    }
     return finish_conv(c);
   }
   void solve_ux(const Arr<double>& u, const Arr<double>& v) {  // navier.rs:622-633
     Arr<T> rhs = ux->to_ortho();
     Arr<T> gp = pres0->gradient(1, 0, scale);
     for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] - mulr(gp.d[i], dt);
-    Arr<T> cv = conv_vel(*ux, u, v);
+    Arr<T> cv = conv_vel(*ux, u, v, u);
     for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] - mulr(cv.d[i], dt);
     ux->vhat = solver[0]->solve<T>(rhs);
   }
@@ -1115,7 +1131,7 @@ template <class T> struct Navier2D {  // src/navier/navier.rs:153-195
     Arr<T> gp = pres0->gradient(0, 1, scale);
     for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] - mulr(gp.d[i], dt);
     for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(buoy.d[i], dt);
-    Arr<T> cv = conv_vel(*uy, u, v);
+    Arr<T> cv = conv_vel(*uy, u, v, v);
     for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] - mulr(cv.d[i], dt);
     uy->vhat = solver[1]->solve<T>(rhs);
   }
@@ -1438,6 +1454,22 @@ int orc_navier_eval(void* p, int what, double* out) {
       v = what == 0 ? n.eval_nu() : what == 1 ? n.eval_nuvol() : what == 2 ? n.eval_re() : n.div_norm();
     }
     *out = v;
+  })
+}
+// navier.solid = Some([mask, value]) (physical [nx, ny] each); mask == nullptr: None
+int orc_navier_set_solid(void* p, const double* mask, const double* value) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    auto apply = [&](auto& n) {
+      n.solid_on = mask != nullptr;
+      if (!mask) return;
+      const int n0 = n.temp->v.n0, n1 = n.temp->v.n1;
+      n.solid_m = Arr<double>(n0, n1);
+      n.solid_v = Arr<double>(n0, n1);
+      std::memcpy(n.solid_m.d.data(), mask, sizeof(double) * n0 * n1);
+      std::memcpy(n.solid_v.d.data(), value, sizeof(double) * n0 * n1);
+    };
+    if (h->periodic) apply(*h->c); else apply(*h->r);
   })
 }
 // statistics: enable (Statistics::new), update (0/1 in *updated), arrays, scalars

@@ -37,6 +37,7 @@ SIGNATURES = {
     "rbc_set_time": (_i, [_vp, _d]),
     "rbc_eval": (_i, [_vp, _i, _dp]),
     "rbc_exit": (_i, [_vp, _ip]),
+    "rbc_set_solid": (_i, [_vp, _dp, _dp]),
     "rbc_stats_enable": (_i, [_vp, _d, _d]),
     "rbc_stats_update": (_i, [_vp, _ip]),
     "rbc_stats_get": (_i, [_vp, _i, _i, _dp]),

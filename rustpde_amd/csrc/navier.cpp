@@ -1202,6 +1202,8 @@ class Navier {
     if (which == 0) xop(bx_p, "DS", P0.p, ny, G.p, W9.p);   // d/dx pres
     else yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p);      // d/dy pres
     conv_pair(bx_u, U);
+    if (solid_on) ew_solid_damp(st, CONV.p, SOLID_M.p, which == 0 ? UXP.p : UYP.p, nullptr, nullptr, -1.0 / 1e-2,
+                                (long)nx * ny);  // navier.rs:579-584, :603-608 (eta = 1e-2)
     forward_dealias_conv();
     ew_rhs_vel(st, RHS.p, A.p, G.p, buoy, CV.p, dt, nbn(no));
     solve(which, RHS.p, U);
@@ -1210,7 +1212,7 @@ class Navier {
   // and implicit solves batched (the two use the same operators; each reads only the
   // physical ux, uy and pres of the step start, so the result equals the two calls in turn)
   bool pair_ok() const {
-    return use_batch && !periodic && vel_same && solver[0]->kind == ADI && solver[1]->kind == ADI;
+    return !solid_on && use_batch && !periodic && vel_same && solver[0]->kind == ADI && solver[1]->kind == ADI;
   }
   // confined mixed step: the convective terms (their dense transforms and products) in FP32,
   // everything else FP64; it runs with the field pairs
@@ -1321,6 +1323,10 @@ class Navier {
     } else {
       conv_pair(bx_t, T.p);
       ew_acc_prod2(st, CONV.p, UXP.p, DTBCX.p, UYP.p, DTBCY.p, nbn(np));
+      if (solid_on) {  // navier.rs:551-560: temp.backward(), then the damp of T + T_bc - value
+        backward(RBC_TEMP);
+        ew_solid_damp(st, CONV.p, SOLID_M.p, Tv.p, BCv.p, SOLID_V.p, -1.0 / 1e-2, np);
+      }
       forward_dealias_conv();
     }
     ew_rhs_temp(st, RHS.p, A.p, GXX.p, GYY.p, CV.p, dt, nbn(no));
@@ -1332,7 +1338,7 @@ class Navier {
   }
   // ----------------------------------------------------------- batched periodic step
   bool batch_ok() const {
-    return nb == 1 && use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
+    return !solid_on && nb == 1 && use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
            use_bands && fdma_rows_lds_supported(by_u.m, by_u.n) && tdma_rows_lds_supported(by_u.m) &&
            by_phi.n == by_u.n;
   }
@@ -1701,6 +1707,25 @@ class Navier {
     double tot = 0.0;
     for (int j = 0; j < ny; ++j) tot += (s[j] / len0) * wy_dx[j] / len1;
     return tot;
+  }
+  // ------------------------------------------------------------ solid obstacle
+  // navier.solid = Some([mask, value]) (navier.rs:189-190): the penalisation terms enter the
+  // convective terms of the field-by-field step (the batched and fused paths do not take them)
+  bool solid_on = false;
+  DBuf SOLID_M, SOLID_V;
+  void set_solid(const double* mask, const double* value) {
+    HIPCHECK(hipStreamSynchronize(st));
+    drop_graph();
+    gpres_valid = false;
+    if (!mask) {
+      solid_on = false;
+      return;
+    }
+    if (!value) throw std::invalid_argument("solid: value array required with the mask");
+    const size_t np = (size_t)nx * ny;
+    SOLID_M.upload(std::vector<double>(mask, mask + np));
+    SOLID_V.upload(std::vector<double>(value, value + np));
+    solid_on = true;
   }
   // -------------------------------------------------------------- statistics
   // Statistics (src/navier/statistics.rs:10-153): the running average of the total
@@ -2307,6 +2332,13 @@ int rbc_eval(rbc_navier* h, int what, double* out) {
       case RBC_DIV: *out = n.div_norm(); break;
       default: throw std::invalid_argument("unknown diagnostic");
     }
+  });
+}
+int rbc_set_solid(rbc_navier* h, const double* mask, const double* value) {
+  return guarded([&] {
+    NEED(h);
+    if (h->member) throw std::invalid_argument("solid masks of ensemble instances are not supported");
+    h->n->set_solid(mask, value);
   });
 }
 int rbc_stats_enable(rbc_navier* h, double save_stat, double write_stat) {

@@ -289,6 +289,15 @@ EW_KERNEL(ew_update_pres, (double* p, const double* d, const double* phi, double
             p[i] = q + phi[i] * inv_dt;
           })
 EW_KERNEL(ew_scale, (double* out, const double* a, double f, long n), out[i] = a[i] * f)
+// solid interaction of a convective term (navier.rs:551-560, :579-584, :603-608): the term
+// minus damp = (c mask) s, c = -1 / eta, s = (q + bc) - value for the temperature (val given)
+// and s = q (the velocity component) otherwise
+EW_KERNEL(ew_solid_damp,
+          (double* conv, const double* mask, const double* q, const double* bc, const double* val, double c, long n),
+          {
+            const double s = val ? (q[i] + bc[i]) - val[i] : q[i];
+            conv[i] = conv[i] - (c * mask[i]) * s;
+          })
 // running average of the statistics (statistics.rs:136-138): (avg w + x) / (w + 1)
 EW_KERNEL(ew_running_avg, (double* avg, const double* x, double w, long n), avg[i] = (avg[i] * w + x[i]) / (w + 1.0))
 EW_KERNEL(ew_speed, (double* out, const double* a, const double* b, double f, long n),
@@ -308,6 +317,10 @@ EW_KERNEL(ew_nuvol, (double* out, const double* d, const double* t, const double
   } while (0)
 
 void ew_add(hipStream_t s, double* out, const double* a, const double* b, long n) { EW_LAUNCH(ew_add, n, out, a, b); }
+void ew_solid_damp(hipStream_t s, double* conv, const double* mask, const double* q, const double* bc,
+                   const double* val, double c, long n) {
+  EW_LAUNCH(ew_solid_damp, n, conv, mask, q, bc, val, c);
+}
 void ew_running_avg(hipStream_t s, double* avg, const double* x, double w, long n) {
   EW_LAUNCH(ew_running_avg, n, avg, x, w);
 }

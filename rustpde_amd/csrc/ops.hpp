@@ -157,6 +157,8 @@ void col_weighted_sum(hipStream_t s, const double* v, int n0, int n1, const doub
 // hold >= 258 doubles (out[2..258) receives per-block partial sums)
 void dot_and_norm(hipStream_t s, const double* a, const double* w, int n, const double* b, long nb, double* out);
 // v = sqrt(a^2 + b^2) * f
+void ew_solid_damp(hipStream_t s, double* conv, const double* mask, const double* q, const double* bc,
+                   const double* val, double c, long n);
 void ew_running_avg(hipStream_t s, double* avg, const double* x, double w, long n);
 void ew_speed(hipStream_t s, double* out, const double* a, const double* b, double f, long n);
 // v = ((d + uyt / ka) * 2) * sy   with uyt = t * uy

@@ -204,6 +204,24 @@ class Navier2D:
         checkpoint.read_navier(self, filename)
         print(" <== %r" % filename)
 
+    @property
+    def solid(self):
+        """navier.solid (navier.rs:189-190): [mask, value] physical arrays, or None."""
+        return getattr(self, "_solid", None)
+
+    @solid.setter
+    def solid(self, masks):
+        if masks is None:
+            check(_lib.load().rbc_set_solid(self._h, None, None))
+            self._solid = None
+            return
+        m = np.ascontiguousarray(masks[0], dtype=float)
+        v = np.ascontiguousarray(masks[1], dtype=float)
+        if m.shape != (self.nx, self.ny) or v.shape != (self.nx, self.ny):
+            raise ValueError("solid masks must be [nx, ny] = %s" % ((self.nx, self.ny),))
+        check(_lib.load().rbc_set_solid(self._h, dptr(m), dptr(v)))
+        self._solid = [m, v]
+
     def init_statistics(self, save_stat, write_stat):
         """navier.statistics = Some(Statistics::new(&navier, save_stat, write_stat))."""
         from .statistics import Statistics
