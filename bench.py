@@ -405,6 +405,8 @@ def main():
                          "timed beside it and the field / Nu differences after 10 steps are reported")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the statistics / snapshot / solid-obstacle legs (profiler runs of the step alone)")
     ap.add_argument("--nu-time", type=float, default=0.0, metavar="T",
                     help="instead of timing: the mixed and the FP64 step over T time units, Nu averaged "
                          "(sampled once per time unit)")
@@ -508,11 +510,13 @@ def main():
     if traffic is not None:
         roof["traffic"] = traffic["bytes_per_launch"]
         roof["traffic_source"] = traffic["source"]
-    out["statistics_and_snapshot"] = stats_and_snapshot(nav)
-    out["solid_step"] = solid_step(args.config, rustpde_amd)
+    if not args.no_extras:
+        out["statistics_and_snapshot"] = stats_and_snapshot(nav)
+        out["solid_step"] = solid_step(args.config, rustpde_amd)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"], out["statistics_and_snapshot"]["stats_update_cpu_ms"] = cpu_baseline(
-            args.config, args.cpu_budget)
+        out["cpu_baseline"], stats_cpu_ms = cpu_baseline(args.config, args.cpu_budget)
+        if not args.no_extras:
+            out["statistics_and_snapshot"]["stats_update_cpu_ms"] = stats_cpu_ms
     if args.copy_probe > 0:
         copy_probe(L, args.copy_probe)
     print(json.dumps(out), flush=True)

@@ -32,10 +32,12 @@ using chains::WAVE;
 // from the other plane's value x: r = 0 gives (a 0 - x k) / sc (x = b), r = 1 gives
 // (x k + b 0) / sc (x = a); the products with 0 only decide the sign of a zero result.
 // isc = 1 / sc when sc is a power of two (else 0): (y sc + 0) / (sc sc) is then exactly y isc.
+// P2: sc is known to be a power of two (isc != 0), which drops the division path.
+template <bool P2>
 __device__ __forceinline__ double fdiff_plane(double x, int r, double kk, double sc, double isc) {
 #pragma clang fp contract(off)
   double v = r ? x * kk : 0.0 - x * kk;
-  if (sc != 1.0) v = isc != 0.0 ? v * isc : (v * sc) / (sc * sc + 0.0 * 0.0);
+  if (sc != 1.0) v = (P2 || isc != 0.0) ? v * isc : (v * sc) / (sc * sc + 0.0 * 0.0);
   return v;
 }
 
@@ -54,9 +56,13 @@ __device__ __forceinline__ double tap_l(int kind, const double* __restrict__ coe
   return -1.0 * (double)(i * i) / (double)((long long)j * j);
 }
 
-template <int SLICE>
+// RBC: the periodic Rayleigh-Benard bases (velocity Dirichlet, pseudo pressure Neumann)
+// with a power-of-two x scale, fixed at compile time so that only their tap and scale code
+// is emitted (the general kernel selects them at run time; the arithmetic is the same).
+template <int SLICE, bool RBC>
 __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #pragma clang fp contract(off)
+  const int su_kind = RBC ? 1 : a.su_kind, sp_kind = RBC ? 2 : a.sp_kind;
   extern __shared__ double lds[];
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
   const int r = w >> 1, p = w & 1;  // plane (0 real, 1 imaginary) and chain parity of this wavefront
@@ -69,10 +75,10 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #define ROWU(pl) ((size_t)((pl) ? mx + k : k) * mu)
 #define ROWN(pl) ((size_t)((pl) ? mx + k : k) * ny)
   // stencil taps of the velocity (u) and pressure (p) bases
-#define SU0(j) tap_d(a.su_kind, a.su, (j), mu)
-#define SU1(j) tap_l(a.su_kind, a.su, (j), ny, mu)
-#define SP0(j) tap_d(a.sp_kind, a.sp, (j), mu)
-#define SP1(j) tap_l(a.sp_kind, a.sp, (j), ny, mu)
+#define SU0(j) tap_d(su_kind, a.su, (j), mu)
+#define SU1(j) tap_l(su_kind, a.su, (j), ny, mu)
+#define SP0(j) tap_d(sp_kind, a.sp, (j), mu)
+#define SP1(j) tap_l(sp_kind, a.sp, (j), ny, mu)
 
   // rows of up to 2 SLICE WAVE + 1 values: unrolled staging bounds (all loads before stores)
   constexpr int QW = (2 * SLICE * WAVE + 8 + WAVE - 1) / WAVE;  // per wavefront
@@ -99,8 +105,8 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     const double* Xo = XA(1 - r);
     // x part of entry i: (ik ux / sc)_r at i and i - 2 with the stencil taps
     auto divx = [&](int i) {
-      const double c0 = fdiff_plane(Xo[pad(i + 2)], r, kk, a.kscale, a.isc);
-      const double c2 = fdiff_plane(Xo[pad(i)], r, kk, a.kscale, a.isc);
+      const double c0 = fdiff_plane<RBC>(Xo[pad(i + 2)], r, kk, a.kscale, a.isc);
+      const double c2 = fdiff_plane<RBC>(Xo[pad(i)], r, kk, a.kscale, a.isc);
       return c0 * SU0(i) + c2 * SU1(i);
     };
 #pragma unroll
@@ -241,9 +247,9 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
   // the from_ortho tables of the velocity basis: diag, low2 and off_{i-2} = diag_i low2_{i-2}
   // are 1, -1, -1 for the Dirichlet basis (0 for off at the chain start)
-#define TB_DG(q) (a.su_kind == 1 ? 1.0 : TB(0, q))
-#define TB_L2(q) (a.su_kind == 1 ? -1.0 : TB(1, q))
-#define TB_OF(q) (a.su_kind == 1 ? (t0 + (q) >= 1 ? -1.0 : 0.0) : TB(2, q))
+#define TB_DG(q) (su_kind == 1 ? 1.0 : TB(0, q))
+#define TB_L2(q) (su_kind == 1 ? -1.0 : TB(1, q))
+#define TB_OF(q) (su_kind == 1 ? (t0 + (q) >= 1 ? -1.0 : 0.0) : TB(2, q))
   {
     double vx[SLICE], vy[SLICE];
     {
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       double cs[SLICE + 2], ta[SLICE + 1], tb[SLICE + 1];
 #pragma unroll
       for (int j = 0; j < SLICE + 2; ++j)
-        cs[j] = (cnt > 0 && j <= cnt + 1) ? fdiff_plane(Xo[pad(i0 + 2 * j)], r, kk, a.kscale, a.isc) : 0.0;
+        cs[j] = (cnt > 0 && j <= cnt + 1) ? fdiff_plane<RBC>(Xo[pad(i0 + 2 * j)], r, kk, a.kscale, a.isc) : 0.0;
 #pragma unroll
       for (int j = 0; j < SLICE + 1; ++j) {
         ta[j] = (cnt > 0 && j <= cnt) ? SP0(i0 + 2 * j) : 0.0;
@@ -379,10 +385,14 @@ void pressure_rows(hipStream_t s, const PressureArgs& args) {
   a.isc = (a.kscale > 0.0 && std::frexp(a.kscale, &ex) == 0.5) ? 1.0 / a.kscale : 0.0;
   const int perd = ((a.ny - 1) / 2 + WAVE - 1) / WAVE, sl = a.per > perd ? a.per : perd;
   const size_t lds = (size_t)6 * a.bl * sizeof(double);
-  if (sl <= 4)
-    RBC_LAUNCH(pressure_rows_kernel<4>, dim3(a.mx), dim3(256), lds, s, a);
-  else
-    RBC_LAUNCH(pressure_rows_kernel<8>, dim3(a.mx), dim3(256), lds, s, a);
+  const bool rbc = a.su_kind == 1 && a.sp_kind == 2 && a.isc != 0.0;
+  if (sl <= 4) {
+    if (rbc) RBC_LAUNCH((pressure_rows_kernel<4, true>), dim3(a.mx), dim3(256), lds, s, a);
+    else RBC_LAUNCH((pressure_rows_kernel<4, false>), dim3(a.mx), dim3(256), lds, s, a);
+  } else {
+    if (rbc) RBC_LAUNCH((pressure_rows_kernel<8, true>), dim3(a.mx), dim3(256), lds, s, a);
+    else RBC_LAUNCH((pressure_rows_kernel<8, false>), dim3(a.mx), dim3(256), lds, s, a);
+  }
   check("pressure_rows");
 }
 
