@@ -12,7 +12,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "build", "librbc_oracle.so")
+_LD_PATH = os.path.join(_HERE, "build", "librbc_oracle_ld.so")
 _lib = None
+_ldlib = None
 
 # base kinds (funspace constructors): chebyshev, cheb_dirichlet, cheb_neumann,
 # cheb_dirichlet_bc, fourier_r2c
@@ -68,6 +70,9 @@ def lib():
         L.orc_stats_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_stats_info.argtypes = [ctypes.c_void_p, _dp]
         L.orc_navier_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_navier_conv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_navier_conv_term.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_navier_finish_conv.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_navier_set_solid.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
@@ -257,6 +262,31 @@ class Navier2D:
     def get_time(self):
         return lib().orc_navier_time(self._h)
 
+    def conv(self, which):
+        """Dealiased convective term of temp / ux / uy from the current state (navier.rs:538-616),
+        spectral array of the `field` space."""
+        shp = self._shape("field", True)
+        out = np.zeros(shp, dtype=complex if self.periodic else float)
+        _check(lib().orc_navier_conv(self._h, FIELDS[which], _p(out.view(float))))
+        return out
+
+    def conv_term(self, which, deriv, u):
+        """conv_term(field, u, deriv) (conv_term.rs:22-42): u * d^deriv field, physical [nx, ny]."""
+        shp = self._shape("field", False)
+        ua = np.ascontiguousarray(u, dtype=float)
+        assert ua.shape == shp, (ua.shape, shp)
+        out = np.zeros(shp)
+        _check(lib().orc_navier_conv_term(self._h, FIELDS[which], int(deriv[0]), int(deriv[1]), _p(ua), _p(out)))
+        return out
+
+    def finish_conv(self, phys):
+        """forward transform in the `field` space, then dealias (navier.rs:562-568, 1022-1032)."""
+        a = np.ascontiguousarray(phys, dtype=float)
+        assert a.shape == self._shape("field", False)
+        out = np.zeros(self._shape("field", True), dtype=complex if self.periodic else float)
+        _check(lib().orc_navier_finish_conv(self._h, _p(a), _p(out.view(float))))
+        return out
+
     def set_time(self, t):
         _check(lib().orc_navier_set_time(self._h, float(t)))
 
@@ -325,3 +355,78 @@ class Navier2D:
         f = None if fwd is None else _p(np.ascontiguousarray(fwd, dtype=float))
         b = None if bwd is None else _p(np.ascontiguousarray(bwd, dtype=float))
         _check(lib().orc_navier_tensor_set(self._h, s, _p(lam), f, b))
+
+
+def ldlib():
+    """The oracle evaluated in long double (oracle/rbc_oracle_ld.cpp)."""
+    global _ldlib
+    if _ldlib is None:
+        if not os.path.exists(_LD_PATH):
+            raise RuntimeError("long double oracle missing: run `make -C oracle`")
+        L = ctypes.CDLL(_LD_PATH)
+        L.orcld_last_error.restype = ctypes.c_char_p
+        L.orcld_navier_create.restype = ctypes.c_void_p
+        L.orcld_navier_create.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int] * 2
+        L.orcld_navier_destroy.argtypes = [ctypes.c_void_p]
+        L.orcld_navier_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orcld_navier_set.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.orcld_navier_forward.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orcld_navier_set_velocity.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3
+        L.orcld_navier_update.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        _ldlib = L
+    return _ldlib
+
+
+class NavierLD:
+    """The oracle's Navier2D evaluated in long double; fields read and written as float64 /
+    complex128 arrays (rounded on the way out).  Shapes as the FP64 oracle's."""
+
+    def __init__(self, nx, ny, ra, pr, dt, aspect, adiabatic=True, periodic=False, threads=None):
+        L = ldlib()
+        if threads:
+            L.orc_set_threads(int(threads))
+        self.periodic = periodic
+        self._shapes = Navier2D(nx, ny, ra, pr, dt, aspect, adiabatic, periodic)  # shapes only
+        self._h = L.orcld_navier_create(nx, ny, ra, pr, dt, aspect, int(adiabatic), int(periodic))
+        if not self._h:
+            raise RuntimeError("oracle (long double): " + L.orcld_last_error().decode())
+
+    @classmethod
+    def new(cls, nx, ny, ra, pr, dt, aspect, adiabatic, threads=None):
+        return cls(nx, ny, ra, pr, dt, aspect, adiabatic, False, threads)
+
+    @classmethod
+    def new_periodic(cls, nx, ny, ra, pr, dt, aspect, threads=None):
+        return cls(nx, ny, ra, pr, dt, aspect, True, True, threads)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _ldlib is not None:
+            _ldlib.orcld_navier_destroy(self._h)
+            self._h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError("oracle (long double): " + ldlib().orcld_last_error().decode())
+
+    def get(self, which, spectral=True):
+        shp = self._shapes._shape(which, spectral)
+        out = np.zeros(shp, dtype=complex if (spectral and self.periodic) else float)
+        self._chk(ldlib().orcld_navier_get(self._h, FIELDS[which], int(spectral), _p(out.view(float))))
+        return out
+
+    def set(self, which, arr, spectral=True):
+        shp = self._shapes._shape(which, spectral)
+        a = np.ascontiguousarray(arr, dtype=complex if (spectral and self.periodic) else float)
+        assert a.shape == shp, (a.shape, shp)
+        self._chk(ldlib().orcld_navier_set(self._h, FIELDS[which], int(spectral), _p(a.view(float))))
+
+    def set_phys_forward(self, which, v):
+        self.set(which, v, spectral=False)
+        self._chk(ldlib().orcld_navier_forward(self._h, FIELDS[which]))
+
+    def set_velocity(self, amp, m, n):
+        self._chk(ldlib().orcld_navier_set_velocity(self._h, amp, m, n))
+
+    def update(self, nsteps=1):
+        self._chk(ldlib().orcld_navier_update(self._h, int(nsteps)))

@@ -578,7 +578,7 @@ vecd hessenberg_inverse_iteration(const Arr<double>& h, double lam, double anorm
   for (int i = 0; i < n; ++i) u(i, i) -= lam;
   std::vector<int> swp(n, 0);
   vecd mult(n, 0.0);
-  const double tiny = 2.220446049250313e-16 * std::max(anorm, 1e-300);
+  const double tiny = 2.220446049250313e-16 * std::max(anorm, (double)1e-300);
   for (int k = 0; k < n; ++k) {
     if (k + 1 < n && std::fabs(u(k + 1, k)) > std::fabs(u(k, k))) {
       for (int j = k; j < n; ++j) std::swap(u(k, j), u(k + 1, j));
@@ -704,7 +704,7 @@ vecd inverse_iteration(const Arr<double>& a, double lam, double anorm) {
   double shift = lam;
   for (int i = 0; i < n; ++i) lu(i, i) -= shift;
   std::vector<int> piv(n);
-  const double tiny = 2.220446049250313e-16 * std::max(anorm, 1e-300);
+  const double tiny = 2.220446049250313e-16 * std::max(anorm, (double)1e-300);
   for (int c = 0; c < n; ++c) {
     int p = c;
     for (int i = c + 1; i < n; ++i) if (std::fabs(lu(i, c)) > std::fabs(lu(p, c))) p = i;
@@ -1439,6 +1439,52 @@ int orc_navier_set_temperature(void* p, double amp, double m, double n) {
 int orc_navier_update(void* p, int nsteps) {
   Handle* h = (Handle*)p;
   GUARD({ for (int s = 0; s < nsteps; ++s) { if (h->periodic) h->c->update(); else h->r->update(); } })
+}
+// The convective term of field `which` (0 temp, 1 ux, 2 uy) from the current state, as the
+// step forms it (navier.rs:538-616): ux and uy backward, then conv_temp / conv_ux / conv_uy,
+// dealiased, in the `field` space (spectral [nx, ny] real or [nx/2+1, ny] complex).
+int orc_navier_conv(void* p, int which, double* out) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    auto run = [&](auto& n) {
+      n.ux->backward();
+      n.uy->backward();
+      auto u = n.ux->v;
+      auto v = n.uy->v;
+      if (which == 0) copy_out(n.conv_temp(u, v), out);
+      else if (which == 1) copy_out(n.conv_vel(*n.ux, u, v, u), out);
+      else if (which == 2) copy_out(n.conv_vel(*n.uy, u, v, v), out);
+      else throw std::invalid_argument("conv: which must be 0, 1 or 2");
+    };
+    if (h->periodic) run(*h->c); else run(*h->r);
+  })
+}
+// conv_term(field which, u, [d0, d1]) (conv_term.rs:22-42) with the handle's scale: physical
+// [nx, ny] in and out.
+int orc_navier_conv_term(void* p, int which, int d0, int d1, const double* u, double* out) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    auto run = [&](auto& n) {
+      auto* f = pick(n, which);
+      Arr<double> ua(f->v.n0, f->v.n1);
+      std::memcpy(ua.d.data(), u, sizeof(double) * ua.d.size());
+      copy_out(n.conv_term(*f, ua, d0, d1), out);
+    };
+    if (h->periodic) run(*h->c); else run(*h->r);
+  })
+}
+// finish_conv (navier.rs:562-568): forward transform of a physical [nx, ny] array in the
+// `field` space followed by dealias (navier.rs:1022-1032) when the handle's dealias is on.
+int orc_navier_finish_conv(void* p, const double* phys, double* out) {
+  Handle* h = (Handle*)p;
+  GUARD({
+    auto run = [&](auto& n) {
+      Arr<double> a(n.field->v.n0, n.field->v.n1);
+      std::memcpy(a.d.data(), phys, sizeof(double) * a.d.size());
+      copy_out(n.finish_conv(a), out);
+    };
+    if (h->periodic) run(*h->c); else run(*h->r);
+  })
 }
 double orc_navier_time(void* p) { Handle* h = (Handle*)p; return h->periodic ? h->c->time : h->r->time; }
 // 0 Nu, 1 Nuvol, 2 Re, 3 |div|_2

@@ -60,3 +60,65 @@ def test_ensemble_instance_rules():
     ens.update(1)
     assert np.isfinite(ens[1].eval_nu())
     ens.close()
+
+
+def _oracle_c4(oracle, descs, steps):
+    """The oracle's fields after `steps` steps of every C4 instance (seeded as the benchmark
+    seeds them: set_velocity(0.2, 1, 1), uniform 1e-2 perturbation with seed 1000 + i),
+    with each instance's oracle set-up (fieldbc, Poisson eigen-decomposition)."""
+    out = []
+    for i, d in enumerate(descs):
+        o = oracle.Navier2D.new(*d)
+        o.set_velocity(0.2, 1.0, 1.0)
+        o.set_phys_forward("temp", seeded.interior_perturbation(d[0], d[1], 1e-2, 1000 + i, kind="uniform",
+                                                                periodic=False))
+        setup = (o.get("fieldbc"), [o.tensor_setup(s) for s in range(4)])
+        o.update(steps)
+        out.append((setup, {k: o.get(k) for k in ("temp", "ux", "uy", "pres")}, o.eval_nu()))
+    return out
+
+
+def _c4_ensemble(descs, orc, mixed):
+    ens = Ensemble(descs)
+    for i, d in enumerate(descs):
+        nav = ens[i]
+        bc, tens = orc[i][0]
+        nav.set_temp_bc(bc)
+        for s, t in enumerate(tens):
+            if t is not None:
+                nav.set_tensor(s, *t)
+        nav.set_velocity(0.2, 1.0, 1.0)
+        nav.temp.v = seeded.interior_perturbation(d[0], d[1], 1e-2, 1000 + i, kind="uniform", periodic=False)
+        nav.temp.forward()
+    if mixed:
+        ens.set_option(Navier2D.OPT_MIXED, 1)
+    return ens
+
+
+def test_c4_workload_vs_oracle(oracle):
+    """The C4 workload itself (96 confined instances, N in {64, 96, 128, 192}, Ra
+    log-uniform in [1e4, 1e7], dt = 0.01 * 64 / N) against the oracle instance by instance
+    after 10 steps: the FP64 ensemble within 1e-12, the mixed ensemble within 1e-5 (fields)
+    and 1e-5 (Nu)."""
+    import bench
+    descs = bench.c4_descs()
+    sizes = sorted({d[0] for d in descs})
+    assert sizes == [64, 96, 128, 192]
+    print("case C4 workload: %d instances, sizes %s, Ra %.2e..%.2e" %
+          (len(descs), sizes, min(d[2] for d in descs), max(d[2] for d in descs)), flush=True)
+    orc = _oracle_c4(oracle, descs, 10)
+    for mixed, tol in ((False, 1e-12), (True, 1e-5)):
+        ens = _c4_ensemble(descs, orc, mixed)
+        ens.update(10)
+        worst = {}
+        for i, d in enumerate(descs):
+            nav = ens[i]
+            for k, proxy in (("temp", nav.temp), ("ux", nav.ux), ("uy", nav.uy), ("pres", nav.pres[0])):
+                e = rel_err(proxy.vhat, orc[i][1][k])
+                worst[k] = max(worst.get(k, 0.0), e)
+                assert e < tol, (mixed, i, d, k, e)
+            nu = nav.eval_nu()
+            worst["nu"] = max(worst.get("nu", 0.0), abs(nu - orc[i][2]) / abs(orc[i][2]))
+            assert abs(nu - orc[i][2]) < max(tol, 1e-12) * abs(orc[i][2]), (mixed, i, nu, orc[i][2])
+        print("C4 mixed=%s worst relative errors over the instances: %s" % (mixed, worst), flush=True)
+        ens.close()
