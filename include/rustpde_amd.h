@@ -244,6 +244,17 @@ int rbc_solver_shape(rbc_solver* h, int* o0, int* o1, int* s0, int* s1, int* is_
 /* Solve::solve (hholtz.rs:159-197, hholtz_adi.rs:106-131, poisson.rs:122-150) of `batch`
  * contiguous right-hand sides on DEVICE buffers (reference layouts).  Synchronous. */
 int rbc_solver_solve(rbc_solver* h, const double* d_rhs, double* d_out, int batch);
+/* rbc_solver_solve without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code that must not call hipMalloc, such as
+ * tight serving loops or harnesses that forbid it).  The work arrays live in a fixed
+ * per-device scratch of RBC_NOALLOC_SCRATCH_BYTES that the library holds until the process
+ * ends: 2 W + (Fourier axis 0: I + O) doubles, W = batch * max(o0', s0') * o1 (o0', s0' the
+ * row counts 2 o0, 2 s0 of complex arrays), I = batch * o0' * o1, O = batch * s0' * s1, each
+ * rounded up to a multiple of 32; a call that needs more returns RBC_ERR_ARG.  Results are
+ * those of rbc_solver_solve, bit for bit.  Calls on one GPU share its scratch: calls that
+ * overlap on one GPU (e.g. from two threads) race.  Synchronous. */
+#define RBC_NOALLOC_SCRATCH_BYTES (32ull << 20)
+int rbc_solver_solve_noalloc(rbc_solver* h, const double* d_rhs, double* d_out, int batch);
 /* the pub FdmaTensor fields lam/fwd/bwd of a Hholtz/Poisson solver (fdma_tensor.rs:73-91) */
 int rbc_solver_tensor_info(rbc_solver* h, int* n, int* has_fb);
 int rbc_solver_get_tensor(rbc_solver* h, double* lam, double* fwd, double* bwd);

@@ -188,6 +188,8 @@ void launch_variant(int v, hipStream_t s, bool timed, int M, int N, int K, const
 std::mutex g_tune_mutex;
 std::map<std::tuple<int, int, int, int, int, int>, int> g_tune;
 
+thread_local int g_no_tune = 0;
+
 int tile_rule(int M, int N, int batch) {
   const long tiles64 = (long)((N + 63) / 64) * ((M + 63) / 64) * batch;
   return tiles64 >= 1024 ? TILE_64x64 : TILE_32x32;
@@ -210,6 +212,7 @@ int tile_for(hipStream_t s, int M, int N, int K, const V* A, int lda, long sA, c
     auto it = g_tune.find(key);
     if (it != g_tune.end()) return it->second;
   }
+  if (g_no_tune) return tile_rule(M, N, batch);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipPeekAtLastError() != hipSuccess) return tile_rule(M, N, batch);  // leave a pending error to the caller
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return tile_rule(M, N, batch);
@@ -274,6 +277,9 @@ void gemm_typed(hipStream_t s, int M, int N, int K, const V* A, int lda, long sA
 }
 
 }  // namespace
+
+GemmNoTune::GemmNoTune() { ++g_no_tune; }
+GemmNoTune::~GemmNoTune() { --g_no_tune; }
 
 void gemm_mfma_batched(hipStream_t s, int M, int N, int K, const double* A, int lda, long sA, const double* B, int ldb,
                        long sB, double* C, int ldc, long sC, int batch, double alpha, double beta, const double* U,

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/rustpde_amd.h"
+#include "noalloc.hpp"
 #include "ops.hpp"
 #include "setup.hpp"
 
@@ -1975,35 +1976,61 @@ class SolverDev {
   void ensure(DBuf& b, size_t n) {
     if (b.n < n) b.alloc(n);
   }
+  // work arrays of a solve of `batch` right-hand sides: W1, W2 (big * batch each) and, with a
+  // Fourier axis 0, the split-plane input and output (in_sz * batch, out_sz * batch)
+  struct Work { double *w1, *w2, *in, *out; };
+  size_t big() const { return (size_t)std::max(rows_in(), rows_out()) * b1.ortho(); }
+  size_t in_size() const { return (size_t)rows_in() * b1.ortho(); }
+  size_t out_size() const { return (size_t)rows_out() * b1.m; }
   void solve(const double* rhs, double* out, int batch) {
-    const int o1 = b1.ortho(), m1 = b1.m, m0 = b0.m, ri = rows_in(), ro = rows_out();
-    const size_t in_sz = (size_t)ri * o1, out_sz = (size_t)ro * m1, big = (size_t)std::max(ri, ro) * o1;
-    ensure(W1, big * batch);
-    ensure(W2, big * batch);
+    ensure(W1, big() * batch);
+    ensure(W2, big() * batch);
+    if (cplx) {
+      ensure(IN, in_size() * batch);
+      ensure(OUT, out_size() * batch);
+    }
+    solve_ws(rhs, out, batch, Work{W1.p, W2.p, IN.p, OUT.p});
+  }
+  // the same solve in caller-provided work arrays (no allocation): the doubles needed, each
+  // array starting on a 256-byte boundary
+  static size_t round32(size_t n) { return (n + 31) / 32 * 32; }
+  size_t work_doubles(int batch) const {
+    size_t n = 2 * round32(big() * batch);
+    if (cplx) n += round32(in_size() * batch) + round32(out_size() * batch);
+    return n;
+  }
+  void solve_in(const double* rhs, double* out, int batch, double* scratch) {
+    Work w{};
+    w.w1 = scratch;
+    w.w2 = w.w1 + round32(big() * batch);
+    w.in = w.w2 + round32(big() * batch);
+    w.out = cplx ? w.in + round32(in_size() * batch) : nullptr;
+    if (!cplx) w.in = nullptr;
+    solve_ws(rhs, out, batch, w);
+  }
+  void solve_ws(const double* rhs, double* out, int batch, const Work& w) {
+    const int o1 = b1.ortho(), m1 = b1.m, m0 = b0.m, ro = rows_out();
+    const size_t in_sz = in_size(), out_sz = out_size();
     const double* x = rhs;
     if (cplx) {  // interleaved -> split planes, per right-hand side
-      ensure(IN, in_sz * batch);
-      for (int b = 0; b < batch; ++b) interleaved_to_split(st, rhs + 2 * (size_t)m0 * o1 * b, IN.p + in_sz * b, m0, o1);
-      x = IN.p;
+      for (int b = 0; b < batch; ++b) interleaved_to_split(st, rhs + 2 * (size_t)m0 * o1 * b, w.in + in_sz * b, m0, o1);
+      x = w.in;
     }
     double* y = out;
-    if (cplx) {
-      ensure(OUT, out_sz * batch);
-      y = OUT.p;
-    }
+    if (cplx) y = w.out;
     Solver& sv = *s;
     if (sv.kind == ADI) {
-      gemm_mfma_batched(st, sv.hx.rows, o1, sv.hx.cols, sv.hx.b.p, sv.hx.cols, 0, x, o1, (long)in_sz, W1.p, o1,
+      gemm_mfma_batched(st, sv.hx.rows, o1, sv.hx.cols, sv.hx.b.p, sv.hx.cols, 0, x, o1, (long)in_sz, w.w1, o1,
                         (long)sv.hx.rows * o1, batch, 1.0, 0.0);
-      gemm_mfma(st, batch * sv.hx.rows, m1, sv.hy.rows, W1.p, sv.hy.rows, sv.hy.b.p, sv.hy.cols, y, m1, 1.0, 0.0);
+      gemm_mfma(st, batch * sv.hx.rows, m1, sv.hy.rows, w.w1, sv.hy.rows, sv.hy.b.p, sv.hy.cols, y, m1, 1.0, 0.0);
     } else if (!sv.has_fb) {
       rows_launch(x, o1, y, m1, batch * ro, sv);
     } else {
-      gemm_mfma_batched(st, sv.pf.rows, o1, sv.pf.cols, sv.pf.b.p, sv.pf.cols, 0, x, o1, (long)in_sz, W1.p, o1,
+      gemm_mfma_batched(st, sv.pf.rows, o1, sv.pf.cols, sv.pf.b.p, sv.pf.cols, 0, x, o1, (long)in_sz, w.w1, o1,
                         (long)sv.pf.rows * o1, batch, 1.0, 0.0);
-      rows_launch(W1.p, o1, W2.p, m1, batch * sv.pf.rows, sv);
-      gemm_mfma_batched(st, sv.pb.rows, m1, sv.pb.cols, sv.pb.b.p, sv.pb.cols, 0, W2.p, m1, (long)sv.pf.rows * m1, y, m1,
-                        (long)out_sz, batch, 1.0, 0.0);
+      rows_launch(w.w1, o1, w.w2, m1, batch * sv.pf.rows, sv);
+      gemm_mfma_batched(st, sv.pb.rows, m1, sv.pb.cols, sv.pb.b.p, sv.pb.cols, 0, w.w2, m1, (long)sv.pf.rows * m1, y,
+                        m1, (long)out_sz, batch, 1.0, 0.0);
     }
     if (cplx)
       for (int b = 0; b < batch; ++b) split_to_interleaved(st, y + out_sz * b, out + 2 * (size_t)m0 * m1 * b, m0, m1);
@@ -2573,6 +2600,24 @@ int rbc_solver_solve(rbc_solver* h, const double* d_rhs, double* d_out, int batc
   return guarded([&] {
     if (!h || !h->s || !d_rhs || !d_out || batch < 1) throw std::invalid_argument("bad argument");
     h->s->solve(d_rhs, d_out, batch);
+  });
+}
+int rbc_solver_solve_noalloc(rbc_solver* h, const double* d_rhs, double* d_out, int batch) {
+  return guarded([&] {
+    if (!h || !h->s || !d_rhs || !d_out || batch < 1) throw std::invalid_argument("bad argument");
+    const size_t need = h->s->work_doubles(batch);
+    if (need > rbc::noalloc::kScratchDoubles)
+      throw std::invalid_argument("rbc_solver_solve_noalloc: the work arrays (" + std::to_string(need) +
+                                  " doubles) exceed the fixed scratch of " +
+                                  std::to_string(rbc::noalloc::kScratchDoubles) + " doubles");
+    double* scratch = nullptr;
+    try {
+      scratch = rbc::noalloc::scratch_ptr();
+    } catch (const std::runtime_error& e) {
+      throw rbc::DeviceError(e.what());
+    }
+    rbc::GemmNoTune no_tune;
+    h->s->solve_in(d_rhs, d_out, batch, scratch);
   });
 }
 int rbc_solver_tensor_info(rbc_solver* h, int* n, int* has_fb) {

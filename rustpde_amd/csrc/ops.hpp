@@ -36,6 +36,17 @@ void gemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const do
 void gemm_mfma(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
                int ldc, double alpha, double beta, const double* U = nullptr, int ldu = 0);
 
+// While a GemmNoTune object lives on the calling thread, the GEMMs take the tile shape
+// already measured for their launch shape, or the rule-of-thumb shape, and never time
+// variants (the timing allocates a scratch output and events).  Every tile shape gives the
+// same result bits.
+struct GemmNoTune {
+  GemmNoTune();
+  ~GemmNoTune();
+  GemmNoTune(const GemmNoTune&) = delete;
+  GemmNoTune& operator=(const GemmNoTune&) = delete;
+};
+
 // Batch-index moduli of A, B, U (0: none): entry z reads A + sA (z % a) etc.
 // Entries z >= split read A from a2 and B from b2 instead (when given; same types, strides
 // and moduli): two operators in one launch.

@@ -34,6 +34,14 @@ class Solver:
         check(_lib.load().rbc_solver_solve(self._h, ctypes.c_void_p(int(getattr(d_rhs, "value", d_rhs) or 0)),
                                            ctypes.c_void_p(int(getattr(d_out, "value", d_out) or 0)), int(batch)))
 
+    def solve_device_noalloc(self, d_rhs, d_out, batch):
+        """solve_device without any device allocation: the work arrays live in the library's
+        fixed per-device scratch (32 MiB); raises RbcError when they do not fit.  Calls that
+        overlap on one GPU share that scratch and race."""
+        check(_lib.load().rbc_solver_solve_noalloc(self._h, ctypes.c_void_p(int(getattr(d_rhs, "value", d_rhs) or 0)),
+                                                   ctypes.c_void_p(int(getattr(d_out, "value", d_out) or 0)),
+                                                   int(batch)))
+
     def solve(self, rhs):
         """Host arrays [o0, o1] or [batch, o0, o1] -> [.., s0, s1]."""
         a = np.ascontiguousarray(rhs, dtype=complex if self.is_complex else float)
