@@ -134,11 +134,39 @@ def nu_run(cfg, t_end, lib_mod):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(cfg, budget_s):
+def _cpu_info():
+    """CPU model, logical CPUs and the oracle's compiler and flags (oracle/Makefile)."""
+    import platform
+    import re
+    import subprocess
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    flags = "unknown"
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as f:
+            m = re.search(r"^CXXFLAGS \?= (.*)$", f.read(), re.M)
+            flags = m.group(1) if m else flags
+    except OSError:
+        pass
+    try:
+        cxx = subprocess.run(["g++", "-dumpfullversion"], capture_output=True, text=True, timeout=10).stdout.strip()
+    except (OSError, subprocess.SubprocessError):
+        cxx = "unknown"
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "compiler": "g++ " + cxx, "flags": flags}
+
+
+def _cpu_rate(cfg, threads, budget_s):
+    """Oracle steps/s at `threads` threads: one warm-up step, then samples of k steps (k such
+    that a sample takes >= 50 ms) until the budget is spent (at least 3 samples); the median
+    sample rate."""
     from oracle import oracle as O
-    if not os.path.exists(O._LIB_PATH):
-        O.build()
-    threads = max(1, min(16, os.cpu_count() or 1))
     O.set_threads(threads)
 
     class _M:
@@ -148,18 +176,43 @@ def cpu_baseline(cfg, budget_s):
     t0 = time.perf_counter()
     o.update(1)
     one = time.perf_counter() - t0
-    steps = max(2, min(200, int(budget_s / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    o.update(steps)
-    el = time.perf_counter() - t0
+    k = max(1, int(0.05 / max(one, 1e-9)) + 1)
+    rates, t_start = [], time.perf_counter()
+    while len(rates) < 3 or (time.perf_counter() - t_start < budget_s and len(rates) < 10):
+        t0 = time.perf_counter()
+        o.update(k)
+        rates.append(k / (time.perf_counter() - t0))
+    rates.sort()
+    return rates[len(rates) // 2], len(rates), k, o
+
+
+def cpu_baseline(cfg, budget_s):
+    """The oracle (the FP64 restatement of the reference algorithm) on the host: at all the
+    host's CPUs available to this process (OMP_NUM_THREADS when set, the box's CPU share) and
+    at one thread, OPENBLAS_NUM_THREADS=1 exported (README.md:13-16 of the reference; the
+    restatement links no BLAS)."""
+    from oracle import oracle as O
+    if not os.path.exists(O._LIB_PATH):
+        O.build()
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    cpus = os.cpu_count() or 1
+    env_threads = os.environ.get("OMP_NUM_THREADS", "")
+    threads = max(1, min(cpus, int(env_threads))) if env_threads.isdigit() else cpus
+    rate, nsamp, k, o = _cpu_rate(cfg, threads, budget_s)
     # one statistics update of the oracle at the same state (statistics.rs:130-153)
     o.stats_enable(1.0, 10.0)
     t1 = time.perf_counter()
     o.stats_update()
     stats_ms = 1e3 * (time.perf_counter() - t1)
-    return {"value": steps / el, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "%d steps of the same workload after 1 warm step (oracle/rbc_oracle.cpp, OpenMP over lanes)"
-                      % steps}, stats_ms
+    rate1, nsamp1, k1, _ = _cpu_rate(cfg, 1, budget_s)
+    info = _cpu_info()
+    info.update({"OPENBLAS_NUM_THREADS": "1", "OMP_NUM_THREADS": env_threads or None})
+    return {"value": rate, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "median of %d samples of %d steps after 1 warm-up step (the oracle, oracle/rbc_oracle.cpp: the "
+                      "reference algorithm restated in C++, OpenMP over lanes; not the Rust binary)" % (nsamp, k),
+            "single_thread": {"value": rate1, "unit": "steps/s", "cores": 1,
+                              "sample": "median of %d samples of %d steps" % (nsamp1, k1)},
+            "host": info}, stats_ms
 
 
 def solid_step(cfg, lib_mod, steps=20):
@@ -453,14 +506,14 @@ def main():
     if mixed:
         nav.set_option(nav.OPT_MIXED, 1)
     elapsed = timed_steps(nav)
-    extra = {}
+    extra = {"per_step_calls": per_step_calls(nav, args.steps)}
     if mixed:
         nav64 = make(args.config, rustpde_amd)
         el64 = timed_steps(nav64)
         del nav64
         extra["fp64_steps_per_s"] = args.steps / el64
         extra["mixed_speedup_vs_fp64"] = el64 / elapsed
-        extra["mixed_error_vs_fp64_after_10_steps"] = mixed_errors(args.config, rustpde_amd)
+        extra["mixed_error_vs_fp64"] = mixed_errors(args.config, rustpde_amd)
 
     # the same K steps launched one kernel at a time with HIP events recorded on the
     # handle's stream around every launch: per-kernel-class durations for the roofline
@@ -522,24 +575,63 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def mixed_errors(cfg, lib_mod):
-    """Relative max-norm differences of the mixed step from the FP64 step after 10 steps from
-    the same seeded fields (spectral coefficients), and of Nu."""
+def per_step_calls(nav, steps):
+    """Steps/s of `steps` calls of update() with one step each, as integrate() and the
+    reference's criterion bench (benches/benchmark_navier.rs:34) make them: from the second
+    call on each call replays the captured step."""
+    nav.update(1)
+    nav.update(1)
+    nav.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nav.update(1)
+    nav.synchronize()
+    el = time.perf_counter() - t0
+    return {"steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "calls": steps}
+
+
+def mixed_errors(cfg, lib_mod, develop=1000):
+    """Relative max-norm differences of the mixed step from the FP64 step (spectral
+    coefficients, per field) and of Nu, from a moving state: the seeded fields with
+    set_velocity(0.05, 2, 1) (a convecting flow, so the FP32 convective terms are live),
+    advanced `develop` FP64 steps; then 10 and 100 steps of each precision from that state."""
     import numpy as np
-    a = make(cfg, lib_mod)
-    b = make(cfg, lib_mod)
-    b.set_option(b.OPT_MIXED, 1)
-    a.update(10)
-    b.update(10)
-    err = {}
-    for name in ("temp", "ux", "uy"):
-        x, y = getattr(a, name).vhat, getattr(b, name).vhat
-        err[name] = float(np.abs(x - y).max() / np.abs(x).max())
-    x, y = a.pres[0].vhat, b.pres[0].vhat
-    err["pres"] = float(np.abs(x - y).max() / np.abs(x).max())
-    nua, nub = a.eval_nu(), b.eval_nu()
-    err["nu"] = abs(nua - nub) / abs(nua)
-    return err
+    src = make(cfg, lib_mod)
+    if CONFIGS[cfg][9] is None:
+        src.set_velocity(0.05, 2.0, 1.0)
+    src.update(develop)
+    state = {k: getattr(src, k).vhat for k in ("temp", "ux", "uy")}
+    state["pres"] = src.pres[0].vhat
+    t_state = src.get_time()
+    del src
+
+    def start(mixed):
+        nav = make(cfg, lib_mod)
+        for k in ("temp", "ux", "uy"):
+            getattr(nav, k).vhat = state[k]
+        nav.pres[0].vhat = state["pres"]
+        nav.time = t_state
+        if mixed:
+            nav.set_option(nav.OPT_MIXED, 1)
+        return nav
+
+    a, b = start(False), start(True)
+    out = {"start": "seeded fields + set_velocity(0.05, 2, 1) after %d FP64 steps (t = %.3f)" % (develop, t_state)}
+    done = 0
+    for n in (10, 100):
+        a.update(n - done)
+        b.update(n - done)
+        done = n
+        err = {}
+        for name in ("temp", "ux", "uy"):
+            x, y = getattr(a, name).vhat, getattr(b, name).vhat
+            err[name] = float(np.abs(x - y).max() / np.abs(x).max())
+        x, y = a.pres[0].vhat, b.pres[0].vhat
+        err["pres"] = float(np.abs(x - y).max() / np.abs(x).max())
+        nua, nub = a.eval_nu(), b.eval_nu()
+        err["nu"] = abs(nua - nub) / abs(nua)
+        out["after_%d_steps" % n] = err
+    return out
 
 
 def committed_traffic(cfg, kernel):

@@ -521,40 +521,54 @@ class Navier {
   size_t FS = 0;          // slot stride of the batched buffers
   DBuf STATE, PHYS2;      // [ux, uy, temp] spectral and [ux, uy] physical, one slot each
   hipGraphExec_t gexec = nullptr;
+  int eager_steps = 0;     // eager steps since the captured step was last dropped
+  bool graph_gpres = false;  // the captured step reads d/dy pres left by the step before it
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     gexec = nullptr;
+    eager_steps = 0;
   }
-  // Integrate::update nsteps times: one eager step, then a captured step replayed
+  // capture one step into gexec (the captured launches do not run)
+  void capture_step() {
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const double t_saved = time;
+    try {
+      update();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      time = t_saved;
+      throw;
+    }
+    time = t_saved;
+    graph_gpres = gpres_valid;  // set by the captured step when its fused pressure stage forms d/dy pres
+    HIPCHECK(hipStreamEndCapture(st, &g));
+    hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { gexec = nullptr; throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+  }
+  // Integrate::update nsteps times.  After a set-up change the first step runs eagerly (it
+  // tunes the GEMM tiles and, after a host write, forms d/dy pres on its own); every later
+  // step -- in this call or in a later one, so that one step per call as integrate() makes
+  // them reaches it from the second call on -- replays one captured step.
   void update_n(int nsteps) {
     int done = 0;
-    if (!gpres_valid) drop_graph();  // the captured step relies on d/dy pres from the last step
-    if (use_graph && !timer.on && nsteps > 1 && !gexec) {
+    if (gexec && graph_gpres && !gpres_valid) drop_graph();  // d/dy pres was not left by a step
+    const bool graph = use_graph && !timer.on;
+    if (graph && !gexec && eager_steps == 0 && done < nsteps) {
       update();
+      ++eager_steps;
       ++done;
-      hipGraph_t g = nullptr;
-      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      const double t_saved = time;
-      try {
-        update();
-      } catch (...) {
-        (void)hipStreamEndCapture(st, &g);
-        if (g) (void)hipGraphDestroy(g);
-        time = t_saved;
-        throw;
-      }
-      time = t_saved;
-      HIPCHECK(hipStreamEndCapture(st, &g));
-      hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) { gexec = nullptr; throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
     }
+    if (graph && !gexec && done < nsteps) capture_step();
     for (; done < nsteps; ++done) {
-      if (use_graph && !timer.on && gexec) {
+      if (graph && gexec) {
         HIPCHECK(hipGraphLaunch(gexec, st));
         time += dt;
       } else {
         update();
+        ++eager_steps;
       }
     }
   }  // Fourier x transforms by the FFT kernels (dense DFT GEMMs otherwise)
@@ -1403,13 +1417,16 @@ class Navier {
     const int my = by_u.m;
     if (use_fuse) {
       // values of ux, uy, temp (W2) and their x-derivatives (W1) in one launch
-      timer.begin(st, 2, 8.0 * 6.0 * my * ((double)nx + 2.0 * mx()));
+      // bytes: the spectra of ux, uy, temp read once, six physical arrays written
+      timer.begin(st, 2, 8.0 * my * (3.0 * 2.0 * mx() + 6.0 * nx));
       irfft_cols_pair(st, UX.p, W2.p, W1.p, nx, my, 1, scale[0], 3, (long)FS, (long)FS);
       timer.end(st);
       CHEBY_BWD_B(by_u, W2.p, UXP.p, nx, 0, nullptr, 0.0, 2);  // ux, uy in physical space
       // conv = d/dy q uy + dq/dx ux (conv_term.rs:35-41), + u . grad(fieldbc) for temp
       const DBand& band = stencil_band(by_u);
-      timer.begin(st, 11, 8.0 * 3.0 * nx * (2.0 * my + 3.0 * ny) + 8.0 * 4.0 * nx * ny);
+      // bytes, distinct arrays once: x-derivatives and values of the three fields, ux, uy,
+      // the two fieldbc gradient arrays read, three convective terms written
+      timer.begin(st, 11, 8.0 * nx * (6.0 * my + 7.0 * ny));
       cheb_y_conv(st, W1.p, W2.p, my, my, CONV.p, ny, nx, ny, band.coef.p, 1.0 / scale[1], UXP.p, UYP.p, ny, 3,
                   (long)FS, (long)FS, UXP.p, DTBCX.p, UYP.p, DTBCY.p, 2);
       timer.end(st);
@@ -1457,14 +1474,14 @@ class Navier {
     if (use_fuse) {
       // values (W2f) and x-derivatives (W1f) of ux, uy, temp in one launch, then ux, uy
       // in physical space and the convective terms in one launch each
-      timer.begin(st, 2, 6.0 * my * fx);
+      timer.begin(st, 2, my * (8.0 * 3.0 * mx2 + 4.0 * 6.0 * nx));  // spectra once, six FP32 outputs
       irfft_cols_pair_f32(st, UX.p, W2f, W1f, nx, my, 1, scale[0], 3, F, F);
       timer.end(st);
       timer.begin(st, 4, 4.0 * 2.0 * nx * ((double)my + ny));
       cheb_y_backward_f32(st, W2f, my, my, UXPf, ny, nx, ny, band.coef.p, 0, 1.0 / scale[1], nullptr, ny, 0.0, 2, F, F,
                           0, nullptr, nullptr, nullptr, nullptr, -1);
       timer.end(st);
-      timer.begin(st, 11, 4.0 * 3.0 * nx * (2.0 * my + 3.0 * ny) + 24.0 * nx * ny);
+      timer.begin(st, 11, 4.0 * nx * (6.0 * my + 5.0 * ny) + 16.0 * nx * ny);  // FP32 fields, FP64 fieldbc
       cheb_y_conv_f32(st, W1f, W2f, my, my, CONVf, ny, nx, ny, band.coef.p, 1.0 / scale[1], UXPf, UYPf, ny, 3, F, F,
                       UXPf, DTBCX.p, UYPf, DTBCY.p, 2);
       timer.end(st);
@@ -1504,9 +1521,10 @@ class Navier {
       pa.sp_kind = by_phi.k == CD ? 1 : by_phi.k == CN ? 2 : 0;
       pa.dP = G.p + FS;  // d/dy of the new pres for the next step
       const double rows = 2.0 * mx();
-      // bytes: + the new pres rows read back and d/dy pres written (2 n per row)
-      timer.begin(st, 10, 8.0 * rows * (2.0 * by_u.m + ny) + 8.0 * rows * (3.0 * by_u.m + ny) +
-                              8.0 * 4.0 * mx() * (double)by_phi.m + 8.0 * rows * 2.0 * ny);
+      // bytes, distinct arrays once per row: ux, uy, pres read; ux, uy, pseu, pres and d/dy
+      // pres written; the four diagonal arrays of the row's Poisson set
+      timer.begin(st, 10, 8.0 * rows * (2.0 * by_u.m + ny) + 8.0 * rows * (3.0 * by_u.m + 2.0 * ny) +
+                              8.0 * 4.0 * mx() * (double)by_phi.m);
       pressure_rows(st, pa);
       timer.end(st);
       gpres_valid = true;
