@@ -99,6 +99,26 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
   //     i - 2 of plane r of ik ux / sc) as the unfused kernels add the two terms
   const int Cd = N / 2, perd = (Cd + WAVE - 1) / WAVE, t0d = lane * perd;
   const int cntd = max(0, min(perd, Cd - t0d));
+  const int per = a.per, t0 = lane * per;
+  // the Poisson phase's P_y taps and forward-elimination multipliers, issued here so that
+  // their round trip overlaps the divergence (the P_y band of the pinv has no tap at -2:
+  // the Rayleigh-Benard instantiation does not load it)
+  const size_t tp = (size_t)2 * per * WAVE;
+  const size_t sb = (size_t)k * (8 * (size_t)per * WAVE);
+#define DG(arr, q) a.pdiag[sb + ((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
+  double pc0[SLICE], pc1[SLICE], pc2[SLICE], pc3[SLICE], plo[SLICE];
+  auto load_poisson = [&]() {
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      const size_t b = ((size_t)p * per + q) * WAVE + lane;
+      pc0[q] = a.pybp[b];
+      pc1[q] = RBC ? 0.0 : a.pybp[tp + b];
+      pc2[q] = a.pybp[2 * tp + b];
+      pc3[q] = a.pybp[3 * tp + b];
+      plo[q] = DG(0, q);
+    }
+  };
+  if constexpr (RBC) load_poisson();  // the general instantiation loads them in (3)
   {
     double v[SLICE];
     const double* Y = YB(r);
@@ -139,45 +159,37 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
   }
   __syncthreads();
   // (3) pseu = Poisson rows of div (P_y taps, then the Fdma of set k); pseu(0,0) = 0
-  const int per = a.per, t0 = lane * per;
   const int lenu = p == 0 ? (mu + 1) / 2 : mu / 2;
   const int cnt = max(0, min(per, lenu - t0));
+  if constexpr (!RBC) load_poisson();
   {
-    double v[SLICE];
-    const size_t tp = (size_t)2 * per * WAVE;
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) {
-      const size_t b = ((size_t)p * per + q) * WAVE + lane;
-      const double c0 = a.pybp[b], c1 = a.pybp[tp + b], c2 = a.pybp[2 * tp + b], c3 = a.pybp[3 * tp + b];
-      double val = 0.0;
-      if (q < cnt) {
-        const int j = p + 2 * (t0 + q) + 2;
-        const double* D = DV(r);
-        val = c0 * D[pad(j)];
-        val = val + c1 * D[pad(j - 2)];
-        val = val + c2 * D[pad(j + 2)];
-        val = val + c3 * D[pad(j + 4)];
-      }
-      v[q] = val;
-    }
-    const size_t sb = (size_t)k * (8 * (size_t)per * WAVE);
-#define DG(arr, q) a.pdiag[sb + ((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
-    {
-      double lo[SLICE];
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) lo[q] = DG(0, q);
-      chains::fdma_forward(v, lo, lane, t0, cnt);
-    }
-    {
-      double rd[SLICE], g1[SLICE], g2[SLICE];  // 1 / dia, up1, up2
+    double rd[SLICE], g1[SLICE], g2[SLICE];  // 1 / dia, up1, up2
+    auto load_back = [&]() {
 #pragma unroll
       for (int q = 0; q < SLICE; ++q) {
         rd[q] = DG(1, q);
         g1[q] = DG(2, q);
         g2[q] = DG(3, q);
       }
-      chains::fdma_backward(v, rd, g1, g2, lane, t0, cnt, lenu);
+    };
+    if constexpr (RBC) load_back();  // issued before the band and the forward pass
+    double v[SLICE];
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      double val = 0.0;
+      if (q < cnt) {
+        const int j = p + 2 * (t0 + q) + 2;
+        const double* D = DV(r);
+        val = pc0[q] * D[pad(j)];
+        if (!RBC) val = val + pc1[q] * D[pad(j - 2)];
+        val = val + pc2[q] * D[pad(j + 2)];
+        val = val + pc3[q] * D[pad(j + 4)];
+      }
+      v[q] = val;
     }
+    chains::fdma_forward(v, plo, lane, t0, cnt);
+    if constexpr (!RBC) load_back();
+    chains::fdma_backward(v, rd, g1, g2, lane, t0, cnt, lenu);
 #undef DG
     if (k == 0 && p == 0 && lane == 0) v[0] = 0.0;  // navier.rs:710-715
 #pragma unroll
@@ -185,7 +197,9 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       if (q < cnt) YB(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
   }
   __syncthreads();
-  // (4) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt (half block h: plane h)
+  // (4) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt (half block h: plane h); the new
+  //     pres values stay in registers for (8)
+  double pn[QH];
   {
     double pv[QH], c0[QH], c1[QH];
     double* prow = a.P0 + ROWN(h);
@@ -200,14 +214,28 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #pragma unroll
     for (int q = 0; q < QH; ++q) {
       const int i = th + q * 128;
+      pn[q] = 0.0;
       if (i >= ny) continue;
       if (i < mu) a.P1[ROWU(h) + i] = Pp[pad(i + 2)];
       const double w3 = Pp[pad(i + 2)] * c0[q] + Pp[pad(i)] * c1[q];
       const double t = pv[q] - DV(h)[pad(i + 2)] * a.nu;
-      prow[i] = t + w3 * a.inv_dt;
+      pn[q] = t + w3 * a.inv_dt;
+      prow[i] = pn[q];
     }
   }
   __syncthreads();
+  // the from_ortho tables of the projections (1 / den, w), issued here so that their round
+  // trip overlaps the derivative below
+#define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
+  double trn[SLICE], twv[SLICE];
+  auto load_tdma = [&]() {
+#pragma unroll
+    for (int q = 0; q < SLICE; ++q) {
+      trn[q] = TB(3, q);
+      twv[q] = TB(4, q);
+    }
+  };
+  if constexpr (RBC) load_tdma();  // the general instantiation loads them in (6)
   // (5) d/dy S_p(pseu) into DV (the chains of (2b); div is no longer needed)
   {
     double v[SLICE];
@@ -244,7 +272,6 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
   // (6) both projections, from_ortho of g = S_p(ik pseu / sc) (ux) and of g = DV (uy):
   //     d_i = g_i diag_i + g_{i+2} low2_i, then the tdma recurrences with the tables shared
   //     by the two right-hand sides
-#define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
   // the from_ortho tables of the velocity basis: diag, low2 and off_{i-2} = diag_i low2_{i-2}
   // are 1, -1, -1 for the Dirichlet basis (0 for off at the chain start)
 #define TB_DG(q) (su_kind == 1 ? 1.0 : TB(0, q))
@@ -286,15 +313,12 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       }
     }
     {
-      double of[SLICE], rn[SLICE], wv[SLICE];  // off_{i-2}, 1 / den_i, w_i
+      double of[SLICE];  // off_{i-2}; 1 / den_i and w_i were loaded before (5)
+      if constexpr (!RBC) load_tdma();
 #pragma unroll
-      for (int q = 0; q < SLICE; ++q) {
-        of[q] = TB_OF(q);
-        rn[q] = TB(3, q);
-        wv[q] = TB(4, q);
-      }
-      chains::tdma_forward2(vx, vy, of, rn, lane, t0, cnt);
-      chains::tdma_backward2(vx, vy, wv, lane, t0, cnt, p, mu);
+      for (int q = 0; q < SLICE; ++q) of[q] = TB_OF(q);
+      chains::tdma_forward2(vx, vy, of, trn, lane, t0, cnt);
+      chains::tdma_backward2(vx, vy, twv, lane, t0, cnt, p, mu);
     }
     // ux = x (-1) + ux in place (XA still holds the staged ux); the uy projection replaces
     // the entries of DV that only this wavefront reads
@@ -320,10 +344,21 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
   chains::unstage_row<QH>(DV(h), a.UY + ROWU(h), true, mu, th, 128, true);
   if (a.dP) {
     // (8) d/dy of the updated pres (ortho.rs:107-125, the chains of (2) on the Chebyshev
-    //     coefficients) for the next step's right-hand sides: pres rows (written in (4)) ->
-    //     YB, derivative -> XA -> dP
+    //     coefficients) for the next step's right-hand sides: pres rows (kept from (4)) ->
+    //     YB (element i at pad(i + 2), zero guards as stage_row leaves them), derivative ->
+    //     XA -> dP
+    //     (the general instantiation reloads them: it has no registers to spare)
     __syncthreads();
-    chains::stage_row<QH>(YB(h), a.P0 + ROWN(h), true, ny, th, 128);
+    if constexpr (RBC) {
+#pragma unroll
+      for (int q = 0; q < QH; ++q) {
+        const int i = th + q * 128;
+        if (i < ny) YB(h)[pad(i + 2)] = pn[q];
+      }
+      if (th < 6) YB(h)[pad(th < 2 ? th : ny + th)] = 0.0;
+    } else {
+      chains::stage_row<QH>(YB(h), a.P0 + ROWN(h), true, ny, th, 128);
+    }
     __syncthreads();
     {
       double v[SLICE];
