@@ -96,8 +96,9 @@ int rbc_eval(rbc_navier* h, int what, double* out);
  * value are physical [nx, ny] arrays; mask = NULL removes the obstacle (None).  The
  * penalisation (eta = 1e-2) enters the convective terms of ux, uy and temp
  * (navier.rs:551-560, 579-584, 603-608).  A handle with a mask runs the field-by-field FP64
- * step: the batched, fused and mixed-precision paths do not take the terms.  Not available
- * for ensemble instances. */
+ * step (the batched and fused paths, equivalent in FP64, do not take the terms): setting a
+ * mask while RBC_OPT_MIXED is on, or RBC_OPT_MIXED while a mask is set, returns RBC_ERR_ARG.
+ * Not available for ensemble instances. */
 int rbc_set_solid(rbc_navier* h, const double* mask, const double* value);
 
 /* Statistics (src/navier/statistics.rs:10-153; Navier2D::statistics, navier.rs:193-194),

@@ -76,6 +76,22 @@ def lib():
         L.orc_navier_set_solid.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_adjoint_create.restype = ctypes.c_void_p
+        L.orc_adjoint_create.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int] * 2
+        L.orc_adjoint_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_adjoint_part.restype = ctypes.c_void_p
+        L.orc_adjoint_part.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_adjoint_update.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_adjoint_time.restype = ctypes.c_double
+        L.orc_adjoint_time.argtypes = [ctypes.c_void_p]
+        L.orc_adjoint_residual.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_adjoint_unsmoothed.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_adjoint_norms.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_adjoint_exit.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.orc_adjoint_smoother.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_adjoint_set_residual.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_adjoint_advance.argtypes = [ctypes.c_void_p]
+        L.orc_adjoint_conv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -215,6 +231,14 @@ class Navier2D:
     @classmethod
     def new(cls, nx, ny, ra, pr, dt, aspect, adiabatic):
         return cls(nx, ny, ra, pr, dt, aspect, adiabatic, False)
+
+    @classmethod
+    def _from_handle(cls, h, periodic, owner):
+        self = cls.__new__(cls)
+        self.periodic = periodic
+        self._h = h
+        self._owner = owner  # keeps the object the handle shares alive
+        return self
 
     @classmethod
     def new_periodic(cls, nx, ny, ra, pr, dt, aspect):
@@ -430,3 +454,96 @@ class NavierLD:
 
     def update(self, nsteps=1):
         self._chk(ldlib().orcld_navier_update(self._h, int(nsteps)))
+
+
+class NavierAdjoint:
+    """Oracle Navier2DAdjoint (navier_adjoint.rs).  `fields` is a Navier2D view of the adjoint
+    fields [0] (temp, ux, uy, pres, pseu, fieldbc; diagnostics and tensors of the adjoint's
+    Poisson solver), `navier` the Navier-Stokes step whose residual is descended."""
+
+    def __init__(self, nx, ny, ra, pr, dt, aspect, adiabatic=True, periodic=False):
+        L = lib()
+        self.periodic = periodic
+        self._h = L.orc_adjoint_create(nx, ny, ra, pr, dt, aspect, int(adiabatic), int(periodic))
+        if not self._h:
+            raise RuntimeError("oracle: " + L.orc_last_error().decode())
+        self.fields = Navier2D._from_handle(L.orc_adjoint_part(self._h, 0), periodic, self)
+        self.navier = Navier2D._from_handle(L.orc_adjoint_part(self._h, 1), periodic, self)
+
+    @classmethod
+    def new(cls, nx, ny, ra, pr, dt, aspect, adiabatic):
+        return cls(nx, ny, ra, pr, dt, aspect, adiabatic, False)
+
+    @classmethod
+    def new_periodic(cls, nx, ny, ra, pr, dt, aspect):
+        return cls(nx, ny, ra, pr, dt, aspect, True, True)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            for part in ("fields", "navier"):
+                obj = self.__dict__.pop(part, None)
+                if obj is not None and obj._h:
+                    _lib.orc_navier_destroy(obj._h)
+                    obj._h = None
+            _lib.orc_adjoint_destroy(self._h)
+            self._h = None
+
+    def update(self, nsteps=1):
+        _check(lib().orc_adjoint_update(self._h, int(nsteps)))
+
+    def get_time(self):
+        return lib().orc_adjoint_time(self._h)
+
+    def residual(self, which):
+        """smoothed residual [1] of 'temp' | 'ux' | 'uy' (spectral)"""
+        shp = self.fields._shape(which, True)
+        out = np.zeros(shp, dtype=complex if self.periodic else float)
+        _check(lib().orc_adjoint_residual(self._h, {"temp": 0, "ux": 1, "uy": 2}[which], _p(out.view(float))))
+        return out
+
+    def set_residual(self, which, arr):
+        """replace the smoothed residual [1] of 'temp' | 'ux' | 'uy' (spectral)"""
+        shp = self.fields._shape(which, True)
+        a = np.ascontiguousarray(arr, dtype=complex if self.periodic else float)
+        assert a.shape == shp, (a.shape, shp)
+        _check(lib().orc_adjoint_set_residual(self._h, {"temp": 0, "ux": 1, "uy": 2}[which], _p(a.view(float))))
+
+    def advance(self):
+        """the update without recomputing the residuals (navier_adjoint.rs:766-788 after :765)"""
+        _check(lib().orc_adjoint_advance(self._h))
+
+    def conv(self, which):
+        """conv_temp / conv_ux / conv_uy (navier_adjoint.rs:488-578) of the current state"""
+        shp = self.fields._shape("field", True)
+        out = np.zeros(shp, dtype=complex if self.periodic else float)
+        _check(lib().orc_adjoint_conv(self._h, {"temp": 0, "ux": 1, "uy": 2}[which], _p(out.view(float))))
+        return out
+
+    def unsmoothed(self, which):
+        """fields_unsmoothed of 'ux' | 'uy' | 'temp' (ortho space of the field)"""
+        shp = self.fields._shape("field", True)
+        out = np.zeros(shp, dtype=complex if self.periodic else float)
+        _check(lib().orc_adjoint_unsmoothed(self._h, {"ux": 0, "uy": 1, "temp": 2}[which], _p(out.view(float))))
+        return out
+
+    def norms(self):
+        """[|ux1|, |uy1|, |temp1|, |unsmoothed ux|, |unsmoothed uy|, |unsmoothed temp|]"""
+        o = np.zeros(6)
+        _check(lib().orc_adjoint_norms(self._h, _p(o)))
+        return o
+
+    def exit(self):
+        r = ctypes.c_int()
+        _check(lib().orc_adjoint_exit(self._h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def smoother_tensor(self, s):
+        """(lam, fwd, bwd) of smoother s (0 ux, 1 uy, 2 temp)"""
+        hp = ctypes.c_void_p()
+        _check(lib().orc_adjoint_smoother(self._h, int(s), ctypes.byref(hp)))
+        n, fb = ctypes.c_int(), ctypes.c_int()
+        _check(lib().orc_solver_tensor(hp, None, None, None, ctypes.byref(n), ctypes.byref(fb)))
+        m = n.value
+        lam, fwd, bwd = np.zeros(m), np.zeros((m, m)), np.zeros((m, m))
+        _check(lib().orc_solver_tensor(hp, _p(lam), _p(fwd), _p(bwd), ctypes.byref(n), ctypes.byref(fb)))
+        return (lam, fwd, bwd) if fb.value else (lam, None, None)

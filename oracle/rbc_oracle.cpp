@@ -1307,10 +1307,150 @@ template <> void Navier2D<cd>::bc_rbc(int nx, int ny) {
 // ------------------------------------------------------------------ C interface
 thread_local std::string g_err;
 
+// NavierAdjoint (src/navier/navier_adjoint.rs:128-1068): the adjoint descent of Farazmand
+// (2016) for steady states.  `navier` is the Navier-Stokes step whose residual is descended
+// (dt_navier = 1e-2, navier_adjoint.rs:231-232); `a` holds the adjoint fields [0] (temp, ux,
+// uy, pres, pseu), the `field` space, fieldbc and the pressure Poisson solver, which are the
+// same objects a Navier2D of the adjoint's dt holds; temp1, ux1, uy1 are the smoothed
+// residuals [1] and smoother the Hholtz smoothers (1 - c lap), c = 1 / scale^2 (:247-268).
+template <class T> struct NavierAdjoint {
+  std::shared_ptr<Navier2D<T>> navier, a;
+  std::unique_ptr<Field2<T>> temp1, ux1, uy1;
+  std::unique_ptr<Solver2> smoother[3];
+  Arr<T> unsm[3];  // fields_unsmoothed (navier_adjoint.rs:151-152)
+  double dt, time = 0.0, res_tol = 1e-8;  // RES_TOL (:114)
+  NavierAdjoint(int nx, int ny, double ra, double pr, double dt_, double aspect, bool adiabatic, bool per) : dt(dt_) {
+    navier.reset(new Navier2D<T>(nx, ny, ra, pr, 1e-2, aspect, adiabatic, per));
+    a.reset(new Navier2D<T>(nx, ny, ra, pr, dt_, aspect, adiabatic, per));
+    temp1.reset(new Field2<T>(a->temp->space));
+    ux1.reset(new Field2<T>(a->ux->space));
+    uy1.reset(new Field2<T>(a->uy->space));
+    const double w = 1e0;  // weight_laplacian
+    const double cs[2] = {w / std::pow(a->scale[0], 2.0), w / std::pow(a->scale[1], 2.0)};
+    smoother[0].reset(new Solver2(S_HHOLTZ, ux1->space.b0, ux1->space.b1, cs));
+    smoother[1].reset(new Solver2(S_HHOLTZ, uy1->space.b0, uy1->space.b1, cs));
+    smoother[2].reset(new Solver2(S_HHOLTZ, temp1->space.b0, temp1->space.b1, cs));
+    for (auto& u : unsm) u = Arr<T>(a->field->vhat.n0, a->field->vhat.n1);
+  }
+  Field2<T>& f1(int which) { return which == 0 ? *temp1 : which == 1 ? *ux1 : *uy1; }
+  static void addto(Arr<double>& x, const Arr<double>& y) { for (size_t i = 0; i < x.d.size(); ++i) x.d[i] = x.d[i] + y.d[i]; }
+  Arr<T> conv_ux(const Arr<double>& u, const Arr<double>& v, const Arr<double>& t) {  // :488-522
+    Arr<double> c = a->conv_term(*ux1, u, 1, 0);
+    addto(c, a->conv_term(*ux1, v, 0, 1));
+    addto(c, a->conv_term(*ux1, u, 1, 0));
+    addto(c, a->conv_term(*uy1, v, 1, 0));
+    addto(c, a->conv_term(*temp1, t, 1, 0));
+    addto(c, a->conv_term(*temp1, a->fieldbc->v, 1, 0));
+    return a->finish_conv(c);
+  }
+  Arr<T> conv_uy(const Arr<double>& u, const Arr<double>& v, const Arr<double>& t) {  // :525-559
+    Arr<double> c = a->conv_term(*uy1, u, 1, 0);
+    addto(c, a->conv_term(*uy1, v, 0, 1));
+    addto(c, a->conv_term(*ux1, u, 0, 1));
+    addto(c, a->conv_term(*uy1, v, 0, 1));
+    addto(c, a->conv_term(*temp1, t, 0, 1));
+    addto(c, a->conv_term(*temp1, a->fieldbc->v, 0, 1));
+    return a->finish_conv(c);
+  }
+  Arr<T> conv_temp(const Arr<double>& u, const Arr<double>& v) {  // :562-578
+    Arr<double> c = a->conv_term(*temp1, u, 1, 0);
+    addto(c, a->conv_term(*temp1, v, 0, 1));
+    return a->finish_conv(c);
+  }
+  // solve_ux / solve_uy (:581-626): explicit, rhs = to_ortho(u) - dt dp + dt conv + dt nu lap(u1)
+  void solve_vel(int which, const Arr<double>& u, const Arr<double>& v, const Arr<double>& t) {
+    Field2<T>& f0 = which == 1 ? *a->ux : *a->uy;
+    Field2<T>& g1 = which == 1 ? *ux1 : *uy1;
+    Arr<T> rhs = f0.to_ortho();
+    Arr<T> gp = a->pres0->gradient(which == 1 ? 1 : 0, which == 1 ? 0 : 1, a->scale);
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] - mulr(gp.d[i], dt);
+    Arr<T> cv = which == 1 ? conv_ux(u, v, t) : conv_uy(u, v, t);
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(cv.d[i], dt);
+    Arr<T> lx = g1.gradient(2, 0, a->scale), ly = g1.gradient(0, 2, a->scale);
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(mulr(lx.d[i], dt), a->nu);
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(mulr(ly.d[i], dt), a->nu);
+    f0.from_ortho(rhs);
+  }
+  void solve_temp(const Arr<double>& u, const Arr<double>& v) {  // :629-648
+    Arr<T> rhs = a->temp->to_ortho();
+    Arr<T> cv = conv_temp(u, v);
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(cv.d[i], dt);
+    Arr<T> buoy = uy1->to_ortho();
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(buoy.d[i], dt);
+    Arr<T> lx = temp1->gradient(2, 0, a->scale), ly = temp1->gradient(0, 2, a->scale);
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(mulr(lx.d[i], dt), a->ka);
+    for (size_t i = 0; i < rhs.d.size(); ++i) rhs.d[i] = rhs.d[i] + mulr(mulr(ly.d[i], dt), a->ka);
+    a->temp->from_ortho(rhs);
+  }
+  void update_residual() {  // :727-750
+    Field2<T>* nf[3] = {navier->temp.get(), navier->ux.get(), navier->uy.get()};
+    Field2<T>* af[3] = {a->temp.get(), a->ux.get(), a->uy.get()};
+    for (int k = 0; k < 3; ++k) nf[k]->vhat = af[k]->vhat;
+    navier->update();
+    for (int k = 0; k < 3; ++k) {
+      Arr<T>& x = nf[k]->vhat;
+      for (size_t i = 0; i < x.d.size(); ++i) x.d[i] = divr(x.d[i] - af[k]->vhat.d[i], navier->dt);
+    }
+    // unsmoothed residuals (ux, uy, temp) and the smoothed ones, negated
+    const int order[3] = {1, 2, 0};  // fields_unsmoothed[0..3] = ux, uy, temp
+    for (int j = 0; j < 3; ++j) {
+      const int k = order[j];
+      unsm[j] = nf[k]->to_ortho();
+      Field2<T>& g = f1(k);
+      g.vhat = smoother[j]->solve<T>(unsm[j]);
+      for (auto& x : g.vhat.d) x = mulr(x, -1.0);
+    }
+  }
+  void update() {  // Integrate::update (:760-790)
+    a->ux->backward();
+    a->uy->backward();
+    a->temp->backward();
+    Arr<double> u = a->ux->v, v = a->uy->v, t = a->temp->v;
+    update_residual();
+    advance(u, v, t);
+  }
+  // the update after update_residual: explicit velocity and temperature updates from the
+  // physical fields u, v, t of the step start and the residuals [1], with the projection
+  void advance(const Arr<double>& u, const Arr<double>& v, const Arr<double>& t) {
+    solve_vel(1, u, v, t);
+    solve_vel(2, u, v, t);
+    Arr<T> div = a->divergence();
+    a->solve_pres(div);
+    a->project_velocity(1.0);
+    Arr<T> po = a->pres1->to_ortho();  // update_pres (:720-724): no viscous term
+    const double inv_dt = 1.0 / dt;
+    for (size_t i = 0; i < po.d.size(); ++i) a->pres0->vhat.d[i] = a->pres0->vhat.d[i] + mulr(po.d[i], inv_dt);
+    solve_temp(u, v);
+    time += dt;
+  }
+  static double norm(const Arr<T>& x) {  // norm_l2_f64 / norm_l2_c64 (:902-913)
+    double s = 0.0;
+    for (const auto& v : x.d) s += std::norm(std::complex<double>(v));
+    return std::sqrt(s);
+  }
+  // [|ux1|, |uy1|, |temp1|, |unsmoothed ux|, |uy|, |temp|] (callback / exit, :846-880)
+  void norms(double* out) {
+    out[0] = norm(ux1->vhat);
+    out[1] = norm(uy1->vhat);
+    out[2] = norm(temp1->vhat);
+    for (int j = 0; j < 3; ++j) out[3 + j] = norm(unsm[j]);
+  }
+  bool exit() {  // :882-897
+    Arr<T> div = a->divergence();
+    if (std::isnan(norm(div))) return true;
+    return norm(ux1->vhat) + norm(uy1->vhat) + norm(temp1->vhat) < res_tol;
+  }
+};
+
 struct Handle {
   bool periodic;
-  std::unique_ptr<Navier2D<double>> r;
-  std::unique_ptr<Navier2D<cd>> c;
+  std::shared_ptr<Navier2D<double>> r;
+  std::shared_ptr<Navier2D<cd>> c;
+};
+struct AHandle {
+  bool periodic;
+  std::unique_ptr<NavierAdjoint<double>> r;
+  std::unique_ptr<NavierAdjoint<cd>> c;
 };
 
 template <class T> Field2<T>* pick(Navier2D<T>& n, int which) {
@@ -1440,6 +1580,94 @@ int orc_navier_update(void* p, int nsteps) {
   Handle* h = (Handle*)p;
   GUARD({ for (int s = 0; s < nsteps; ++s) { if (h->periodic) h->c->update(); else h->r->update(); } })
 }
+// ------------------------------------------------------------ Navier2DAdjoint
+void* orc_adjoint_create(int nx, int ny, double ra, double pr, double dt, double aspect, int adiabatic, int periodic) {
+  try {
+    AHandle* h = new AHandle();
+    h->periodic = periodic != 0;
+    if (h->periodic) h->c.reset(new NavierAdjoint<cd>(nx, ny, ra, pr, dt, aspect, adiabatic != 0, true));
+    else h->r.reset(new NavierAdjoint<double>(nx, ny, ra, pr, dt, aspect, adiabatic != 0, false));
+    return h;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void orc_adjoint_destroy(void* p) { delete (AHandle*)p; }
+// a Navier2D handle sharing part 0 (the adjoint fields [0], their space, fieldbc and the
+// pressure solver) or part 1 (the Navier-Stokes step of the residual) of the adjoint
+void* orc_adjoint_part(void* p, int part) {
+  AHandle* a = (AHandle*)p;
+  Handle* h = new Handle();
+  h->periodic = a->periodic;
+  if (a->periodic) h->c = part ? a->c->navier : a->c->a;
+  else h->r = part ? a->r->navier : a->r->a;
+  return h;
+}
+int orc_adjoint_update(void* p, int nsteps) {
+  AHandle* h = (AHandle*)p;
+  GUARD({ for (int s = 0; s < nsteps; ++s) { if (h->periodic) h->c->update(); else h->r->update(); } })
+}
+double orc_adjoint_time(void* p) { AHandle* h = (AHandle*)p; return h->periodic ? h->c->time : h->r->time; }
+// the smoothed residual [1] of field which (0 temp, 1 ux, 2 uy), spectral
+int orc_adjoint_residual(void* p, int which, double* out) {
+  AHandle* h = (AHandle*)p;
+  GUARD({ if (h->periodic) copy_out(h->c->f1(which).vhat, out); else copy_out(h->r->f1(which).vhat, out); })
+}
+// the unsmoothed residual (fields_unsmoothed[j], j = 0 ux, 1 uy, 2 temp), ortho space
+int orc_adjoint_unsmoothed(void* p, int j, double* out) {
+  AHandle* h = (AHandle*)p;
+  GUARD({ if (h->periodic) copy_out(h->c->unsm[j], out); else copy_out(h->r->unsm[j], out); })
+}
+int orc_adjoint_set_residual(void* p, int which, const double* in) {
+  AHandle* h = (AHandle*)p;
+  GUARD({ if (h->periodic) copy_in(h->c->f1(which).vhat, in); else copy_in(h->r->f1(which).vhat, in); })
+}
+// the update without update_residual (the residuals [1] as they are), time += dt
+int orc_adjoint_advance(void* p) {
+  AHandle* h = (AHandle*)p;
+  GUARD({
+    auto run = [&](auto& ad) {
+      ad.a->ux->backward();
+      ad.a->uy->backward();
+      ad.a->temp->backward();
+      Arr<double> u = ad.a->ux->v, v = ad.a->uy->v, t = ad.a->temp->v;
+      ad.advance(u, v, t);
+    };
+    if (h->periodic) run(*h->c); else run(*h->r);
+  })
+}
+// the adjoint convective term of field which (0 temp, 1 ux, 2 uy) from the current fields
+// and residuals, dealiased, in the `field` space
+int orc_adjoint_conv(void* p, int which, double* out) {
+  AHandle* h = (AHandle*)p;
+  GUARD({
+    auto run = [&](auto& ad) {
+      ad.a->ux->backward();
+      ad.a->uy->backward();
+      ad.a->temp->backward();
+      Arr<double> u = ad.a->ux->v, v = ad.a->uy->v, t = ad.a->temp->v;
+      if (which == 0) copy_out(ad.conv_temp(u, v), out);
+      else if (which == 1) copy_out(ad.conv_ux(u, v, t), out);
+      else copy_out(ad.conv_uy(u, v, t), out);
+    };
+    if (h->periodic) run(*h->c); else run(*h->r);
+  })
+}
+int orc_adjoint_norms(void* p, double* out) {
+  AHandle* h = (AHandle*)p;
+  GUARD({ if (h->periodic) h->c->norms(out); else h->r->norms(out); })
+}
+int orc_adjoint_exit(void* p, int* out) {
+  AHandle* h = (AHandle*)p;
+  GUARD({ *out = (h->periodic ? h->c->exit() : h->r->exit()) ? 1 : 0; })
+}
+// smoother s (0 ux, 1 uy, 2 temp): its FdmaTensor lam / fwd / bwd (as orc_solver_tensor)
+int orc_adjoint_smoother(void* p, int s, void** out) {
+  AHandle* h = (AHandle*)p;
+  GUARD({ *out = h->periodic ? (void*)h->c->smoother[s].get() : (void*)h->r->smoother[s].get(); })
+}
+
 // The convective term of field `which` (0 temp, 1 ux, 2 uy) from the current state, as the
 // step forms it (navier.rs:538-616): ux and uy backward, then conv_temp / conv_ux / conv_uy,
 // dealiased, in the `field` space (spectral [nx, ny] real or [nx/2+1, ny] complex).

@@ -10,7 +10,8 @@ datasets the reference writes to its HDF5 file, under the same paths:
   hdf5-interface/src/lib.rs:195-232);
 * ``x``, ``dx``, ``y``, ``dy`` (coordinates and cell widths, field/write.rs:88-92);
 * scalars as one-element arrays (write_scalar_to_hdf5, hdf5-interface/src/lib.rs:257-268):
-  ``time``, ``ra``, ``pr``, ``nu``, ``kappa``.
+  ``time``, ``ra``, ``pr``, ``nu``, ``kappa``;
+* ``solid/mask`` when an obstacle is set (navier.rs:995-998).
 
 A converter to HDF5 is a one-to-one copy of these datasets.  Reading restores temp, ux,
 uy and pres[0] from their ``vhat`` (broadcast to the handle's shape when the file holds
@@ -63,12 +64,16 @@ def broadcast_2d(old, new):
     return new
 
 
-def coords(n, periodic, scale):
-    """Grid points of a base times the axis scale: Chebyshev -cos(pi j / (n - 1))
-    (funspace chebyshev/ortho.rs:71-80), Fourier 2 pi j / n (fourier/c2c.rs:63-66);
-    navier.rs:500-515 scales them."""
-    j = np.arange(n, dtype=float)
-    x = 2.0 * math.pi * j / n if periodic else -np.cos(math.pi * j / (n - 1.0))
+def coords(n, periodic, scale=1.0):
+    """Grid points of a base times the axis scale: Chebyshev x_i = -sin(pi (m - 2 i) / (2 m)),
+    m = n - 1 (funspace chebyshev/ortho.rs:71-80), Fourier 0 + (2 pi / n) i
+    (fourier/c2c.rs:63-66); navier.rs:500-515 scales them."""
+    i = np.arange(n, dtype=float)
+    if periodic:
+        x = 0.0 + (2.0 * math.pi / n) * i
+    else:
+        m = float(n - 1)
+        x = -np.sin(math.pi * (m - 2.0 * i) / (2.0 * m))
     return x * scale
 
 
@@ -100,9 +105,12 @@ def write_navier(nav, filename):
             v = v + nav.fieldbc.v
         data[g + "/v"] = v
         data[g + "/vhat"] = f.vhat
-    x = coords(nav.nx, nav.periodic, nav.scale[0])
-    y = coords(nav.ny, False, nav.scale[1])
-    data.update({"x": x, "dx": get_dx(x, nav.periodic), "y": y, "dy": get_dx(y, False)})
+    # field.rs:134-163 forms dx from the unscaled points; navier.rs:500-515 scales both
+    x0, y0 = coords(nav.nx, nav.periodic), coords(nav.ny, False)
+    data.update({"x": x0 * nav.scale[0], "dx": get_dx(x0, nav.periodic) * nav.scale[0],
+                 "y": y0 * nav.scale[1], "dy": get_dx(y0, False) * nav.scale[1]})
+    if getattr(nav, "solid", None) is not None:  # navier.rs:995-998
+        data["solid/mask"] = np.asarray(nav.solid[0], dtype=float)
     data.update({"time": nav.get_time(), "ra": nav.ra, "pr": nav.pr, "nu": nav.nu, "kappa": nav.ka})
     write_container(filename, data)
 

@@ -1741,6 +1741,9 @@ class Navier {
       return;
     }
     if (!value) throw std::invalid_argument("solid: value array required with the mask");
+    if (mixed)
+      throw std::invalid_argument("solid: the penalised step is the field-by-field FP64 step; switch RBC_OPT_MIXED "
+                                  "off first");
     const size_t np = (size_t)nx * ny;
     SOLID_M.upload(std::vector<double>(mask, mask + np));
     SOLID_V.upload(std::vector<double>(value, value + np));
@@ -2503,6 +2506,9 @@ void apply_option(Navier& n, int option, int value, bool alloc = true) {
     case RBC_OPT_BATCH: n.use_batch = value != 0; break;
     case RBC_OPT_FUSE: n.use_fuse = value != 0; break;
     case RBC_OPT_MIXED:
+      if (value && n.solid_on)
+        throw std::invalid_argument("RBC_OPT_MIXED: a handle with a solid obstacle runs the field-by-field FP64 step; "
+                                    "remove the obstacle first");
       n.mixed = value != 0;
       if (alloc && n.mixed && n.periodic && n.MIX.n < (11 * n.FS + 1) / 2) n.MIX.alloc((11 * n.FS + 1) / 2);
       if (alloc && n.mixed && !n.periodic) n.alloc_mixc();

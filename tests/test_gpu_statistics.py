@@ -59,23 +59,43 @@ def test_statistics_vs_oracle(oracle, periodic):
 def test_restart_continues(oracle, periodic, tmp_path):
     print("case restart periodic=%s" % periodic, flush=True)
     from rustpde_amd import Navier2D
-    a, _ = _pair(oracle, periodic)
+    a, o = _pair(oracle, periodic)
     b, _ = _pair(oracle, periodic)
     a.update(5)
+    o.update(10)  # the oracle runs the 10 steps without a restart
     f = str(tmp_path / "snap.npz")
+    temp_v = a.temp.v + a.fieldbc.v
+    ux_v = a.ux.v
     a.write(f)
     a.update(5)
     b.read(f)
     assert b.get_time() == pytest.approx(5 * a.get_dt(), abs=1e-15)
     b.update(5)
-    for fa, fb in ((a.temp, b.temp), (a.ux, b.ux), (a.uy, b.uy), (a.pres[0], b.pres[0])):
+    for fa, fb, name in ((a.temp, b.temp, "temp"), (a.ux, b.ux, "ux"), (a.uy, b.uy, "uy"),
+                         (a.pres[0], b.pres[0], "pres")):
         # the first step after a read forms d/dy pres with its own kernel (periodic), so
         # the continuation agrees to rounding rather than bit for bit
         assert rel_err(fb.vhat, fa.vhat) < 1e-13
+        assert rel_err(fb.vhat, o.get(name)) < 1e-12, name  # the restarted run against the oracle
     assert a.get_time() == b.get_time()
-    # temp/v of the snapshot carries the boundary field (navier.rs:987-990)
+    # the snapshot's datasets (navier.rs:983-1013, field/write.rs:82-115): temp/v carries the
+    # boundary field, v and vhat per group, coordinates and cell widths scaled by the aspect
     d = np.load(f)
     assert d["temp/v"].shape == (a.nx, a.ny) and d["time"].shape == (1,)
+    assert np.array_equal(d["temp/v"], temp_v) and np.array_equal(d["ux/v"], ux_v)
+    sx = a.scale[0]
+    if periodic:
+        assert np.allclose(d["x"], 2 * np.pi * np.arange(a.nx) / a.nx * sx, rtol=0, atol=1e-14)
+        assert np.allclose(d["dx"], 2 * np.pi / a.nx * sx, rtol=0, atol=1e-14)
+    else:
+        assert np.allclose(d["x"], -np.cos(np.pi * np.arange(a.nx) / (a.nx - 1)) * sx, rtol=0, atol=1e-14)
+        assert abs(d["dx"].sum() - 2.0 * sx) < 1e-13  # the cells partition [-sx, sx]
+        assert d["dx"][0] == pytest.approx((d["x"][1] - d["x"][0]) / 2, abs=1e-15)
+    assert np.allclose(d["y"], -np.cos(np.pi * np.arange(a.ny) / (a.ny - 1)), rtol=0, atol=1e-14)
+    assert abs(d["dy"].sum() - 2.0) < 1e-13
+    for k in ("time", "ra", "pr", "nu", "kappa"):
+        assert d[k].shape == (1,)
+    assert d["nu"][0] == a.nu and d["kappa"][0] == a.ka and "solid/mask" not in d
     # another resolution: the overlap of the spectral arrays is taken, the rest stays zero
     if periodic:
         c = Navier2D.new_periodic(160, 129, 1e6, 1.0, 0.004, 2.0)
