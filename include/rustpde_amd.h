@@ -206,6 +206,41 @@ int rbc_set_option(rbc_navier* h, int option, int value);
 int rbc_timing_enable(rbc_navier* h, int on);
 int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work);
 
+/* --------------------------------------------------------------- Navier2DAdjoint
+ * The adjoint descent for steady states (src/navier/navier_adjoint.rs:128-1068; Farazmand,
+ * J. Fluid Mech. 795, 2016), on the device.  Navier2DAdjoint::new / new_periodic
+ * (navier_adjoint.rs:187-289, 305-447): the adjoint step of size dt descends the residual
+ * (NS(q) - q) / dt_navier of a Navier2D step with dt_navier = 1e-2 (:231-232), smoothed by
+ * the Hholtz operator (1 - lap / scale^2) and negated (:727-750); velocity and temperature
+ * are updated explicitly (:581-648) and the velocity projected (:651-724).
+ * rbc_adjoint_part gives views owned by the adjoint: part 0 the adjoint fields (temp, ux, uy,
+ * pres[0], pres[1], fieldbc; .v / .vhat, forward / backward, set_velocity / set_temperature,
+ * rbc_eval diagnostics eval_nu / eval_nuvol / eval_re (:936-972), the tensors of its solvers:
+ * s = 0, 1, 2 the ux, uy, temp smoothers, 3 the pressure Poisson solver), part 1 the
+ * Navier2D step of the residual (its fields and pressure).  rbc_update and rbc_destroy refuse
+ * the views. */
+typedef struct rbc_adjoint rbc_adjoint;
+int rbc_adjoint_create_confined(int nx, int ny, double ra, double pr, double dt, double aspect, int adiabatic,
+                                rbc_adjoint** out);
+int rbc_adjoint_create_periodic(int nx, int ny, double ra, double pr, double dt, double aspect, rbc_adjoint** out);
+int rbc_adjoint_destroy(rbc_adjoint* h);
+int rbc_adjoint_part(rbc_adjoint* h, int part, rbc_navier** out);
+/* Integrate::update of the adjoint (navier_adjoint.rs:760-790), nsteps times. */
+int rbc_adjoint_update(rbc_adjoint* h, int nsteps);
+/* Integrate::get_time / get_dt; time is settable (reset_time, :987-991). */
+double rbc_adjoint_get_time(rbc_adjoint* h);
+double rbc_adjoint_get_dt(rbc_adjoint* h);
+int rbc_adjoint_set_time(rbc_adjoint* h, double time);
+/* Residual of field which (RBC_TEMP, RBC_UX, RBC_UY): kind 0 the smoothed residual temp[1] /
+ * ux[1] / uy[1] (spectral, the field's shape), kind 1 fields_unsmoothed (ortho space of the
+ * `field` space: [nx, ny] real or [nx/2 + 1, ny] complex). */
+int rbc_adjoint_residual(rbc_adjoint* h, int which, int kind, double* host_out);
+/* out[6] = |ux1|, |uy1|, |temp1|, |unsmoothed ux|, |uy|, |temp| (L2 of the coefficients, as
+ * callback prints them, :846-880). */
+int rbc_adjoint_norms(rbc_adjoint* h, double* out);
+/* Integrate::exit (:882-897): *out = 1 if |div| is NaN or |ux1| + |uy1| + |temp1| < 1e-8. */
+int rbc_adjoint_exit(rbc_adjoint* h, int* out);
+
 /* ------------------------------------------------------------- Space2 (funspace) */
 typedef struct fs_space2 fs_space2;
 /* Base kinds: funspace constructors chebyshev, cheb_dirichlet, cheb_neumann,

@@ -66,6 +66,17 @@ SIGNATURES = {
     "rbc_solver_tensor_info": (_i, [_vp, _ip, _ip]),
     "rbc_solver_get_tensor": (_i, [_vp, _dp, _dp, _dp]),
     "rbc_solver_set_tensor": (_i, [_vp, _dp, _dp, _dp]),
+    "rbc_adjoint_create_confined": (_i, [_i, _i, _d, _d, _d, _d, _i, ctypes.POINTER(_vp)]),
+    "rbc_adjoint_create_periodic": (_i, [_i, _i, _d, _d, _d, _d, ctypes.POINTER(_vp)]),
+    "rbc_adjoint_destroy": (_i, [_vp]),
+    "rbc_adjoint_part": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
+    "rbc_adjoint_update": (_i, [_vp, _i]),
+    "rbc_adjoint_get_time": (_d, [_vp]),
+    "rbc_adjoint_get_dt": (_d, [_vp]),
+    "rbc_adjoint_set_time": (_i, [_vp, _d]),
+    "rbc_adjoint_residual": (_i, [_vp, _i, _i, _dp]),
+    "rbc_adjoint_norms": (_i, [_vp, _dp]),
+    "rbc_adjoint_exit": (_i, [_vp, _ip]),
     "rbc_dev_malloc": (_i, [ctypes.POINTER(_vp), ctypes.c_ulonglong]),
     "rbc_dev_free": (_i, [_vp]),
     "rbc_memcpy_h2d": (_i, [_vp, _vp, ctypes.c_ulonglong]),

@@ -2197,6 +2197,173 @@ class Ensemble {
   }
 };
 
+// ------------------------------------------------------------------ adjoint descent
+// Navier2DAdjoint (src/navier/navier_adjoint.rs:128-1068): Farazmand's adjoint descent for
+// steady states.  `ns` is the Navier-Stokes step whose residual (NS(q) - q) / dt_navier is
+// descended (dt_navier = 1e-2, navier_adjoint.rs:231-232); `ad` is a Navier of the adjoint's
+// dt whose fields T, UX, UY, P0, P1 are the adjoint fields [0] and whose transforms, work
+// arrays, fieldbc, pressure solver (Poisson, c = 1 / scale^2, the adjoint's) and diagnostics
+// serve the adjoint; its three other solver slots hold the Hholtz smoothers (1 - c lap),
+// c = 1 / scale^2 (:247-268).  Both share one stream, and every array stays on the device.
+class Adjoint {
+ public:
+  std::unique_ptr<Navier> ad, ns;
+  DBuf F1[3];        // smoothed residuals [1] of temp, ux, uy (spectral, the fields' layout)
+  DBuf UNS[3];       // unsmoothed residuals (fields_unsmoothed: ux, uy, temp; ortho space)
+  DBuf U, V, TP;     // physical ux, uy, temp of the step start
+  DBuf B, LXX, LYY;  // buoyancy and the residuals' second derivatives (ortho space)
+  double time = 0.0, dt = 0.0, res_tol = 1e-8;  // RES_TOL (:114)
+
+  Adjoint(bool per, int nx, int ny, double ra, double pr, double dt_, double aspect, bool adiabatic) : dt(dt_) {
+    ad = std::make_unique<Navier>(per, nx, ny, ra, pr, dt_, aspect, adiabatic);
+    ns = std::make_unique<Navier>(per, nx, ny, ra, pr, 1e-2, aspect, adiabatic, nullptr, 0, ad->st);
+    Navier& n = *ad;
+    const double w = 1e0;  // weight_laplacian
+    const double cs[2] = {w / std::pow(n.scale[0], 2.0), w / std::pow(n.scale[1], 2.0)};
+    n.solver[0] = std::shared_ptr<Solver>(make_solver(HHOLTZ, n.bx_u, n.by_u, cs));
+    n.solver[1] = std::shared_ptr<Solver>(make_solver(HHOLTZ, n.bx_u, n.by_u, cs));
+    n.solver[2] = std::shared_ptr<Solver>(make_solver(HHOLTZ, n.bx_t, n.by_u, cs));
+    for (DBuf* b : {&F1[0], &F1[1], &F1[2], &UNS[0], &UNS[1], &UNS[2], &U, &V, &TP, &B, &LXX, &LYY}) b->alloc(n.FS);
+  }
+  ~Adjoint() {
+    if (ad) (void)hipStreamSynchronize(ad->st);
+    ns.reset();  // shares ad's stream
+  }
+  long spec_len(const Basis& bx) const { return (long)ad->rows_spec(bx) * ad->by_u.m; }
+  long ortho_len() const { return (long)ad->rows_ortho(ad->bx_f) * ad->ny; }
+  // CONV = M .* field.backward(gradient(F, [d0, d1])) + beta CONV  (conv_term.rs:22-42)
+  void conv_acc(const Basis& bx, const double* F, int d0, int d1, const double* M, double beta) {
+    Navier& n = *ad;
+    const double* a = n.xop(bx, d0 ? "DS" : "S", F, n.by_u.m, n.W1.p, n.W9.p);
+    const double* g = n.yop(n.by_u, d1 ? "DS" : "S", a, n.rows_ortho(bx), n.W2.p);
+    const double* b = n.xop(n.bx_f, "BS", g, n.ny, n.W3.p, n.W9.p);
+    n.yop(n.by_p, "BS", b, n.nx, n.CONV.p, M, beta);
+  }
+  // lap parts of a residual with x basis bx: LXX = d2/dx2, LYY = d2/dy2 (ortho space)
+  void laplace_parts(const Basis& bx, const double* F) {
+    Navier& n = *ad;
+    const int ro = n.rows_ortho(bx);
+    const double* a = n.xop(bx, "DDS", F, n.by_u.m, n.W1.p, n.W9.p);
+    const double* r = n.yop(n.by_u, "S", a, ro, LXX.p);
+    if (r != LXX.p) HIPCHECK(hipMemcpyAsync(LXX.p, r, sizeof(double) * ortho_len(), hipMemcpyDeviceToDevice, n.st));
+    a = n.xop(bx, "S", F, n.by_u.m, n.W1.p, n.W9.p);
+    r = n.yop(n.by_u, "DDS", a, ro, LYY.p);
+    if (r != LYY.p) HIPCHECK(hipMemcpyAsync(LYY.p, r, sizeof(double) * ortho_len(), hipMemcpyDeviceToDevice, n.st));
+  }
+  // q = from_ortho(RHS) in the basis (bx, cheb_dirichlet)
+  void from_ortho_into(const Basis& bx, double* q) {
+    Navier& n = *ad;
+    const double* a = n.xop(bx, "FO", n.RHS.p, n.ny, n.W1.p, n.W9.p);
+    const double* r = n.yop(n.by_u, "FO", a, n.rows_spec(bx), q);
+    if (r != q) HIPCHECK(hipMemcpyAsync(q, r, sizeof(double) * spec_len(bx), hipMemcpyDeviceToDevice, n.st));
+  }
+  void update_residual() {  // navier_adjoint.rs:727-750
+    Navier &n = *ad, &s = *ns;
+    const long lu = spec_len(n.bx_u), lt = spec_len(n.bx_t);
+    HIPCHECK(hipMemcpyAsync(s.UX.p, n.UX.p, sizeof(double) * lu, hipMemcpyDeviceToDevice, n.st));
+    HIPCHECK(hipMemcpyAsync(s.UY.p, n.UY.p, sizeof(double) * lu, hipMemcpyDeviceToDevice, n.st));
+    HIPCHECK(hipMemcpyAsync(s.T.p, n.T.p, sizeof(double) * lt, hipMemcpyDeviceToDevice, n.st));
+    s.update_n(1);
+    ew_residual(n.st, s.UX.p, s.UX.p, n.UX.p, s.dt, lu);
+    ew_residual(n.st, s.UY.p, s.UY.p, n.UY.p, s.dt, lu);
+    ew_residual(n.st, s.T.p, s.T.p, n.T.p, s.dt, lt);
+    s.to_ortho(s.bx_u, s.by_u, s.UX.p, UNS[0].p);
+    s.to_ortho(s.bx_u, s.by_u, s.UY.p, UNS[1].p);
+    s.to_ortho(s.bx_t, s.by_u, s.T.p, UNS[2].p);
+    // smoothed and negated: ux1, uy1, temp1 (F1[1], F1[2], F1[0])
+    n.solve(0, UNS[0].p, F1[1].p);
+    n.solve(1, UNS[1].p, F1[2].p);
+    n.solve(2, UNS[2].p, F1[0].p);
+    ew_scale(n.st, F1[1].p, F1[1].p, -1.0, lu);
+    ew_scale(n.st, F1[2].p, F1[2].p, -1.0, lu);
+    ew_scale(n.st, F1[0].p, F1[0].p, -1.0, lt);
+  }
+  // solve_ux / solve_uy (navier_adjoint.rs:581-626)
+  void solve_vel(int which) {
+    Navier& n = *ad;
+    double* q = which ? n.UY.p : n.UX.p;
+    const double* f = which ? F1[2].p : F1[1].p;
+    n.to_ortho(n.bx_u, n.by_u, q, n.A.p);
+    const double* gp = which ? n.yop(n.by_p, "DS", n.P0.p, n.rows_ortho(n.bx_p), n.G.p)
+                             : n.xop(n.bx_p, "DS", n.P0.p, n.ny, n.G.p, n.W9.p);
+    if (!which) {  // + ux dux1/dx + uy dux1/dy + ux dux1/dx + uy duy1/dx + T dT1/dx + T_bc dT1/dx
+      conv_acc(n.bx_u, F1[1].p, 1, 0, U.p, 0.0);
+      conv_acc(n.bx_u, F1[1].p, 0, 1, V.p, 1.0);
+      conv_acc(n.bx_u, F1[1].p, 1, 0, U.p, 1.0);
+      conv_acc(n.bx_u, F1[2].p, 1, 0, V.p, 1.0);
+      conv_acc(n.bx_t, F1[0].p, 1, 0, TP.p, 1.0);
+      conv_acc(n.bx_t, F1[0].p, 1, 0, n.BCv.p, 1.0);
+    } else {  // + ux duy1/dx + uy duy1/dy + ux dux1/dy + uy duy1/dy + T dT1/dy + T_bc dT1/dy
+      conv_acc(n.bx_u, F1[2].p, 1, 0, U.p, 0.0);
+      conv_acc(n.bx_u, F1[2].p, 0, 1, V.p, 1.0);
+      conv_acc(n.bx_u, F1[1].p, 0, 1, U.p, 1.0);
+      conv_acc(n.bx_u, F1[2].p, 0, 1, V.p, 1.0);
+      conv_acc(n.bx_t, F1[0].p, 0, 1, TP.p, 1.0);
+      conv_acc(n.bx_t, F1[0].p, 0, 1, n.BCv.p, 1.0);
+    }
+    n.forward_dealias_conv();
+    laplace_parts(n.bx_u, f);
+    ew_adj_rhs(n.st, n.RHS.p, n.A.p, gp, n.CV.p, nullptr, LXX.p, LYY.p, dt, n.nu, ortho_len());
+    from_ortho_into(n.bx_u, q);
+  }
+  void solve_temp() {  // navier_adjoint.rs:629-648
+    Navier& n = *ad;
+    n.to_ortho(n.bx_t, n.by_u, n.T.p, n.A.p);
+    conv_acc(n.bx_t, F1[0].p, 1, 0, U.p, 0.0);
+    conv_acc(n.bx_t, F1[0].p, 0, 1, V.p, 1.0);
+    n.forward_dealias_conv();
+    n.to_ortho(n.bx_u, n.by_u, F1[2].p, B.p);  // adjoint buoyancy: uy1
+    laplace_parts(n.bx_t, F1[0].p);
+    ew_adj_rhs(n.st, n.RHS.p, n.A.p, nullptr, n.CV.p, B.p, LXX.p, LYY.p, dt, n.ka, ortho_len());
+    from_ortho_into(n.bx_t, n.T.p);
+  }
+  void update() {  // Integrate::update (navier_adjoint.rs:760-790)
+    Navier& n = *ad;
+    n.backward_vel(n.UX.p, U.p);
+    n.backward_vel(n.UY.p, V.p);
+    {
+      const double* a = n.xop(n.bx_t, "BS", n.T.p, n.by_u.m, n.W1.p, n.W9.p);
+      const double* r = n.yop(n.by_u, "BS", a, n.nx, TP.p);
+      if (r != TP.p) HIPCHECK(hipMemcpyAsync(TP.p, r, sizeof(double) * n.nx * n.ny, hipMemcpyDeviceToDevice, n.st));
+    }
+    update_residual();
+    solve_vel(0);
+    solve_vel(1);
+    n.divergence(n.DIV.p);
+    n.solve_pres();
+    n.project_velocity();
+    // update_pres (:720-724): pres += to_ortho(pseu) / dt, no viscous term
+    n.to_ortho(n.bx_phi, n.by_phi, n.P1.p, n.W3.p);
+    ew_axpy(n.st, n.P0.p, n.P0.p, n.W3.p, 1.0 / dt, (long)n.rows_ortho(n.bx_p) * n.ny);
+    solve_temp();
+    n.gpres_valid = false;
+    time += dt;
+  }
+  static double norm(const std::vector<double>& v) {  // norm_l2_f64 / norm_l2_c64 (:902-913)
+    double s = 0.0;
+    for (double x : v) s += x * x;
+    return std::sqrt(s);
+  }
+  // [|ux1|, |uy1|, |temp1|, |unsmoothed ux|, |uy|, |temp|] (callback / exit, :846-897)
+  void norms(double* out) {
+    Navier& n = *ad;
+    HIPCHECK(hipStreamSynchronize(n.st));
+    out[0] = norm(F1[1].download(spec_len(n.bx_u)));
+    out[1] = norm(F1[2].download(spec_len(n.bx_u)));
+    out[2] = norm(F1[0].download(spec_len(n.bx_t)));
+    for (int j = 0; j < 3; ++j) out[3 + j] = norm(UNS[j].download(ortho_len()));
+  }
+  bool exit() {  // :882-897
+    Navier& n = *ad;
+    n.divergence(n.DIV.p);
+    HIPCHECK(hipStreamSynchronize(n.st));
+    if (std::isnan(norm(n.DIV.download(ortho_len())))) return true;
+    double r[6];
+    norms(r);
+    return r[0] + r[1] + r[2] < res_tol;
+  }
+};
+
 }  // namespace rbc
 
 // =================================================================== C ABI
@@ -2217,6 +2384,10 @@ struct fs_space2 {
 };
 struct rbc_solver {
   std::unique_ptr<rbc::SolverDev> s;
+};
+struct rbc_adjoint {
+  std::unique_ptr<rbc::Adjoint> a;
+  rbc_navier parts[2];  // views of the adjoint fields and of the Navier-Stokes step (owned here)
 };
 
 static thread_local std::string g_err;
@@ -2282,7 +2453,7 @@ int rbc_create_periodic(int nx, int ny, double ra, double pr, double dt, double 
 }
 int rbc_destroy(rbc_navier* h) {
   return guarded([&] {
-    if (h && h->member) throw std::invalid_argument("an ensemble instance is destroyed with its ensemble");
+    if (h && h->member) throw std::invalid_argument("an ensemble instance or adjoint part is destroyed with its owner");
     delete h;
   });
 }
@@ -2349,7 +2520,7 @@ int rbc_set_temperature(rbc_navier* h, double amp, double m, double n) {
 int rbc_update(rbc_navier* h, int nsteps) {
   return guarded([&] {
     NEED(h);
-    if (h->member) throw std::invalid_argument("ensemble instances advance with rbc_ensemble_update");
+    if (h->member) throw std::invalid_argument("ensemble instances and adjoint parts advance with their owner");
     if (nsteps < 0) throw std::invalid_argument("nsteps < 0");
     h->n->update_n(nsteps);
   });
@@ -2624,6 +2795,81 @@ int rbc_solver_solve(rbc_solver* h, const double* d_rhs, double* d_out, int batc
   return guarded([&] {
     if (!h || !h->s || !d_rhs || !d_out || batch < 1) throw std::invalid_argument("bad argument");
     h->s->solve(d_rhs, d_out, batch);
+  });
+}
+static int adjoint_create(bool per, int nx, int ny, double ra, double pr, double dt, double aspect, int adiabatic,
+                          rbc_adjoint** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null out");
+    auto h = std::make_unique<rbc_adjoint>();
+    h->a = std::make_unique<rbc::Adjoint>(per, nx, ny, ra, pr, dt, aspect, adiabatic != 0);
+    h->parts[0].n = h->a->ad.get();
+    h->parts[1].n = h->a->ns.get();
+    h->parts[0].member = h->parts[1].member = true;
+    *out = h.release();
+  });
+}
+int rbc_adjoint_create_confined(int nx, int ny, double ra, double pr, double dt, double aspect, int adiabatic,
+                                rbc_adjoint** out) {
+  return adjoint_create(false, nx, ny, ra, pr, dt, aspect, adiabatic, out);
+}
+int rbc_adjoint_create_periodic(int nx, int ny, double ra, double pr, double dt, double aspect, rbc_adjoint** out) {
+  return adjoint_create(true, nx, ny, ra, pr, dt, aspect, 1, out);
+}
+int rbc_adjoint_destroy(rbc_adjoint* h) {
+  return guarded([&] { delete h; });
+}
+int rbc_adjoint_part(rbc_adjoint* h, int part, rbc_navier** out) {
+  return guarded([&] {
+    if (!h || !h->a || !out || part < 0 || part > 1) throw std::invalid_argument("bad argument");
+    *out = &h->parts[part];
+  });
+}
+int rbc_adjoint_update(rbc_adjoint* h, int nsteps) {
+  return guarded([&] {
+    if (!h || !h->a || nsteps < 0) throw std::invalid_argument("bad argument");
+    for (int k = 0; k < nsteps; ++k) h->a->update();
+  });
+}
+double rbc_adjoint_get_time(rbc_adjoint* h) { return h && h->a ? h->a->time : 0.0; }
+double rbc_adjoint_get_dt(rbc_adjoint* h) { return h && h->a ? h->a->dt : 0.0; }
+int rbc_adjoint_set_time(rbc_adjoint* h, double time) {
+  return guarded([&] {
+    if (!h || !h->a) throw std::invalid_argument("null handle");
+    h->a->time = time;
+  });
+}
+int rbc_adjoint_residual(rbc_adjoint* h, int which, int kind, double* host_out) {
+  return guarded([&] {
+    if (!h || !h->a || !host_out || which < 0 || which > 2 || kind < 0 || kind > 1)
+      throw std::invalid_argument("bad argument");
+    rbc::Adjoint& A = *h->a;
+    Navier& n = *A.ad;
+    HIPCHECK(hipStreamSynchronize(n.st));
+    // kind 0: the smoothed residual [1] in the field's spectral layout; kind 1: the
+    // unsmoothed one in the ortho space of the `field` space (fields_unsmoothed: ux, uy, temp)
+    const rbc::Basis& bx = kind == 0 ? (which == RBC_TEMP ? n.bx_t : n.bx_u) : n.bx_f;
+    const int r = kind == 0 ? n.rows_spec(bx) : n.rows_ortho(bx), c = kind == 0 ? n.by_u.m : n.ny;
+    const int j = which == RBC_TEMP ? 2 : which - 1;
+    std::vector<double> v = (kind == 0 ? A.F1[which] : A.UNS[j]).download((size_t)r * c);
+    if (!n.periodic) {
+      std::memcpy(host_out, v.data(), v.size() * sizeof(double));
+      return;
+    }
+    const size_t pl = v.size() / 2;  // split planes -> interleaved complex
+    for (size_t i = 0; i < pl; ++i) { host_out[2 * i] = v[i]; host_out[2 * i + 1] = v[pl + i]; }
+  });
+}
+int rbc_adjoint_norms(rbc_adjoint* h, double* out) {
+  return guarded([&] {
+    if (!h || !h->a || !out) throw std::invalid_argument("bad argument");
+    h->a->norms(out);
+  });
+}
+int rbc_adjoint_exit(rbc_adjoint* h, int* out) {
+  return guarded([&] {
+    if (!h || !h->a || !out) throw std::invalid_argument("bad argument");
+    *out = h->a->exit() ? 1 : 0;
   });
 }
 int rbc_solver_solve_noalloc(rbc_solver* h, const double* d_rhs, double* d_out, int batch) {

@@ -186,6 +186,22 @@ EW_KERNEL(ew_rhs_temp, (double* out, const double* a, const double* bx, const do
           out[i] = ((a[i] + bx[i]) + by[i]) - c[i] * dt)
 EW_KERNEL(ew_acc_prod, (double* out, const double* a, const double* b, long n), out[i] = out[i] + a[i] * b[i])
 EW_KERNEL(ew_project, (double* u, const double* f, double c, long n), u[i] = f[i] * (-c) + u[i])
+// the explicit right-hand sides of the adjoint descent (navier_adjoint.rs:581-648):
+// ((((a - g dt) + c dt) + b dt) + (lx dt) k) + (ly dt) k, g and b optional
+EW_KERNEL(ew_adj_rhs, (double* out, const double* a, const double* g, const double* c, const double* b,
+                       const double* lx, const double* ly, double dt, double k, long n),
+          {
+            double r = a[i];
+            if (g) r = r - g[i] * dt;
+            r = r + c[i] * dt;
+            if (b) r = r + b[i] * dt;
+            r = r + (lx[i] * dt) * k;
+            out[i] = r + (ly[i] * dt) * k;
+          })
+// (a - b) / dt (navier_adjoint.rs:731-737)
+EW_KERNEL(ew_residual, (double* out, const double* a, const double* b, double dt, long n), out[i] = (a[i] - b[i]) / dt)
+// a + b f
+EW_KERNEL(ew_axpy, (double* out, const double* a, const double* b, double f, long n), out[i] = a[i] + b[i] * f)
 EW_KERNEL(ew_acc_prod2, (double* out, const double* a, const double* b, const double* c, const double* d, long n),
           {
             const double t = out[i] + a[i] * b[i];
@@ -353,6 +369,16 @@ void ew_acc_prod(hipStream_t s, double* out, const double* a, const double* b, l
   EW_LAUNCH(ew_acc_prod, n, out, a, b);
 }
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n) { EW_LAUNCH(ew_project, n, u, f, c); }
+void ew_adj_rhs(hipStream_t s, double* out, const double* a, const double* g, const double* c, const double* b,
+                const double* lx, const double* ly, double dt, double k, long n) {
+  EW_LAUNCH(ew_adj_rhs, n, out, a, g, c, b, lx, ly, dt, k);
+}
+void ew_residual(hipStream_t s, double* out, const double* a, const double* b, double dt, long n) {
+  EW_LAUNCH(ew_residual, n, out, a, b, dt);
+}
+void ew_axpy(hipStream_t s, double* out, const double* a, const double* b, double f, long n) {
+  EW_LAUNCH(ew_axpy, n, out, a, b, f);
+}
 void ew_acc_prod2(hipStream_t s, double* out, const double* a, const double* b, const double* c, const double* d,
                   long n) {
   EW_LAUNCH(ew_acc_prod2, n, out, a, b, c, d);

@@ -153,6 +153,11 @@ void rhs3_fused_cv32(hipStream_t s, const double* U, int my, const double* stenc
                      const double* thbc, const float* cv, const double* gxx, const double* gyy, double* out, int mx,
                      int ny, double kscale, double dt, long F);
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n);              // f (-c) + u
+// ((((a - g dt) + c dt) + b dt) + (lx dt) k) + (ly dt) k; g, b may be null
+void ew_adj_rhs(hipStream_t s, double* out, const double* a, const double* g, const double* c, const double* b,
+                const double* lx, const double* ly, double dt, double k, long n);
+void ew_residual(hipStream_t s, double* out, const double* a, const double* b, double dt, long n);  // (a - b) / dt
+void ew_axpy(hipStream_t s, double* out, const double* a, const double* b, double f, long n);       // a + b f
 void ew_update_pres(hipStream_t s, double* p, const double* div, const double* phi, double nu, double inv_dt,
                     long n);  // (p - div nu) + phi inv_dt
 void ew_scale(hipStream_t s, double* out, const double* a, double f, long n);              // a f

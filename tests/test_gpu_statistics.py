@@ -64,6 +64,8 @@ def test_restart_continues(oracle, periodic, tmp_path):
     a.update(5)
     o.update(10)  # the oracle runs the 10 steps without a restart
     f = str(tmp_path / "snap.npz")
+    a.temp.backward()
+    a.ux.backward()
     temp_v = a.temp.v + a.fieldbc.v
     ux_v = a.ux.v
     a.write(f)
