@@ -300,6 +300,20 @@ def bench_hholtz_batch(args):
            "roofline": {"bound": "mfma", "achieved": flops / el / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                         "frac": flops / el / 1e12 / PEAK_FP64_TFLOPS, "traffic": None,
                         "kernel": "whole solve (eigen-GEMMs + rows + banded), GEMM flops only"}}
+    # the mixed form: FP32 eigen-transforms with one FP64-residual refinement (two chained
+    # solves), and the residual of both forms' solutions (first 8 right-hand sides)
+    out["fp64_residual"] = g.residual(f[:8], _download(L, dout, (8, n - 2, n - 2)))
+    g.set_mixed(2)
+    g.solve_device(din.p, dout.p, nb)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.solve_device(din.p, dout.p, nb)
+    el32 = (time.perf_counter() - t0) / reps
+    out["mixed_refined"] = {"rhs_per_s": nb / el32, "ms_per_batch": 1e3 * el32, "speedup_vs_fp64": el / el32,
+                            "residual": g.residual(f[:8], _download(L, dout, (8, n - 2, n - 2))),
+                            "note": "fwd / bwd eigen-transforms as FP32 matrix-core GEMMs, FP64 row solves, one "
+                                    "refinement step against the FP64 residual of the tensor equation (2 solves)"}
+    g.set_mixed(0)
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -314,6 +328,15 @@ def bench_hholtz_batch(args):
         out["cpu_baseline"] = {"value": cnt / (time.perf_counter() - t0), "unit": "rhs/s", "cores": threads,
                                "kind": "port", "sample": "%d right-hand sides of the same batch" % cnt}
     print(json.dumps(out), flush=True)
+
+
+def _download(L, dev, shape):
+    import numpy as np
+
+    from rustpde_amd import _lib
+    out = np.zeros(shape)
+    _lib.check(L.rbc_memcpy_d2h(out.ctypes.data, dev.p, out.nbytes))
+    return out
 
 
 def c4_descs():

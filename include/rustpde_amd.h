@@ -280,6 +280,16 @@ int rbc_solver_shape(rbc_solver* h, int* o0, int* o1, int* s0, int* s1, int* is_
 /* Solve::solve (hholtz.rs:159-197, hholtz_adi.rs:106-131, poisson.rs:122-150) of `batch`
  * contiguous right-hand sides on DEVICE buffers (reference layouts).  Synchronous. */
 int rbc_solver_solve(rbc_solver* h, const double* d_rhs, double* d_out, int batch);
+/* Mixed precision for solvers whose axis 0 is Chebyshev (Hholtz, Poisson; no counterpart in
+ * the reference, which solves in FP64): iters > 0 runs the eigen-transforms fwd = Q^-1 C_x^-1
+ * and bwd = Q (fdma_tensor.rs:195-234) as FP32 matrix-core GEMMs with FP64 row solves, and
+ * chains iters such solves as iterative refinement against the FP64 residual of the tensor
+ * equation (A_x (x) C_y + C_x (x) (A_y + alpha C_y)) u = P_x f P_y^T (fdma_tensor.rs:16-70,
+ * all matrices banded); iters = 0 restores the FP64 solve.  rbc_solver_residual gives the
+ * max-norm of that residual for solutions d_sol of right-hand sides d_rhs, relative to the
+ * max-norm of P_x f P_y^T (synchronous). */
+int rbc_solver_set_mixed(rbc_solver* h, int iters);
+int rbc_solver_residual(rbc_solver* h, const double* d_rhs, const double* d_sol, int batch, double* out);
 /* rbc_solver_solve without any device allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code that must not call hipMalloc, such as
  * tight serving loops or harnesses that forbid it).  The work arrays live in a fixed
@@ -288,7 +298,8 @@ int rbc_solver_solve(rbc_solver* h, const double* d_rhs, double* d_out, int batc
  * row counts 2 o0, 2 s0 of complex arrays), I = batch * o0' * o1, O = batch * s0' * s1, each
  * rounded up to a multiple of 32; a call that needs more returns RBC_ERR_ARG.  Results are
  * those of rbc_solver_solve, bit for bit.  Calls on one GPU share its scratch: calls that
- * overlap on one GPU (e.g. from two threads) race.  Synchronous. */
+ * overlap on one GPU (e.g. from two threads) race.  Synchronous.  Refused (RBC_ERR_ARG) while
+ * the mixed form (rbc_solver_set_mixed) is on. */
 #define RBC_NOALLOC_SCRATCH_BYTES (32ull << 20)
 int rbc_solver_solve_noalloc(rbc_solver* h, const double* d_rhs, double* d_out, int batch);
 /* the pub FdmaTensor fields lam/fwd/bwd of a Hholtz/Poisson solver (fdma_tensor.rs:73-91) */

@@ -63,6 +63,8 @@ SIGNATURES = {
     "rbc_solver_shape": (_i, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "rbc_solver_solve": (_i, [_vp, _vp, _vp, _i]),
     "rbc_solver_solve_noalloc": (_i, [_vp, _vp, _vp, _i]),
+    "rbc_solver_set_mixed": (_i, [_vp, _i]),
+    "rbc_solver_residual": (_i, [_vp, _vp, _vp, _i, _dp]),
     "rbc_solver_tensor_info": (_i, [_vp, _ip, _ip]),
     "rbc_solver_get_tensor": (_i, [_vp, _dp, _dp, _dp]),
     "rbc_solver_set_tensor": (_i, [_vp, _dp, _dp, _dp]),

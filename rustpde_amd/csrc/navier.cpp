@@ -273,6 +273,7 @@ struct Solver {
   bool has_fb = false;
   std::vector<double> lam, fwd, bwd;  // fwd/bwd m x m (row-major) when has_fb
   std::vector<double> px;             // P_x (m x n) in double, for composing fwd * P_x
+  std::vector<double> ax, cx;         // A_x, C_x (m x m) when axis 0 is not diagonal
   DOp pf, pb, py;
   DBand pyb;  // P_y as a banded matvec
   std::vector<double> pyb_taps;  // its taps on the host, [4][m]
@@ -364,6 +365,9 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
     LMat C(m0, m0), Am(m0, m0);
     for (size_t i = 0; i < C.a.size(); ++i) C.a[i] = g0.a.a[i];
     for (size_t i = 0; i < Am.a.size(); ++i) Am.a[i] = A0[i];
+    s->ax = A0;
+    s->cx.resize(C.a.size());
+    for (size_t i = 0; i < C.a.size(); ++i) s->cx[i] = (double)g0.a.a[i];
     LMat Ci = inverse(C);
     std::vector<double> xmat = (Ci * Am).to_double();
     std::vector<double> q, qi;
@@ -2004,6 +2008,10 @@ class SolverDev {
   size_t in_size() const { return (size_t)rows_in() * b1.ortho(); }
   size_t out_size() const { return (size_t)rows_out() * b1.m; }
   void solve(const double* rhs, double* out, int batch) {
+    if (mixed_iters > 0) {
+      solve_mixed(rhs, out, batch);
+      return;
+    }
     ensure(W1, big() * batch);
     ensure(W2, big() * batch);
     if (cplx) {
@@ -2011,6 +2019,112 @@ class SolverDev {
       ensure(OUT, out_size() * batch);
     }
     solve_ws(rhs, out, batch, Work{W1.p, W2.p, IN.p, OUT.p});
+  }
+  // ---- mixed precision (a Chebyshev axis 0: Hholtz, Poisson).  The eigen-transforms
+  // fwd = Q^-1 C_x^-1 and bwd = Q run as FP32 matrix-core GEMMs (FP64 in, FP64 out, FP32
+  // products), the per-row solves stay FP64, and `mixed_iters` such solves are chained as
+  // iterative refinement against the FP64 residual of the tensor equation
+  //     (A_x (x) C_y + C_x (x) (A_y + alpha C_y)) u = P_x f P_y^T    (fdma_tensor.rs:16-70),
+  // all of whose matrices are banded (so the residual is O(n^2) band passes).
+  int mixed_iters = 0;
+  bool mixed_ready = false;
+  DBuf F32, B32;                   // fwd, bwd as FP32 (m0 x m0)
+  DBand pxb, axb, cxb, cyb, aycb;  // P_x; A_x, C_x; C_y and A_y + alpha C_y
+  DBuf MR, MX, MV, MT, ML;         // P f P^T, transform buffer, solution, band temporaries
+  size_t mixed_batch = 0;
+  static void band_from(const std::vector<double>& a, int r, int c, DBand& b) {
+    LMat m(r, c);
+    for (size_t i = 0; i < m.a.size(); ++i) m.a[i] = a[i];
+    make_matvec_band(m, b);
+  }
+  static std::vector<double> dense_of(const Band4& f) {
+    const int n = f.n;
+    std::vector<double> a((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) a[(size_t)i * n + i] = f.dia[i];
+    for (int i = 0; i + 2 < n; ++i) a[(size_t)(i + 2) * n + i] = f.low[i];
+    for (int i = 0; i + 2 < n; ++i) a[(size_t)i * n + i + 2] = f.up1[i];
+    for (int i = 0; i + 4 < n; ++i) a[(size_t)i * n + i + 4] = f.up2[i];
+    return a;
+  }
+  void set_mixed(int iters) {
+    if (iters < 0 || iters > 8) throw std::invalid_argument("mixed iterations must be in [0, 8]");
+    if (iters > 0 && (s->kind == ADI || !s->has_fb))
+      throw std::invalid_argument("the mixed form needs eigen-transforms (Hholtz / Poisson with a Chebyshev axis 0)");
+    mixed_iters = iters;
+    if (!iters || mixed_ready) return;
+    Solver& sv = *s;
+    const int m0 = b0.m, m1 = b1.m;
+    if (!fdma_rows_wave_supported(m1)) throw std::invalid_argument("mixed form: row length not supported");
+    std::vector<float> f(sv.fwd.begin(), sv.fwd.end()), b(sv.bwd.begin(), sv.bwd.end());
+    F32.alloc((f.size() + 1) / 2);
+    B32.alloc((b.size() + 1) / 2);
+    HIPCHECK(hipMemcpy(F32.p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(B32.p, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
+    band_from(sv.px, m0, b0.ortho(), pxb);
+    band_from(sv.ax, m0, m0, axb);
+    band_from(sv.cx, m0, m0, cxb);
+    band_from(dense_of(sv.fc), m1, m1, cyb);
+    band_from(dense_of(sv.fa.add_scaled(sv.fc, sv.alpha)), m1, m1, aycb);
+    mixed_ready = true;
+  }
+  // out = R - L(V) for `batch` entries [m0, m1] (L the tensor operator above)
+  void mixed_residual(const double* R, const double* V, double* out, int batch) {
+    const int m0 = b0.m, m1 = b1.m;
+    const long e = (long)m0 * m1;
+    band_x(st, V, MT.p, m1, m0, axb.coef.p, axb.mask.p, 0.0, batch, e, e);
+    band_y(st, MT.p, m1, ML.p, m1, m0, m1, cyb.coef.p, cyb.mask.p, 0.0, batch, e, e);
+    band_x(st, V, MT.p, m1, m0, cxb.coef.p, cxb.mask.p, 0.0, batch, e, e);
+    band_y(st, MT.p, m1, ML.p, m1, m0, m1, aycb.coef.p, aycb.mask.p, 1.0, batch, e, e);
+    ew_axpy(st, out, R, ML.p, -1.0, e * batch);
+  }
+  void solve_mixed(const double* f, double* u, int batch) {
+    Solver& sv = *s;
+    const int m0 = b0.m, m1 = b1.m, o0 = b0.ortho(), o1 = b1.ortho();
+    const long e = (long)m0 * m1;
+    if (mixed_batch < (size_t)batch) {
+      for (DBuf* b : {&MR, &MX, &MV, &MT, &ML}) b->alloc((size_t)std::max(e, (long)m0 * o1) * batch);
+      mixed_batch = batch;
+    }
+    // R = P_x f P_y^T
+    band_x(st, f, MT.p, o1, m0, pxb.coef.p, pxb.mask.p, 0.0, batch, (long)o0 * o1, (long)m0 * o1);
+    band_y(st, MT.p, o1, MR.p, m1, m0, m1, sv.pyb.coef.p, sv.pyb.mask.p, 0.0, batch, (long)m0 * o1, e);
+    for (int it = 0; it < mixed_iters; ++it) {
+      const double* src = it == 0 ? MR.p : ML.p;  // the residual of the previous iterate
+      gemm_mfma_f32_batched(st, m0, m1, m0, reinterpret_cast<const float*>(F32.p), m0, 0, src, true, m1, e, MX.p,
+                            true, m1, e, batch, 1.0, 0.0, nullptr, 0, 0);
+      fdma_rows_wave(st, MX.p, m1, m0, m1, m0, sv.low.p, sv.dia.p, sv.up1.p, sv.up2.p, batch, e);
+      double* dst = it == 0 ? u : MT.p;
+      gemm_mfma_f32_batched(st, m0, m1, m0, reinterpret_cast<const float*>(B32.p), m0, 0, MX.p, true, m1, e, dst,
+                            true, m1, e, batch, 1.0, 0.0, nullptr, 0, 0);
+      if (it > 0) ew_add(st, u, u, MT.p, e * batch);
+      if (it + 1 < mixed_iters) mixed_residual(MR.p, u, ML.p, batch);
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+  }
+  // max-norm of P_x f P_y^T - L(u) relative to that of P_x f P_y^T, per batch of solutions u
+  double residual(const double* f, const double* u, int batch) {
+    if (!s->has_fb || s->kind == ADI) throw std::invalid_argument("residual: tensor solvers with a Chebyshev axis 0");
+    const int saved = mixed_iters;
+    set_mixed(1);  // tables
+    mixed_iters = saved;
+    Solver& sv = *s;
+    const int m0 = b0.m, m1 = b1.m, o0 = b0.ortho(), o1 = b1.ortho();
+    const long e = (long)m0 * m1;
+    if (mixed_batch < (size_t)batch) {
+      for (DBuf* b : {&MR, &MX, &MV, &MT, &ML}) b->alloc((size_t)std::max(e, (long)m0 * o1) * batch);
+      mixed_batch = batch;
+    }
+    band_x(st, f, MT.p, o1, m0, pxb.coef.p, pxb.mask.p, 0.0, batch, (long)o0 * o1, (long)m0 * o1);
+    band_y(st, MT.p, o1, MR.p, m1, m0, m1, sv.pyb.coef.p, sv.pyb.mask.p, 0.0, batch, (long)m0 * o1, e);
+    mixed_residual(MR.p, u, MV.p, batch);
+    HIPCHECK(hipStreamSynchronize(st));
+    std::vector<double> r = MR.download((size_t)e * batch), d = MV.download((size_t)e * batch);
+    double rm = 0.0, dm = 0.0;
+    for (size_t i = 0; i < r.size(); ++i) {
+      rm = std::max(rm, std::fabs(r[i]));
+      dm = std::max(dm, std::fabs(d[i]));
+    }
+    return dm / std::max(rm, 1e-300);
   }
   // the same solve in caller-provided work arrays (no allocation): the doubles needed, each
   // array starting on a 256-byte boundary
@@ -2875,6 +2989,8 @@ int rbc_adjoint_exit(rbc_adjoint* h, int* out) {
 int rbc_solver_solve_noalloc(rbc_solver* h, const double* d_rhs, double* d_out, int batch) {
   return guarded([&] {
     if (!h || !h->s || !d_rhs || !d_out || batch < 1) throw std::invalid_argument("bad argument");
+    if (h->s->mixed_iters > 0)
+      throw std::invalid_argument("rbc_solver_solve_noalloc: not available with the mixed form (rbc_solver_set_mixed)");
     const size_t need = h->s->work_doubles(batch);
     if (need > rbc::noalloc::kScratchDoubles)
       throw std::invalid_argument("rbc_solver_solve_noalloc: the work arrays (" + std::to_string(need) +
@@ -2888,6 +3004,18 @@ int rbc_solver_solve_noalloc(rbc_solver* h, const double* d_rhs, double* d_out, 
     }
     rbc::GemmNoTune no_tune;
     h->s->solve_in(d_rhs, d_out, batch, scratch);
+  });
+}
+int rbc_solver_set_mixed(rbc_solver* h, int iters) {
+  return guarded([&] {
+    if (!h || !h->s) throw std::invalid_argument("null handle");
+    h->s->set_mixed(iters);
+  });
+}
+int rbc_solver_residual(rbc_solver* h, const double* d_rhs, const double* d_sol, int batch, double* out) {
+  return guarded([&] {
+    if (!h || !h->s || !d_rhs || !d_sol || !out || batch < 1) throw std::invalid_argument("bad argument");
+    *out = h->s->residual(d_rhs, d_sol, batch);
   });
 }
 int rbc_solver_tensor_info(rbc_solver* h, int* n, int* has_fb) {

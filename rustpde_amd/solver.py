@@ -42,6 +42,23 @@ class Solver:
                                                    ctypes.c_void_p(int(getattr(d_out, "value", d_out) or 0)),
                                                    int(batch)))
 
+    def set_mixed(self, iters):
+        """FP32 eigen-transforms with `iters` chained solves of FP64-residual refinement (0: FP64)"""
+        check(_lib.load().rbc_solver_set_mixed(self._h, int(iters)))
+
+    def residual(self, rhs, sol):
+        """relative max-norm of P_x f P_y^T - L(u) for host arrays rhs [.., o0, o1], sol [.., s0, s1]"""
+        a = np.ascontiguousarray(rhs, dtype=float)
+        u = np.ascontiguousarray(sol, dtype=float)
+        batch = 1 if a.ndim == 2 else a.shape[0]
+        din, dsol = _Dev(a.nbytes), _Dev(u.nbytes)
+        L = _lib.load()
+        check(L.rbc_memcpy_h2d(din.p, a.ctypes.data, a.nbytes))
+        check(L.rbc_memcpy_h2d(dsol.p, u.ctypes.data, u.nbytes))
+        out = ctypes.c_double()
+        check(L.rbc_solver_residual(self._h, din.p, dsol.p, batch, ctypes.byref(out)))
+        return out.value
+
     def solve(self, rhs):
         """Host arrays [o0, o1] or [batch, o0, o1] -> [.., s0, s1]."""
         a = np.ascontiguousarray(rhs, dtype=complex if self.is_complex else float)

@@ -85,3 +85,37 @@ def test_solver_kinds_vs_oracle(oracle, case):
     u = g.solve(f)
     for b in range(3):
         assert rel_err(u[b], o.solve(f[b])) < 1e-12, (b, rel_err(u[b], o.solve(f[b])))
+
+
+def test_hholtz_batch_512_mixed_refined(oracle):
+    """FP32 eigen-transforms with FP64-residual refinement (rbc_solver_set_mixed): two chained
+    solves bring the tensor equation's residual to below 1e-12 (the FP64 solve's own is
+    ~5e-10 at this size), and the solution agrees with the FP64 solve to that solve's error;
+    one solve alone stays at the FP32 level."""
+    from rustpde_amd import HHOLTZ, RbcError, Solver
+    print("case hholtz batch 512^2 mixed FP32 + refinement", flush=True)
+    o = oracle.Solver("hholtz", oracle.CD, 512, oracle.CD, 512, [1e-3, 1e-3])
+    g64 = Solver(HHOLTZ, (1, 512), (1, 512), (1e-3, 1e-3))
+    g32 = Solver(HHOLTZ, (1, 512), (1, 512), (1e-3, 1e-3))
+    for g in (g64, g32):
+        g.set_tensor(*o.tensor_setup())
+    f = batch_rhs(512, 4)
+    u64 = g64.solve(f)
+    g32.set_mixed(1)
+    u1 = g32.solve(f)
+    g32.set_mixed(2)
+    u2 = g32.solve(f)
+    for b in range(4):
+        r64, r1, r2 = (tensor_residual(u[b], f[b], 1e-3) for u in (u64, u1, u2))
+        d2 = g32.residual(f[b], u2[b])
+        print("rhs %d residual fp64 %.3e, fp32 %.3e, fp32 + refinement %.3e (device %.3e); |u2 - u64| %.3e"
+              % (b, r64, r1, r2, d2, rel_err(u2[b], u64[b])), flush=True)
+        assert r2 < 1e-12 and r2 <= r64, (r2, r64)
+        assert 1e-9 < r1 < 1e-5
+        assert rel_err(u2[b], u64[b]) < 1e-8
+        assert abs(d2 - r2) < 0.5 * r2 + 1e-15
+    g32.set_mixed(0)
+    assert np.array_equal(g32.solve(f), u64)
+    fourier = Solver(HHOLTZ, (4, 64), (1, 65), (1e-3, 1e-3))  # fourier_r2c x cheb_dirichlet
+    with pytest.raises(RbcError):
+        fourier.set_mixed(2)
