@@ -47,7 +47,7 @@ WORKLOAD_NAMES = {
 KERNEL_CLASSES = {0: "gemm_mfma_kernel", 1: "fdma_rows_lds_kernel", 2: "irfft_cols_kernel", 3: "other kernels",
                   4: "cheb_y_bwd_kernel", 5: "rfft_cols_kernel", 6: "cheb_y_fwd_kernel", 7: "cheb_y_diff_kernel",
                   8: "tdma_rows_lds_kernel", 9: "band_y_kernel", 10: "pressure_rows_kernel",
-                  11: "cheb_y_conv_kernel", 12: "gemm_mfma_kernel (FP32)"}
+                  11: "cheb_y_conv_kernel", 12: "gemm_mfma_kernel (FP32)", 13: "cheb_y_conv3_kernel"}
 MIXED_CLASS = 3
 
 

@@ -165,7 +165,12 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *                  batched step: FFT/DCT kernels in FP32; confined step with RBC_OPT_BATCH
  *                  on (and ensemble groups, rbc_ensemble_set_option): the dense transforms
  *                  as FP32 matrix-core GEMMs (with RBC_OPT_BATCH off the confined step
- *                  stays FP64).  Fields agree with the FP64 step to ~1e-9 after 10 steps. */
+ *                  stays FP64).  Fields agree with the FP64 step to ~1e-9 after 10 steps.
+ *   RBC_OPT_CONV3  periodic batched mixed step with RBC_OPT_FUSE: the y-stage of the FP32
+ *                  convective terms (physical ux, uy, the three terms and their dealiased
+ *                  forward y-transforms) as one launch per x-row (default 1; 0 = three
+ *                  launches: y-backward of ux, uy, the convective-term launch, the forward
+ *                  launch).  The FP64 step always runs the three launches. */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2
@@ -174,6 +179,7 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
 #define RBC_OPT_BATCH 5
 #define RBC_OPT_FUSE 6
 #define RBC_OPT_MIXED 7
+#define RBC_OPT_CONV3 8
 int rbc_set_option(rbc_navier* h, int option, int value);
 
 /* Kernel timing of a handle (no counterpart in the reference, whose timing protocol is
@@ -189,6 +195,7 @@ int rbc_set_option(rbc_navier* h, int option, int value);
  *   RBC_K_PRESSURE pressure_rows_kernel (the fused pressure stage); RBC_K_CONV
  *   cheb_y_conv_kernel (the convective term's y transforms and products, one launch);
  *   RBC_K_GEMM32 gemm_mfma_kernel in FP32 (the confined mixed step's convective terms);
+ *   RBC_K_CONV3 cheb_y_conv3_kernel (the whole y-stage of the convective terms, one launch);
  *   RBC_K_ELEMENTWISE the remaining timed kernels (Fourier derivative, fallbacks). */
 #define RBC_K_GEMM 0
 #define RBC_K_ROWSOLVE 1
@@ -203,6 +210,7 @@ int rbc_set_option(rbc_navier* h, int option, int value);
 #define RBC_K_PRESSURE 10
 #define RBC_K_CONV 11
 #define RBC_K_GEMM32 12
+#define RBC_K_CONV3 13
 int rbc_timing_enable(rbc_navier* h, int on);
 int rbc_timing_read(rbc_navier* h, int kernel_class, long long* launches, double* total_ms, double* work);
 

@@ -12,6 +12,17 @@ namespace chains {
 
 constexpr int WAVE = 64;
 
+// Workgroup barrier that orders LDS accesses only: the fences cover the local address
+// space, so the wave's outstanding global loads (prefetched operands) and stores stay in
+// flight across it (__syncthreads first waits for every outstanding global access of the
+// wave).  For phases that exchange data between the threads of a workgroup through LDS
+// only; data passed through global memory needs __syncthreads.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // padded LDS index of row element i (stored at pad(i + 2)): one pad double per 16
 __host__ __device__ __forceinline__ int pad(int j) { return j + (j >> 4); }
 

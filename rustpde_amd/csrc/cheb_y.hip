@@ -336,6 +336,222 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_conv_kernel(const Tin* __res
   cheb_y_bwd_body<L, true, true, Tin, V, Tout>(dx, ldi, mi, out, ldo, rows, coef, 1, inv_sc, ux, ldu, 1.0, tw, si, so,
                                                0, xp, val, uy);
 }
+// The whole convective stage along y of the periodic step for P x-rows per block, all three
+// fields (slots 0, 1, 2 = ux, uy, temp of dx / val, stride si) in the Dirichlet basis
+// (composite_stencil.rs: s_k = c_k - c_{k-2}, taps d_k = 1 for k < m, l_k = -1 for
+// 2 <= k < m + 2, formed in place with the table's values):
+//   transform 0: (val_ux, val_uy) -> the physical velocity ux_r, uy_r, kept in LDS (the
+//                values cheb_y_bwd_kernel stores for the convective launch);
+//   transform f + 1: (dx_f, d/dy val_f) -> out_f = d/dy(val_f) .* uy + 1 (dx_f .* ux), plus
+//                temp's u . grad(fieldbc) with the gradient arrays f1 (x; null where it is
+//                zero) and f2 (y), row stride ldf (0 when the boundary field is constant in x);
+//   then the three terms go through the dealiased forward transform (cheb_y_fwd_kernel's
+//                epilogue, rows k >= cut zero) as the pairs (out_ux, out_uy) and
+//                (out_temp, 0), and `out` receives the y-spectral rows.
+// Each value is formed with the arithmetic of the launches it replaces (cheb_y_bwd_kernel,
+// cheb_y_conv_kernel, cheb_y_fwd_kernel); only the pairing of rows inside the complex
+// transforms differs, which moves the results by rounding only.  The physical velocity and
+// the convective terms never leave the chip, and the rows of the next field are loaded
+// while the current one is transformed.  FP32 (the mixed step), four blocks per CU.
+template <int L, typename Tin, typename V, typename Tout>
+__global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
+    const Tin* __restrict__ dx, const Tin* __restrict__ val, int ldi, int mi, Tout* __restrict__ out, int ldo, int rows,
+    double inv_sc_d, const double* __restrict__ f1, const double* __restrict__ f2, long ldf, int cut,
+    const double* __restrict__ tw, long si, long so) {
+#pragma clang fp contract(off)
+  using G = Geo<L>;
+  constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
+  constexpr int Q = (P * n + T - 1) / T;
+  constexpr double invN = 1.0 / (double)N;  // exact: N is a power of two
+  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
+  __shared__ V uxs[P * n], uys[P * n];  // the physical velocity of the block's rows
+  const V inv_sc = (V)inv_sc_d;
+  const int tid = threadIdx.x, row0 = P * blockIdx.x;
+  // raw coefficients c_kk, c_kk-2 of row r of one slot (zero outside the row)
+  struct Raw {
+    V c0[Q], c2[Q];
+  };
+  auto load = [&](const Tin* __restrict__ base, int shift, Raw& w) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int p = e / n, r = row0 + p, kk = e - p * n + shift;
+      const bool ok = e < P * n && r < rows && kk < n;
+      const Tin* row = base + (size_t)r * ldi;
+      w.c0[q] = (ok && kk < mi) ? (V)row[kk] : (V)0;
+      w.c2[q] = (ok && kk >= 2 && kk - 2 < mi) ? (V)row[kk - 2] : (V)0;
+    }
+  };
+  // the stencil value d c_kk + l c_kk-2 with the Dirichlet taps at kk (d = 1 below m = n - 2
+  // and 0 above; l = -1 for 2 <= kk < n, 0 below): for kk >= n both rows load zeros, as the
+  // reference's absent entries
+  auto sval = [&](const Raw& w, int q, int kk) -> V {
+    const V d = kk < n - 2 ? (V)1 : (V)0, l = (kk >= 2 && kk < n) ? (V)-1 : (V)0;
+    return d * w.c0[q] + l * w.c2[q];
+  };
+  // even extension: element k of transform p at k and L - k
+  auto put = [&](int q, V a, V b) {
+    const int e = tid + q * T;
+    if (e >= P * n) return;
+    const int p = e / n, k = e - p * n;
+    re[lp(p * L + k)] = a;
+    im[lp(p * L + k)] = b;
+    if (k > 0 && k < N) {
+      re[lp(p * L + L - k)] = a;
+      im[lp(p * L + L - k)] = b;
+    }
+  };
+  // dealiased forward transform of the pair (a, b) into out slots fa (and fa + 1 with hasb)
+  auto forward = [&](const V (&a)[Q], const V (&b)[Q], int fa, bool hasb) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) put(q, a[q], hasb ? b[q] : (V)0);
+    __syncthreads();
+    lds_dft<L, T, G::E>(re, im, tw, fftlds::opaque_tid());
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int p = e / n, k = e - p * n, r = row0 + p;
+      if (e >= P * n || r >= rows) continue;
+      const V s = (V)(((k & 1) ? -invN : invN) * ((k == 0 || k == N) ? 0.5 : 1.0));
+      out[fa * so + (size_t)r * ldo + k] = (Tout)(k < cut ? re[lp(p * L + k)] * s : (V)0);
+      if (hasb) out[(fa + 1) * so + (size_t)r * ldo + k] = (Tout)(k < cut ? im[lp(p * L + k)] * s : (V)0);
+    }
+  };
+  // ---- transform 0: the physical velocity (times = 0: u_k = (-1)^k w_k c_k); the rows of
+  //      field 0 are loaded alongside
+  Raw ra, rb;
+  {
+    Raw va, vb;
+    load(val, 0, va);
+    load(val + si, 0, vb);
+    load(dx, 0, ra);
+    load(val, 1, rb);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int k = e - (e / n) * n;
+      const V w = (V)(((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5));
+      put(q, sval(va, q, k) * w, sval(vb, q, k) * w);
+    }
+  }
+  __syncthreads();
+  lds_dft<L, T, G::E>(re, im, tw, fftlds::opaque_tid());
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = tid + q * T;
+    const int p = e / n, j = e - p * n;
+    if (e < P * n) {
+      uxs[e] = (V)(Tout)re[lp(p * L + j)];
+      uys[e] = (V)(Tout)im[lp(p * L + j)];
+    }
+  }
+  V res0[Q], res1[Q];  // the convective terms of ux, then uy (temp reuses res0)
+  // ---- transforms 1..3: the convective terms
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    __syncthreads();  // the previous transform's results are read
+    // (1) component a: s_k of dx_f; component b: g_k of the derivative recurrence of val_f,
+    //     g_0 = s_1, g_k = 2 (k + 1) s_{k+1}, g_{n-1} = 0
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      if (e >= P * n) continue;
+      const int p = e / n, k = e - p * n;
+      V b = sval(rb, q, k + 1);
+      if (k == n - 1) b = 0;
+      else if (k > 0) b = (V)(2.0 * (double)(k + 1)) * b;
+      re[lp(p * L + k)] = sval(ra, q, k);
+      im[lp(p * L + k)] = b;
+    }
+    // the next field's rows (or temp's fieldbc gradient), in flight during this transform
+    V g1[Q], g2[Q];
+    if (f < 2) {
+      load(dx + (f + 1) * si, 0, ra);
+      load(val + (f + 1) * si, 1, rb);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int e = tid + q * T;
+        const int p = e / n, j = e - p * n, r = row0 + p;
+        const bool ok = e < P * n && r < rows;
+        const size_t o = (size_t)r * ldf + j;
+        g1[q] = (ok && f1) ? (V)f1[o] : (V)0;
+        g2[q] = (ok && f2) ? (V)f2[o] : (V)0;
+      }
+    }
+    __syncthreads();
+    // (2) d_k = g_k + d_{k+2} along the chains 1, 3, .., N-1 and 2, 4, .., N of component b,
+    //     then d_0 = g_0 + d_2 / 2; one wavefront per chain
+    {
+      constexpr int C = N / 2, SEG = (C + 63) / 64, NCH = 2 * P;
+      const int lane = tid & 63, wave = tid >> 6;
+      for (int ch = wave; ch < NCH; ch += T / 64) {
+        const int p = ch >> 1, par = ch & 1;
+        const int zb = p * L + (par ? 1 : 2);
+        const int t0 = lane * SEG;
+        V tot = 0;
+#pragma unroll
+        for (int q = SEG - 1; q >= 0; --q)
+          if (t0 + q < C) tot = im[lp(zb + 2 * (t0 + q))] + tot;
+        V inc = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const V o = __shfl_down(inc, off);
+          if (lane + off < 64) inc = inc + o;
+        }
+        V acc = __shfl_down(inc, 1);
+        if (lane == 63) acc = 0;
+#pragma unroll
+        for (int q = SEG - 1; q >= 0; --q) {
+          const int t = t0 + q;
+          if (t < C) {
+            acc = im[lp(zb + 2 * t)] + acc;
+            im[lp(zb + 2 * t)] = acc;
+          }
+        }
+        if (par == 0 && lane == 0) im[lp(p * L)] = im[lp(p * L)] + acc / (V)2;
+      }
+    }
+    __syncthreads();
+    // (3) u_k = (-1)^k w_k c_k (b: / sc) and the even extension
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      if (e >= P * n) continue;
+      const int p = e / n, k = e - p * n;
+      const V a = re[lp(p * L + k)];
+      V b = im[lp(p * L + k)];
+      if (inv_sc != (V)1) b = b * inv_sc;
+      const V w = (V)(((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5));
+      put(q, a * w, b * w);
+    }
+    __syncthreads();
+    lds_dft<L, T, G::E>(re, im, tw, fftlds::opaque_tid());
+    // (4) out_f = d/dy(val_f) .* uy + 1 (dx_f .* ux) (+ temp's fieldbc terms)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int p = e / n, j = e - p * n, r = row0 + p;
+      V v = 0;
+      if (e < P * n && r < rows) {
+        const V ux = uxs[e], uy = uys[e];
+        const V xa = (V)(Tout)(re[lp(p * L + j)] * ux);
+        v = im[lp(p * L + j)] * uy;
+        v = v + (V)1 * xa;
+        if (f == 2) {
+          if (f1) v = v + ux * g1[q];
+          if (f2) v = v + uy * g2[q];
+        }
+      }
+      if (f == 1) res1[q] = (V)(Tout)v;
+      else res0[q] = (V)(Tout)v;
+    }
+    if (f == 1) forward(res0, res1, 0, true);  // (out_ux, out_uy)
+    if (f == 2) forward(res0, res1, 2, false);  // (out_temp, 0)
+  }
+}
+
 // the same without the transform: Chebyshev coefficients of the derivative
 template <int L>
 __global__ __launch_bounds__(Geo<L>::T) void cheb_y_diff_kernel(const double* __restrict__ in, int ldi, int mi,
@@ -429,6 +645,15 @@ void launch_conv(hipStream_t s, const Tin* dx, const Tin* val, int ldi, int mi, 
 }
 
 template <int L, typename Tin, typename V, typename Tout>
+void launch_conv3(hipStream_t s, const Tin* dx, const Tin* val, int ldi, int mi, Tout* out, int ldo, int rows,
+                  double inv_sc, const double* f1, const double* f2, long ldf, int cut, long si, long so) {
+  using G = Geo<L>;
+  const unsigned blocks = (unsigned)((rows + G::P - 1) / G::P);
+  RBC_LAUNCH((cheb_y_conv3_kernel<L, Tin, V, Tout>), dim3(blocks), dim3(G::T), 0, s, dx, val, ldi, mi, out, ldo, rows,
+             inv_sc, f1, f2, ldf, cut, fft_table(L), si, so);
+}
+
+template <int L, typename Tin, typename V, typename Tout>
 void launch_fwd(hipStream_t s, const Tin* in, int ldi, Tout* out, int ldo, int rows, int cut, int batch, long si,
                 long so) {
   using G = Geo<L>;
@@ -499,6 +724,25 @@ void cheb_y_conv_f32(hipStream_t s, const float* dx, const float* val, int ldi, 
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_CONV32)
 #undef RBC_CONV32
   check("cheb_y_conv_f32");
+}
+
+namespace {
+void conv3_check(int n, int rows, int mi, const void* f2, long ldf, int cut, long si, long so, int ldo) {
+  if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_conv3: unsupported length");
+  if (rows < 1 || mi != n - 2) throw std::invalid_argument("cheb_y_conv3: bad rows or input length");
+  if (ldf < 0 || (f2 == nullptr && ldf != 0)) throw std::invalid_argument("cheb_y_conv3: bad fieldbc stride");
+  if (cut < 0 || cut > n || si < 0 || so < 0 || ldo < n) throw std::invalid_argument("cheb_y_conv3: bad output layout");
+}
+}  // namespace
+
+void cheb_y_conv3_f32(hipStream_t s, const float* dx, const float* val, int ldi, int mi, float* out, int ldo, int rows,
+                      int n, double inv_sc, const double* f1, const double* f2, long ldf, int cut, long si, long so) {
+  conv3_check(n, rows, mi, f2, ldf, cut, si, so, ldo);
+#define RBC_CONV3F(LL) \
+  launch_conv3<LL, float, float, float>(s, dx, val, ldi, mi, out, ldo, rows, inv_sc, f1, f2, ldf, cut, si, so)
+  RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_CONV3F)
+#undef RBC_CONV3F
+  check("cheb_y_conv3_f32");
 }
 
 void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,

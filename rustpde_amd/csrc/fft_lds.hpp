@@ -64,7 +64,7 @@ __device__ __forceinline__ void dft8(T* r, T* i) {
 
 // one Stockham pass of radix R over the P transforms of length L held in (re, im)
 template <int L, int R, int T, int E, typename V>
-__device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double* __restrict__ tw) {
+__device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double* __restrict__ tw, int tid) {
   static_assert(E % (R * T) == 0 && E % L == 0, "block geometry");
   constexpr int NB = L / R;          // butterflies per transform
   constexpr int PER = E / (R * T);   // butterflies per thread
@@ -72,7 +72,7 @@ __device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double
   int dst[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int g = threadIdx.x + q * T;
+    const int g = tid + q * T;
     const int p = g / NB, j = g - p * NB;
     const int k = j & (Ns - 1);
     const int base = p * L + j;
@@ -122,17 +122,28 @@ __device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
+// tid: the thread's index in the block (a caller that runs several transforms can pass a
+// value the compiler cannot prove equal between them, so that each transform forms its
+// LDS addresses afresh instead of holding one set live across the others)
 template <int L, int T, int E, typename V>
-__device__ __forceinline__ void lds_dft(V* re, V* im, const double* __restrict__ tw) {
+__device__ __forceinline__ void lds_dft(V* re, V* im, const double* __restrict__ tw, int tid = threadIdx.x) {
   constexpr int NUM8 = ilog2(L) / 3, REM = ilog2(L) % 3;
   int Ns = 1;
 #pragma unroll
   for (int s = 0; s < NUM8; ++s) {
-    stockham_pass<L, 8, T, E>(re, im, Ns, tw);
+    stockham_pass<L, 8, T, E>(re, im, Ns, tw, tid);
     Ns *= 8;
   }
-  if constexpr (REM == 2) stockham_pass<L, 4, T, E>(re, im, Ns, tw);
-  else if constexpr (REM == 1) stockham_pass<L, 2, T, E>(re, im, Ns, tw);
+  if constexpr (REM == 2) stockham_pass<L, 4, T, E>(re, im, Ns, tw, tid);
+  else if constexpr (REM == 1) stockham_pass<L, 2, T, E>(re, im, Ns, tw, tid);
+}
+
+// threadIdx.x behind an empty asm statement: a value the compiler cannot merge with other
+// uses of the thread index
+__device__ __forceinline__ int opaque_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
 }
 
 // block index remapped so that consecutive logical blocks run on the same XCD (blocks

@@ -397,7 +397,7 @@ static std::unique_ptr<Solver> make_solver(SKind kind, const Basis& b0, const Ba
 // the launch's algorithmic work (flops for GEMMs, bytes for memory-bound kernels).
 class KernelTimer {
  public:
-  static constexpr int NCAT = 13;  // the RBC_K_* classes of the C ABI
+  static constexpr int NCAT = 14;  // the RBC_K_* classes of the C ABI
   bool on = false;
   struct Rec { int cat; double work; hipEvent_t a, b; };
   ~KernelTimer() { clear(); for (auto e : pool_) (void)hipEventDestroy(e); }
@@ -470,6 +470,8 @@ class Navier {
   DBuf Tv, UXv, UYv, P0v, P1v, BCv;  // physical
   // constants derived from fieldbc
   DBuf THAT_BC, DTBCX, DTBCY, GXX, GYY;
+  // the physical fieldbc gradient: d/dx identically zero, d/dy the same in every x row
+  bool bc_dx_zero = false, bc_dy_xconst = false;
   // workspaces
   DBuf THAT, UXP, UYP, A, G, CONV, CV, RHS, DIV, W1, W2, W3, W4, W9, RED;
   std::vector<double> wx_dx, wy_dx;  // cell widths of `field` (unscaled, field.rs:135-163)
@@ -482,6 +484,7 @@ class Navier {
   bool use_dct = true;    // Chebyshev y transforms by FFT-based DCT-I kernels
   bool use_batch = true;  // periodic step: the three fields' transforms and solves batched
   bool use_fuse = true;   // periodic step: the pressure stage as one fused kernel
+  bool use_conv3 = true;  // periodic fused mixed step: the y-stage of the convective terms in one launch
   // ensemble group: this handle (the group's leader) advances nb instances of the same size
   // whose buffers are slots nb * bs apart; per-instance nu and ADI operators in slabs
   int nb = 1;
@@ -1139,8 +1142,12 @@ class Navier {
     for (auto& v : a) v = (v * dt) * ka;
     for (auto& v : b) v = (v * dt) * ka;
     THAT_BC.upload(that.to_double());
-    DTBCX.upload(dxp.to_double());
-    DTBCY.upload(dyp.to_double());
+    const std::vector<double> hdx = dxp.to_double(), hdy = dyp.to_double();
+    DTBCX.upload(hdx);
+    DTBCY.upload(hdy);
+    bc_dx_zero = std::all_of(hdx.begin(), hdx.end(), [](double v) { return v == 0.0; });
+    bc_dy_xconst = true;
+    for (size_t i = (size_t)ny; i < hdy.size() && bc_dy_xconst; ++i) bc_dy_xconst = hdy[i] == hdy[i % (size_t)ny];
     GXX.upload(a);
     GYY.upload(b);
     // physical fieldbc (Field2::v after bc_rbc's backward)
@@ -1481,6 +1488,18 @@ class Navier {
       timer.begin(st, 2, my * (8.0 * 3.0 * mx2 + 4.0 * 6.0 * nx));  // spectra once, six FP32 outputs
       irfft_cols_pair_f32(st, UX.p, W2f, W1f, nx, my, 1, scale[0], 3, F, F);
       timer.end(st);
+      if (conv3_ok()) {
+        // the y-stage in one launch (CONVf: y-spectral FP32 convective terms), then the
+        // x-transforms into W1f's slots
+        timer.begin(st, 13, conv3_bytes(4.0, 4.0));
+        cheb_y_conv3_f32(st, W1f, W2f, my, my, CONVf, ny, nx, ny, 1.0 / scale[1],
+                         bc_dx_zero ? nullptr : DTBCX.p, DTBCY.p, conv3_ldf(), ny * 2 / 3, F, F);
+        timer.end(st);
+        timer.begin(st, 5, 3.0 * ny * (4.0 * nx + 4.0 * mx2));
+        rfft_cols_f32f(st, CONVf, W1f, nx, ny, bx_f.m * 2 / 3, 3, F, F);
+        timer.end(st);
+        return;
+      }
       timer.begin(st, 4, 4.0 * 2.0 * nx * ((double)my + ny));
       cheb_y_backward_f32(st, W2f, my, my, UXPf, ny, nx, ny, band.coef.p, 0, 1.0 / scale[1], nullptr, ny, 0.0, 2, F, F,
                           0, nullptr, nullptr, nullptr, nullptr, -1);
@@ -1504,6 +1523,27 @@ class Navier {
     timer.begin(st, 5, 3.0 * ny * (4.0 * nx + 4.0 * mx2));
     rfft_cols_f32f(st, W2f, W1f, nx, ny, bx_f.m * 2 / 3, 3, F, F);
     timer.end(st);
+  }
+  // the one-launch y-stage of the FP32 convective terms: DCT rows and the Dirichlet stencil
+  // for the three fields (the periodic Rayleigh-Benard bases).  The FP64 step keeps its
+  // three launches: the one-launch kernel holds twice the registers in FP64 and runs two
+  // blocks per CU, which measured slower.
+  bool conv3_ok() const {
+    return use_conv3 && cheb_y_supported(ny) && by_u.k == CD && by_u.n == ny && by_u.m == ny - 2 && dirichlet_taps(by_u);
+  }
+  // the stencil of b is the Dirichlet one the conv3 kernel forms in place (d = 1, l = -1)
+  static bool dirichlet_taps(const Basis& b) {
+    for (int i = 0; i < b.m; ++i)
+      if (b.diag[i] != 1.0 || b.low2[i] != -1.0) return false;
+    return true;
+  }
+  long conv3_ldf() const { return bc_dy_xconst && bc_dx_zero ? 0 : (long)ny; }
+  // bytes, distinct arrays once: x-derivatives and values of the three fields read, the
+  // three y-spectral convective terms written (element sizes bi in, bo out), temp's
+  // fieldbc gradient (FP64; one row when constant in x, x part skipped when zero)
+  double conv3_bytes(double bi, double bo) const {
+    const double frows = conv3_ldf() ? (double)nx : 1.0;
+    return bi * nx * 6.0 * by_u.m + bo * nx * 3.0 * ny + 8.0 * frows * ny * (bc_dx_zero ? 1.0 : 2.0);
   }
   void solves_and_pressure() {
     const int my = by_u.m, R2 = 2 * mx();
@@ -2790,6 +2830,7 @@ void apply_option(Navier& n, int option, int value, bool alloc = true) {
     case RBC_OPT_DCT: n.use_dct = value != 0; break;
     case RBC_OPT_BATCH: n.use_batch = value != 0; break;
     case RBC_OPT_FUSE: n.use_fuse = value != 0; break;
+    case RBC_OPT_CONV3: n.use_conv3 = value != 0; break;
     case RBC_OPT_MIXED:
       if (value && n.solid_on)
         throw std::invalid_argument("RBC_OPT_MIXED: a handle with a solid obstacle runs the field-by-field FP64 step; "

@@ -280,6 +280,14 @@ void cheb_y_conv_f32(hipStream_t s, const float* dx, const float* val, int ldi, 
                      int n, const double* coef, double inv_sc, const float* ux, const float* uy, int ldu, int batch,
                      long si, long so, const float* e1 = nullptr, const double* f1 = nullptr,
                      const float* e2 = nullptr, const double* f2 = nullptr, int xslot = -1);
+// the y-stage of the three FP32 convective terms (ux, uy, temp in slots 0, 1, 2 of dx / val,
+// stride si; Dirichlet y basis, mi = n - 2 coefficients) in one launch: the physical
+// velocity from val's slots 0, 1, then out_f = d/dy(val_f) .* uy + dx_f .* ux (+ ux f1 +
+// uy f2 for temp: the fieldbc gradient, row stride ldf, 0 when constant in x; f1 null when
+// zero) and their dealiased forward transforms (rows k >= cut zero) into out [3 slots of
+// stride so, rows, n]
+void cheb_y_conv3_f32(hipStream_t s, const float* dx, const float* val, int ldi, int mi, float* out, int ldo, int rows,
+                      int n, double inv_sc, const double* f1, const double* f2, long ldf, int cut, long si, long so);
 void cheb_y_diff(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
                  const double* coef, double inv_sc, double beta, int batch = 1, long si = 0, long so = 0);
 

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     const int j = tid < 4 ? (tid >> 1) : ny + 2 + ((tid - 4) >> 1);
     DV(tid & 1)[pad(j)] = 0.0;
   }
-  __syncthreads();
+  chains::lds_barrier();
   // (2) div = S_u(ik ux / sc) + d/dy S_u(uy) for i < ny.  The derivative: g_i = 2 (i + 1) s_{i+1}
   //     summed from the top along the chains 1, 3, .., N - 1 and 2, 4, .., N, then
   //     d_0 = s_1 + d_2 / 2 (ortho.rs:107-125); each entry then adds the x part (taps i,
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       DV(r)[pad(2)] = d + 1.0 * divx(0);
     }
   }
-  __syncthreads();
+  chains::lds_barrier();
   // (3) pseu = Poisson rows of div (P_y taps, then the Fdma of set k); pseu(0,0) = 0
   const int lenu = p == 0 ? (mu + 1) / 2 : mu / 2;
   const int cnt = max(0, min(per, lenu - t0));
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     for (int q = 0; q < SLICE; ++q)
       if (q < cnt) YB(r)[pad(p + 2 * (t0 + q) + 2)] = v[q];
   }
-  __syncthreads();
+  chains::lds_barrier();
   // (4) pseu -> HBM; pres = (pres - div nu) + S_p(pseu) / dt (half block h: plane h); the new
   //     pres values stay in registers for (8)
   double pn[QH];
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       prow[i] = pn[q];
     }
   }
-  __syncthreads();
+  chains::lds_barrier();
   // the from_ortho tables of the projections (1 / den, w), issued here so that their round
   // trip overlaps the derivative below
 #define TB(arr, q) a.ttab[((size_t)((arr) * 2 + p) * per + (q)) * WAVE + lane]
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
       DV(r)[pad(2)] = d;
     }
   }
-  __syncthreads();
+  chains::lds_barrier();
   // (6) both projections, from_ortho of g = S_p(ik pseu / sc) (ux) and of g = DV (uy):
   //     d_i = g_i diag_i + g_{i+2} low2_i, then the tdma recurrences with the tables shared
   //     by the two right-hand sides
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #undef SU1
 #undef SP0
 #undef SP1
-  __syncthreads();
+  chains::lds_barrier();
   // (7) the updated ux rows -> HBM; uy = x (-1) + uy (uy reloaded: its staged copy became pseu)
   chains::unstage_row<QH>(XA(h), a.UX + ROWU(h), true, mu, th, 128, false);
   chains::unstage_row<QH>(DV(h), a.UY + ROWU(h), true, mu, th, 128, true);
@@ -347,8 +347,10 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     //     coefficients) for the next step's right-hand sides: pres rows (kept from (4)) ->
     //     YB (element i at pad(i + 2), zero guards as stage_row leaves them), derivative ->
     //     XA -> dP
-    //     (the general instantiation reloads them: it has no registers to spare)
-    __syncthreads();
+    //     (the general instantiation reloads them: it has no registers to spare, and its
+    //     barrier waits for the pres stores of (4))
+    if constexpr (RBC) chains::lds_barrier();
+    else __syncthreads();
     if constexpr (RBC) {
 #pragma unroll
       for (int q = 0; q < QH; ++q) {
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
     } else {
       chains::stage_row<QH>(YB(h), a.P0 + ROWN(h), true, ny, th, 128);
     }
-    __syncthreads();
+    chains::lds_barrier();
     {
       double v[SLICE];
       const double* Y = YB(r);
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
         XA(r)[pad(2)] = d;
       }
     }
-    __syncthreads();
+    chains::lds_barrier();
     chains::unstage_row<QH>(XA(h), a.dP + ROWN(h), true, ny, th, 128, false);
   }
 #undef XA

@@ -240,7 +240,7 @@ class Navier2D:
         self.fieldbc.vhat = vhat
 
     # ---- execution options (equivalent implementations; for A/B checks)
-    OPT_GRAPH, OPT_MFMA, OPT_FFT, OPT_BANDS, OPT_DCT, OPT_BATCH, OPT_FUSE, OPT_MIXED = 0, 1, 2, 3, 4, 5, 6, 7
+    OPT_GRAPH, OPT_MFMA, OPT_FFT, OPT_BANDS, OPT_DCT, OPT_BATCH, OPT_FUSE, OPT_MIXED, OPT_CONV3 = 0, 1, 2, 3, 4, 5, 6, 7, 8
 
     def set_option(self, option, value):
         check(_lib.load().rbc_set_option(self._h, int(option), int(value)))

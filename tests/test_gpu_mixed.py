@@ -39,6 +39,26 @@ def test_mixed_fields_vs_oracle(oracle, shape):
     assert abs(g.eval_nu() - o.eval_nu()) < 1e-5 * abs(o.eval_nu())
 
 
+def test_mixed_conv3_off_vs_oracle(oracle):
+    """the mixed step with the convective y-stage as three launches (RBC_OPT_CONV3 off)
+    against the oracle, and against the one-launch form: they differ only by the pairing of
+    rows inside the FP32 transforms"""
+    print("case mixed conv3 off", flush=True)
+    g, o = _periodic(oracle, 128, 129, 1e7, 2e-3, 21)
+    h, _ = _periodic(oracle, 128, 129, 1e7, 2e-3, 21)
+    g.set_option(MIXED, 1)
+    h.set_option(MIXED, 1)
+    h.set_option(8, 0)
+    g.update(10)
+    h.update(10)
+    o.update(10)
+    e = compare(h, o)
+    for k in ("temp", "ux", "uy", "pres"):
+        assert e[k] < 1e-5, (k, e)
+    for fg, fh in ((g.temp, h.temp), (g.ux, h.ux), (g.uy, h.uy)):
+        assert rel_err(fg.vhat, fh.vhat) < 1e-6
+
+
 def test_mixed_vs_fp64_gpu_and_graph(oracle):
     """mixed against the FP64 GPU step; graph replays reproduce eager mixed steps bit for bit."""
     print("case mixed vs fp64 gpu", flush=True)

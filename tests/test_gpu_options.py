@@ -3,7 +3,9 @@
 Options (rbc_set_option): HIP-graph replay, FP64 MFMA GEMMs, FFT kernels for the
 Fourier axis, banded kernels for stencils/preconditioners, FFT-based DCT-I kernels for
 the Chebyshev y axis (active when ny - 1 is a power of two: the periodic 128x129 case
-and the confined 48x65 case below).  Tolerances as in
+and the confined 48x65 case below), the batched and fused periodic step, and the
+one-launch y-stage of the convective terms (conv3; its off form is the three launches it
+replaces).  Tolerances as in
 test_gpu_navier.py (1e-12 for T/ux/uy/p after the steps taken).
 """
 import numpy as np
@@ -13,7 +15,7 @@ from helpers import compare, init_pair
 
 pytestmark = pytest.mark.gpu
 
-OPTS = {"graph": 0, "mfma": 1, "fft": 2, "bands": 3, "dct": 4, "batch": 5, "fuse": 6}
+OPTS = {"graph": 0, "mfma": 1, "fft": 2, "bands": 3, "dct": 4, "batch": 5, "fuse": 6, "conv3": 8}
 
 
 def _pair(oracle, periodic):
@@ -32,7 +34,7 @@ def _pair(oracle, periodic):
 
 
 @pytest.mark.parametrize("periodic", [False, True])
-@pytest.mark.parametrize("off", [None, "graph", "mfma", "fft", "bands", "dct", "batch", "fuse"])
+@pytest.mark.parametrize("off", [None, "graph", "mfma", "fft", "bands", "dct", "batch", "fuse", "conv3"])
 def test_option_parity(oracle, periodic, off):
     print("case options periodic=%s off=%s" % (periodic, off), flush=True)
     g, o = _pair(oracle, periodic)
