@@ -482,7 +482,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the statistics / snapshot / solid-obstacle legs (profiler runs of the step alone)")
+                    help="skip the mixed-error, statistics / snapshot and solid-obstacle legs (profiler runs of "
+                         "the step alone)")
     ap.add_argument("--nu-time", type=float, default=0.0, metavar="T",
                     help="instead of timing: the mixed and the FP64 step over T time units, Nu averaged "
                          "(sampled once per time unit)")
@@ -536,7 +537,8 @@ def main():
         del nav64
         extra["fp64_steps_per_s"] = args.steps / el64
         extra["mixed_speedup_vs_fp64"] = el64 / elapsed
-        extra["mixed_error_vs_fp64"] = mixed_errors(args.config, rustpde_amd)
+        if not args.no_extras:
+            extra["mixed_error_vs_fp64"] = mixed_errors(args.config, rustpde_amd)
 
     # the same K steps launched one kernel at a time with HIP events recorded on the
     # handle's stream around every launch: per-kernel-class durations for the roofline

@@ -365,6 +365,7 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
   constexpr double invN = 1.0 / (double)N;  // exact: N is a power of two
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
   __shared__ V uxs[P * n], uys[P * n];  // the physical velocity of the block's rows
+  __shared__ V cs0[P * n], cs1[P * n];  // the convective terms of ux, then uy (temp reuses cs0)
   const V inv_sc = (V)inv_sc_d;
   const int tid = threadIdx.x, row0 = P * blockIdx.x;
   // raw coefficients c_kk, c_kk-2 of row r of one slot (zero outside the row)
@@ -401,11 +402,14 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
       im[lp(p * L + L - k)] = b;
     }
   };
-  // dealiased forward transform of the pair (a, b) into out slots fa (and fa + 1 with hasb)
-  auto forward = [&](const V (&a)[Q], const V (&b)[Q], int fa, bool hasb) {
+  // dealiased forward transform of the pair (cs0, cs1) into out slots fa (and fa + 1 with hasb)
+  auto forward = [&](int fa, bool hasb) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < Q; ++q) put(q, a[q], hasb ? b[q] : (V)0);
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      if (e < P * n) put(q, cs0[e], hasb ? cs1[e] : (V)0);
+    }
     __syncthreads();
     lds_dft<L, T, G::E>(re, im, tw, fftlds::opaque_tid());
 #pragma unroll
@@ -446,7 +450,6 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
       uys[e] = (V)(Tout)im[lp(p * L + j)];
     }
   }
-  V res0[Q], res1[Q];  // the convective terms of ux, then uy (temp reuses res0)
   // ---- transforms 1..3: the convective terms
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
@@ -475,9 +478,10 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
         const int e = tid + q * T;
         const int p = e / n, j = e - p * n, r = row0 + p;
         const bool ok = e < P * n && r < rows;
-        const size_t o = (size_t)r * ldf + j;
-        g1[q] = (ok && f1) ? (V)f1[o] : (V)0;
-        g2[q] = (ok && f2) ? (V)f2[o] : (V)0;
+        const size_t o = (size_t)min(r, rows - 1) * ldf + min(j, n - 1);  // always valid
+        const V a1 = f1 ? (V)f1[o] : (V)0, a2 = f2 ? (V)f2[o] : (V)0;
+        g1[q] = ok ? a1 : (V)0;
+        g2[q] = ok ? a2 : (V)0;
       }
     }
     __syncthreads();
@@ -544,11 +548,10 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
           if (f2) v = v + uy * g2[q];
         }
       }
-      if (f == 1) res1[q] = (V)(Tout)v;
-      else res0[q] = (V)(Tout)v;
+      if (e < P * n) (f == 1 ? cs1 : cs0)[e] = (V)(Tout)v;
     }
-    if (f == 1) forward(res0, res1, 0, true);  // (out_ux, out_uy)
-    if (f == 2) forward(res0, res1, 2, false);  // (out_temp, 0)
+    if (f == 1) forward(0, true);   // (out_ux, out_uy)
+    if (f == 2) forward(2, false);  // (out_temp, 0)
   }
 }
 

@@ -472,6 +472,8 @@ class Navier {
   DBuf THAT_BC, DTBCX, DTBCY, GXX, GYY;
   // the physical fieldbc gradient: d/dx identically zero, d/dy the same in every x row
   bool bc_dx_zero = false, bc_dy_xconst = false;
+  // rows of THAT_BC, GXX and GYY (spectral, [rows, ny]) before which every value is +0.0
+  int bc_rows = 0;
   // workspaces
   DBuf THAT, UXP, UYP, A, G, CONV, CV, RHS, DIV, W1, W2, W3, W4, W9, RED;
   std::vector<double> wx_dx, wy_dx;  // cell widths of `field` (unscaled, field.rs:135-163)
@@ -1150,6 +1152,17 @@ class Navier {
     for (size_t i = (size_t)ny; i < hdy.size() && bc_dy_xconst; ++i) bc_dy_xconst = hdy[i] == hdy[i % (size_t)ny];
     GXX.upload(a);
     GYY.upload(b);
+    {
+      const std::vector<double> t = that.to_double();
+      const size_t rows = t.size() / (size_t)ny;
+      bc_rows = 0;
+      for (size_t r = 0; r < rows; ++r)
+        for (size_t i = 0; i < (size_t)ny; ++i) {
+          const size_t j = r * ny + i;
+          for (double v : {t[j], a[j], b[j]})
+            if (v != 0.0 || std::signbit(v)) bc_rows = (int)r + 1;
+        }
+    }
     // physical fieldbc (Field2::v after bc_rbc's backward)
     LMat v = host_y(by_bc, "BS", host_x(xb, "BS", V, sx));
     BCv.upload(v.to_double());
@@ -1408,16 +1421,18 @@ class Navier {
     // d/dy pres, then the right-hand sides (navier.rs:622-674) with to_ortho of ux, uy, temp,
     // the buoyancy temp + fieldbc (navier.rs:740-742) and d/dx pres formed in the same kernel
     if (!gpres_valid || !pressure_fused_ok()) yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
-    // bytes: ux, uy, temp coefficients, pres, d/dy pres, fieldbc, gxx, gyy and three
-    // convective terms read, three right-hand sides written
+    // bytes: ux, uy, temp coefficients, pres, d/dy pres and three convective terms read,
+    // three right-hand sides written; fieldbc, gxx, gyy in their bc_rows nonzero rows
+    const double bcb = 8.0 * 3.0 * bc_rows * ny;
     if (mixed) {  // FP32 convective terms (conv_mixed)
-      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 8.0 * ny) + 4.0 * 2.0 * mx() * 3.0 * ny);
+      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 5.0 * ny) + 4.0 * 2.0 * mx() * 3.0 * ny + bcb);
       rhs3_fused_cv32(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p,
-                      reinterpret_cast<const float*>(MIX.p), GXX.p, GYY.p, RHS.p, mx(), ny, scale[0], dt, (long)F);
+                      reinterpret_cast<const float*>(MIX.p), GXX.p, GYY.p, RHS.p, mx(), ny, scale[0], dt, (long)F,
+                      bc_rows);
     } else {
-      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 11.0 * ny));
+      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 8.0 * ny) + bcb);
       rhs3_fused(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p, CV.p, GXX.p, GYY.p, RHS.p, mx(), ny,
-                 scale[0], dt, (long)F);
+                 scale[0], dt, (long)F, bc_rows);
     }
     timer.end(st);
     solves_and_pressure();

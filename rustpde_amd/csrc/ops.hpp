@@ -145,13 +145,14 @@ void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const
 // The same with the to_ortho stencil of ux, uy, temp (U: slots of [2 mx, my], taps
 // stencil[0, ny) and [ny, 2 ny)), the buoyancy temp + fieldbc (thbc) and d/dx pres (from P0,
 // Fourier x) formed on the fly; g1 = d/dy pres; all other arrays [2 mx, ny], slot stride F.
+// thbc, gxx, gyy are read in rows < bcr only: every later row must hold +0.0.
 void rhs3_fused(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                 const double* thbc, const double* cv, const double* gxx, const double* gyy, double* out, int mx, int ny,
-                double kscale, double dt, long F);
+                double kscale, double dt, long F, int bcr);
 // the same with FP32 convective terms (the mixed-precision step)
 void rhs3_fused_cv32(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                      const double* thbc, const float* cv, const double* gxx, const double* gyy, double* out, int mx,
-                     int ny, double kscale, double dt, long F);
+                     int ny, double kscale, double dt, long F, int bcr);
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n);              // f (-c) + u
 // ((((a - g dt) + c dt) + b dt) + (lx dt) k) + (ly dt) k; g, b may be null
 void ew_adj_rhs(hipStream_t s, double* out, const double* a, const double* g, const double* c, const double* b,
