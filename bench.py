@@ -253,7 +253,7 @@ def stats_and_snapshot(nav, updates=20):
     nav.synchronize()
     upd_ms = 1e3 * (time.perf_counter() - t0) / updates
     with tempfile.TemporaryDirectory() as d:
-        f = os.path.join(d, "snap.npz")
+        f = os.path.join(d, "snap" + checkpoint._SUFFIX)
         t0 = time.perf_counter()
         checkpoint.write_navier(nav, f)
         w_ms = 1e3 * (time.perf_counter() - t0)
@@ -263,9 +263,10 @@ def stats_and_snapshot(nav, updates=20):
         nav.synchronize()
         r_ms = 1e3 * (time.perf_counter() - t0)
     return {"stats_update_ms": upd_ms, "stats_updates_timed": updates, "snapshot_write_ms": w_ms,
-            "snapshot_read_ms": r_ms, "snapshot_bytes": size,
+            "snapshot_read_ms": r_ms, "snapshot_bytes": size, "snapshot_format": checkpoint._SUFFIX,
             "note": "statistics update: wall time per call with the queue drained at the end (FP64 on the "
-                    "device); snapshot: device -> host -> .npz container in a temporary directory and back"}
+                    "device); snapshot: device -> host -> file (HDF5 through libhdf5, or .npz where it is "
+                    "absent) in a temporary directory and back"}
 
 
 def bench_hholtz_batch(args):

@@ -1,8 +1,6 @@
 """Checkpoint / restart of a GPU Navier2D (Navier2D::write / read, navier.rs:961-1008).
 
-libhdf5 is not available to this library, so the snapshot is a NumPy ``.npz`` container
-(a zip of ``.npy`` arrays, loaded with ``allow_pickle=False``) holding exactly the
-datasets the reference writes to its HDF5 file, under the same paths:
+A snapshot holds exactly the datasets the reference writes, under the same paths:
 
 * per field group (``temp``, ``ux``, ``uy``, ``pres``): ``<g>/v`` (physical; ``temp/v``
   includes the boundary field, navier.rs:987-990) and ``<g>/vhat`` (spectral; complex
@@ -13,15 +11,21 @@ datasets the reference writes to its HDF5 file, under the same paths:
   ``time``, ``ra``, ``pr``, ``nu``, ``kappa``;
 * ``solid/mask`` when an obstacle is set (navier.rs:995-998).
 
-A converter to HDF5 is a one-to-one copy of these datasets.  Reading restores temp, ux,
-uy and pres[0] from their ``vhat`` (broadcast to the handle's shape when the file holds
-another resolution, field/read.rs:62-77 and :112-121) and the time.
+The container follows the file name: ``.h5`` is an HDF5 file written and read through the
+HDF5 C library (rustpde_amd.h5; the reference's files read here and ours there), anything
+else a NumPy ``.npz`` archive of the same datasets (loaded with ``allow_pickle=False``),
+for hosts without libhdf5.  ``callback`` names its files with ``_SUFFIX``: ``.h5`` where the
+library is found.  Reading restores temp, ux, uy and pres[0] from their ``vhat``
+(broadcast to the handle's shape when the file holds another resolution,
+field/read.rs:62-77 and :112-121) and the time.
 """
 import math
 
 import numpy as np
 
-_SUFFIX = ".npz"
+from . import h5
+
+_SUFFIX = ".h5" if h5.available() else ".npz"
 
 
 def write_container(filename, data):
@@ -36,22 +40,28 @@ def write_container(filename, data):
             flat[k + "_im"] = np.ascontiguousarray(a.imag)
         else:
             flat[k] = a
+    if str(filename).endswith(".h5"):
+        h5.write(str(filename), flat)
+        return
     with open(filename, "wb") as f:  # a file object: numpy appends no suffix
         np.savez(f, **flat)
 
 
 def read_container(filename):
     """{dataset path: array} with _re/_im pairs joined into complex arrays."""
+    if str(filename).endswith(".h5"):
+        z = h5.read(str(filename))
+    else:
+        with np.load(filename, allow_pickle=False) as f:
+            z = {k: f[k] for k in f.keys()}
     out = {}
-    with np.load(filename, allow_pickle=False) as z:
-        keys = list(z.keys())
-        for k in keys:
-            if k.endswith("_im") and k[:-3] + "_re" in keys:
-                continue
-            if k.endswith("_re") and k[:-3] + "_im" in keys:
-                out[k[:-3]] = z[k] + 1j * z[k[:-3] + "_im"]
-            else:
-                out[k] = z[k]
+    for k in z:
+        if k.endswith("_im") and k[:-3] + "_re" in z:
+            continue
+        if k.endswith("_re") and k[:-3] + "_im" in z:
+            out[k[:-3]] = z[k] + 1j * z[k[:-3] + "_im"]
+        else:
+            out[k] = z[k]
     return out
 
 
@@ -90,7 +100,7 @@ def get_dx(x, periodic):
 
 
 def flow_filename(time):
-    """data/flow{:0>8.2}.h5 of navier.rs:782, with this container's suffix."""
+    """data/flow{:0>8.2}.h5 of navier.rs:782 (the .npz suffix on hosts without libhdf5)."""
     return "flow%s%s" % (format(time, "0>8.2f"), _SUFFIX)
 
 

@@ -49,9 +49,10 @@ def test_solid_vs_oracle(oracle, periodic, graph, tmp_path):
         assert e[k] < 1e-12, (k, e)
     assert e["pseu"] < 1e-7, e
     # a snapshot carries the mask (navier.rs:995-998)
-    f = str(tmp_path / "solid.npz")
+    from rustpde_amd import checkpoint
+    f = str(tmp_path / ("solid" + checkpoint._SUFFIX))
     g.write(f)
-    assert np.array_equal(np.load(f)["solid/mask"], masks[0])
+    assert np.array_equal(checkpoint.read_container(f)["solid/mask"], masks[0])
     # the obstacle changes the flow (the same step without it differs)
     h, _, _ = _pair(oracle, periodic)
     h.update(6)
