@@ -168,9 +168,10 @@ int rbc_set_tensor(rbc_navier* h, int s, const double* lam, const double* fwd, c
  *                  stays FP64).  Fields agree with the FP64 step to ~1e-9 after 10 steps.
  *   RBC_OPT_CONV3  periodic batched mixed step with RBC_OPT_FUSE: the y-stage of the FP32
  *                  convective terms (physical ux, uy, the three terms and their dealiased
- *                  forward y-transforms) as one launch per x-row (default 1; 0 = three
- *                  launches: y-backward of ux, uy, the convective-term launch, the forward
- *                  launch).  The FP64 step always runs the three launches. */
+ *                  forward y-transforms) as one launch per x-row: 1 (default) when nx is at
+ *                  least four times the GPU's CU count, 2 always, 0 never (three launches:
+ *                  y-backward of ux, uy, the convective-term launch, the forward launch).
+ *                  The FP64 step always runs the three launches. */
 #define RBC_OPT_GRAPH 0
 #define RBC_OPT_MFMA 1
 #define RBC_OPT_FFT 2

@@ -486,7 +486,9 @@ class Navier {
   bool use_dct = true;    // Chebyshev y transforms by FFT-based DCT-I kernels
   bool use_batch = true;  // periodic step: the three fields' transforms and solves batched
   bool use_fuse = true;   // periodic step: the pressure stage as one fused kernel
-  bool use_conv3 = true;  // periodic fused mixed step: the y-stage of the convective terms in one launch
+  // periodic fused mixed step: the y-stage of the convective terms in one launch (0 off, 1 when
+  // its blocks fill every CU four times, 2 always)
+  int use_conv3 = 1;
   // ensemble group: this handle (the group's leader) advances nb instances of the same size
   // whose buffers are slots nb * bs apart; per-instance nu and ADI operators in slabs
   int nb = 1;
@@ -1544,7 +1546,24 @@ class Navier {
   // three launches: the one-launch kernel holds twice the registers in FP64 and runs two
   // blocks per CU, which measured slower.
   bool conv3_ok() const {
-    return use_conv3 && cheb_y_supported(ny) && by_u.k == CD && by_u.n == ny && by_u.m == ny - 2 && dirichlet_taps(by_u);
+    if (!use_conv3 || !cheb_y_supported(ny) || by_u.k != CD || by_u.n != ny || by_u.m != ny - 2 ||
+        !dirichlet_taps(by_u))
+      return false;
+    // one block per x-row runs six transforms in turn, four blocks per CU: below four blocks
+    // per CU the three separate launches are faster (measured at 256 x 257: 14,530 against
+    // 13,880 steps/s; at 1024 x 1025 the one launch wins, 5,558 against 5,465)
+    return use_conv3 == 2 || nx >= 4 * device_cus();
+  }
+  static int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+      int dev = 0, v = 0;
+      if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+        cus = v;
+      else
+        cus = 256;
+    }
+    return cus;
   }
   // the stencil of b is the Dirichlet one the conv3 kernel forms in place (d = 1, l = -1)
   static bool dirichlet_taps(const Basis& b) {
@@ -2845,7 +2864,10 @@ void apply_option(Navier& n, int option, int value, bool alloc = true) {
     case RBC_OPT_DCT: n.use_dct = value != 0; break;
     case RBC_OPT_BATCH: n.use_batch = value != 0; break;
     case RBC_OPT_FUSE: n.use_fuse = value != 0; break;
-    case RBC_OPT_CONV3: n.use_conv3 = value != 0; break;
+    case RBC_OPT_CONV3:
+      if (value < 0 || value > 2) throw std::invalid_argument("RBC_OPT_CONV3: 0, 1 or 2");
+      n.use_conv3 = value;
+      break;
     case RBC_OPT_MIXED:
       if (value && n.solid_on)
         throw std::invalid_argument("RBC_OPT_MIXED: a handle with a solid obstacle runs the field-by-field FP64 step; "
