@@ -1118,7 +1118,14 @@ class Navier {
     // backward then forward of the Space2(x, cheb_dirichlet_bc(ny))
     LMat v = host_y(by_bc, "BS", host_x(xb, "BS", vh, (ld)scale[0]));
     LMat vh2 = host_x(xb, "FS", host_y(by_bc, "FS", v), (ld)scale[0]);
-    set_bc_spec(vh2.to_double());
+    std::vector<double> out = vh2.to_double();
+    if (periodic) {
+      // The boundary values are the same in every x row, and a real FFT of a constant row has
+      // exact zeros at k != 0 (its butterflies subtract equal values), as in the reference's
+      // and the oracle's fieldbc; the dense long-double DFT above leaves ~1e-19 there instead
+      for (size_t i = 2; i < out.size(); ++i) out[i] = 0.0;
+    }
+    set_bc_spec(out);
   }
 
   // fieldbc.vhat (device layout [rows_ortho(x), 2]) and all derived constants
@@ -1145,8 +1152,17 @@ class Navier {
     std::vector<double> a = gxx.to_double(), b = gyy.to_double();
     for (auto& v : a) v = (v * dt) * ka;
     for (auto& v : b) v = (v * dt) * ka;
-    THAT_BC.upload(that.to_double());
-    const std::vector<double> hdx = dxp.to_double(), hdy = dyp.to_double();
+    std::vector<double> t = that.to_double(), hdx = dxp.to_double(), hdy = dyp.to_double();
+    // a Fourier x basis and a fieldbc with only the k = 0 row (constant in x): every derived
+    // array is zero beyond that row and the x-gradient is zero; they are stored as +0.0, the
+    // zeros the reference's transforms form for them (up to the sign of a zero)
+    bool xconst = periodic;
+    for (size_t i = 2; i < vh.size() && xconst; ++i) xconst = vh[i] == 0.0;
+    if (xconst) {
+      for (size_t i = (size_t)ny; i < t.size(); ++i) t[i] = a[i] = b[i] = 0.0;
+      std::fill(hdx.begin(), hdx.end(), 0.0);
+    }
+    THAT_BC.upload(t);
     DTBCX.upload(hdx);
     DTBCY.upload(hdy);
     bc_dx_zero = std::all_of(hdx.begin(), hdx.end(), [](double v) { return v == 0.0; });
@@ -1155,7 +1171,6 @@ class Navier {
     GXX.upload(a);
     GYY.upload(b);
     {
-      const std::vector<double> t = that.to_double();
       const size_t rows = t.size() / (size_t)ny;
       bc_rows = 0;
       for (size_t r = 0; r < rows; ++r)
