@@ -44,11 +44,13 @@ struct CGeo {
   static constexpr int m = N + 1;  // half-spectrum length
 };
 
-// Tin: input type, V: arithmetic (and LDS) type, Tout: output type
+// Tin: input type, V: arithmetic (and LDS) type, Tout: output type.  Columns j >= nzc are
+// zero on input (the rows a y-dealiasing cut cleared): a block whose columns all lie there
+// writes their zero spectra without loading or transforming.
 template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
                                                                int cols, int cut, const double* __restrict__ tw,
-                                                               long sx, long sy) {
+                                                               long sx, long sy, int nzc) {
   using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, m = G::m;
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
@@ -56,6 +58,15 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(cons
   Y += sy * blockIdx.y;
   const int tid = threadIdx.x;
   const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
+  if (j0 >= nzc) {
+    for (int e = tid; e < m * C; e += T) {
+      const int c = e % C, k = e / C, j = j0 + c;
+      if (j >= cols) continue;
+      Y[(size_t)k * cols + j] = (Tout)0;
+      Y[(size_t)(m + k) * cols + j] = (Tout)0;
+    }
+    return;
+  }
   // load: row t, column c -> z_t of pair c / 2 (real part for even c, imaginary for odd)
   constexpr int QL = L * C / T;
   V v[QL];
@@ -194,11 +205,11 @@ void check_n(int n) {
 }
 
 template <int L, typename Tin, typename V, typename Tout>
-void launch_rfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int cut, int batch, long sx, long sy) {
+void launch_rfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int cut, int batch, long sx, long sy, int nzc) {
   using G = CGeo<L, sizeof(V)>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
   RBC_LAUNCH((rfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut,
-             fft_table(L), sx, sy);
+             fft_table(L), sx, sy, nzc < 0 ? cols : nzc);
 }
 template <int L, typename Tin, typename V, typename Tout>
 void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, double sc, int batch, long sx, long sy,
@@ -227,11 +238,12 @@ bool fft_supported(int n) { return n >= 8 && n <= 2048 && (n & (n - 1)) == 0; }
     default: throw std::invalid_argument("fft: unsupported length");        \
   }
 
-void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch, long sx, long sy) {
+void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch, long sx, long sy,
+               int nzc) {
   check_n(n);
   if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("rfft_cols: bad batch or cols");
   cut = std::min(cut, n / 2 + 1);
-#define RBC_RFFT(LL) launch_rfft<LL, double, double, double>(s, X, Y, cols, cut, batch, sx, sy)
+#define RBC_RFFT(LL) launch_rfft<LL, double, double, double>(s, X, Y, cols, cut, batch, sx, sy, nzc)
   RBC_FFT_SWITCH(n, RBC_RFFT)
 #undef RBC_RFFT
   check("rfft_cols");
@@ -273,17 +285,18 @@ void rfft_cols_f32(hipStream_t s, const float* X, double* Y, int n, int cols, in
   check_n(n);
   if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("rfft_cols_f32: bad batch or cols");
   cut = std::min(cut, n / 2 + 1);
-#define RBC_RFFT32(LL) launch_rfft<LL, float, float, double>(s, X, Y, cols, cut, batch, sx, sy)
+#define RBC_RFFT32(LL) launch_rfft<LL, float, float, double>(s, X, Y, cols, cut, batch, sx, sy, -1)
   RBC_FFT_SWITCH(n, RBC_RFFT32)
 #undef RBC_RFFT32
   check("rfft_cols_f32");
 }
 
-void rfft_cols_f32f(hipStream_t s, const float* X, float* Y, int n, int cols, int cut, int batch, long sx, long sy) {
+void rfft_cols_f32f(hipStream_t s, const float* X, float* Y, int n, int cols, int cut, int batch, long sx, long sy,
+                    int nzc) {
   check_n(n);
   if (batch < 1 || batch > 65535 || cols < 1) throw std::invalid_argument("rfft_cols_f32f: bad batch or cols");
   cut = std::min(cut, n / 2 + 1);
-#define RBC_RFFT32F(LL) launch_rfft<LL, float, float, float>(s, X, Y, cols, cut, batch, sx, sy)
+#define RBC_RFFT32F(LL) launch_rfft<LL, float, float, float>(s, X, Y, cols, cut, batch, sx, sy, nzc)
   RBC_FFT_SWITCH(n, RBC_RFFT32F)
 #undef RBC_RFFT32F
   check("rfft_cols_f32f");

@@ -1404,9 +1404,11 @@ class Navier {
     irfft_cols(st, in, out, nx, cols, times, sc, batch, (long)FS, (long)FS);
     timer.end(st);
   }
-  void RFFT_B(const double* in, double* out, int cols, int cut, int batch) {
-    timer.begin(st, 5, 8.0 * batch * cols * ((double)nx + 2.0 * mx()));
-    rfft_cols(st, in, out, nx, cols, cut, batch, (long)FS, (long)FS);
+  // nzc >= 0: the input columns j >= nzc are zero (the y-dealiasing cut), only written as zeros
+  void RFFT_B(const double* in, double* out, int cols, int cut, int batch, int nzc = -1) {
+    const double tc = nzc >= 0 ? (double)nzc : (double)cols;  // transformed columns
+    timer.begin(st, 5, 8.0 * batch * (tc * nx + cols * 2.0 * mx()));
+    rfft_cols(st, in, out, nx, cols, cut, batch, (long)FS, (long)FS, nzc);
     timer.end(st);
   }
   // Chebyshev y backward of composite by (times derivatives): out = x .* U + beta out
@@ -1438,18 +1440,21 @@ class Navier {
     // d/dy pres, then the right-hand sides (navier.rs:622-674) with to_ortho of ux, uy, temp,
     // the buoyancy temp + fieldbc (navier.rs:740-742) and d/dx pres formed in the same kernel
     if (!gpres_valid || !pressure_fused_ok()) yop(by_p, "DS", P0.p, rows_ortho(bx_p), G.p + F);
-    // bytes: ux, uy, temp coefficients, pres, d/dy pres and three convective terms read,
-    // three right-hand sides written; fieldbc, gxx, gyy in their bc_rows nonzero rows
+    // bytes: ux, uy, temp coefficients, pres, d/dy pres read, three right-hand sides
+    // written; fieldbc, gxx, gyy in their bc_rows nonzero rows; the three convective terms
+    // inside the dealiasing cuts (bx_f.m 2/3 wavenumbers, ny 2/3 y-modes: zero beyond)
     const double bcb = 8.0 * 3.0 * bc_rows * ny;
+    const int cutx = bx_f.m * 2 / 3, cuty = ny * 2 / 3;
+    const double cvn = 3.0 * 2.0 * cutx * cuty;
     if (mixed) {  // FP32 convective terms (conv_mixed)
-      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 5.0 * ny) + 4.0 * 2.0 * mx() * 3.0 * ny + bcb);
+      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 5.0 * ny) + 4.0 * cvn + bcb);
       rhs3_fused_cv32(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p,
                       reinterpret_cast<const float*>(MIX.p), GXX.p, GYY.p, RHS.p, mx(), ny, scale[0], dt, (long)F,
-                      bc_rows);
+                      bc_rows, cutx, cuty);
     } else {
-      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 8.0 * ny) + bcb);
+      timer.begin(st, 3, 8.0 * 2.0 * mx() * (3.0 * my + 5.0 * ny) + 8.0 * cvn + bcb);
       rhs3_fused(st, UX.p, my, stencil_band(by_u).coef.p, P0.p, G.p + F, THAT_BC.p, CV.p, GXX.p, GYY.p, RHS.p, mx(), ny,
-                 scale[0], dt, (long)F, bc_rows);
+                 scale[0], dt, (long)F, bc_rows, cutx, cuty);
     }
     timer.end(st);
     solves_and_pressure();
@@ -1474,7 +1479,7 @@ class Navier {
                   (long)FS, (long)FS, UXP.p, DTBCX.p, UYP.p, DTBCY.p, 2);
       timer.end(st);
       CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
-      RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
+      RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3, ny * 2 / 3);  // y-modes >= the cut are zero
       return;
     }
     // ux, uy in physical space
@@ -1527,8 +1532,9 @@ class Navier {
         cheb_y_conv3_f32(st, W1f, W2f, my, my, CONVf, ny, nx, ny, 1.0 / scale[1],
                          bc_dx_zero ? nullptr : DTBCX.p, DTBCY.p, conv3_ldf(), ny * 2 / 3, F, F);
         timer.end(st);
-        timer.begin(st, 5, 3.0 * ny * (4.0 * nx + 4.0 * mx2));
-        rfft_cols_f32f(st, CONVf, W1f, nx, ny, bx_f.m * 2 / 3, 3, F, F);
+        const int nzc = ny * 2 / 3;  // y-modes >= the cut are zero: written, not transformed
+        timer.begin(st, 5, 3.0 * (4.0 * nx * nzc + 4.0 * mx2 * ny));
+        rfft_cols_f32f(st, CONVf, W1f, nx, ny, bx_f.m * 2 / 3, 3, F, F, nzc);
         timer.end(st);
         return;
       }
@@ -1552,8 +1558,9 @@ class Navier {
     cheb_y_forward_f32(st, CONVf, ny, W2f, ny, nx, ny, ny * 2 / 3, 3, F, F);
     timer.end(st);
     // FP32 convective terms into W1f's slots (free after the y transforms)
-    timer.begin(st, 5, 3.0 * ny * (4.0 * nx + 4.0 * mx2));
-    rfft_cols_f32f(st, W2f, W1f, nx, ny, bx_f.m * 2 / 3, 3, F, F);
+    const int nzc = ny * 2 / 3;  // y-modes >= the cut are zero: written, not transformed
+    timer.begin(st, 5, 3.0 * (4.0 * nx * nzc + 4.0 * mx2 * ny));
+    rfft_cols_f32f(st, W2f, W1f, nx, ny, bx_f.m * 2 / 3, 3, F, F, nzc);
     timer.end(st);
   }
   // the one-launch y-stage of the FP32 convective terms: DCT rows and the Dirichlet stencil

@@ -236,14 +236,16 @@ __global__ void ew_rhs3_kernel(double* out, const double* a, const double* g, co
 // as FP64, so the FP32 values enter exactly as the FP64 copy of them would).
 // thbc, gxx and gyy are read in rows r < bcr only; the caller guarantees that all later rows
 // hold +0.0, which is then added as a literal (a boundary field constant in x has only the
-// k = 0 row in Fourier space), with the same result bits.
+// k = 0 row in Fourier space), with the same result bits.  Likewise the convective terms are
+// read for wavenumbers k < cutx and y-modes i < cuty only: the dealiased transforms leave
+// zeros beyond both cuts.
 template <typename CV>
 __global__ void rhs3_fused_kernel(const double* __restrict__ U, int my, const double* __restrict__ sc0,
                                   const double* __restrict__ sc1, const double* __restrict__ P0,
                                   const double* __restrict__ g1, const double* __restrict__ thbc,
                                   const CV* __restrict__ cv, const double* __restrict__ gxx,
                                   const double* __restrict__ gyy, double* __restrict__ out, int mx, int ny, double kscale,
-                                  double dt, long F, int bcr) {
+                                  double dt, long F, int bcr, int cutx, int cuty) {
 #pragma clang fp contract(off)
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)2 * mx * ny) return;
@@ -277,30 +279,31 @@ __global__ void rhs3_fused_kernel(const double* __restrict__ U, int my, const do
   }
   const double gx = imag ? nb : na;
   const long j = (long)r * ny + i;
+  const bool cvz = k >= cutx || i >= cuty;  // beyond the dealiasing cuts: zero terms
   const bool bc = r < bcr;
   const double tb = bc ? thbc[j] : 0.0, gx2 = bc ? gxx[j] : 0.0, gy2 = bc ? gyy[j] : 0.0;
   const double buoy = a[2] + tb;
   double r0 = a[0] - gx * dt;
-  out[j] = r0 - (double)cv[j] * dt;
+  out[j] = r0 - (cvz ? 0.0 : (double)cv[j]) * dt;
   double r1 = a[1] - g1[j] * dt;
   r1 = r1 + buoy * dt;
-  out[F + j] = r1 - (double)cv[F + j] * dt;
-  out[2 * F + j] = ((a[2] + gx2) + gy2) - (double)cv[2 * F + j] * dt;
+  out[F + j] = r1 - (cvz ? 0.0 : (double)cv[F + j]) * dt;
+  out[2 * F + j] = ((a[2] + gx2) + gy2) - (cvz ? 0.0 : (double)cv[2 * F + j]) * dt;
 }
 void rhs3_fused(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                 const double* thbc, const double* cv, const double* gxx, const double* gyy, double* out, int mx, int ny,
-                double kscale, double dt, long F, int bcr) {
+                double kscale, double dt, long F, int bcr, int cutx, int cuty) {
   const long n = (long)2 * mx * ny;
   RBC_LAUNCH(rhs3_fused_kernel<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s, U, my, stencil, stencil + ny, P0,
-             g1, thbc, cv, gxx, gyy, out, mx, ny, kscale, dt, F, bcr);
+             g1, thbc, cv, gxx, gyy, out, mx, ny, kscale, dt, F, bcr, cutx, cuty);
   check_launch("rhs3_fused");
 }
 void rhs3_fused_cv32(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                      const double* thbc, const float* cv, const double* gxx, const double* gyy, double* out, int mx,
-                     int ny, double kscale, double dt, long F, int bcr) {
+                     int ny, double kscale, double dt, long F, int bcr, int cutx, int cuty) {
   const long n = (long)2 * mx * ny;
   RBC_LAUNCH(rhs3_fused_kernel<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, U, my, stencil, stencil + ny, P0,
-             g1, thbc, cv, gxx, gyy, out, mx, ny, kscale, dt, F, bcr);
+             g1, thbc, cv, gxx, gyy, out, mx, ny, kscale, dt, F, bcr, cutx, cuty);
   check_launch("rhs3_fused_cv32");
 }
 

@@ -145,14 +145,15 @@ void ew_rhs3(hipStream_t s, double* out, const double* a, const double* g, const
 // The same with the to_ortho stencil of ux, uy, temp (U: slots of [2 mx, my], taps
 // stencil[0, ny) and [ny, 2 ny)), the buoyancy temp + fieldbc (thbc) and d/dx pres (from P0,
 // Fourier x) formed on the fly; g1 = d/dy pres; all other arrays [2 mx, ny], slot stride F.
-// thbc, gxx, gyy are read in rows < bcr only: every later row must hold +0.0.
+// thbc, gxx, gyy are read in rows < bcr only: every later row must hold +0.0; cv is read for
+// wavenumbers < cutx and y-modes < cuty only (zero beyond: the dealiasing cuts).
 void rhs3_fused(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                 const double* thbc, const double* cv, const double* gxx, const double* gyy, double* out, int mx, int ny,
-                double kscale, double dt, long F, int bcr);
+                double kscale, double dt, long F, int bcr, int cutx, int cuty);
 // the same with FP32 convective terms (the mixed-precision step)
 void rhs3_fused_cv32(hipStream_t s, const double* U, int my, const double* stencil, const double* P0, const double* g1,
                      const double* thbc, const float* cv, const double* gxx, const double* gyy, double* out, int mx,
-                     int ny, double kscale, double dt, long F, int bcr);
+                     int ny, double kscale, double dt, long F, int bcr, int cutx, int cuty);
 void ew_project(hipStream_t s, double* u, const double* f, double c, long n);              // f (-c) + u
 // ((((a - g dt) + c dt) + b dt) + (lx dt) k) + (ly dt) k; g, b may be null
 void ew_adj_rhs(hipStream_t s, double* out, const double* a, const double* g, const double* c, const double* b,
@@ -190,11 +191,14 @@ bool fft_supported(int n);
 // values, rfft from FP32 values to FP64 spectra (arithmetic in FP32).
 void rfft_cols_f32(hipStream_t s, const float* X, double* Y, int n, int cols, int cut, int batch = 1, long sx = 0,
                    long sy = 0);
-void rfft_cols_f32f(hipStream_t s, const float* X, float* Y, int n, int cols, int cut, int batch, long sx, long sy);
+// nzc >= 0: the columns j >= nzc are zero on input (a y-dealiasing cut); their blocks only
+// write the zero spectra
+void rfft_cols_f32f(hipStream_t s, const float* X, float* Y, int n, int cols, int cut, int batch, long sx, long sy,
+                    int nzc = -1);
 void irfft_cols_f32(hipStream_t s, const double* X, float* Y, int n, int cols, int times, double sc, int batch = 1,
                     long sx = 0, long sy = 0);
 void rfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int cut, int batch = 1, long sx = 0,
-               long sy = 0);
+               long sy = 0, int nzc = -1);
 void irfft_cols(hipStream_t s, const double* X, double* Y, int n, int cols, int times, double sc, int batch = 1,
                 long sx = 0, long sy = 0);
 // values (entries [0, nin): times 0, sc 1 -> Y0) and (i k)^times1 / sc1 (-> Y1) of the same
