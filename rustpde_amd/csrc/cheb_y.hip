@@ -109,14 +109,17 @@ __device__ __forceinline__ V stencil_value(const StencilLoad<V>& v, bool composi
 // Tin: input type, V: arithmetic (and LDS) type, Tout: type of out and U
 // Extra products added to one batch entry after the epilogue: out += E1 F1, then out += E2 F2
 // (temp's u . grad(fieldbc) terms, navier.rs:547-550); E of the output type, F FP64, both
-// [rows, ldu], shared by the batch.
+// [rows, ldu], shared by the batch.  The convective launch also takes F with its own row
+// stride ldf (0: one row for all, a boundary field constant in x) and f1 null (a zero x
+// gradient: the term is left out).
 template <typename Tout>
 struct ExtraProducts {
   const Tout* e1;
   const double* f1;
   const Tout* e2;
   const double* f2;
-  int slot;  // batch entry that receives them (-1: none)
+  int slot;        // batch entry that receives them (-1: none)
+  long ldf = -1;   // row stride of f1, f2 (-1: ldu)
 };
 
 // DUAL: the two components of a complex transform are row r of `in` (no derivative) and
@@ -259,9 +262,9 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
       V v = im[lp(p * L + j)] * ub[q];
       v = v + (V)1 * xa;
       if (extra) {
-        const size_t o = (size_t)r * ldu + j;
-        v = v + (V)xp.e1[o] * (V)xp.f1[o];
-        v = v + (V)xp.e2[o] * (V)xp.f2[o];
+        const size_t o = (size_t)r * ldu + j, of = xp.ldf < 0 ? o : (size_t)r * xp.ldf + j;
+        if (xp.f1) v = v + (V)xp.e1[o] * (V)xp.f1[of];
+        v = v + (V)xp.e2[o] * (V)xp.f2[of];
       }
       out[(size_t)r * ldo + j] = (Tout)v;
     }
@@ -702,13 +705,14 @@ void cheb_y_backward(hipStream_t s, const double* in, int ldi, int mi, double* o
 
 void cheb_y_conv(hipStream_t s, const double* dx, const double* val, int ldi, int mi, double* out, int ldo, int rows,
                  int n, const double* coef, double inv_sc, const double* ux, const double* uy, int ldu, int batch,
-                 long si, long so, const double* e1, const double* f1, const double* e2, const double* f2, int xslot) {
+                 long si, long so, const double* e1, const double* f1, const double* e2, const double* f2, int xslot,
+                 long ldf) {
   if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_conv: unsupported length");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_conv: bad batch or rows");
   if (mi > n || (!coef && mi < 1) || !ux || !uy) throw std::invalid_argument("cheb_y_conv: bad arguments");
 #define RBC_CONV(LL)                                                                                            \
   launch_conv<LL, double, double, double>(s, dx, val, ldi, mi, out, ldo, rows, coef, inv_sc, ux, uy, ldu, batch, si, \
-                                          so, ExtraProducts<double>{e1, f1, e2, f2, xslot})
+                                          so, ExtraProducts<double>{e1, f1, e2, f2, xslot, ldf})
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_CONV)
 #undef RBC_CONV
   check("cheb_y_conv");

@@ -1472,11 +1472,13 @@ class Navier {
       CHEBY_BWD_B(by_u, W2.p, UXP.p, nx, 0, nullptr, 0.0, 2);  // ux, uy in physical space
       // conv = d/dy q uy + dq/dx ux (conv_term.rs:35-41), + u . grad(fieldbc) for temp
       const DBand& band = stencil_band(by_u);
-      // bytes, distinct arrays once: x-derivatives and values of the three fields, ux, uy,
-      // the two fieldbc gradient arrays read, three convective terms written
-      timer.begin(st, 11, 8.0 * nx * (6.0 * my + 7.0 * ny));
+      // bytes, distinct arrays once: x-derivatives and values of the three fields, ux, uy
+      // read, three convective terms written, the fieldbc gradient (one row when constant
+      // in x, the x part left out when zero)
+      const double fbr = conv3_ldf() ? (double)nx : 1.0;
+      timer.begin(st, 11, 8.0 * nx * (6.0 * my + 5.0 * ny) + 8.0 * fbr * ny * (bc_dx_zero ? 1.0 : 2.0));
       cheb_y_conv(st, W1.p, W2.p, my, my, CONV.p, ny, nx, ny, band.coef.p, 1.0 / scale[1], UXP.p, UYP.p, ny, 3,
-                  (long)FS, (long)FS, UXP.p, DTBCX.p, UYP.p, DTBCY.p, 2);
+                  (long)FS, (long)FS, UXP.p, bc_dx_zero ? nullptr : DTBCX.p, UYP.p, DTBCY.p, 2, conv3_ldf());
       timer.end(st);
       CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
       RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3, ny * 2 / 3);  // y-modes >= the cut are zero

@@ -277,10 +277,11 @@ void cheb_y_forward_f32(hipStream_t s, const float* in, int ldi, float* out, int
 // fused on load: out [rows, n] = d / sc + beta out, by the same LDS suffix scans.
 // convective term of `batch` fields from their x-derivatives and values (x physical, y composite
 // spectral, stencil coef): out = d/dy(val) .* uy + dx .* ux, one complex transform per row pair
+// (f1, f2 with row stride ldf: -1 = ldu, 0 = one row for all; f1 null: no x term)
 void cheb_y_conv(hipStream_t s, const double* dx, const double* val, int ldi, int mi, double* out, int ldo, int rows,
                  int n, const double* coef, double inv_sc, const double* ux, const double* uy, int ldu, int batch,
                  long si, long so, const double* e1 = nullptr, const double* f1 = nullptr, const double* e2 = nullptr,
-                 const double* f2 = nullptr, int xslot = -1);
+                 const double* f2 = nullptr, int xslot = -1, long ldf = -1);
 void cheb_y_conv_f32(hipStream_t s, const float* dx, const float* val, int ldi, int mi, float* out, int ldo, int rows,
                      int n, const double* coef, double inv_sc, const float* ux, const float* uy, int ldu, int batch,
                      long si, long so, const float* e1 = nullptr, const double* f1 = nullptr,
