@@ -721,13 +721,13 @@ void cheb_y_conv(hipStream_t s, const double* dx, const double* val, int ldi, in
 void cheb_y_conv_f32(hipStream_t s, const float* dx, const float* val, int ldi, int mi, float* out, int ldo, int rows,
                      int n, const double* coef, double inv_sc, const float* ux, const float* uy, int ldu, int batch,
                      long si, long so, const float* e1, const double* f1, const float* e2, const double* f2,
-                     int xslot) {
+                     int xslot, long ldf) {
   if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_conv_f32: unsupported length");
   if (batch < 1 || batch > 65535 || rows < 1) throw std::invalid_argument("cheb_y_conv_f32: bad batch or rows");
   if (mi > n || (!coef && mi < 1) || !ux || !uy) throw std::invalid_argument("cheb_y_conv_f32: bad arguments");
 #define RBC_CONV32(LL)                                                                                       \
   launch_conv<LL, float, float, float>(s, dx, val, ldi, mi, out, ldo, rows, coef, inv_sc, ux, uy, ldu, batch, si, \
-                                       so, ExtraProducts<float>{e1, f1, e2, f2, xslot})
+                                       so, ExtraProducts<float>{e1, f1, e2, f2, xslot, ldf})
   RBC_CHEB_Y_SWITCH(2 * (n - 1), RBC_CONV32)
 #undef RBC_CONV32
   check("cheb_y_conv_f32");

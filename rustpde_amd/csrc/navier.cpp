@@ -1544,9 +1544,11 @@ class Navier {
       cheb_y_backward_f32(st, W2f, my, my, UXPf, ny, nx, ny, band.coef.p, 0, 1.0 / scale[1], nullptr, ny, 0.0, 2, F, F,
                           0, nullptr, nullptr, nullptr, nullptr, -1);
       timer.end(st);
-      timer.begin(st, 11, 4.0 * nx * (6.0 * my + 5.0 * ny) + 16.0 * nx * ny);  // FP32 fields, FP64 fieldbc
+      // FP32 fields, the FP64 fieldbc gradient (one row when constant in x, no x part when zero)
+      const double fbr = conv3_ldf() ? (double)nx : 1.0;
+      timer.begin(st, 11, 4.0 * nx * (6.0 * my + 5.0 * ny) + 8.0 * fbr * ny * (bc_dx_zero ? 1.0 : 2.0));
       cheb_y_conv_f32(st, W1f, W2f, my, my, CONVf, ny, nx, ny, band.coef.p, 1.0 / scale[1], UXPf, UYPf, ny, 3, F, F,
-                      UXPf, DTBCX.p, UYPf, DTBCY.p, 2);
+                      UXPf, bc_dx_zero ? nullptr : DTBCX.p, UYPf, DTBCY.p, 2, conv3_ldf());
       timer.end(st);
     } else {
       irfft(0, 1.0, 2);

@@ -285,7 +285,7 @@ void cheb_y_conv(hipStream_t s, const double* dx, const double* val, int ldi, in
 void cheb_y_conv_f32(hipStream_t s, const float* dx, const float* val, int ldi, int mi, float* out, int ldo, int rows,
                      int n, const double* coef, double inv_sc, const float* ux, const float* uy, int ldu, int batch,
                      long si, long so, const float* e1 = nullptr, const double* f1 = nullptr,
-                     const float* e2 = nullptr, const double* f2 = nullptr, int xslot = -1);
+                     const float* e2 = nullptr, const double* f2 = nullptr, int xslot = -1, long ldf = -1);
 // the y-stage of the three FP32 convective terms (ux, uy, temp in slots 0, 1, 2 of dx / val,
 // stride si; Dirichlet y basis, mi = n - 2 coefficients) in one launch: the physical
 // velocity from val's slots 0, 1, then out_f = d/dy(val_f) .* uy + dx_f .* ux (+ ux f1 +
