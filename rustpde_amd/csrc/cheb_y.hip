@@ -39,6 +39,19 @@ void check(const char* what) {
 
 std::mutex g_mutex;
 std::map<std::pair<int, int>, double*> g_tables;
+std::map<std::pair<int, int>, float*> g_tables32;
+
+// e^{-2 pi i t / L}, t < L, interleaved (re, im), rounded from long double
+std::vector<double> table_values(int L) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  std::vector<double> t(2 * (size_t)L);
+  for (int i = 0; i < L; ++i) {
+    const long double a = -2.0L * pi * (long double)i / (long double)L;
+    t[2 * i] = (double)cosl(a);
+    t[2 * i + 1] = (double)sinl(a);
+  }
+  return t;
+}
 
 }  // namespace
 
@@ -49,18 +62,30 @@ const double* fft_table(int L) {
   auto key = std::make_pair(dev, L);
   auto it = g_tables.find(key);
   if (it != g_tables.end()) return it->second;
-  const long double pi = 3.141592653589793238462643383279502884L;
-  std::vector<double> t(2 * (size_t)L);
-  for (int i = 0; i < L; ++i) {
-    const long double a = -2.0L * pi * (long double)i / (long double)L;
-    t[2 * i] = (double)cosl(a);
-    t[2 * i + 1] = (double)sinl(a);
-  }
+  const std::vector<double> t = table_values(L);
   double* d = nullptr;
   if (hipMalloc(&d, t.size() * sizeof(double)) != hipSuccess) throw std::runtime_error("cheb_y: table allocation failed");
   if (hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     throw std::runtime_error("cheb_y: table upload failed");
   g_tables.emplace(key, d);
+  return d;
+}
+
+const float* fft_table_f32(int L) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(g_mutex);
+  auto key = std::make_pair(dev, L);
+  auto it = g_tables32.find(key);
+  if (it != g_tables32.end()) return it->second;
+  const std::vector<double> t = table_values(L);
+  std::vector<float> f(t.size());
+  for (size_t i = 0; i < t.size(); ++i) f[i] = (float)t[i];
+  float* d = nullptr;
+  if (hipMalloc(&d, f.size() * sizeof(float)) != hipSuccess) throw std::runtime_error("cheb_y: table allocation failed");
+  if (hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    throw std::runtime_error("cheb_y: table upload failed");
+  g_tables32.emplace(key, d);
   return d;
 }
 
@@ -132,7 +157,7 @@ template <int L, bool transform, bool DUAL, typename Tin, typename V, typename T
 __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out,
                                                 int ldo, int rows, const double* __restrict__ coef, int times,
                                                 double inv_sc_d, const Tout* __restrict__ U, int ldu, double beta_d,
-                                                const double* __restrict__ tw, long si, long so, long su,
+                                                const V* __restrict__ tw, long si, long so, long su,
                                                 ExtraProducts<Tout> xp, const Tin* __restrict__ in2 = nullptr,
                                                 const Tout* __restrict__ U2 = nullptr) {
   using G = Geo<L>;
@@ -322,7 +347,7 @@ template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__(Geo<L>::T) void cheb_y_bwd_kernel(
     const Tin* __restrict__ in, int ldi, int mi, Tout* __restrict__ out, int ldo, int rows,
     const double* __restrict__ coef, int times, double inv_sc, const Tout* __restrict__ U, int ldu, double beta,
-    const double* __restrict__ tw, long si, long so, long su, ExtraProducts<Tout> xp) {
+    const V* __restrict__ tw, long si, long so, long su, ExtraProducts<Tout> xp) {
   cheb_y_bwd_body<L, true, false, Tin, V, Tout>(in, ldi, mi, out, ldo, rows, coef, times, inv_sc, U, ldu, beta, tw, si,
                                                 so, su, xp);
 }
@@ -334,7 +359,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_conv_kernel(const Tin* __res
                                                                 int rows, const double* __restrict__ coef,
                                                                 double inv_sc, const Tout* __restrict__ ux,
                                                                 const Tout* __restrict__ uy, int ldu,
-                                                                const double* __restrict__ tw, long si, long so,
+                                                                const V* __restrict__ tw, long si, long so,
                                                                 ExtraProducts<Tout> xp) {
   cheb_y_bwd_body<L, true, true, Tin, V, Tout>(dx, ldi, mi, out, ldo, rows, coef, 1, inv_sc, ux, ldu, 1.0, tw, si, so,
                                                0, xp, val, uy);
@@ -360,7 +385,7 @@ template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
     const Tin* __restrict__ dx, const Tin* __restrict__ val, int ldi, int mi, Tout* __restrict__ out, int ldo, int rows,
     double inv_sc_d, const double* __restrict__ f1, const double* __restrict__ f2, long ldf, int cut,
-    const double* __restrict__ tw, long si, long so) {
+    const V* __restrict__ tw, long si, long so) {
 #pragma clang fp contract(off)
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
@@ -572,7 +597,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_diff_kernel(const double* __
 template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __restrict__ in, int ldi,
                                                                Tout* __restrict__ out, int ldo, int rows, int cut,
-                                                               const double* __restrict__ tw, long si, long so) {
+                                                               const V* __restrict__ tw, long si, long so) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
   __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
@@ -630,7 +655,7 @@ void launch_bwd(hipStream_t s, const Tin* in, int ldi, int mi, Tout* out, int ld
   (void)mask;  // the zero coefficients of absent taps stand for the mask
   if (transform) {
     RBC_LAUNCH((cheb_y_bwd_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo,
-               rows, coef, times, inv_sc, U, ldu, beta, fft_table(L), si, so, su, xp);
+               rows, coef, times, inv_sc, U, ldu, beta, twiddles<V>(L), si, so, su, xp);
   } else {
     if constexpr (std::is_same<Tin, double>::value && std::is_same<Tout, double>::value)
       RBC_LAUNCH(cheb_y_diff_kernel<L>, dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, mi, out, ldo, rows, coef,
@@ -647,7 +672,7 @@ void launch_conv(hipStream_t s, const Tin* dx, const Tin* val, int ldi, int mi, 
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + G::P - 1) / G::P);
   RBC_LAUNCH((cheb_y_conv_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, dx, val, ldi, mi, out, ldo,
-             rows, coef, inv_sc, ux, uy, ldu, fft_table(L), si, so, xp);
+             rows, coef, inv_sc, ux, uy, ldu, twiddles<V>(L), si, so, xp);
 }
 
 template <int L, typename Tin, typename V, typename Tout>
@@ -656,7 +681,7 @@ void launch_conv3(hipStream_t s, const Tin* dx, const Tin* val, int ldi, int mi,
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + G::P - 1) / G::P);
   RBC_LAUNCH((cheb_y_conv3_kernel<L, Tin, V, Tout>), dim3(blocks), dim3(G::T), 0, s, dx, val, ldi, mi, out, ldo, rows,
-             inv_sc, f1, f2, ldf, cut, fft_table(L), si, so);
+             inv_sc, f1, f2, ldf, cut, twiddles<V>(L), si, so);
 }
 
 template <int L, typename Tin, typename V, typename Tout>
@@ -665,7 +690,7 @@ void launch_fwd(hipStream_t s, const Tin* in, int ldi, Tout* out, int ldo, int r
   using G = Geo<L>;
   const unsigned blocks = (unsigned)((rows + 2 * G::P - 1) / (2 * G::P));
   RBC_LAUNCH((cheb_y_fwd_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, in, ldi, out, ldo, rows, cut,
-             fft_table(L), si, so);
+             twiddles<V>(L), si, so);
 }
 
 }  // namespace

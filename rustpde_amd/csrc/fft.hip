@@ -49,7 +49,7 @@ struct CGeo {
 // writes their zero spectra without loading or transforming.
 template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
-                                                               int cols, int cut, const double* __restrict__ tw,
+                                                               int cols, int cut, const V* __restrict__ tw,
                                                                long sx, long sy, int nzc) {
   using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, m = G::m;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(cons
 template <int L, typename Tin, typename V, typename Tout>
 __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
                                                                 int cols, int times, double sc_d,
-                                                                const double* __restrict__ tw, long sx, long sy,
+                                                                const V* __restrict__ tw, long sx, long sy,
                                                                 Tout* __restrict__ Y1, int split, int times1,
                                                                 double sc1) {
 #pragma clang fp contract(off)
@@ -209,7 +209,7 @@ void launch_rfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int cut, int ba
   using G = CGeo<L, sizeof(V)>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
   RBC_LAUNCH((rfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, cut,
-             fft_table(L), sx, sy, nzc < 0 ? cols : nzc);
+             twiddles<V>(L), sx, sy, nzc < 0 ? cols : nzc);
 }
 template <int L, typename Tin, typename V, typename Tout>
 void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, double sc, int batch, long sx, long sy,
@@ -217,7 +217,7 @@ void launch_irfft(hipStream_t s, const Tin* X, Tout* Y, int cols, int times, dou
   using G = CGeo<L, sizeof(V)>;
   const unsigned blocks = (unsigned)((cols + G::C - 1) / G::C);
   RBC_LAUNCH((irfft_cols_kernel<L, Tin, V, Tout>), dim3(blocks, batch), dim3(G::T), 0, s, X, Y, cols, times, sc,
-             fft_table(L), sx, sy, Y1, split, times1, sc1);
+             twiddles<V>(L), sx, sy, Y1, split, times1, sc1);
 }
 
 }  // namespace

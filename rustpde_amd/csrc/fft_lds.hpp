@@ -3,7 +3,7 @@
 // radix-4 or radix-2 pass, natural order in and out, forward sign (e^{-2 pi i jk/L}).
 // A block holds E / L transforms of length L at stride L in two arrays (re, im); each of
 // its T threads handles E / T = 8 values per pass.  Twiddles come from a table of
-// e^{-2 pi i t / L}, t < L, interleaved (re, im).
+// e^{-2 pi i t / L}, t < L, interleaved (re, im), in the arithmetic type or FP64.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -62,10 +62,19 @@ __device__ __forceinline__ void dft8(T* r, T* i) {
   r[7] = e3r - o3r; i[7] = e3i - o3i;
 }
 
-// one Stockham pass of radix R over the P transforms of length L held in (re, im)
-template <int L, int R, int T, int E, typename V>
-__device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double* __restrict__ tw, int tid) {
-  static_assert(E % (R * T) == 0 && E % L == 0, "block geometry");
+// one Stockham pass of radix R over the P transforms of length L held in (re, im), the
+// NS-th (NS = 1, 8, 64, ..: the length of the sub-transforms already done).  The LDS index
+// of every element a thread touches is its base index's plus a constant, since
+// lp(a + c) = lp(a) + lp(c) whenever (a mod 16) + (c mod 16) < 16: the reads take
+// a = p L + j (j < L / R) and c = r L / R; the writes a = p L + (j / NS) NS R + k (k < NS)
+// and c = r NS, and for NS = 1 (a mod 16 a multiple of R, r < R) and NS = 8 (a mod 16 =
+// k < 8, c mod 16 in {0, 8}) that holds as for NS a multiple of 16 (L = 8: p L mod 16 is
+// 0 or 8 and the offsets stay below 8).  So each thread forms
+// two LDS indices per butterfly and the rest are immediate offsets.
+template <int L, int R, int T, int E, int NS, typename V, typename TW>
+__device__ __forceinline__ void stockham_pass(V* re, V* im, const TW* __restrict__ tw, int tid) {
+  static_assert(E % (R * T) == 0 && E % L == 0 && (L % 16 == 0 || L == 8), "block geometry");
+  static_assert(NS == 1 || NS == 8 || NS % 16 == 0, "pass stride");
   constexpr int NB = L / R;          // butterflies per transform
   constexpr int PER = E / (R * T);   // butterflies per thread
   V xr[PER][R], xi[PER][R];
@@ -74,68 +83,74 @@ __device__ __forceinline__ void stockham_pass(V* re, V* im, int Ns, const double
   for (int q = 0; q < PER; ++q) {
     const int g = tid + q * T;
     const int p = g / NB, j = g - p * NB;
-    const int k = j & (Ns - 1);
-    const int base = p * L + j;
+    const int k = j & (NS - 1);
+    const int src = lp(p * L + j);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      xr[q][r] = re[lp(base + r * NB)];
-      xi[q][r] = im[lp(base + r * NB)];
+      xr[q][r] = re[src + lp(r * NB)];
+      xi[q][r] = im[src + lp(r * NB)];
     }
-    // twiddles e^{-2 pi i r k / (Ns R)} = table[r k L / (Ns R)]: powers 1, 2, 4 from the
-    // table, the others as products of two of them
-    const int t1 = k * (L / (Ns * R));
-    V wr[R], wi[R];
-    wr[0] = 1;
-    wi[0] = 0;
-    if constexpr (R >= 2) { wr[1] = (V)tw[2 * t1]; wi[1] = (V)tw[2 * t1 + 1]; }
-    if constexpr (R >= 4) {
-      wr[2] = (V)tw[4 * t1]; wi[2] = (V)tw[4 * t1 + 1];
-      wr[3] = wr[1] * wr[2] - wi[1] * wi[2]; wi[3] = wr[1] * wi[2] + wi[1] * wr[2];
-    }
-    if constexpr (R == 8) {
-      wr[4] = (V)tw[8 * t1]; wi[4] = (V)tw[8 * t1 + 1];
-      wr[5] = wr[1] * wr[4] - wi[1] * wi[4]; wi[5] = wr[1] * wi[4] + wi[1] * wr[4];
-      wr[6] = wr[2] * wr[4] - wi[2] * wi[4]; wi[6] = wr[2] * wi[4] + wi[2] * wr[4];
-      wr[7] = wr[3] * wr[4] - wi[3] * wi[4]; wi[7] = wr[3] * wi[4] + wi[3] * wr[4];
-    }
+    if constexpr (NS > 1) {
+      // twiddles e^{-2 pi i r k / (NS R)} = table[r k L / (NS R)]: powers 1, 2, 4 from the
+      // table, the others as products of two of them (the first pass has k = 0: none)
+      const int t1 = k * (L / (NS * R));
+      V wr[R], wi[R];
+      wr[0] = 1;
+      wi[0] = 0;
+      if constexpr (R >= 2) { wr[1] = (V)tw[2 * t1]; wi[1] = (V)tw[2 * t1 + 1]; }
+      if constexpr (R >= 4) {
+        wr[2] = (V)tw[4 * t1]; wi[2] = (V)tw[4 * t1 + 1];
+        wr[3] = wr[1] * wr[2] - wi[1] * wi[2]; wi[3] = wr[1] * wi[2] + wi[1] * wr[2];
+      }
+      if constexpr (R == 8) {
+        wr[4] = (V)tw[8 * t1]; wi[4] = (V)tw[8 * t1 + 1];
+        wr[5] = wr[1] * wr[4] - wi[1] * wi[4]; wi[5] = wr[1] * wi[4] + wi[1] * wr[4];
+        wr[6] = wr[2] * wr[4] - wi[2] * wi[4]; wi[6] = wr[2] * wi[4] + wi[2] * wr[4];
+        wr[7] = wr[3] * wr[4] - wi[3] * wi[4]; wi[7] = wr[3] * wi[4] + wi[3] * wr[4];
+      }
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      const V ar = xr[q][r], ai = xi[q][r];
-      xr[q][r] = ar * wr[r] - ai * wi[r];
-      xi[q][r] = ar * wi[r] + ai * wr[r];
+      for (int r = 1; r < R; ++r) {
+        const V ar = xr[q][r], ai = xi[q][r];
+        xr[q][r] = ar * wr[r] - ai * wi[r];
+        xi[q][r] = ar * wi[r] + ai * wr[r];
+      }
     }
     if constexpr (R == 8) dft8(xr[q], xi[q]);
     else if constexpr (R == 4) dft4(xr[q][0], xi[q][0], xr[q][1], xi[q][1], xr[q][2], xi[q][2], xr[q][3], xi[q][3]);
     else dft2(xr[q], xi[q]);
-    dst[q] = p * L + (j / Ns) * Ns * R + k;
+    dst[q] = lp(p * L + (j / NS) * NS * R + k);
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; ++q)
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      re[lp(dst[q] + r * Ns)] = xr[q][r];
-      im[lp(dst[q] + r * Ns)] = xi[q][r];
+      re[dst[q] + lp(r * NS)] = xr[q][r];
+      im[dst[q] + lp(r * NS)] = xi[q][r];
     }
   __syncthreads();
 }
 
-constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+// the passes from sub-transform length NS on: radix 8 while it divides, then one radix-4 or
+// radix-2 pass
+template <int L, int T, int E, int NS, typename V, typename TW>
+__device__ __forceinline__ void dft_passes(V* re, V* im, const TW* __restrict__ tw, int tid) {
+  if constexpr (NS * 8 <= L) {
+    stockham_pass<L, 8, T, E, NS>(re, im, tw, tid);
+    dft_passes<L, T, E, NS * 8>(re, im, tw, tid);
+  } else if constexpr (NS * 4 == L) {
+    stockham_pass<L, 4, T, E, NS>(re, im, tw, tid);
+  } else if constexpr (NS * 2 == L) {
+    stockham_pass<L, 2, T, E, NS>(re, im, tw, tid);
+  }
+}
 
 // tid: the thread's index in the block (a caller that runs several transforms can pass a
 // value the compiler cannot prove equal between them, so that each transform forms its
 // LDS addresses afresh instead of holding one set live across the others)
-template <int L, int T, int E, typename V>
-__device__ __forceinline__ void lds_dft(V* re, V* im, const double* __restrict__ tw, int tid = threadIdx.x) {
-  constexpr int NUM8 = ilog2(L) / 3, REM = ilog2(L) % 3;
-  int Ns = 1;
-#pragma unroll
-  for (int s = 0; s < NUM8; ++s) {
-    stockham_pass<L, 8, T, E>(re, im, Ns, tw, tid);
-    Ns *= 8;
-  }
-  if constexpr (REM == 2) stockham_pass<L, 4, T, E>(re, im, Ns, tw, tid);
-  else if constexpr (REM == 1) stockham_pass<L, 2, T, E>(re, im, Ns, tw, tid);
+template <int L, int T, int E, typename V, typename TW>
+__device__ __forceinline__ void lds_dft(V* re, V* im, const TW* __restrict__ tw, int tid = threadIdx.x) {
+  dft_passes<L, T, E, 1>(re, im, tw, tid);
 }
 
 // threadIdx.x behind an empty asm statement: a value the compiler cannot merge with other
@@ -156,7 +171,16 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 
 }  // namespace fftlds
 
-// e^{-2 pi i t / L}, t < L, interleaved, cached per (device, L) (defined in cheb_y.hip)
+// e^{-2 pi i t / L}, t < L, interleaved, cached per (device, L) (defined in cheb_y.hip);
+// the FP32 table holds the FP64 table's values rounded to FP32, so that the FP32 kernels
+// read the twiddles they would convert
 const double* fft_table(int L);
+const float* fft_table_f32(int L);
+// the table in the arithmetic type V of a kernel
+template <typename V>
+inline const V* twiddles(int L) {
+  if constexpr (sizeof(V) == sizeof(float)) return fft_table_f32(L);
+  else return fft_table(L);
+}
 
 }  // namespace rbc
