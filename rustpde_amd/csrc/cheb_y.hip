@@ -162,7 +162,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
                                                 const Tout* __restrict__ U2 = nullptr) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
-  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
+  __shared__ __attribute__((aligned(16))) V zmem[2 * lp_size(G::E)];
+  const fftlds::LdsZ<V, lp_size(G::E)> z{zmem};
   const V inv_sc = (V)inv_sc_d, beta = (V)beta_d;
   in += si * blockIdx.y;
   if constexpr (DUAL) in2 += si * blockIdx.y;
@@ -201,8 +202,7 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
       if (k == n - 1) sb = 0;
       else if (k > 0) sb = (V)(2.0 * (double)(k + 1)) * sb;
     }
-    re[lp(p * L + k)] = sa;
-    im[lp(p * L + k)] = sb;
+    z.st(lp(p * L + k), sa, sb);
   }
   __syncthreads();
   if (times) {
@@ -213,13 +213,12 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     const int lane = tid & 63, wave = tid >> 6;
     for (int ch = wave; ch < NCH; ch += T / 64) {
       const int p = DUAL ? ch >> 1 : ch >> 2, comp = DUAL ? 1 : (ch >> 1) & 1, par = ch & 1;
-      V* z = comp ? im : re;
-      const int zb = p * L + (par ? 1 : 2);  // chain element t at z[lp(zb + 2 t)]
+      const int zb = p * L + (par ? 1 : 2);  // chain element t at z.c(comp, lp(zb + 2 t))
       const int t0 = lane * SEG;
       V tot = 0;
 #pragma unroll
       for (int q = SEG - 1; q >= 0; --q)
-        if (t0 + q < C) tot = z[lp(zb + 2 * (t0 + q))] + tot;
+        if (t0 + q < C) tot = z.c(comp, lp(zb + 2 * (t0 + q))) + tot;
       V inc = tot;  // inclusive suffix sum over lanes
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
@@ -232,12 +231,12 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
       for (int q = SEG - 1; q >= 0; --q) {
         const int t = t0 + q;
         if (t < C) {
-          acc = z[lp(zb + 2 * t)] + acc;
-          z[lp(zb + 2 * t)] = acc;
+          acc = z.c(comp, lp(zb + 2 * t)) + acc;
+          z.c(comp, lp(zb + 2 * t)) = acc;
         }
       }
       if (par == 0 && lane == 0) {  // acc = d_2
-        z[lp(p * L)] = z[lp(p * L)] + acc / (V)2;
+        z.c(comp, lp(p * L)) = z.c(comp, lp(p * L)) + acc / (V)2;
       }
     }
     __syncthreads();
@@ -245,22 +244,19 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   // (3) u_k = (-1)^k w_k c_k / sc^times and the even extension u_{L-k} = u_k
   for (int e = tid; e < P * n && transform; e += T) {
     const int p = e / n, k = e - p * n;
-    V a = re[lp(p * L + k)], b = im[lp(p * L + k)];
+    V a, b;
+    z.ld(lp(p * L + k), a, b);
     if (times_a && inv_sc != (V)1) a = a * inv_sc;
     if (times_b && inv_sc != (V)1) b = b * inv_sc;
     const V w = (V)(((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5));
     a = a * w;
     b = b * w;
-    re[lp(p * L + k)] = a;
-    im[lp(p * L + k)] = b;
-    if (k > 0 && k < N) {
-      re[lp(p * L + L - k)] = a;
-      im[lp(p * L + L - k)] = b;
-    }
+    z.st(lp(p * L + k), a, b);
+    if (k > 0 && k < N) z.st(lp(p * L + L - k), a, b);
   }
   if constexpr (transform) {
     __syncthreads();
-    lds_dft<L, T, G::E>(re, im, tw);
+    lds_dft<L, T, G::E>(z, tw);
   }
   // (4) x_j = Re / Im Z_j, j <= N, or the coefficients d_j / sc without the transform;
   //     out = x .* U + beta out (loads first), plus the extra products on one entry
@@ -283,8 +279,10 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
       const int p = e / n, j = e - p * n;
       const int r = row0 + p;
       if (r >= rows) continue;
-      const V xa = (V)(Tout)(re[lp(p * L + j)] * ua[q]);  // the value the first launch stores
-      V v = im[lp(p * L + j)] * ub[q];
+      V za, zb;
+      z.ld(lp(p * L + j), za, zb);
+      const V xa = (V)(Tout)(za * ua[q]);  // the value the first launch stores
+      V v = zb * ub[q];
       v = v + (V)1 * xa;
       if (extra) {
         const size_t o = (size_t)r * ldu + j, of = xp.ldf < 0 ? o : (size_t)r * xp.ldf + j;
@@ -313,7 +311,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     if (e >= P * n) continue;
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
-    V va = re[lp(p * L + j)], vb = im[lp(p * L + j)];
+    V va, vb;
+    z.ld(lp(p * L + j), va, vb);
     if (!transform && inv_sc != (V)1) {
       va = va * inv_sc;
       vb = vb * inv_sc;
@@ -391,7 +390,8 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
   constexpr int Q = (P * n + T - 1) / T;
   constexpr double invN = 1.0 / (double)N;  // exact: N is a power of two
-  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
+  __shared__ __attribute__((aligned(16))) V zmem[2 * lp_size(G::E)];
+  const fftlds::LdsZ<V, lp_size(G::E)> z{zmem};
   __shared__ V uxs[P * n], uys[P * n];  // the physical velocity of the block's rows
   __shared__ V cs0[P * n], cs1[P * n];  // the convective terms of ux, then uy (temp reuses cs0)
   const V inv_sc = (V)inv_sc_d;
@@ -423,12 +423,8 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
     const int e = tid + q * T;
     if (e >= P * n) return;
     const int p = e / n, k = e - p * n;
-    re[lp(p * L + k)] = a;
-    im[lp(p * L + k)] = b;
-    if (k > 0 && k < N) {
-      re[lp(p * L + L - k)] = a;
-      im[lp(p * L + L - k)] = b;
-    }
+    z.st(lp(p * L + k), a, b);
+    if (k > 0 && k < N) z.st(lp(p * L + L - k), a, b);
   };
   // dealiased forward transform of the pair (cs0, cs1) into out slots fa (and fa + 1 with hasb)
   auto forward = [&](int fa, bool hasb) {
@@ -439,15 +435,17 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
       if (e < P * n) put(q, cs0[e], hasb ? cs1[e] : (V)0);
     }
     __syncthreads();
-    lds_dft<L, T, G::E>(re, im, tw, fftlds::opaque_tid());
+    lds_dft<L, T, G::E>(z, tw, fftlds::opaque_tid());
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int e = tid + q * T;
       const int p = e / n, k = e - p * n, r = row0 + p;
       if (e >= P * n || r >= rows) continue;
       const V s = (V)(((k & 1) ? -invN : invN) * ((k == 0 || k == N) ? 0.5 : 1.0));
-      out[fa * so + (size_t)r * ldo + k] = (Tout)(k < cut ? re[lp(p * L + k)] * s : (V)0);
-      if (hasb) out[(fa + 1) * so + (size_t)r * ldo + k] = (Tout)(k < cut ? im[lp(p * L + k)] * s : (V)0);
+      V za, zb;
+      z.ld(lp(p * L + k), za, zb);
+      out[fa * so + (size_t)r * ldo + k] = (Tout)(k < cut ? za * s : (V)0);
+      if (hasb) out[(fa + 1) * so + (size_t)r * ldo + k] = (Tout)(k < cut ? zb * s : (V)0);
     }
   };
   // ---- transform 0: the physical velocity (times = 0: u_k = (-1)^k w_k c_k); the rows of
@@ -468,14 +466,16 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
     }
   }
   __syncthreads();
-  lds_dft<L, T, G::E>(re, im, tw, fftlds::opaque_tid());
+  lds_dft<L, T, G::E>(z, tw, fftlds::opaque_tid());
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int e = tid + q * T;
     const int p = e / n, j = e - p * n;
     if (e < P * n) {
-      uxs[e] = (V)(Tout)re[lp(p * L + j)];
-      uys[e] = (V)(Tout)im[lp(p * L + j)];
+      V za, zb;
+      z.ld(lp(p * L + j), za, zb);
+      uxs[e] = (V)(Tout)za;
+      uys[e] = (V)(Tout)zb;
     }
   }
   // ---- transforms 1..3: the convective terms
@@ -492,8 +492,7 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
       V b = sval(rb, q, k + 1);
       if (k == n - 1) b = 0;
       else if (k > 0) b = (V)(2.0 * (double)(k + 1)) * b;
-      re[lp(p * L + k)] = sval(ra, q, k);
-      im[lp(p * L + k)] = b;
+      z.st(lp(p * L + k), sval(ra, q, k), b);
     }
     // the next field's rows (or temp's fieldbc gradient), in flight during this transform
     V g1[Q], g2[Q];
@@ -525,7 +524,7 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
         V tot = 0;
 #pragma unroll
         for (int q = SEG - 1; q >= 0; --q)
-          if (t0 + q < C) tot = im[lp(zb + 2 * (t0 + q))] + tot;
+          if (t0 + q < C) tot = z.i(lp(zb + 2 * (t0 + q))) + tot;
         V inc = tot;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -538,11 +537,11 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
         for (int q = SEG - 1; q >= 0; --q) {
           const int t = t0 + q;
           if (t < C) {
-            acc = im[lp(zb + 2 * t)] + acc;
-            im[lp(zb + 2 * t)] = acc;
+            acc = z.i(lp(zb + 2 * t)) + acc;
+            z.i(lp(zb + 2 * t)) = acc;
           }
         }
-        if (par == 0 && lane == 0) im[lp(p * L)] = im[lp(p * L)] + acc / (V)2;
+        if (par == 0 && lane == 0) z.i(lp(p * L)) = z.i(lp(p * L)) + acc / (V)2;
       }
     }
     __syncthreads();
@@ -552,14 +551,14 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
       const int e = tid + q * T;
       if (e >= P * n) continue;
       const int p = e / n, k = e - p * n;
-      const V a = re[lp(p * L + k)];
-      V b = im[lp(p * L + k)];
+      V a, b;
+      z.ld(lp(p * L + k), a, b);
       if (inv_sc != (V)1) b = b * inv_sc;
       const V w = (V)(((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == N) ? 1.0 : 0.5));
       put(q, a * w, b * w);
     }
     __syncthreads();
-    lds_dft<L, T, G::E>(re, im, tw, fftlds::opaque_tid());
+    lds_dft<L, T, G::E>(z, tw, fftlds::opaque_tid());
     // (4) out_f = d/dy(val_f) .* uy + 1 (dx_f .* ux) (+ temp's fieldbc terms)
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -568,8 +567,10 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
       V v = 0;
       if (e < P * n && r < rows) {
         const V ux = uxs[e], uy = uys[e];
-        const V xa = (V)(Tout)(re[lp(p * L + j)] * ux);
-        v = im[lp(p * L + j)] * uy;
+        V za, zb;
+        z.ld(lp(p * L + j), za, zb);
+        const V xa = (V)(Tout)(za * ux);
+        v = zb * uy;
         v = v + (V)1 * xa;
         if (f == 2) {
           if (f1) v = v + ux * g1[q];
@@ -600,7 +601,8 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
                                                                const V* __restrict__ tw, long si, long so) {
   using G = Geo<L>;
   constexpr int N = G::N, n = G::n, P = G::P, T = G::T;
-  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
+  __shared__ __attribute__((aligned(16))) V zmem[2 * lp_size(G::E)];
+  const fftlds::LdsZ<V, lp_size(G::E)> z{zmem};
   in += si * blockIdx.y;
   out += so * blockIdx.y;
   const int tid = threadIdx.x, row0 = 2 * P * blockIdx.x;
@@ -621,15 +623,11 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
     if (e >= P * n) continue;
     const int p = e / n, k = e - p * n;
     const V a = va[q], b = vb[q];
-    re[lp(p * L + k)] = a;
-    im[lp(p * L + k)] = b;
-    if (k > 0 && k < N) {
-      re[lp(p * L + L - k)] = a;
-      im[lp(p * L + L - k)] = b;
-    }
+    z.st(lp(p * L + k), a, b);
+    if (k > 0 && k < N) z.st(lp(p * L + L - k), a, b);
   }
   __syncthreads();
-  lds_dft<L, T, G::E>(re, im, tw);
+  lds_dft<L, T, G::E>(z, tw);
   constexpr double invN = 1.0 / (double)N;  // exact: N is a power of two
   for (int e = tid; e < P * n; e += T) {
     const int p = e / n, k = e - p * n;
@@ -639,7 +637,7 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
     for (int h = 0; h < 2; ++h) {
       const int r = ra + h;
       if (r >= rows) continue;
-      const V v = h ? im[lp(p * L + k)] : re[lp(p * L + k)];
+      const V v = z.c(h, lp(p * L + k));
       out[(size_t)r * ldo + k] = (Tout)(k < cut ? v * s : (V)0);
     }
   }

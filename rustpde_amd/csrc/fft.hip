@@ -53,7 +53,8 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(cons
                                                                long sx, long sy, int nzc) {
   using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, m = G::m;
-  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
+  __shared__ __attribute__((aligned(16))) V zmem[2 * lp_size(G::E)];
+  const fftlds::LdsZ<V, lp_size(G::E)> z{zmem};
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
   const int tid = threadIdx.x;
@@ -78,16 +79,18 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(cons
 #pragma unroll
   for (int q = 0; q < QL; ++q) {
     const int e = tid + q * T, c = e % C, t = e / C;
-    ((c & 1) ? im : re)[lp((c >> 1) * L + t)] = v[q];
+    z.c(c & 1, lp((c >> 1) * L + t)) = v[q];
   }
   __syncthreads();
-  lds_dft<L, T, G::E>(re, im, tw);
+  lds_dft<L, T, G::E>(z, tw);
   // split: A_k = (Z_k + conj Z_{L-k}) / 2, B_k = (Z_k - conj Z_{L-k}) / (2i)
   for (int e = tid; e < m * C; e += T) {
     const int c = e % C, k = e / C, j = j0 + c;
     if (j >= cols) continue;
     const int p = (c >> 1) * L, k2 = (L - k) & (L - 1);
-    const V zr = re[lp(p + k)], zi = im[lp(p + k)], yr = re[lp(p + k2)], yi = im[lp(p + k2)];
+    V zr, zi, yr, yi;
+    z.ld(lp(p + k), zr, zi);
+    z.ld(lp(p + k2), yr, yi);
     const V half = (V)0.5;
     V xr, xi;
     if (c & 1) {
@@ -118,7 +121,8 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
 #pragma clang fp contract(off)
   using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
-  __shared__ V re[lp_size(G::E)], im[lp_size(G::E)];
+  __shared__ __attribute__((aligned(16))) V zmem[2 * lp_size(G::E)];
+  const fftlds::LdsZ<V, lp_size(G::E)> z{zmem};
   int b = blockIdx.y;
   if (split > 0 && b >= split) {
     b -= split;
@@ -175,22 +179,20 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
     const V ar = odd ? pa : a, ai = odd ? pb : b, br = odd ? a : pa, bi = odd ? b : pb;
     const int zb = (c >> 1) * L;
     if (!odd) {
-      re[lp(zb + k)] = ar - bi;
-      im[lp(zb + k)] = -(ai + br);
+      z.st(lp(zb + k), ar - bi, -(ai + br));
     } else if (k > 0 && k < N) {
-      re[lp(zb + L - k)] = ar + bi;
-      im[lp(zb + L - k)] = ai - br;
+      z.st(lp(zb + L - k), ar + bi, ai - br);
     }
   }
   __syncthreads();
-  lds_dft<L, T, G::E>(re, im, tw);
+  lds_dft<L, T, G::E>(z, tw);
   // (3) a_j = Re W_j / L, b_j = -Im W_j / L with W = DFT(conj Z)
   const V inv = (V)(1.0 / (double)L);
   for (int e = tid; e < L * C; e += T) {
     const int c = e % C, t = e / C, j = j0 + c;
     if (j >= cols) continue;
     const int p = (c >> 1) * L;
-    Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -im[lp(p + t)] * inv : re[lp(p + t)] * inv);
+    Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -z.i(lp(p + t)) * inv : z.r(lp(p + t)) * inv);
   }
 }
 

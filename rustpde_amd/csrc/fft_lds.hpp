@@ -1,7 +1,7 @@
 // In-LDS complex DFTs for the FFT-based kernels (FP64, or FP32 in the mixed-precision
 // step; T is the arithmetic type): radix-8 Stockham passes plus one
 // radix-4 or radix-2 pass, natural order in and out, forward sign (e^{-2 pi i jk/L}).
-// A block holds E / L transforms of length L at stride L in two arrays (re, im); each of
+// A block holds E / L transforms of length L at stride L in one complex buffer (LdsZ); each of
 // its T threads handles E / T = 8 values per pass.  Twiddles come from a table of
 // e^{-2 pi i t / L}, t < L, interleaved (re, im), in the arithmetic type or FP64.
 #pragma once
@@ -10,11 +10,52 @@
 namespace rbc {
 namespace fftlds {
 
-// LDS index of element a of the transform buffers: one pad element per 16 spreads the
-// strided accesses of the passes and of the chain scans over the banks
-__host__ __device__ constexpr int lp(int a) { return a + (a >> 4); }
+// LDS index of element a of the transform buffers: three pad elements per 32.  The LDS
+// serves a wavefront's access in half-waves of 32 lanes, and lanes whose element indices
+// agree mod 32 conflict (b32 and b64 accesses alike, measured on MI355X).  With this
+// padding the passes' reads (32 consecutive elements) and the writes of stride 8 are
+// conflict-free, and the writes of the second radix-8 pass (64 u + k, u < 4, k < 8)
+// conflict two-way; one pad per 16 conflicts two-way on every access.
+__host__ __device__ constexpr int lp(int a) { return a + 3 * (a >> 5); }
 // LDS elements of a buffer of E values
 __host__ __device__ constexpr int lp_size(int E) { return lp(E) + 1; }
+
+// A block's complex buffer of NE elements in LDS: (re, im) pairs (FP32: one 8-byte access
+// per element, which moves twice the bytes of a 4-byte access in the same LDS time) or two
+// arrays re[NE], im[NE] (FP64).  m points at 2 NE values, aligned to 16 bytes.
+template <typename V>
+struct alignas(2 * sizeof(V)) cpair {
+  V r, i;
+};
+#ifndef RBC_LDS_PAIRS_FP64
+#define RBC_LDS_PAIRS_FP64 0
+#endif
+template <typename V, int NE, bool PAIRS = (sizeof(V) == 4 || RBC_LDS_PAIRS_FP64)>
+struct LdsZ {
+  using value_type = V;
+  V* m;
+  __device__ __forceinline__ V& r(int e) const { return PAIRS ? m[2 * e] : m[e]; }
+  __device__ __forceinline__ V& i(int e) const { return PAIRS ? m[2 * e + 1] : m[NE + e]; }
+  __device__ __forceinline__ V& c(int comp, int e) const { return PAIRS ? m[2 * e + comp] : m[comp * NE + e]; }
+  __device__ __forceinline__ void ld(int e, V& a, V& b) const {
+    if constexpr (PAIRS) {
+      const cpair<V> v = reinterpret_cast<const cpair<V>*>(m)[e];
+      a = v.r;
+      b = v.i;
+    } else {
+      a = m[e];
+      b = m[NE + e];
+    }
+  }
+  __device__ __forceinline__ void st(int e, V a, V b) const {
+    if constexpr (PAIRS) {
+      reinterpret_cast<cpair<V>*>(m)[e] = cpair<V>{a, b};
+    } else {
+      m[e] = a;
+      m[NE + e] = b;
+    }
+  }
+};
 
 constexpr double SQH = 0.70710678118654752440;  // sqrt(1/2), rounded to T where used
 
@@ -65,16 +106,17 @@ __device__ __forceinline__ void dft8(T* r, T* i) {
 // one Stockham pass of radix R over the P transforms of length L held in (re, im), the
 // NS-th (NS = 1, 8, 64, ..: the length of the sub-transforms already done).  The LDS index
 // of every element a thread touches is its base index's plus a constant, since
-// lp(a + c) = lp(a) + lp(c) whenever (a mod 16) + (c mod 16) < 16: the reads take
+// lp(a + c) = lp(a) + lp(c) whenever (a mod 32) + (c mod 32) < 32: the reads take
 // a = p L + j (j < L / R) and c = r L / R; the writes a = p L + (j / NS) NS R + k (k < NS)
-// and c = r NS, and for NS = 1 (a mod 16 a multiple of R, r < R) and NS = 8 (a mod 16 =
-// k < 8, c mod 16 in {0, 8}) that holds as for NS a multiple of 16 (L = 8: p L mod 16 is
-// 0 or 8 and the offsets stay below 8).  So each thread forms
+// and c = r NS, and for NS = 1 (a mod 32 a multiple of R, r < R) and NS = 8 (a mod 32 =
+// k < 8 when 8 R is a multiple of 32, c mod 32 = 8 r; R = 2: a mod 32 < 24, c < 16) that
+// holds as for NS a multiple of 32 (L < 32: p L mod 32 plus the offsets stays below 32).  So each thread forms
 // two LDS indices per butterfly and the rest are immediate offsets.
-template <int L, int R, int T, int E, int NS, typename V, typename TW>
-__device__ __forceinline__ void stockham_pass(V* re, V* im, const TW* __restrict__ tw, int tid) {
+template <int L, int R, int T, int E, int NS, typename ZB, typename TW>
+__device__ __forceinline__ void stockham_pass(const ZB& z, const TW* __restrict__ tw, int tid) {
+  using V = typename ZB::value_type;
   static_assert(E % (R * T) == 0 && E % L == 0 && (L % 16 == 0 || L == 8), "block geometry");
-  static_assert(NS == 1 || NS == 8 || NS % 16 == 0, "pass stride");
+  static_assert(NS == 1 || NS == 8 || NS % 32 == 0, "pass stride");
   constexpr int NB = L / R;          // butterflies per transform
   constexpr int PER = E / (R * T);   // butterflies per thread
   V xr[PER][R], xi[PER][R];
@@ -86,10 +128,7 @@ __device__ __forceinline__ void stockham_pass(V* re, V* im, const TW* __restrict
     const int k = j & (NS - 1);
     const int src = lp(p * L + j);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      xr[q][r] = re[src + lp(r * NB)];
-      xi[q][r] = im[src + lp(r * NB)];
-    }
+    for (int r = 0; r < R; ++r) z.ld(src + lp(r * NB), xr[q][r], xi[q][r]);
     if constexpr (NS > 1) {
       // twiddles e^{-2 pi i r k / (NS R)} = table[r k L / (NS R)]: powers 1, 2, 4 from the
       // table, the others as products of two of them (the first pass has k = 0: none)
@@ -124,33 +163,30 @@ __device__ __forceinline__ void stockham_pass(V* re, V* im, const TW* __restrict
 #pragma unroll
   for (int q = 0; q < PER; ++q)
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      re[dst[q] + lp(r * NS)] = xr[q][r];
-      im[dst[q] + lp(r * NS)] = xi[q][r];
-    }
+    for (int r = 0; r < R; ++r) z.st(dst[q] + lp(r * NS), xr[q][r], xi[q][r]);
   __syncthreads();
 }
 
 // the passes from sub-transform length NS on: radix 8 while it divides, then one radix-4 or
 // radix-2 pass
-template <int L, int T, int E, int NS, typename V, typename TW>
-__device__ __forceinline__ void dft_passes(V* re, V* im, const TW* __restrict__ tw, int tid) {
+template <int L, int T, int E, int NS, typename ZB, typename TW>
+__device__ __forceinline__ void dft_passes(const ZB& z, const TW* __restrict__ tw, int tid) {
   if constexpr (NS * 8 <= L) {
-    stockham_pass<L, 8, T, E, NS>(re, im, tw, tid);
-    dft_passes<L, T, E, NS * 8>(re, im, tw, tid);
+    stockham_pass<L, 8, T, E, NS>(z, tw, tid);
+    dft_passes<L, T, E, NS * 8>(z, tw, tid);
   } else if constexpr (NS * 4 == L) {
-    stockham_pass<L, 4, T, E, NS>(re, im, tw, tid);
+    stockham_pass<L, 4, T, E, NS>(z, tw, tid);
   } else if constexpr (NS * 2 == L) {
-    stockham_pass<L, 2, T, E, NS>(re, im, tw, tid);
+    stockham_pass<L, 2, T, E, NS>(z, tw, tid);
   }
 }
 
 // tid: the thread's index in the block (a caller that runs several transforms can pass a
 // value the compiler cannot prove equal between them, so that each transform forms its
 // LDS addresses afresh instead of holding one set live across the others)
-template <int L, int T, int E, typename V, typename TW>
-__device__ __forceinline__ void lds_dft(V* re, V* im, const TW* __restrict__ tw, int tid = threadIdx.x) {
-  dft_passes<L, T, E, 1>(re, im, tw, tid);
+template <int L, int T, int E, typename ZB, typename TW>
+__device__ __forceinline__ void lds_dft(const ZB& z, const TW* __restrict__ tw, int tid = threadIdx.x) {
+  dft_passes<L, T, E, 1>(z, tw, tid);
 }
 
 // threadIdx.x behind an empty asm statement: a value the compiler cannot merge with other
