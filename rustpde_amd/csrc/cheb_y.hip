@@ -105,6 +105,23 @@ struct Geo {
   static constexpr int n = N + 1;
 };
 
+// A raw buffer over one row of `count` values: element i reads as the value for 0 <= i <
+// count and as 0 outside (the buffer's range check on the byte offset, which a negative i
+// exceeds as an unsigned value), so guarded loads take no branch or select.  The row must be
+// the same for the whole wavefront: the descriptor lives in scalar registers.
+template <typename T>
+struct BufRow {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ BufRow(const T* row, int count)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(row), 0, count * (int)sizeof(T), 0x00020000)) {}
+  __device__ __forceinline__ T operator[](int i) const {
+    if constexpr (sizeof(T) == 4)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, i * 4, 0, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, i * 8, 0, 0));
+  }
+};
+
 // stencil value s_k = d_k c_k + l_k c_{k-2} of one row (composite_stencil.rs:207-229);
 // d = coef[0, n), l = coef[n, 2n) hold zeros where the stencil has no entry, which leaves
 // the sum of the present terms unchanged.  Without coef the row is Chebyshev already.
@@ -177,15 +194,35 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   //     All loads of a thread are issued before any is used.
   constexpr int Q = (P * n + T - 1) / T;
   StencilLoad<V> la[Q], lb[Q];
+  if constexpr (P == 1) {
+    // one row per component: raw buffers whose range checks give the zeros of the guarded
+    // form (c_k for k >= mi, c_{k-2} for k < 2 or k - 2 >= min(mi, n - 2), the taps for
+    // k >= n, where both coefficients are zero and the tap's value does not matter)
+    const int ra = row0, rb = DUAL ? ra : ra + 1;
+    const int m2 = mi < n - 2 ? mi : n - 2;
+    const Tin* pa = in + (size_t)(ra < rows ? ra : 0) * ldi;
+    const Tin* pb = (DUAL ? in2 : in) + (size_t)(rb < rows ? rb : 0) * ldi;
+    const BufRow<Tin> a0(pa, ra < rows ? mi : 0), a2(pa, ra < rows && coef ? m2 : 0);
+    const BufRow<Tin> b0(pb, rb < rows ? mi : 0), b2(pb, rb < rows && coef ? m2 : 0);
+    const BufRow<double> cd(coef, coef ? n : 0), cl(coef ? coef + n : coef, coef ? n : 0);
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int e = tid + q * T;
-    const int p = e / n, k = e - p * n;
-    const int ra = DUAL ? row0 + p : row0 + 2 * p, rb = DUAL ? ra : ra + 1;
-    const int ka = (e < P * n) ? (times_a ? k + 1 : k) : n;  // k = n loads nothing
-    const int kb = (e < P * n) ? (times_b ? k + 1 : k) : n;
-    la[q] = stencil_load<V>(in + (size_t)ra * ldi, ra < rows, ka, mi, n, coef);
-    lb[q] = stencil_load<V>((DUAL ? in2 : in) + (size_t)rb * ldi, rb < rows, kb, mi, n, coef);
+    for (int q = 0; q < Q; ++q) {
+      const int k = tid + q * T;
+      const int ka = times_a ? k + 1 : k, kb = times_b ? k + 1 : k;
+      la[q] = StencilLoad<V>{(V)a0[ka], (V)a2[ka - 2], (V)cd[ka], (V)cl[ka]};
+      lb[q] = StencilLoad<V>{(V)b0[kb], (V)b2[kb - 2], (V)cd[kb], (V)cl[kb]};
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int p = e / n, k = e - p * n;
+      const int ra = DUAL ? row0 + p : row0 + 2 * p, rb = DUAL ? ra : ra + 1;
+      const int ka = (e < P * n) ? (times_a ? k + 1 : k) : n;  // k = n loads nothing
+      const int kb = (e < P * n) ? (times_b ? k + 1 : k) : n;
+      la[q] = stencil_load<V>(in + (size_t)ra * ldi, ra < rows, ka, mi, n, coef);
+      lb[q] = stencil_load<V>((DUAL ? in2 : in) + (size_t)rb * ldi, rb < rows, kb, mi, n, coef);
+    }
   }
   const bool comp = coef != nullptr;
 #pragma unroll
@@ -263,14 +300,24 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
   const bool extra = xp.slot == (int)blockIdx.y;
   if constexpr (DUAL) {
     V ua[Q], ub[Q];
+    if constexpr (P == 1) {
+      const int r = row0 < rows ? row0 : 0, cnt = row0 < rows ? n : 0;
+      const BufRow<Tout> bu(U + (size_t)r * ldu, cnt), bu2(U2 + (size_t)r * ldu, cnt);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int e = tid + q * T;
-      const int p = e / n, j = e - p * n;
-      const int r = row0 + p;
-      const bool ok = e < P * n && r < rows;
-      ua[q] = ok ? (V)U[(size_t)r * ldu + j] : (V)0;
-      ub[q] = ok ? (V)U2[(size_t)r * ldu + j] : (V)0;
+      for (int q = 0; q < Q; ++q) {
+        ua[q] = (V)bu[tid + q * T];
+        ub[q] = (V)bu2[tid + q * T];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int e = tid + q * T;
+        const int p = e / n, j = e - p * n;
+        const int r = row0 + p;
+        const bool ok = e < P * n && r < rows;
+        ua[q] = ok ? (V)U[(size_t)r * ldu + j] : (V)0;
+        ub[q] = ok ? (V)U2[(size_t)r * ldu + j] : (V)0;
+      }
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -294,16 +341,34 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
     return;
   }
   V ua[Q], ub[Q], oa[Q], ob[Q];
+  if constexpr (P == 1) {
+    // the values of lanes past the row or of rows past `rows` are never stored, and ua / oa
+    // are used only with U / beta != 0: raw buffers stand for the guards
+    const int ra = row0, rb = ra + 1;
+    const int ca = ra < rows ? n : 0, cb = rb < rows ? n : 0;
+    const size_t ia = (size_t)(ra < rows ? ra : 0), ib = (size_t)(rb < rows ? rb : 0);
+    const BufRow<Tout> bua(U ? U + ia * ldu : U, U ? ca : 0), bub(U ? U + ib * ldu : U, U ? cb : 0);
+    const BufRow<Tout> boa(out + ia * ldo, beta != (V)0 ? ca : 0), bob(out + ib * ldo, beta != (V)0 ? cb : 0);
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int e = tid + q * T;
-    const int p = e / n, j = e - p * n;
-    const int ra = row0 + 2 * p, rb = ra + 1;
-    const bool ok = e < P * n;
-    ua[q] = (U && ok && ra < rows) ? (V)U[(size_t)ra * ldu + j] : (V)1;
-    ub[q] = (U && ok && rb < rows) ? (V)U[(size_t)rb * ldu + j] : (V)1;
-    oa[q] = (beta != (V)0 && ok && ra < rows) ? (V)out[(size_t)ra * ldo + j] : (V)0;
-    ob[q] = (beta != (V)0 && ok && rb < rows) ? (V)out[(size_t)rb * ldo + j] : (V)0;
+    for (int q = 0; q < Q; ++q) {
+      const int j = tid + q * T;
+      ua[q] = (V)bua[j];
+      ub[q] = (V)bub[j];
+      oa[q] = (V)boa[j];
+      ob[q] = (V)bob[j];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int p = e / n, j = e - p * n;
+      const int ra = row0 + 2 * p, rb = ra + 1;
+      const bool ok = e < P * n;
+      ua[q] = (U && ok && ra < rows) ? (V)U[(size_t)ra * ldu + j] : (V)1;
+      ub[q] = (U && ok && rb < rows) ? (V)U[(size_t)rb * ldu + j] : (V)1;
+      oa[q] = (beta != (V)0 && ok && ra < rows) ? (V)out[(size_t)ra * ldo + j] : (V)0;
+      ob[q] = (beta != (V)0 && ok && rb < rows) ? (V)out[(size_t)rb * ldo + j] : (V)0;
+    }
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -401,14 +466,26 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
     V c0[Q], c2[Q];
   };
   auto load = [&](const Tin* __restrict__ base, int shift, Raw& w) {
+    if constexpr (P == 1) {
+      // one row per block: a raw buffer over its mi = n - 2 values returns the zeros of
+      // kk >= mi and of kk - 2 < 0 by its range check (kk >= n gives kk - 2 >= mi)
+      const BufRow<Tin> row(base + (size_t)(row0 < rows ? row0 : 0) * ldi, row0 < rows ? mi : 0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int e = tid + q * T;
-      const int p = e / n, r = row0 + p, kk = e - p * n + shift;
-      const bool ok = e < P * n && r < rows && kk < n;
-      const Tin* row = base + (size_t)r * ldi;
-      w.c0[q] = (ok && kk < mi) ? (V)row[kk] : (V)0;
-      w.c2[q] = (ok && kk >= 2 && kk - 2 < mi) ? (V)row[kk - 2] : (V)0;
+      for (int q = 0; q < Q; ++q) {
+        const int kk = tid + q * T + shift;
+        w.c0[q] = (V)row[kk];
+        w.c2[q] = (V)row[kk - 2];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int e = tid + q * T;
+        const int p = e / n, r = row0 + p, kk = e - p * n + shift;
+        const bool ok = e < P * n && r < rows && kk < n;
+        const Tin* row = base + (size_t)r * ldi;
+        w.c0[q] = (ok && kk < mi) ? (V)row[kk] : (V)0;
+        w.c2[q] = (ok && kk >= 2 && kk - 2 < mi) ? (V)row[kk - 2] : (V)0;
+      }
     }
   };
   // the stencil value d c_kk + l c_kk-2 with the Dirichlet taps at kk (d = 1 below m = n - 2
@@ -608,14 +685,25 @@ __global__ __launch_bounds__(Geo<L>::T) void cheb_y_fwd_kernel(const Tin* __rest
   const int tid = threadIdx.x, row0 = 2 * P * blockIdx.x;
   constexpr int Q = (P * n + T - 1) / T;
   V va[Q], vb[Q];
+  if constexpr (P == 1) {
+    const int ra = row0, rb = ra + 1;
+    const BufRow<Tin> a(in + (size_t)(ra < rows ? ra : 0) * ldi, ra < rows ? n : 0);
+    const BufRow<Tin> b(in + (size_t)(rb < rows ? rb : 0) * ldi, rb < rows ? n : 0);
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int e = tid + q * T;
-    const int p = e / n, k = e - p * n;
-    const int ra = row0 + 2 * p, rb = ra + 1;
-    const bool ok = e < P * n;
-    va[q] = (ok && ra < rows) ? (V)in[(size_t)ra * ldi + k] : (V)0;
-    vb[q] = (ok && rb < rows) ? (V)in[(size_t)rb * ldi + k] : (V)0;
+    for (int q = 0; q < Q; ++q) {
+      va[q] = (V)a[tid + q * T];
+      vb[q] = (V)b[tid + q * T];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int e = tid + q * T;
+      const int p = e / n, k = e - p * n;
+      const int ra = row0 + 2 * p, rb = ra + 1;
+      const bool ok = e < P * n;
+      va[q] = (ok && ra < rows) ? (V)in[(size_t)ra * ldi + k] : (V)0;
+      vb[q] = (ok && rb < rows) ? (V)in[(size_t)rb * ldi + k] : (V)0;
+    }
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
