@@ -25,6 +25,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "bufrow.hpp"
 #include "fft_lds.hpp"
 #include "ops.hpp"
 
@@ -103,23 +104,6 @@ struct Geo {
   static constexpr int T = E / 8;
   static constexpr int N = L / 2;
   static constexpr int n = N + 1;
-};
-
-// A raw buffer over one row of `count` values: element i reads as the value for 0 <= i <
-// count and as 0 outside (the buffer's range check on the byte offset, which a negative i
-// exceeds as an unsigned value), so guarded loads take no branch or select.  The row must be
-// the same for the whole wavefront: the descriptor lives in scalar registers.
-template <typename T>
-struct BufRow {
-  __amdgpu_buffer_rsrc_t rs;
-  __device__ __forceinline__ BufRow(const T* row, int count)
-      : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(row), 0, count * (int)sizeof(T), 0x00020000)) {}
-  __device__ __forceinline__ T operator[](int i) const {
-    if constexpr (sizeof(T) == 4)
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, i * 4, 0, 0));
-    else
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, i * 8, 0, 0));
-  }
 };
 
 // stencil value s_k = d_k c_k + l_k c_{k-2} of one row (composite_stencil.rs:207-229);
