@@ -53,8 +53,9 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(cons
                                                                long sx, long sy, int nzc) {
   using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, m = G::m;
-  __shared__ __attribute__((aligned(16))) V zmem[2 * lp_size(G::E)];
-  const fftlds::LdsZ<V, lp_size(G::E)> z{zmem};
+  constexpr int S = fftlds::spread_stride(L), NE = fftlds::spread_size(L, G::PP);  // transform p at p S
+  __shared__ __attribute__((aligned(16))) V zmem[2 * NE];
+  const fftlds::LdsZ<V, NE> z{zmem};
   X += sx * blockIdx.y;
   Y += sy * blockIdx.y;
   const int tid = threadIdx.x;
@@ -79,18 +80,18 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(cons
 #pragma unroll
   for (int q = 0; q < QL; ++q) {
     const int e = tid + q * T, c = e % C, t = e / C;
-    z.c(c & 1, lp((c >> 1) * L + t)) = v[q];
+    z.c(c & 1, (c >> 1) * S + lp(t)) = v[q];
   }
   __syncthreads();
-  lds_dft<L, T, G::E>(z, tw);
+  lds_dft<L, T, G::E, S>(z, tw);
   // split: A_k = (Z_k + conj Z_{L-k}) / 2, B_k = (Z_k - conj Z_{L-k}) / (2i)
   for (int e = tid; e < m * C; e += T) {
     const int c = e % C, k = e / C, j = j0 + c;
     if (j >= cols) continue;
-    const int p = (c >> 1) * L, k2 = (L - k) & (L - 1);
+    const int p = (c >> 1) * S, k2 = (L - k) & (L - 1);
     V zr, zi, yr, yi;
-    z.ld(lp(p + k), zr, zi);
-    z.ld(lp(p + k2), yr, yi);
+    z.ld(p + lp(k), zr, zi);
+    z.ld(p + lp(k2), yr, yi);
     const V half = (V)0.5;
     V xr, xi;
     if (c & 1) {
@@ -121,8 +122,9 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
 #pragma clang fp contract(off)
   using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
-  __shared__ __attribute__((aligned(16))) V zmem[2 * lp_size(G::E)];
-  const fftlds::LdsZ<V, lp_size(G::E)> z{zmem};
+  constexpr int S = fftlds::spread_stride(L), NE = fftlds::spread_size(L, G::PP);  // transform p at p S
+  __shared__ __attribute__((aligned(16))) V zmem[2 * NE];
+  const fftlds::LdsZ<V, NE> z{zmem};
   int b = blockIdx.y;
   if (split > 0 && b >= split) {
     b -= split;
@@ -177,22 +179,22 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
     if (e >= m * C) continue;
     const bool odd = (c & 1) != 0;
     const V ar = odd ? pa : a, ai = odd ? pb : b, br = odd ? a : pa, bi = odd ? b : pb;
-    const int zb = (c >> 1) * L;
+    const int zb = (c >> 1) * S;
     if (!odd) {
-      z.st(lp(zb + k), ar - bi, -(ai + br));
+      z.st(zb + lp(k), ar - bi, -(ai + br));
     } else if (k > 0 && k < N) {
-      z.st(lp(zb + L - k), ar + bi, ai - br);
+      z.st(zb + lp(L - k), ar + bi, ai - br);
     }
   }
   __syncthreads();
-  lds_dft<L, T, G::E>(z, tw);
+  lds_dft<L, T, G::E, S>(z, tw);
   // (3) a_j = Re W_j / L, b_j = -Im W_j / L with W = DFT(conj Z)
   const V inv = (V)(1.0 / (double)L);
   for (int e = tid; e < L * C; e += T) {
     const int c = e % C, t = e / C, j = j0 + c;
     if (j >= cols) continue;
-    const int p = (c >> 1) * L;
-    Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -z.i(lp(p + t)) * inv : z.r(lp(p + t)) * inv);
+    const int p = (c >> 1) * S;
+    Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -z.i(p + lp(t)) * inv : z.r(p + lp(t)) * inv);
   }
 }
 

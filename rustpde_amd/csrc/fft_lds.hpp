@@ -112,7 +112,7 @@ __device__ __forceinline__ void dft8(T* r, T* i) {
 // k < 8 when 8 R is a multiple of 32, c mod 32 = 8 r; R = 2: a mod 32 < 24, c < 16) that
 // holds as for NS a multiple of 32 (L < 32: p L mod 32 plus the offsets stays below 32).  So each thread forms
 // two LDS indices per butterfly and the rest are immediate offsets.
-template <int L, int R, int T, int E, int NS, typename ZB, typename TW>
+template <int L, int R, int T, int E, int NS, int S, typename ZB, typename TW>
 __device__ __forceinline__ void stockham_pass(const ZB& z, const TW* __restrict__ tw, int tid) {
   using V = typename ZB::value_type;
   static_assert(E % (R * T) == 0 && E % L == 0 && (L % 16 == 0 || L == 8), "block geometry");
@@ -126,7 +126,7 @@ __device__ __forceinline__ void stockham_pass(const ZB& z, const TW* __restrict_
     const int g = tid + q * T;
     const int p = g / NB, j = g - p * NB;
     const int k = j & (NS - 1);
-    const int src = lp(p * L + j);
+    const int src = p * S + lp(j);
 #pragma unroll
     for (int r = 0; r < R; ++r) z.ld(src + lp(r * NB), xr[q][r], xi[q][r]);
     if constexpr (NS > 1) {
@@ -157,7 +157,7 @@ __device__ __forceinline__ void stockham_pass(const ZB& z, const TW* __restrict_
     if constexpr (R == 8) dft8(xr[q], xi[q]);
     else if constexpr (R == 4) dft4(xr[q][0], xi[q][0], xr[q][1], xi[q][1], xr[q][2], xi[q][2], xr[q][3], xi[q][3]);
     else dft2(xr[q], xi[q]);
-    dst[q] = lp(p * L + (j / NS) * NS * R + k);
+    dst[q] = p * S + lp((j / NS) * NS * R + k);
   }
   __syncthreads();
 #pragma unroll
@@ -169,25 +169,36 @@ __device__ __forceinline__ void stockham_pass(const ZB& z, const TW* __restrict_
 
 // the passes from sub-transform length NS on: radix 8 while it divides, then one radix-4 or
 // radix-2 pass
-template <int L, int T, int E, int NS, typename ZB, typename TW>
+template <int L, int T, int E, int NS, int S, typename ZB, typename TW>
 __device__ __forceinline__ void dft_passes(const ZB& z, const TW* __restrict__ tw, int tid) {
   if constexpr (NS * 8 <= L) {
-    stockham_pass<L, 8, T, E, NS>(z, tw, tid);
-    dft_passes<L, T, E, NS * 8>(z, tw, tid);
+    stockham_pass<L, 8, T, E, NS, S>(z, tw, tid);
+    dft_passes<L, T, E, NS * 8, S>(z, tw, tid);
   } else if constexpr (NS * 4 == L) {
-    stockham_pass<L, 4, T, E, NS>(z, tw, tid);
+    stockham_pass<L, 4, T, E, NS, S>(z, tw, tid);
   } else if constexpr (NS * 2 == L) {
-    stockham_pass<L, 2, T, E, NS>(z, tw, tid);
+    stockham_pass<L, 2, T, E, NS, S>(z, tw, tid);
   }
 }
 
 // tid: the thread's index in the block (a caller that runs several transforms can pass a
 // value the compiler cannot prove equal between them, so that each transform forms its
 // LDS addresses afresh instead of holding one set live across the others)
-template <int L, int T, int E, typename ZB, typename TW>
+// transform p of the block starts at element p S of the buffer (S >= lp(L - 1) + 1)
+template <int L, int T, int E, int S = lp(L), typename ZB, typename TW>
 __device__ __forceinline__ void lds_dft(const ZB& z, const TW* __restrict__ tw, int tid = threadIdx.x) {
-  dft_passes<L, T, E, 1>(z, tw, tid);
+  dft_passes<L, T, E, 1, S>(z, tw, tid);
 }
+
+// The stride between the transforms of a block whose threads also touch the same element
+// of several transforms at once (the column kernels' loads and stores: lanes of one row run
+// over the columns, i.e. over the transforms): S = 4 mod 32 puts element t of consecutive
+// transforms on banks 4 apart (8 apart in 32-bit words for 8-byte pairs), where the plain
+// stride lp(L) = 0 mod 32 would put them all on one bank.  Lengths below 256 (many short
+// transforms per block) keep the plain stride, which costs no LDS.
+constexpr int spread_stride(int L) { return L >= 256 ? lp(L) + ((4 - lp(L)) % 32 + 32) % 32 : lp(L); }
+// LDS elements of PT transforms at stride S
+constexpr int spread_size(int L, int PT) { return (PT - 1) * spread_stride(L) + lp(L - 1) + 1; }
 
 // threadIdx.x behind an empty asm statement: a value the compiler cannot merge with other
 // uses of the thread index
