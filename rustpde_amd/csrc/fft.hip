@@ -110,36 +110,19 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void rfft_cols_kernel(cons
   }
 }
 
-// Batch entries b < split (all of them when split = 0) apply (i k)^times / sc and write Y;
-// entries b >= split read input b - split again, apply (i k)^times1 / sc1 and write Y1 (the
-// values and the x-derivatives of the same fields in one launch).
-template <int L, typename Tin, typename V, typename Tout>
-__global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
-                                                                int cols, int times, double sc_d,
-                                                                const V* __restrict__ tw, long sx, long sy,
-                                                                Tout* __restrict__ Y1, int split, int times1,
-                                                                double sc1) {
+// The work of one irfft block on columns j0 .. j0 + C - 1; FULL: all of them lie below cols
+// (every block but the last), so the column guards drop out.
+template <bool FULL, int L, typename Tin, typename V, typename Tout, int S, typename ZB>
+__device__ __forceinline__ void irfft_block(const ZB& z, const Tin* __restrict__ X, Tout* __restrict__ Y, int cols,
+                                            int times, double sc_d, const V* __restrict__ tw, int j0) {
 #pragma clang fp contract(off)
   using G = CGeo<L, sizeof(V)>;
   constexpr int C = G::C, T = G::T, N = G::N, m = G::m;
-  constexpr int S = fftlds::spread_stride(L), NE = fftlds::spread_size(L, G::PP);  // transform p at p S
-  __shared__ __attribute__((aligned(16))) V zmem[2 * NE];
-  const fftlds::LdsZ<V, NE> z{zmem};
-  int b = blockIdx.y;
-  if (split > 0 && b >= split) {
-    b -= split;
-    Y = Y1;
-    times = times1;
-    sc_d = sc1;
-  }
   const V sc = (V)sc_d;
   // sc a power of two (and normal): the complex division by the real sc is a product
   const bool pow2 = sc_d > 0.0 && (__double_as_longlong(sc_d) & 0xFFFFFFFFFFFFFLL) == 0;
   const V isc = (V)(1.0 / sc_d);
-  X += sx * b;
-  Y += sy * b;
   const int tid = threadIdx.x;
-  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
   // (1) half spectra with (i k)^times / sc applied; the two columns of a pair (lanes e and
   //     e ^ 1: C is even) exchange theirs by a lane swap and write conj Z of the pair:
   //     conj Z_k = (Ar - Bi) - i (Ai + Br), conj Z_{L-k} = (Ar + Bi) + i (Ai - Br)
@@ -149,7 +132,7 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
 #pragma unroll
   for (int q = 0; q < QS; ++q) {
     const int e = tid + q * T, c = e % C, k = e / C, j = j0 + c;
-    const bool ok = e < m * C && j < cols;
+    const bool ok = e < m * C && (FULL || j < cols);
     va[q] = ok ? (V)X[(size_t)k * cols + j] : (V)0;
     vb[q] = ok ? (V)X[(size_t)(m + k) * cols + j] : (V)0;
   }
@@ -192,10 +175,38 @@ __global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(con
   const V inv = (V)(1.0 / (double)L);
   for (int e = tid; e < L * C; e += T) {
     const int c = e % C, t = e / C, j = j0 + c;
-    if (j >= cols) continue;
+    if (!FULL && j >= cols) continue;
     const int p = (c >> 1) * S;
     Y[(size_t)t * cols + j] = (Tout)((c & 1) ? -z.i(p + lp(t)) * inv : z.r(p + lp(t)) * inv);
   }
+}
+
+// Batch entries b < split (all of them when split = 0) apply (i k)^times / sc and write Y;
+// entries b >= split read input b - split again, apply (i k)^times1 / sc1 and write Y1 (the
+// values and the x-derivatives of the same fields in one launch).
+template <int L, typename Tin, typename V, typename Tout>
+__global__ __launch_bounds__((CGeo<L, sizeof(V)>::T)) void irfft_cols_kernel(const Tin* __restrict__ X, Tout* __restrict__ Y,
+                                                                int cols, int times, double sc_d,
+                                                                const V* __restrict__ tw, long sx, long sy,
+                                                                Tout* __restrict__ Y1, int split, int times1,
+                                                                double sc1) {
+  using G = CGeo<L, sizeof(V)>;
+  constexpr int C = G::C;
+  constexpr int S = fftlds::spread_stride(L), NE = fftlds::spread_size(L, G::PP);  // transform p at p S
+  __shared__ __attribute__((aligned(16))) V zmem[2 * NE];
+  const fftlds::LdsZ<V, NE> z{zmem};
+  int b = blockIdx.y;
+  if (split > 0 && b >= split) {
+    b -= split;
+    Y = Y1;
+    times = times1;
+    sc_d = sc1;
+  }
+  X += sx * b;
+  Y += sy * b;
+  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * C;
+  if (j0 + C <= cols) irfft_block<true, L, Tin, V, Tout, S>(z, X, Y, cols, times, sc_d, tw, j0);
+  else irfft_block<false, L, Tin, V, Tout, S>(z, X, Y, cols, times, sc_d, tw, j0);
 }
 
 void check(const char* what) {
