@@ -217,8 +217,9 @@ def cpu_baseline(cfg, budget_s):
 
 def solid_step(cfg, lib_mod, steps=20):
     """The step with a solid obstacle (navier.rs:551-608; roughness elements of
-    solid_masks.rs:61-101 on both plates) on the same workload: the field-by-field FP64
-    step with the penalisation in its convective terms."""
+    solid_masks.rs:61-101 on both plates) on the same workload: the FP64 step (batched for a
+    periodic handle, field by field for a confined one) with the penalisation in its
+    convective terms."""
     from rustpde_amd.checkpoint import coords
     from rustpde_amd.solid_masks import solid_roughness_sinusoid
     nav = make(cfg, lib_mod)
@@ -234,7 +235,8 @@ def solid_step(cfg, lib_mod, steps=20):
     nav.close()
     return {"ms_per_step": 1e3 * el / steps, "steps": steps, "finite": ok,
             "mask": "solid_roughness_sinusoid(height 0.1, wavenumber 3)",
-            "note": "field-by-field FP64 step (the batched, fused and mixed paths do not take the terms)"}
+            "note": "FP64 step with the penalisation (batched when periodic; the FP32 convective pipeline has no "
+                    "penalised form)"}
 
 
 def stats_and_snapshot(nav, updates=20):

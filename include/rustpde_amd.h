@@ -95,9 +95,10 @@ int rbc_eval(rbc_navier* h, int what, double* out);
  * src/navier/solid_masks.rs:34-162 are host code, rustpde_amd/solid_masks.py): mask and
  * value are physical [nx, ny] arrays; mask = NULL removes the obstacle (None).  The
  * penalisation (eta = 1e-2) enters the convective terms of ux, uy and temp
- * (navier.rs:551-560, 579-584, 603-608).  A handle with a mask runs the field-by-field FP64
- * step (the batched and fused paths, equivalent in FP64, do not take the terms): setting a
- * mask while RBC_OPT_MIXED is on, or RBC_OPT_MIXED while a mask is set, returns RBC_ERR_ARG.
+ * (navier.rs:551-560, 579-584, 603-608).  A periodic handle with a mask runs the batched FP64
+ * step with the terms, a confined one the field-by-field FP64 step (the confined field triple
+ * does not take them); the FP32 convective pipeline has no penalised form: setting a mask
+ * while RBC_OPT_MIXED is on, or RBC_OPT_MIXED while a mask is set, returns RBC_ERR_ARG.
  * Not available for ensemble instances. */
 int rbc_set_solid(rbc_navier* h, const double* mask, const double* value);
 

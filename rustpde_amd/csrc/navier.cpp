@@ -1394,7 +1394,7 @@ class Navier {
   }
   // ----------------------------------------------------------- batched periodic step
   bool batch_ok() const {
-    return !solid_on && nb == 1 && use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
+    return nb == 1 && use_batch && periodic && vel_same && use_fft && fft_supported(nx) && use_dct && cheb_y_supported(ny) &&
            use_bands && fdma_rows_lds_supported(by_u.m, by_u.n) && tdma_rows_lds_supported(by_u.m) &&
            by_phi.n == by_u.n;
   }
@@ -1480,6 +1480,10 @@ class Navier {
       cheb_y_conv(st, W1.p, W2.p, my, my, CONV.p, ny, nx, ny, band.coef.p, 1.0 / scale[1], UXP.p, UYP.p, ny, 3,
                   (long)FS, (long)FS, UXP.p, bc_dx_zero ? nullptr : DTBCX.p, UYP.p, DTBCY.p, 2, conv3_ldf());
       timer.end(st);
+      if (solid_on) {
+        CHEBY_BWD_B(by_u, W2.p + 2 * FS, Tv.p, nx, 0, nullptr, 0.0, 1);  // temp in physical space
+        solid_damp_batched();
+      }
       CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
       RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3, ny * 2 / 3);  // y-modes >= the cut are zero
       return;
@@ -1492,8 +1496,22 @@ class Navier {
     CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 0, UXP.p, 0.0, 3);
     IRFFT_B(UX.p, W1.p, my, 0, 1.0, 3);
     CHEBY_BWD_B(by_u, W1.p, CONV.p, nx, 1, UYP.p, 1.0, 3, true);  // + u . grad(fieldbc) for temp
+    if (solid_on) {
+      backward(RBC_TEMP);  // temp in physical space
+      solid_damp_batched();
+    }
     CHEBY_FWD_B(CONV.p, W2.p, nx, ny * 2 / 3, 3);  // dealiased forward transforms
     RFFT_B(W2.p, CV.p, ny, bx_f.m * 2 / 3, 3);
+  }
+  // the solid obstacle's penalisation of the three physical convective terms (CONV slots
+  // 0, 1, 2), as the field-by-field step applies it: -mask ux / eta, -mask uy / eta
+  // (navier.rs:579-584, 603-608) and -mask (temp + fieldbc - value) / eta with temp in
+  // physical space (Tv, formed by the caller; navier.rs:551-560), eta = 1e-2
+  void solid_damp_batched() {
+    const long np = (long)nx * ny;
+    ew_solid_damp(st, CONV.p, SOLID_M.p, UXP.p, nullptr, nullptr, -1.0 / 1e-2, np);
+    ew_solid_damp(st, CONV.p + FS, SOLID_M.p, UYP.p, nullptr, nullptr, -1.0 / 1e-2, np);
+    ew_solid_damp(st, CONV.p + 2 * FS, SOLID_M.p, Tv.p, BCv.p, SOLID_V.p, -1.0 / 1e-2, np);
   }
   // the same pipeline with FP32 values and arithmetic (FP64 spectra in, FP32 convective
   // terms out in the first three FP32 slots of MIX)
@@ -1833,7 +1851,9 @@ class Navier {
   }
   // ------------------------------------------------------------ solid obstacle
   // navier.solid = Some([mask, value]) (navier.rs:189-190): the penalisation terms enter the
-  // convective terms of the field-by-field step (the batched and fused paths do not take them)
+  // physical convective terms of the FP64 step (the batched periodic step and the
+  // field-by-field step; the confined field triple and the FP32 convective pipeline do not
+  // take them, so a confined handle with a mask runs field by field and a mixed one refuses it)
   bool solid_on = false;
   DBuf SOLID_M, SOLID_V;
   void set_solid(const double* mask, const double* value) {
@@ -1846,8 +1866,7 @@ class Navier {
     }
     if (!value) throw std::invalid_argument("solid: value array required with the mask");
     if (mixed)
-      throw std::invalid_argument("solid: the penalised step is the field-by-field FP64 step; switch RBC_OPT_MIXED "
-                                  "off first");
+      throw std::invalid_argument("solid: the penalised step is an FP64 step; switch RBC_OPT_MIXED off first");
     const size_t np = (size_t)nx * ny;
     SOLID_M.upload(std::vector<double>(mask, mask + np));
     SOLID_V.upload(std::vector<double>(value, value + np));
@@ -2896,7 +2915,7 @@ void apply_option(Navier& n, int option, int value, bool alloc = true) {
       break;
     case RBC_OPT_MIXED:
       if (value && n.solid_on)
-        throw std::invalid_argument("RBC_OPT_MIXED: a handle with a solid obstacle runs the field-by-field FP64 step; "
+        throw std::invalid_argument("RBC_OPT_MIXED: a handle with a solid obstacle runs an FP64 step; "
                                     "remove the obstacle first");
       n.mixed = value != 0;
       if (alloc && n.mixed && n.periodic && n.MIX.n < (11 * n.FS + 1) / 2) n.MIX.alloc((11 * n.FS + 1) / 2);

@@ -1,7 +1,7 @@
 """Solid obstacles on the device (navier.rs:551-560, :579-584, :603-608) through the C ABI:
 the penalised step against the oracle with the reference's mask builders, 1e-12 relative
-(FP64) after 6 steps, eager and graph-replayed; removing the mask returns to the step
-without it (still matching the oracle)."""
+(FP64) after 6 steps, eager and graph-replayed, in the batched periodic step and field by
+field; removing the mask returns to the step without it (still matching the oracle)."""
 import numpy as np
 import pytest
 
@@ -30,13 +30,14 @@ def _pair(oracle, periodic):
     return g, o, masks
 
 
-@pytest.mark.parametrize("graph", [1, 0])
+@pytest.mark.parametrize("graph,batch", [(1, 1), (0, 1), (1, 0)])
 @pytest.mark.parametrize("periodic", [False, True])
-def test_solid_vs_oracle(oracle, periodic, graph, tmp_path):
+def test_solid_vs_oracle(oracle, periodic, graph, batch, tmp_path):
     from rustpde_amd import RbcError
-    print("case solid periodic=%s graph=%d" % (periodic, graph), flush=True)
+    print("case solid periodic=%s graph=%d batch=%d" % (periodic, graph, batch), flush=True)
     g, o, masks = _pair(oracle, periodic)
     g.set_option(0, graph)
+    g.set_option(g.OPT_BATCH, batch)  # periodic: the batched step takes the terms, or field by field
     g.solid = masks
     o.set_solid(masks)
     with pytest.raises(RbcError):  # the penalised step has no mixed-precision form
