@@ -103,15 +103,15 @@ __device__ __forceinline__ void dft8(T* r, T* i) {
   r[7] = e3r - o3r; i[7] = e3i - o3i;
 }
 
-// one Stockham pass of radix R over the P transforms of length L held in (re, im), the
-// NS-th (NS = 1, 8, 64, ..: the length of the sub-transforms already done).  The LDS index
-// of every element a thread touches is its base index's plus a constant, since
-// lp(a + c) = lp(a) + lp(c) whenever (a mod 32) + (c mod 32) < 32: the reads take
-// a = p L + j (j < L / R) and c = r L / R; the writes a = p L + (j / NS) NS R + k (k < NS)
-// and c = r NS, and for NS = 1 (a mod 32 a multiple of R, r < R) and NS = 8 (a mod 32 =
-// k < 8 when 8 R is a multiple of 32, c mod 32 = 8 r; R = 2: a mod 32 < 24, c < 16) that
-// holds as for NS a multiple of 32 (L < 32: p L mod 32 plus the offsets stays below 32).  So each thread forms
-// two LDS indices per butterfly and the rest are immediate offsets.
+// one Stockham pass of radix R over the transforms of length L held in z (element t of
+// transform p at p S + lp(t)), the pass after sub-transforms of length NS (NS = 1, 8, 64, ..).
+// Every element a thread touches sits at its base index plus a constant, since
+// lp(a + c) = lp(a) + lp(c) whenever (a mod 32) + (c mod 32) < 32: the reads take a = j
+// (j < L / R) and c = r L / R; the writes a = (j / NS) NS R + k (k < NS) and c = r NS, and
+// for NS = 1 (a mod 32 a multiple of R, r < R) and NS = 8 (a mod 32 = k < 8 when 8 R is a
+// multiple of 32, c mod 32 = 8 r; R = 2: a mod 32 < 24, c < 16) that holds as it does for NS
+// a multiple of 32.  So each thread forms two LDS indices per butterfly and the rest are
+// immediate offsets.
 template <int L, int R, int T, int E, int NS, int S, typename ZB, typename TW>
 __device__ __forceinline__ void stockham_pass(const ZB& z, const TW* __restrict__ tw, int tid) {
   using V = typename ZB::value_type;
