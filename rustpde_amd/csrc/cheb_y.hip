@@ -27,6 +27,7 @@
 
 #include "bufrow.hpp"
 #include "fft_lds.hpp"
+#include "wave.hpp"
 #include "ops.hpp"
 
 namespace rbc {
@@ -240,14 +241,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
 #pragma unroll
       for (int q = SEG - 1; q >= 0; --q)
         if (t0 + q < C) tot = z.c(comp, lp(zb + 2 * (t0 + q))) + tot;
-      V inc = tot;  // inclusive suffix sum over lanes
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const V o = __shfl_down(inc, off);
-        if (lane + off < 64) inc = inc + o;
-      }
-      V acc = __shfl_down(inc, 1);
-      if (lane == 63) acc = 0;
+      V acc;  // the chain's sum above this lane's segment
+      wave::suffix_sum(tot, lane, acc);
 #pragma unroll
       for (int q = SEG - 1; q >= 0; --q) {
         const int t = t0 + q;
@@ -586,14 +581,8 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
 #pragma unroll
         for (int q = SEG - 1; q >= 0; --q)
           if (t0 + q < C) tot = z.i(lp(zb + 2 * (t0 + q))) + tot;
-        V inc = tot;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const V o = __shfl_down(inc, off);
-          if (lane + off < 64) inc = inc + o;
-        }
-        V acc = __shfl_down(inc, 1);
-        if (lane == 63) acc = 0;
+        V acc;  // the chain's sum above this lane's segment
+        wave::suffix_sum(tot, lane, acc);
 #pragma unroll
         for (int q = SEG - 1; q >= 0; --q) {
           const int t = t0 + q;
