@@ -531,21 +531,28 @@ class Navier {
   bool gpres_valid = true;
   size_t FS = 0;          // slot stride of the batched buffers
   DBuf STATE, PHYS2;      // [ux, uy, temp] spectral and [ux, uy] physical, one slot each
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;    // one captured step
+  hipGraphExec_t gexec_k = nullptr;  // GRAPH_STEPS captured steps in one graph
+  // Consecutive replays of a graph are separated by its launch (about 8 us between the
+  // last kernel of one replay and the first of the next at 1024x1025, against none between
+  // the kernels inside one): runs of steps replay GRAPH_STEPS steps per launch.
+  static constexpr int GRAPH_STEPS = 8;
   int eager_steps = 0;     // eager steps since the captured step was last dropped
   bool graph_gpres = false;  // the captured step reads d/dy pres left by the step before it
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (gexec_k) (void)hipGraphExecDestroy(gexec_k);
     gexec = nullptr;
+    gexec_k = nullptr;
     eager_steps = 0;
   }
-  // capture one step into gexec (the captured launches do not run)
-  void capture_step() {
+  // capture k steps into one graph (the captured launches do not run)
+  hipGraphExec_t capture_steps(int k) {
     hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const double t_saved = time;
     try {
-      update();
+      for (int i = 0; i < k; ++i) update();
     } catch (...) {
       (void)hipStreamEndCapture(st, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -553,16 +560,33 @@ class Navier {
       throw;
     }
     time = t_saved;
-    graph_gpres = gpres_valid;  // set by the captured step when its fused pressure stage forms d/dy pres
     HIPCHECK(hipStreamEndCapture(st, &g));
-    hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) { gexec = nullptr; throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    if (e != hipSuccess) throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return ex;
+  }
+  // the one-step graph and the GRAPH_STEPS-step graph; every captured step after the first
+  // reads the d/dy pres its predecessor formed, as the one-step graph's replays do
+  void capture_step() {
+    const bool gp_saved = gpres_valid;
+    gexec = capture_steps(1);
+    graph_gpres = gpres_valid;  // set by the captured step when its fused pressure stage forms d/dy pres
+    gpres_valid = gp_saved;
+    try {
+      gexec_k = capture_steps(GRAPH_STEPS);
+    } catch (...) {
+      drop_graph();
+      throw;
+    }
+    gpres_valid = graph_gpres;
   }
   // Integrate::update nsteps times.  After a set-up change the first step runs eagerly (it
   // tunes the GEMM tiles and, after a host write, forms d/dy pres on its own); every later
   // step -- in this call or in a later one, so that one step per call as integrate() makes
-  // them reaches it from the second call on -- replays one captured step.
+  // them reaches it from the second call on -- replays a captured step, GRAPH_STEPS of them
+  // per launch while that many remain.
   void update_n(int nsteps) {
     int done = 0;
     if (gexec && graph_gpres && !gpres_valid) drop_graph();  // d/dy pres was not left by a step
@@ -573,13 +597,16 @@ class Navier {
       ++done;
     }
     if (graph && !gexec && done < nsteps) capture_step();
-    for (; done < nsteps; ++done) {
+    while (done < nsteps) {
       if (graph && gexec) {
-        HIPCHECK(hipGraphLaunch(gexec, st));
-        time += dt;
+        const int k = (gexec_k && nsteps - done >= GRAPH_STEPS) ? GRAPH_STEPS : 1;
+        HIPCHECK(hipGraphLaunch(k > 1 ? gexec_k : gexec, st));
+        for (int i = 0; i < k; ++i) time += dt;  // as k single steps add it
+        done += k;
       } else {
         update();
         ++eager_steps;
+        ++done;
       }
     }
   }  // Fourier x transforms by the FFT kernels (dense DFT GEMMs otherwise)

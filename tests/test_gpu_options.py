@@ -50,18 +50,19 @@ def test_option_parity(oracle, periodic, off):
 
 @pytest.mark.parametrize("periodic", [False, True])
 def test_graph_replay_matches_eager(oracle, periodic):
-    """Graph replays interleaved with eager work reproduce the eager step bit for bit."""
+    """Graph replays (one step or eight per launch) interleaved with eager work reproduce the
+    eager steps bit for bit, and the time advances as in eager steps."""
     print("case graph replay periodic=%s" % periodic, flush=True)
     ge, _ = _pair(oracle, periodic)
     gg, _ = _pair(oracle, periodic)
     ge.set_option(OPTS["graph"], 0)
-    for rep in range(3):
-        ge.update(3)
-        gg.update(3)  # 1 eager + 2 replays the first time, 3 replays after
+    for rep, n in enumerate((3, 3, 3, 19)):
+        ge.update(n)
+        gg.update(n)  # 1 eager + 2 replays the first time, 3 after; 19: two 8-step replays and 3
         _ = gg.temp.v  # eager device work between replays
         for a, b in ((ge.temp, gg.temp), (ge.ux, gg.ux), (ge.uy, gg.uy), (ge.pres[0], gg.pres[0])):
             assert np.array_equal(a.vhat, b.vhat), rep
-    assert abs(ge.get_time() - gg.get_time()) < 1e-15
+    assert ge.get_time() == gg.get_time()
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3])
