@@ -41,30 +41,49 @@ struct RowTables {
 };
 
 // One wavefront per (row, parity chain); the two chains of a row share its LDS buffer.
+// With pairg > 0 (one block per set, entries 0 and 1 on the same tables) the grid is 1-D:
+// blocks [0, 16 pairg) hold entries 0 and 1 interleaved in groups of 16, block b of a group
+// taking set 8 g + (b mod 8) of entry (b / 8) mod 2, so that the two blocks of a set are
+// dispatched 8 apart and land on the same XCD (workgroups go round-robin over the 8 XCDs):
+// the second reads the set's tables from that XCD's L2.  Blocks from 16 pairg on take the
+// further entries, nsets blocks each.
 template <int SLICE>
 __global__ __launch_bounds__(WAVE* WAVES, SLICE <= 8 ? 4 : 2) void fdma_rows_lds_kernel(const double* __restrict__ R, int ldr, int nr,
                                                                     double* __restrict__ X, int ld, int nrows, int m,
                                                                     int nsets, int per, int bl, RowTables tb, long sr,
-                                                                    long sx) {
+                                                                    long sx, int pairg) {
 #pragma clang fp contract(off)
   extern __shared__ double lds[];
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
   const int slot = w >> 1, p = w & 1;  // row slot of the block, chain parity
-  const int ti = tb.shared ? 0 : (int)blockIdx.y;
+  int entry = blockIdx.y, bx = blockIdx.x;
+  if (pairg > 0) {
+    const int pb = 16 * pairg;
+    if (bx < pb) {
+      entry = (bx >> 3) & 1;
+      bx = (bx >> 4) * 8 + (bx & 7);
+      if (bx >= nsets) return;  // the whole block: no barrier reached
+    } else {
+      bx -= pb;
+      entry = 2 + bx / nsets;
+      bx -= (entry - 2) * nsets;
+    }
+  }
+  const int ti = tb.shared ? 0 : entry;
   const double* __restrict__ bandp = tb.band[ti];
   const double* __restrict__ diagp = tb.diag[ti];
-  R += sr * blockIdx.y;
-  X += sx * blockIdx.y;
+  R += sr * entry;
+  X += sx * entry;
   double* buf = lds + slot * bl;
   // rows r and r + nsets share a diagonal set: one block per set when nrows = 2 nsets
   int r, set;
   bool valid;
   if (nrows == 2 * nsets) {
-    set = blockIdx.x;
+    set = bx;
     r = set + slot * nsets;
     valid = true;
   } else {
-    r = blockIdx.x * (WAVES / 2) + slot;
+    r = bx * (WAVES / 2) + slot;
     valid = r < nrows;
     set = valid ? r % nsets : 0;
   }
@@ -218,16 +237,21 @@ static void launch_rows(hipStream_t s, const double* R, int ldr, int nr, double*
   const int per = fdma_rows_lds_per(m), bl = buf_len(nr);
   const unsigned blocks = nrows == 2 * nsets ? (unsigned)nsets : (unsigned)((nrows + 1) / 2);
   const size_t lds = (size_t)(WAVES / 2) * bl * sizeof(double);
-  const dim3 grid(blocks, batch), block(WAVE * WAVES);
+  // entries 0 and 1 on the same tables, one block per set: interleaved 1-D grid
+  const bool paired = !tb.shared && batch >= 2 && nrows == 2 * nsets && tb.band[0] == tb.band[1] &&
+                      tb.diag[0] == tb.diag[1];
+  const int pairg = paired ? (nsets + 7) / 8 : 0;
+  const dim3 grid = paired ? dim3((unsigned)(16 * pairg + (batch - 2) * nsets), 1) : dim3(blocks, batch),
+             block(WAVE * WAVES);
   if (per <= 4)
     RBC_LAUNCH(fdma_rows_lds_kernel<4>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
-                       sr, sx);
+                       sr, sx, pairg);
   else if (per <= 8)
     RBC_LAUNCH(fdma_rows_lds_kernel<8>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
-                       sr, sx);
+                       sr, sx, pairg);
   else
     RBC_LAUNCH(fdma_rows_lds_kernel<16>, grid, block, lds, s, R, ldr, nr, X, ld, nrows, m, nsets, per, bl, tb,
-                       sr, sx);
+                       sr, sx, pairg);
   check("fdma_rows_lds");
 }
 
