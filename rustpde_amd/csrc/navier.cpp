@@ -1653,12 +1653,19 @@ class Navier {
   void solves_and_pressure() {
     const int my = by_u.m, R2 = 2 * mx();
     const size_t F = FS;
-    // implicit solves of ux, uy (one solver for both) and temp in one launch
+    // implicit solves of ux, uy (one solver for both) and temp in one launch; with the fused
+    // pressure stage temp's solve runs in further blocks of the stage's launch instead (the
+    // stage does not touch temp), where they fill the workgroup slots its mx blocks leave free
     const Solver &sv = *solver[0], &stt = *solver[2];
+    const bool fold_temp = pressure_fused_ok() && stt.nsets == mx() && !stt.has_fb;
+    const int nsolve = fold_temp ? 2 : 3;
     const double* bands[3] = {sv.pybp.p, sv.pybp.p, stt.pybp.p};
     const double* diags[3] = {sv.diagp.p, sv.diagp.p, stt.diagp.p};
-    timer.begin(st, 1, 8.0 * (3.0 * (double)R2 * (ny + my) + 2.0 * 4.0 * (double)sv.nsets * my));
-    fdma_rows_lds_multi(st, RHS.p, ny, ny, UX.p, my, R2, my, sv.nsets, bands, diags, 3, (long)F, (long)F);
+    // bytes: right-hand sides read and results written per row, the diagonal sets once per solver
+    const double temp_bytes = 8.0 * ((double)R2 * (ny + my) + 4.0 * (double)stt.nsets * my);
+    timer.begin(st, 1, 8.0 * (2.0 * (double)R2 * (ny + my) + 4.0 * (double)sv.nsets * my) +
+                           (fold_temp ? 0.0 : temp_bytes));
+    fdma_rows_lds_multi(st, RHS.p, ny, ny, UX.p, my, R2, my, sv.nsets, bands, diags, nsolve, (long)F, (long)F);
     timer.end(st);
     // projection (navier.rs:683-721)
     if (pressure_fused_ok()) {
@@ -1669,11 +1676,22 @@ class Navier {
       pa.su_kind = by_u.k == CD ? 1 : by_u.k == CN ? 2 : 0;
       pa.sp_kind = by_phi.k == CD ? 1 : by_phi.k == CN ? 2 : 0;
       pa.dP = G.p + FS;  // d/dy of the new pres for the next step
+      if (fold_temp) {
+        pa.rows_R = RHS.p + 2 * F;
+        pa.rows_X = UX.p + 2 * F;
+        pa.rows_band = stt.pybp.p;
+        pa.rows_diag = stt.diagp.p;
+        pa.rows_ldr = ny;
+        pa.rows_nr = ny;
+        pa.rows_ld = my;
+        pa.rows_m = my;
+        pa.rows_nsets = stt.nsets;
+      }
       const double rows = 2.0 * mx();
       // bytes, distinct arrays once per row: ux, uy, pres read; ux, uy, pseu, pres and d/dy
-      // pres written; the four diagonal arrays of the row's Poisson set
+      // pres written; the four diagonal arrays of the row's Poisson set (and temp's solve)
       timer.begin(st, 10, 8.0 * rows * (2.0 * by_u.m + ny) + 8.0 * rows * (3.0 * by_u.m + 2.0 * ny) +
-                              8.0 * 4.0 * mx() * (double)by_phi.m);
+                              8.0 * 4.0 * mx() * (double)by_phi.m + (fold_temp ? temp_bytes : 0.0));
       pressure_rows(st, pa);
       timer.end(st);
       gpres_valid = true;

@@ -246,6 +246,15 @@ struct PressureArgs {
   // when set: d/dy of the updated pres (Chebyshev derivative / scale_y, rows as P0) for the
   // next step's right-hand sides
   double* dP = nullptr;
+  // when rows_nsets > 0: rows_nsets further blocks of the launch solve the banded rows of
+  // another field, as fdma_rows_lds with one block per set (2 rows_nsets rows of rows_nr
+  // right-hand side values at stride rows_ldr, rows_m results at stride rows_ld); their
+  // arrays must not overlap the stage's
+  const double* rows_R = nullptr;
+  double* rows_X = nullptr;
+  const double *rows_band = nullptr, *rows_diag = nullptr;
+  int rows_ldr = 0, rows_nr = 0, rows_ld = 0, rows_m = 0, rows_nsets = 0;
+  int rows_per = 0, rows_bl = 0;  // set by pressure_rows
 };
 bool pressure_rows_supported(int ny, int mu);
 void pressure_rows(hipStream_t s, const PressureArgs& a);

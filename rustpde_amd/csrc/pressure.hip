@@ -19,6 +19,7 @@
 
 #include "chains.hpp"
 #include "ops.hpp"
+#include "rowsolve.hpp"
 
 namespace rbc {
 
@@ -59,11 +60,19 @@ __device__ __forceinline__ double tap_l(int kind, const double* __restrict__ coe
 // RBC: the periodic Rayleigh-Benard bases (velocity Dirichlet, pseudo pressure Neumann)
 // with a power-of-two x scale, fixed at compile time so that only their tap and scale code
 // is emitted (the general kernel selects them at run time; the arithmetic is the same).
+// Blocks from mx on solve the further field's rows (a.rows_*), which the stage does not touch:
+// they fill the workgroup slots the mx stage blocks leave free.
 template <int SLICE, bool RBC>
 __global__ __launch_bounds__(256, 3) void pressure_rows_kernel(PressureArgs a) {
 #pragma clang fp contract(off)
   const int su_kind = RBC ? 1 : a.su_kind, sp_kind = RBC ? 2 : a.sp_kind;
   extern __shared__ double lds[];
+  if ((int)blockIdx.x >= a.mx) {  // the whole block
+    rowsolve::fdma_block<SLICE>(lds, a.rows_R, a.rows_ldr, a.rows_nr, a.rows_X, a.rows_ld, 2 * a.rows_nsets,
+                                a.rows_m, a.rows_nsets, a.rows_per, a.rows_bl, a.rows_band,
+                                a.rows_diag, (int)blockIdx.x - a.mx);
+    return;
+  }
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
   const int r = w >> 1, p = w & 1;  // plane (0 real, 1 imaginary) and chain parity of this wavefront
   const int k = blockIdx.x, mx = a.mx, ny = a.ny, mu = a.mu, N = ny - 1, bl = a.bl;
@@ -423,12 +432,22 @@ void pressure_rows(hipStream_t s, const PressureArgs& args) {
   const int perd = ((a.ny - 1) / 2 + WAVE - 1) / WAVE, sl = a.per > perd ? a.per : perd;
   const size_t lds = (size_t)6 * a.bl * sizeof(double);
   const bool rbc = a.su_kind == 1 && a.sp_kind == 2 && a.isc != 0.0;
-  if (sl <= 4) {
-    if (rbc) RBC_LAUNCH((pressure_rows_kernel<4, true>), dim3(a.mx), dim3(256), lds, s, a);
-    else RBC_LAUNCH((pressure_rows_kernel<4, false>), dim3(a.mx), dim3(256), lds, s, a);
+  if (a.rows_nsets > 0) {  // the further rows: slices within the stage's SLICE, buffers within its LDS
+    a.rows_per = fdma_rows_lds_per(a.rows_m);
+    a.rows_bl = buf_len(a.rows_nr);
+    if (!fdma_rows_lds_supported(a.rows_m, a.rows_nr) || (a.rows_per <= 4 ? 4 : 8) > (sl <= 4 ? 4 : 8) ||
+        (size_t)(rowsolve::WAVES / 2) * a.rows_bl * sizeof(double) > lds || !a.rows_R || !a.rows_X || !a.rows_diag)
+      throw std::invalid_argument("pressure_rows: unsupported further rows");
   } else {
-    if (rbc) RBC_LAUNCH((pressure_rows_kernel<8, true>), dim3(a.mx), dim3(256), lds, s, a);
-    else RBC_LAUNCH((pressure_rows_kernel<8, false>), dim3(a.mx), dim3(256), lds, s, a);
+    a.rows_nsets = 0;
+  }
+  const dim3 grid((unsigned)(a.mx + a.rows_nsets));
+  if (sl <= 4) {
+    if (rbc) RBC_LAUNCH((pressure_rows_kernel<4, true>), grid, dim3(256), lds, s, a);
+    else RBC_LAUNCH((pressure_rows_kernel<4, false>), grid, dim3(256), lds, s, a);
+  } else {
+    if (rbc) RBC_LAUNCH((pressure_rows_kernel<8, true>), grid, dim3(256), lds, s, a);
+    else RBC_LAUNCH((pressure_rows_kernel<8, false>), grid, dim3(256), lds, s, a);
   }
   check("pressure_rows");
 }

@@ -1,21 +1,7 @@
 // Banded row solves of the tensor solvers (fdma_tensor.rs:195-234 with the per-row Fdma
 // of fdma.rs:101-118), FP64, optionally fused with the preconditioner matvec that
-// precedes them (MatVecFdma along y, matvec.rs:172-193).
-//
-// The offsets -2, 0, 2, 4 of the 4-diagonal systems couple only indices of equal parity,
-// so a row is two independent chains, each solved by one wavefront; lane l owns chain
-// positions [l per, (l + 1) per).  A pass over a chain is a linear recurrence:
-// each lane composes the affine map of its slice, the maps are scanned across the
-// wavefront with shuffles, and each lane replays its slice with the reference's
-// arithmetic from its incoming state (a division by a diagonal is a product with the
-// reciprocal the table holds).
-//
-// Memory layout for coalescing: the right-hand side row is loaded with unit stride into
-// a padded LDS buffer of the wavefront and the slices are read from there (results go
-// back the same way).  Diagonals and band taps are stored on the host in slice order,
-// [array][parity][q][lane] per diagonal set, so lane-parallel loads are contiguous.
-// Absent entries are stored as zeros (band taps, off-diagonals beyond the chain end):
-// adding their zero products leaves every sum as the reference forms it.
+// precedes them (MatVecFdma along y, matvec.rs:172-193): the row-solve kernel around the
+// block body of rowsolve.hpp, and the from_ortho row kernel of the composite bases.
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
@@ -23,6 +9,7 @@
 
 #include "chains.hpp"
 #include "ops.hpp"
+#include "rowsolve.hpp"
 
 namespace rbc {
 
@@ -31,7 +18,7 @@ namespace {
 using chains::buf_len;
 using chains::pad;
 using chains::WAVE;
-constexpr int WAVES = 4;  // wavefronts per block
+using rowsolve::WAVES;
 
 // Tables of up to four batch entries (blockIdx.y), or one shared by all entries.
 struct RowTables {
@@ -40,7 +27,7 @@ struct RowTables {
   int shared;
 };
 
-// One wavefront per (row, parity chain); the two chains of a row share its LDS buffer.
+// One block per set (nrows = 2 nsets) or per two rows, blockIdx.y the batch entry.
 // With pairg > 0 (one block per set, entries 0 and 1 on the same tables) the grid is 1-D:
 // blocks [0, 16 pairg) hold entries 0 and 1 interleaved in groups of 16, block b of a group
 // taking set 8 g + (b mod 8) of entry (b / 8) mod 2, so that the two blocks of a set are
@@ -52,10 +39,7 @@ __global__ __launch_bounds__(WAVE* WAVES, SLICE <= 8 ? 4 : 2) void fdma_rows_lds
                                                                     double* __restrict__ X, int ld, int nrows, int m,
                                                                     int nsets, int per, int bl, RowTables tb, long sr,
                                                                     long sx, int pairg) {
-#pragma clang fp contract(off)
   extern __shared__ double lds[];
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  const int slot = w >> 1, p = w & 1;  // row slot of the block, chain parity
   int entry = blockIdx.y, bx = blockIdx.x;
   if (pairg > 0) {
     const int pb = 16 * pairg;
@@ -70,83 +54,8 @@ __global__ __launch_bounds__(WAVE* WAVES, SLICE <= 8 ? 4 : 2) void fdma_rows_lds
     }
   }
   const int ti = tb.shared ? 0 : entry;
-  const double* __restrict__ bandp = tb.band[ti];
-  const double* __restrict__ diagp = tb.diag[ti];
-  R += sr * entry;
-  X += sx * entry;
-  double* buf = lds + slot * bl;
-  // rows r and r + nsets share a diagonal set: one block per set when nrows = 2 nsets
-  int r, set;
-  bool valid;
-  if (nrows == 2 * nsets) {
-    set = bx;
-    r = set + slot * nsets;
-    valid = true;
-  } else {
-    r = bx * (WAVES / 2) + slot;
-    valid = r < nrows;
-    set = valid ? r % nsets : 0;
-  }
-  // (1) row -> LDS: element i at pad(i + 2); i in [-2, 0) and [nr, nr + 4) read as zero
-  constexpr int MAXQ = (2 * SLICE * WAVE + 8 + 2 * WAVE - 1) / (2 * WAVE);
-  chains::stage_row<MAXQ>(buf, R + (size_t)r * ldr, valid, nr, p * WAVE + lane, 2 * WAVE);
-  __syncthreads();
-  const int len = p == 0 ? (m + 1) / 2 : m / 2;
-  const int t0 = lane * per;
-  const int cnt = max(0, min(per, len - t0));
-  double v[SLICE];
-  // The slice-ordered arrays carry SLICE * WAVE doubles of tail padding, so the loads of
-  // all q < SLICE are in bounds and are issued unconditionally, ahead of their use.
-  const size_t setbase = (size_t)set * (8 * (size_t)per * WAVE);
-#define DG(a, q) diagp[setbase + ((size_t)((a) * 2 + p) * per + (q)) * WAVE + lane]
-  if (bandp) {  // taps at offsets 0, -2, +2, +4, summed in that order
-    const size_t tp = (size_t)2 * per * WAVE;
-    double c[4][SLICE];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int q = 0; q < SLICE; ++q) c[k][q] = bandp[k * tp + ((size_t)p * per + q) * WAVE + lane];
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) {
-      double val = 0.0;
-      if (q < cnt) {
-        const int j = p + 2 * (t0 + q) + 2;
-        val = c[0][q] * buf[pad(j)];
-        val = val + c[1][q] * buf[pad(j - 2)];
-        val = val + c[2][q] * buf[pad(j + 2)];
-        val = val + c[3][q] * buf[pad(j + 4)];
-      }
-      v[q] = val;
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) v[q] = q < cnt ? buf[pad(p + 2 * (t0 + q) + 2)] : 0.0;
-  }
-  // ---- forward elimination and back substitution (diagonal tables of this set)
-  {
-    double lo[SLICE];
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) lo[q] = DG(0, q);
-    chains::fdma_forward(v, lo, lane, t0, cnt);
-  }
-  {
-    double rd[SLICE], g1[SLICE], g2[SLICE];  // 1 / dia, up1, up2
-#pragma unroll
-    for (int q = 0; q < SLICE; ++q) {
-      rd[q] = DG(1, q);
-      g1[q] = DG(2, q);
-      g2[q] = DG(3, q);
-    }
-    chains::fdma_backward(v, rd, g1, g2, lane, t0, cnt, len);
-  }
-#undef DG
-  // (3) slices -> LDS -> row, unit stride
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < SLICE; ++q)
-    if (q < cnt) buf[pad(p + 2 * (t0 + q) + 2)] = v[q];
-  __syncthreads();
-  chains::unstage_row<MAXQ>(buf, X + (size_t)r * ld, valid, m, p * WAVE + lane, 2 * WAVE, false);
+  rowsolve::fdma_block<SLICE>(lds, R + sr * entry, ldr, nr, X + sx * entry, ld, nrows, m, nsets, per, bl, tb.band[ti],
+                              tb.diag[ti], bx);
 }
 
 // from_ortho of a composite basis per row (composite_stencil.rs:250-276, linalg.rs:14-57):
