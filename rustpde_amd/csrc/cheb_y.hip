@@ -556,13 +556,17 @@ __global__ __launch_bounds__(Geo<L>::T, 4) void cheb_y_conv3_kernel(
       load(dx + (f + 1) * si, 0, ra);
       load(val + (f + 1) * si, 1, rb);
     } else {
+      // raw buffers over the gradient rows (an absent one has an empty range and reads as
+      // zero), so that the loads are issued together instead of one guarded round trip each
+      const int fcnt = ldf == 0 ? n : (int)((size_t)(rows - 1) * ldf + n);
+      const BufRow<double> h1(f1, f1 ? fcnt : 0), h2(f2, f2 ? fcnt : 0);
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int e = tid + q * T;
         const int p = e / n, j = e - p * n, r = row0 + p;
         const bool ok = e < P * n && r < rows;
-        const size_t o = (size_t)min(r, rows - 1) * ldf + min(j, n - 1);  // always valid
-        const V a1 = f1 ? (V)f1[o] : (V)0, a2 = f2 ? (V)f2[o] : (V)0;
+        const int o = min(r, rows - 1) * (int)ldf + min(j, n - 1);  // always within the range
+        const V a1 = (V)h1[o], a2 = (V)h2[o];
         g1[q] = ok ? a1 : (V)0;
         g2[q] = ok ? a2 : (V)0;
       }
@@ -822,6 +826,8 @@ void conv3_check(int n, int rows, int mi, const void* f2, long ldf, int cut, lon
   if (!cheb_y_supported(n)) throw std::invalid_argument("cheb_y_conv3: unsupported length");
   if (rows < 1 || mi != n - 2) throw std::invalid_argument("cheb_y_conv3: bad rows or input length");
   if (ldf < 0 || (f2 == nullptr && ldf != 0)) throw std::invalid_argument("cheb_y_conv3: bad fieldbc stride");
+  if (((double)(rows - 1) * (double)ldf + n) * 8.0 > 2147483647.0)  // the gradient rows' byte range, as an int
+    throw std::invalid_argument("cheb_y_conv3: fieldbc rows too large");
   if (cut < 0 || cut > n || si < 0 || so < 0 || ldo < n) throw std::invalid_argument("cheb_y_conv3: bad output layout");
 }
 }  // namespace

@@ -129,12 +129,16 @@ __device__ __forceinline__ void irfft_block(const ZB& z, const Tin* __restrict__
   //     (Im A_0, Im A_N ignored: the half-spectrum convention)
   constexpr int QS = (m * C + T - 1) / T;
   V va[QS], vb[QS];
+  // every load is issued, from element 0 of each plane where the entry lies outside (a
+  // guarded load would become a branch that waits for its own round trip)
 #pragma unroll
   for (int q = 0; q < QS; ++q) {
     const int e = tid + q * T, c = e % C, k = e / C, j = j0 + c;
     const bool ok = e < m * C && (FULL || j < cols);
-    va[q] = ok ? (V)X[(size_t)k * cols + j] : (V)0;
-    vb[q] = ok ? (V)X[(size_t)(m + k) * cols + j] : (V)0;
+    const size_t ia = ok ? (size_t)k * cols + j : 0;
+    const Tin xa = X[ia], xb = X[ia + (size_t)m * cols];
+    va[q] = ok ? (V)xa : (V)0;
+    vb[q] = ok ? (V)xb : (V)0;
   }
 #pragma unroll
   for (int q = 0; q < QS; ++q) {

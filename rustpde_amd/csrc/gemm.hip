@@ -86,6 +86,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = v4{0, 0, 0, 0};
   V ra[NA], rb[NB];
+  // B loads that convert are all issued, from element (0, 0) where the entry lies outside (a
+  // guarded load followed by its conversion would wait for its own round trip)
   auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < NA; ++q) {
@@ -95,7 +97,13 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
       const int e = tid + q * 256, kk = e / BN, nn = e % BN, gk = k0 + kk, gn = n0 + nn;
-      rb[q] = (gk < K && gn < N) ? (V)B[(size_t)gk * ldb + gn] : (V)0;
+      const bool ok = gk < K && gn < N;
+      if constexpr (sizeof(TB) != sizeof(V)) {
+        const TB x = B[ok ? (size_t)gk * ldb + gn : 0];
+        rb[q] = ok ? (V)x : (V)0;
+      } else {
+        rb[q] = ok ? (V)B[(size_t)gk * ldb + gn] : (V)0;
+      }
     }
   };
   gload(0);
@@ -127,6 +135,32 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
     }
   }
   const V va = (V)alpha, vb = (V)beta;
+  // the epilogue's operands U and C, all loads issued before the first is used (from
+  // element (0, 0) where an entry lies outside: a guarded load would wait for its own round
+  // trip); the branches on U and beta are uniform
+  V uv[FM][FN][4], cv[FM][FN][4];
+  if (U) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gm = m0 + wm0 + i * 16 + MF::row(r, lk), gn = n0 + wn0 + j * 16 + lr;
+          uv[i][j][r] = U[gm < M && gn < N ? (size_t)gm * ldu + gn : 0];
+        }
+  }
+  if (beta != 0.0) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gm = m0 + wm0 + i * 16 + MF::row(r, lk), gn = n0 + wn0 + j * 16 + lr;
+          cv[i][j][r] = (V)C[gm < M && gn < N ? (size_t)gm * ldc + gn : 0];
+        }
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -136,8 +170,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
         const int gm = m0 + wm0 + i * 16 + MF::row(r, lk), gn = n0 + wn0 + j * 16 + lr;
         if (gm < M && gn < N) {
           V v = va * acc[i][j][r];
-          if (U) v = v * U[(size_t)gm * ldu + gn];
-          if (beta != 0.0) v = v + vb * (V)C[(size_t)gm * ldc + gn];
+          if (U) v = v * uv[i][j][r];
+          if (beta != 0.0) v = v + vb * cv[i][j][r];
           if (zero00 && gm == 0 && gn == 0) v = (V)0;
           C[(size_t)gm * ldc + gn] = (TC)v;
         }

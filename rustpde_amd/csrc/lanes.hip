@@ -16,6 +16,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "bufrow.hpp"
 #include "ops.hpp"
 
 namespace rbc {
@@ -247,12 +248,13 @@ __global__ void band_xy_kernel(const double* __restrict__ in, int ldi, double* _
 // d_{n-1} = 0, d_0 = p_1 + d_2 / 2: a suffix sum along each parity chain.  Output is
 // scaled by 1/sc and optionally accumulated: out = d / sc [+ beta out].
 __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void cheb_diff_y_kernel(
-    const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows, int n,
+    const double* __restrict__ in, int ldi, int mi, double* __restrict__ out, int ldo, int rows, int n,
     const double* __restrict__ coef, const unsigned char* __restrict__ mask, double inv_sc, double beta, long si,
     long so) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & (WAVE - 1);
-  const int r = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE);
+  // the row of the wavefront (the same in every lane: scalar, so the row buffers below are)
+  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x / WAVE));
   if (r >= rows) return;
   in += si * blockIdx.y;
   out += so * blockIdx.y;
@@ -260,16 +262,23 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void cheb_diff_y_kernel(
   double* o = out + (size_t)r * ldo;
   const int per = ((n + 1) / 2 + WAVE - 1) / WAVE;  // slice length per parity
   const int i0 = 2 * lane * per;                      // first index of this lane's slice
-  // p_j (ortho coefficients) for j in [i0, i0 + 2 per + 1]
+  // p_j (ortho coefficients) for j in [i0, i0 + 2 per + 1].  The row (mi values) and the
+  // stencil taps are read through raw buffers whose range check returns 0 outside, so every
+  // load is issued and the mask only selects among the products (a load guarded by a mask bit
+  // would wait for the mask's round trip and then for its own)
+  const BufRow<double> ar(a, mi), cr(coef, coef ? 2 * n : 0);
   auto pval = [&](int j) -> double {
-    if (j >= n) return 0.0;
-    if (!coef) return a[j];
-    const unsigned mk = mask[j];
+    if (!coef) {
+      const double x = ar[j];
+      return j < n ? x : 0.0;
+    }
+    const unsigned mk = mask[min(j, n - 1)];
+    const double x0 = ar[j], c0 = cr[j], x2 = ar[j - 2], c2 = cr[n + j];
     double acc = 0.0;
     bool first = true;
-    if (mk & 1u) { acc = a[j] * coef[j]; first = false; }
-    if (mk & 2u) { const double t = a[j - 2] * coef[n + j]; acc = first ? t : acc + t; }
-    return acc;
+    if (mk & 1u) { acc = x0 * c0; first = false; }
+    if (mk & 2u) { const double t = x2 * c2; acc = first ? t : acc + t; }
+    return j < n ? acc : 0.0;
   };
   double w[2][SLICE];
   double s[2] = {0.0, 0.0};  // slice sums, accumulated top-down as the reference does
@@ -434,11 +443,13 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void tdma_y_kernel(
 bool cheb_diff_y_supported(int n) { return n >= 6 && (n + 1) / 2 <= WAVE * SLICE; }
 bool tdma_y_supported(int m) { return m >= 4 && (m + 1) / 2 <= WAVE * SLICE; }
 
-void cheb_diff_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, const double* coef,
-                 const unsigned char* mask, double inv_sc, double beta, int batch, long si, long so) {
+void cheb_diff_y(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
+                 const double* coef, const unsigned char* mask, double inv_sc, double beta, int batch, long si,
+                 long so) {
   if (!cheb_diff_y_supported(n)) throw std::invalid_argument("cheb_diff_y: unsupported length");
+  if (mi < 1 || mi > ldi || mi > n || (!coef && mi != n)) throw std::invalid_argument("cheb_diff_y: bad input length");
   const unsigned blocks = (unsigned)((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK);
-  RBC_LAUNCH(cheb_diff_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, out, ldo,
+  RBC_LAUNCH(cheb_diff_y_kernel, dim3(blocks, batch), dim3(WAVE * ROWS_PER_BLOCK), 0, s, in, ldi, mi, out, ldo,
                      rows, n, coef, mask, inv_sc, beta, si, so);
   check("cheb_diff_y");
 }

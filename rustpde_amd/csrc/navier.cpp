@@ -945,8 +945,8 @@ class Navier {
       const bool comp = b.k != CHEB;
       const DBand* band = comp ? &stencil_band(b) : nullptr;
       timer.begin(st, 3, 8.0 * nbt() * (double)rows * ((double)b.m + b.n * (beta != 0.0 ? 2.0 : 1.0)));
-      cheb_diff_y(st, in, b.m, out, b.n, rows, b.n, comp ? band->coef.p : nullptr, comp ? band->mask.p : nullptr, 1.0,
-                  beta, nbt(), dstr(), dstr());
+      cheb_diff_y(st, in, b.m, b.m, out, b.n, rows, b.n, comp ? band->coef.p : nullptr, comp ? band->mask.p : nullptr,
+                  1.0, beta, nbt(), dstr(), dstr());
       timer.end(st);
       return out;
     }

@@ -110,8 +110,9 @@ void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int
 // stencil of band coef/mask fused on load (coef == nullptr: input already ortho);
 // out = d / sc [+ beta out].
 bool cheb_diff_y_supported(int n);
-void cheb_diff_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int n, const double* coef,
-                 const unsigned char* mask, double inv_sc, double beta, int batch = 1, long si = 0, long so = 0);
+void cheb_diff_y(hipStream_t s, const double* in, int ldi, int mi, double* out, int ldo, int rows, int n,
+                 const double* coef, const unsigned char* mask, double inv_sc, double beta, int batch = 1, long si = 0,
+                 long so = 0);
 // from_ortho of every row (composite length m): d = S^T p then the tridiagonal solve with
 // the precomputed w and den of linalg.rs:14-57.
 bool tdma_y_supported(int m);
