@@ -298,6 +298,32 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
         ub[q] = ok ? (V)U2[(size_t)r * ldu + j] : (V)0;
       }
     }
+    // the extra products of the entry, formed ahead with all their loads issued together
+    // (from element 0 where an entry lies outside; the branches are uniform)
+    V x1[Q], x2[Q];
+    if (extra) {
+      auto idx = [&](int q, size_t& o, size_t& of) {
+        const int e = tid + q * T;
+        const int p = e / n, j = e - p * n, r = row0 + p;
+        const bool ok = e < P * n && r < rows;
+        o = ok ? (size_t)r * ldu + j : 0;
+        of = ok ? (xp.ldf < 0 ? o : (size_t)r * xp.ldf + j) : 0;
+      };
+      if (xp.f1) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          size_t o, of;
+          idx(q, o, of);
+          x1[q] = (V)xp.e1[o] * (V)xp.f1[of];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        size_t o, of;
+        idx(q, o, of);
+        x2[q] = (V)xp.e2[o] * (V)xp.f2[of];
+      }
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int e = tid + q * T;
@@ -311,9 +337,8 @@ __device__ __forceinline__ void cheb_y_bwd_body(const Tin* __restrict__ in, int 
       V v = zb * ub[q];
       v = v + (V)1 * xa;
       if (extra) {
-        const size_t o = (size_t)r * ldu + j, of = xp.ldf < 0 ? o : (size_t)r * xp.ldf + j;
-        if (xp.f1) v = v + (V)xp.e1[o] * (V)xp.f1[of];
-        v = v + (V)xp.e2[o] * (V)xp.f2[of];
+        if (xp.f1) v = v + x1[q];
+        v = v + x2[q];
       }
       out[(size_t)r * ldo + j] = (Tout)v;
     }
