@@ -145,24 +145,38 @@ __global__ __launch_bounds__(WAVE* ROWS_PER_BLOCK) void fdma_rows_wave_kernel(
 }
 
 // taps: coef[k * mo + i] for k = 0..3 at input offsets {0, -2, +2, +4}; mask bit k set if
-// tap k is present at output index i
+// tap k is present at output index i.  The input is read through a raw buffer over its extent
+// (reads outside it return 0): every tap kind in `taps` (the union of the mask bits) is loaded
+// for every output, and the output's mask selects among the products, so the loads are issued
+// together (a load guarded by the mask would wait for the mask's round trip, then for its own).
 __global__ void band_y_kernel(const double* __restrict__ in, int ldi, double* __restrict__ out, int ldo, int rows,
-                              int mo, const double* __restrict__ coef, const unsigned char* __restrict__ mask,
-                              double beta, long si, long so) {
+                              int mo, int mi, int taps, const double* __restrict__ coef,
+                              const unsigned char* __restrict__ mask, double beta, long si, long so) {
 #pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)rows * mo) return;
   in += si * blockIdx.y;
   out += so * blockIdx.y;
   const int r = (int)(t / mo), i = (int)(t % mo);
+  const BufRow<double> inb(in, (rows - 1) * ldi + mi);
   const unsigned mk = mask[i];
-  const double* a = in + (size_t)r * ldi;
   const int off[4] = {0, -2, 2, 4};
+  double x[4], c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[k] = 0.0;
+    c[k] = 0.0;
+    if (taps & (1 << k)) {
+      x[k] = inb[r * ldi + i + off[k]];
+      c[k] = coef[k * mo + i];
+    }
+  }
   double acc = 0.0;
   bool first = true;
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (!(mk & (1u << k))) continue;
-    const double term = a[i + off[k]] * coef[k * mo + i];
+    const double term = x[k] * c[k];
     acc = first ? term : acc + term;
     first = false;
   }
@@ -170,22 +184,34 @@ __global__ void band_y_kernel(const double* __restrict__ in, int ldi, double* __
   *o = (beta != 0.0) ? acc + beta * (*o) : acc;
 }
 
-__global__ void band_x_kernel(const double* __restrict__ in, double* __restrict__ out, int cols, int mo,
-                              const double* __restrict__ coef, const unsigned char* __restrict__ mask, double beta,
-                              long si, long so) {
+__global__ void band_x_kernel(const double* __restrict__ in, double* __restrict__ out, int cols, int mo, int mi,
+                              int taps, const double* __restrict__ coef, const unsigned char* __restrict__ mask,
+                              double beta, long si, long so) {
 #pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)cols * mo) return;
   in += si * blockIdx.y;
   out += so * blockIdx.y;
   const int i = (int)(t / cols), j = (int)(t % cols);
+  const BufRow<double> inb(in, mi * cols);
   const unsigned mk = mask[i];
   const int off[4] = {0, -2, 2, 4};
+  double x[4], c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[k] = 0.0;
+    c[k] = 0.0;
+    if (taps & (1 << k)) {
+      x[k] = inb[(i + off[k]) * cols + j];
+      c[k] = coef[k * mo + i];
+    }
+  }
   double acc = 0.0;
   bool first = true;
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (!(mk & (1u << k))) continue;
-    const double term = in[(size_t)(i + off[k]) * cols + j] * coef[k * mo + i];
+    const double term = x[k] * c[k];
     acc = first ? term : acc + term;
     first = false;
   }
@@ -195,30 +221,46 @@ __global__ void band_x_kernel(const double* __restrict__ in, double* __restrict_
 
 // band_x then band_y in one pass (the 2-D composite stencil to_ortho of a field): each
 // output forms the band_x values it needs with band_x's arithmetic and combines them as
-// band_y does.  in [*, ldi] (ldi = the y-input length), out [mox, moy].  Epilogue: out = acc,
+// band_y does.  in [mix, ldi] (miy values per row), out [mox, moy].  Epilogue: out = acc,
 // out = acc + add (as ew_add), or with p0 the pressure update of ew_update_pres:
-// p0 = (p0 - div nu) + acc inv_dt, nu = nuarr[entry] when given.
+// p0 = (p0 - div nu) + acc inv_dt, nu = nuarr[entry] when given.  The inputs of every tap
+// pair in tx x ty are loaded as band_y_kernel loads its taps.
 __global__ void band_xy_kernel(const double* __restrict__ in, int ldi, double* __restrict__ out, int mox, int moy,
-                               const double* __restrict__ cx, const unsigned char* __restrict__ mkx,
-                               const double* __restrict__ cy, const unsigned char* __restrict__ mky, BandXYEpi ep,
-                               long si, long so) {
+                               int mix, int miy, int tx, int ty, const double* __restrict__ cx,
+                               const unsigned char* __restrict__ mkx, const double* __restrict__ cy,
+                               const unsigned char* __restrict__ mky, BandXYEpi ep, long si, long so) {
 #pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)mox * moy) return;
   in += si * blockIdx.y;
   if (ep.split > 0 && (int)blockIdx.y >= ep.split) {  // the second field's x stencil
-    cx = ep.cx2;
-    mkx = ep.mkx2;
+    cx = ep.x2.coef;
+    mkx = ep.x2.mask;
+    mix = ep.x2.mi;
+    tx = ep.x2.taps;
   }
   const int i = (int)(t / moy), j = (int)(t % moy);
+  const BufRow<double> inb(in, (mix - 1) * ldi + miy);
   const unsigned bx = mkx[i], by = mky[j];
   const int off[4] = {0, -2, 2, 4};
-  auto xval = [&](int jj) {
+  double cxv[4], cyv[4], xin[4][4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    cxv[k] = (tx & (1 << k)) ? cx[k * mox + i] : 0.0;
+    cyv[k] = (ty & (1 << k)) ? cy[k * moy + j] : 0.0;
+  }
+#pragma unroll
+  for (int ky = 0; ky < 4; ++ky)
+#pragma unroll
+    for (int kx = 0; kx < 4; ++kx)
+      xin[ky][kx] = ((ty & (1 << ky)) && (tx & (1 << kx))) ? inb[(i + off[kx]) * ldi + j + off[ky]] : 0.0;
+  auto xval = [&](int ky) {
     double acc = 0.0;
     bool first = true;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (!(bx & (1u << k))) continue;
-      const double term = in[(size_t)(i + off[k]) * ldi + jj] * cx[k * mox + i];
+      const double term = xin[ky][k] * cxv[k];
       acc = first ? term : acc + term;
       first = false;
     }
@@ -226,9 +268,10 @@ __global__ void band_xy_kernel(const double* __restrict__ in, int ldi, double* _
   };
   double acc = 0.0;
   bool first = true;
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (!(by & (1u << k))) continue;
-    const double term = xval(j + off[k]) * cy[k * moy + j];
+    const double term = xval(k) * cyv[k];
     acc = first ? term : acc + term;
     first = false;
   }
@@ -474,32 +517,46 @@ void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nset
   check("fdma_rows_wave");
 }
 
-void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int mo, const double* coef,
-            const unsigned char* mask, double beta, int batch, long si, long so) {
-  const long n = (long)rows * mo;
+// the raw buffers of the band kernels span a batch entry's input: its byte size must fit an int
+static void band_extent_check(long elems, const char* what) {
+  if (elems < 1 || elems > 2147483647L / 8) throw std::invalid_argument(std::string(what) + ": input extent");
+}
+
+void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, const BandRef& b, double beta,
+            int batch, long si, long so) {
+  const long n = (long)rows * b.mo;
+  if (n <= 0) return;
+  if (b.mi < 1 || b.mi > ldi) throw std::invalid_argument("band_y: bad input length");
+  band_extent_check((long)(rows - 1) * ldi + b.mi, "band_y");
   RBC_LAUNCH(band_y_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, ldo, rows,
-                     mo, coef, mask, beta, si, so);
+             b.mo, b.mi, b.taps, b.coef, b.mask, beta, si, so);
   check("band_y");
 }
 
-void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int moy, const double* cx,
-             const unsigned char* mkx, const double* cy, const unsigned char* mky, const BandXYEpi& ep, int batch,
-             long si, long so) {
-  const long n = (long)mox * moy;
+void band_xy(hipStream_t s, const double* in, int ldi, double* out, const BandRef& bx, const BandRef& by,
+             const BandXYEpi& ep, int batch, long si, long so) {
+  const long n = (long)bx.mo * by.mo;
   if (n <= 0) return;
   if (batch < 1 || batch > 65535) throw std::invalid_argument("band_xy: bad batch");
   if (ep.p0 && !ep.div) throw std::invalid_argument("band_xy: pressure update without div");
-  if (ep.split > 0 && (!ep.cx2 || !ep.mkx2)) throw std::invalid_argument("band_xy: second stencil missing");
-  RBC_LAUNCH(band_xy_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, mox, moy, cx,
-             mkx, cy, mky, ep, si, so);
+  if (ep.split > 0 && (!ep.x2.coef || !ep.x2.mask || ep.x2.mo != bx.mo))
+    throw std::invalid_argument("band_xy: second stencil missing");
+  if (by.mi < 1 || by.mi > ldi || bx.mi < 1) throw std::invalid_argument("band_xy: bad input lengths");
+  band_extent_check((long)(bx.mi - 1) * ldi + by.mi, "band_xy");
+  if (ep.split > 0) band_extent_check((long)(ep.x2.mi - 1) * ldi + by.mi, "band_xy");
+  RBC_LAUNCH(band_xy_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, ldi, out, bx.mo, by.mo,
+             bx.mi, by.mi, bx.taps, by.taps, bx.coef, bx.mask, by.coef, by.mask, ep, si, so);
   check("band_xy");
 }
 
-void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
-            const unsigned char* mask, double beta, int batch, long si, long so) {
-  const long n = (long)cols * mo;
-  RBC_LAUNCH(band_x_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, out, cols, mo, coef,
-                     mask, beta, si, so);
+void band_x(hipStream_t s, const double* in, double* out, int cols, const BandRef& b, double beta, int batch, long si,
+            long so) {
+  const long n = (long)cols * b.mo;
+  if (n <= 0) return;
+  if (b.mi < 1) throw std::invalid_argument("band_x: bad input length");
+  band_extent_check((long)b.mi * cols, "band_x");
+  RBC_LAUNCH(band_x_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, s, in, out, cols, b.mo, b.mi,
+             b.taps, b.coef, b.mask, beta, si, so);
   check("band_x");
 }
 

@@ -101,7 +101,14 @@ struct DBand {
   DBuf coef;
   DBytes mask;
   int mo = 0, mi = 0;
+  int taps = 0;  // union of the presence bits
+  BandRef ref() const { return BandRef{coef.p, mask.p, mo, mi, taps}; }
 };
+static int tap_union(const std::vector<unsigned char>& mk) {
+  int t = 0;
+  for (unsigned char b : mk) t |= b;
+  return t;
+}
 
 // to_ortho stencil of a composite basis (composite_stencil.rs:207-229): n outputs from m inputs
 static void make_stencil_band(const Basis& b, DBand& band) {
@@ -114,6 +121,7 @@ static void make_stencil_band(const Basis& b, DBand& band) {
   }
   band.mo = n;
   band.mi = m;
+  band.taps = tap_union(mk);
   band.coef.upload(c);
   band.mask.upload(mk);
 }
@@ -132,6 +140,7 @@ static void make_matvec_band(const LMat& a, DBand& band, std::vector<double>* ta
   }
   band.mo = m;
   band.mi = a.c;
+  band.taps = tap_union(mk);
   band.coef.upload(c);
   band.mask.upload(mk);
   if (taps) *taps = c;  // absent taps are zero
@@ -719,12 +728,12 @@ class Navier {
   }
   void BANDY(const DBand& b, const double* in, int ldi, double* out, int ldo, int rows, double beta) {
     timer.begin(st, 9, 8.0 * nbt() * (double)rows * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
-    band_y(st, in, ldi, out, ldo, rows, b.mo, b.coef.p, b.mask.p, beta, nbt(), dstr(), dstr());
+    band_y(st, in, ldi, out, ldo, rows, b.ref(), beta, nbt(), dstr(), dstr());
     timer.end(st);
   }
   void BANDX(const DBand& b, const double* in, double* out, int cols, double beta) {
     timer.begin(st, 3, 8.0 * nbt() * (double)cols * ((double)b.mi + b.mo * (beta != 0.0 ? 2.0 : 1.0)));
-    band_x(st, in, out, cols, b.mo, b.coef.p, b.mask.p, beta, nbt(), dstr(), dstr());
+    band_x(st, in, out, cols, b.ref(), beta, nbt(), dstr(), dstr());
     timer.end(st);
   }
   // cached stencil bands per basis
@@ -1111,12 +1120,10 @@ class Navier {
       FieldPair ft(*this, 3);
       const DBand &su = stencil_band(bx_u), &stt = stencil_band(bx_t), &sy = stencil_band(by_u);
       BandXYEpi ep;
-      ep.cx2 = stt.coef.p;
-      ep.mkx2 = stt.mask.p;
+      ep.x2 = stt.ref();
       ep.split = 2 * nb;
       timer.begin(st, 3, 8.0 * nbt() * ((double)bx_u.m * by_u.m + (double)bx_u.n * by_u.n));
-      band_xy(st, UX.p, by_u.m, A.p, bx_u.n, by_u.n, su.coef.p, su.mask.p, sy.coef.p, sy.mask.p, ep, nbt(), dstr(),
-              dstr());
+      band_xy(st, UX.p, by_u.m, A.p, su.ref(), sy.ref(), ep, nbt(), dstr(), dstr());
       timer.end(st);
     }
     xop(bx_p, "DS", P0.p, ny, G.p, W9.p);                                  // d/dx pres
@@ -1260,7 +1267,7 @@ class Navier {
     const DBand &sx = stencil_band(bx), &sy = stencil_band(by);
     const double n = (double)bx.n * by.n, extra = ep.p0 ? 3.0 : ep.add ? 2.0 : 1.0;
     timer.begin(st, 3, 8.0 * nbt() * ((double)bx.m * by.m + n * extra));
-    band_xy(st, q, by.m, out, bx.n, by.n, sx.coef.p, sx.mask.p, sy.coef.p, sy.mask.p, ep, nbt(), dstr(), dstr());
+    band_xy(st, q, by.m, out, sx.ref(), sy.ref(), ep, nbt(), dstr(), dstr());
     timer.end(st);
   }
   // out = to_ortho(q) (+ add, same layout, when given)
@@ -2235,10 +2242,10 @@ class SolverDev {
   void mixed_residual(const double* R, const double* V, double* out, int batch) {
     const int m0 = b0.m, m1 = b1.m;
     const long e = (long)m0 * m1;
-    band_x(st, V, MT.p, m1, m0, axb.coef.p, axb.mask.p, 0.0, batch, e, e);
-    band_y(st, MT.p, m1, ML.p, m1, m0, m1, cyb.coef.p, cyb.mask.p, 0.0, batch, e, e);
-    band_x(st, V, MT.p, m1, m0, cxb.coef.p, cxb.mask.p, 0.0, batch, e, e);
-    band_y(st, MT.p, m1, ML.p, m1, m0, m1, aycb.coef.p, aycb.mask.p, 1.0, batch, e, e);
+    band_x(st, V, MT.p, m1, axb.ref(), 0.0, batch, e, e);
+    band_y(st, MT.p, m1, ML.p, m1, m0, cyb.ref(), 0.0, batch, e, e);
+    band_x(st, V, MT.p, m1, cxb.ref(), 0.0, batch, e, e);
+    band_y(st, MT.p, m1, ML.p, m1, m0, aycb.ref(), 1.0, batch, e, e);
     ew_axpy(st, out, R, ML.p, -1.0, e * batch);
   }
   void solve_mixed(const double* f, double* u, int batch) {
@@ -2250,8 +2257,8 @@ class SolverDev {
       mixed_batch = batch;
     }
     // R = P_x f P_y^T
-    band_x(st, f, MT.p, o1, m0, pxb.coef.p, pxb.mask.p, 0.0, batch, (long)o0 * o1, (long)m0 * o1);
-    band_y(st, MT.p, o1, MR.p, m1, m0, m1, sv.pyb.coef.p, sv.pyb.mask.p, 0.0, batch, (long)m0 * o1, e);
+    band_x(st, f, MT.p, o1, pxb.ref(), 0.0, batch, (long)o0 * o1, (long)m0 * o1);
+    band_y(st, MT.p, o1, MR.p, m1, m0, sv.pyb.ref(), 0.0, batch, (long)m0 * o1, e);
     for (int it = 0; it < mixed_iters; ++it) {
       const double* src = it == 0 ? MR.p : ML.p;  // the residual of the previous iterate
       gemm_mfma_f32_batched(st, m0, m1, m0, reinterpret_cast<const float*>(F32.p), m0, 0, src, true, m1, e, MX.p,
@@ -2278,8 +2285,8 @@ class SolverDev {
       for (DBuf* b : {&MR, &MX, &MV, &MT, &ML}) b->alloc((size_t)std::max(e, (long)m0 * o1) * batch);
       mixed_batch = batch;
     }
-    band_x(st, f, MT.p, o1, m0, pxb.coef.p, pxb.mask.p, 0.0, batch, (long)o0 * o1, (long)m0 * o1);
-    band_y(st, MT.p, o1, MR.p, m1, m0, m1, sv.pyb.coef.p, sv.pyb.mask.p, 0.0, batch, (long)m0 * o1, e);
+    band_x(st, f, MT.p, o1, pxb.ref(), 0.0, batch, (long)o0 * o1, (long)m0 * o1);
+    band_y(st, MT.p, o1, MR.p, m1, m0, sv.pyb.ref(), 0.0, batch, (long)m0 * o1, e);
     mixed_residual(MR.p, u, MV.p, batch);
     HIPCHECK(hipStreamSynchronize(st));
     std::vector<double> r = MR.download((size_t)e * batch), d = MV.download((size_t)e * batch);
@@ -2342,7 +2349,7 @@ class SolverDev {
       fdma_rows_lds(st, R, ldr, nr, X, ld, nrows, m, sv.nsets, sv.pybp.p, sv.diagp.p);
       return;
     }
-    band_y(st, R, ldr, X, ld, nrows, sv.pyb.mo, sv.pyb.coef.p, sv.pyb.mask.p, 0.0);
+    band_y(st, R, ldr, X, ld, nrows, sv.pyb.ref(), 0.0);
     if (fdma_rows_wave_supported(m)) fdma_rows_wave(st, X, ld, nrows, m, sv.nsets, sv.low.p, sv.dia.p, sv.up1.p, sv.up2.p);
     else fdma_rows(st, X, ld, nrows, m, sv.nsets, sv.low.p, sv.dia.p, sv.up1.p, sv.up2.p);
   }

@@ -80,31 +80,35 @@ void fdma_rows_wave(hipStream_t s, double* X, int ld, int nrows, int m, int nset
                     const double* dia, const double* up1, const double* up2, int batch = 1, long sx = 0);
 
 // Banded operator with taps at input offsets {0, -2, +2, +4} (summed in that order),
-// coefficients coef[k * mo + i] and presence bits mask[i]; out = op(in) [+ beta out].
+// coefficients coef[k * mo + i] and presence bits mask[i] (bit k: tap k); mo outputs from mi
+// inputs, taps the union of the presence bits.  out = op(in) [+ beta out].
+struct BandRef {
+  const double* coef = nullptr;
+  const unsigned char* mask = nullptr;
+  int mo = 0, mi = 0, taps = 0;
+};
 // band_y acts along axis 1 of in[rows, *] (leading dims ldi/ldo), band_x along axis 0
 // of in[*, cols].
-void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, int mo, const double* coef,
-            const unsigned char* mask, double beta, int batch = 1, long si = 0, long so = 0);
-void band_x(hipStream_t s, const double* in, double* out, int cols, int mo, const double* coef,
-            const unsigned char* mask, double beta, int batch = 1, long si = 0, long so = 0);
-// band_x (x stencil cx/mkx, mox outputs) then band_y (cy/mky, moy outputs, input length ldi) in
-// one launch with the two passes' arithmetic; out [mox, moy].  Epilogue (same layout as out):
-// + add when given, or with p0 the pressure update p0 = (p0 - div nu) + result inv_dt
+void band_y(hipStream_t s, const double* in, int ldi, double* out, int ldo, int rows, const BandRef& b, double beta,
+            int batch = 1, long si = 0, long so = 0);
+void band_x(hipStream_t s, const double* in, double* out, int cols, const BandRef& b, double beta, int batch = 1,
+            long si = 0, long so = 0);
+// band_x (x stencil bx) then band_y (y stencil by, input row length by.mi at stride ldi) in
+// one launch with the two passes' arithmetic; out [bx.mo, by.mo].  Epilogue (same layout as
+// out): + add when given, or with p0 the pressure update p0 = (p0 - div nu) + result inv_dt
 // (nu = nuarr[batch entry] when given; out unused).
-// Batch entries >= split (when split > 0) use the second x stencil cx2/mkx2.
+// Batch entries >= split (when split > 0) use the second x stencil x2.
 struct BandXYEpi {
   const double* add = nullptr;
   double* p0 = nullptr;
   const double* div = nullptr;
   const double* nuarr = nullptr;
   double nu = 0.0, inv_dt = 0.0;
-  const double* cx2 = nullptr;
-  const unsigned char* mkx2 = nullptr;
+  BandRef x2{};
   int split = 0;
 };
-void band_xy(hipStream_t s, const double* in, int ldi, double* out, int mox, int moy, const double* cx,
-             const unsigned char* mkx, const double* cy, const unsigned char* mky, const BandXYEpi& ep, int batch,
-             long si, long so);
+void band_xy(hipStream_t s, const double* in, int ldi, double* out, const BandRef& bx, const BandRef& by,
+             const BandXYEpi& ep, int batch, long si, long so);
 
 // Chebyshev d/dy of every row of in[rows, *] (ortho length n), with the composite
 // stencil of band coef/mask fused on load (coef == nullptr: input already ortho);
