@@ -135,34 +135,30 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
     }
   }
   const V va = (V)alpha, vb = (V)beta;
-  // the epilogue's operands U and C, all loads issued before the first is used (from
-  // element (0, 0) where an entry lies outside: a guarded load would wait for its own round
-  // trip); the branches on U and beta are uniform
-  V uv[FM][FN][4], cv[FM][FN][4];
-  if (U) {
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int gm = m0 + wm0 + i * 16 + MF::row(r, lk), gn = n0 + wn0 + j * 16 + lr;
-          uv[i][j][r] = U[gm < M && gn < N ? (size_t)gm * ldu + gn : 0];
-        }
-  }
-  if (beta != 0.0) {
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
+  for (int i = 0; i < FM; ++i) {
+    // the epilogue's operands U and C of this row of MFMA blocks, all loads issued before the
+    // first is used (from element (0, 0) where an entry lies outside: a guarded load would wait
+    // for its own round trip); the branches on U and beta are uniform
+    V uv[FN][4], cv[FN][4];
+    if (U) {
 #pragma unroll
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int gm = m0 + wm0 + i * 16 + MF::row(r, lk), gn = n0 + wn0 + j * 16 + lr;
-          cv[i][j][r] = (V)C[gm < M && gn < N ? (size_t)gm * ldc + gn : 0];
+          uv[j][r] = U[gm < M && gn < N ? (size_t)gm * ldu + gn : 0];
         }
-  }
+    }
+    if (beta != 0.0) {
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gm = m0 + wm0 + i * 16 + MF::row(r, lk), gn = n0 + wn0 + j * 16 + lr;
+          cv[j][r] = (V)C[gm < M && gn < N ? (size_t)gm * ldc + gn : 0];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
@@ -170,29 +166,30 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, con
         const int gm = m0 + wm0 + i * 16 + MF::row(r, lk), gn = n0 + wn0 + j * 16 + lr;
         if (gm < M && gn < N) {
           V v = va * acc[i][j][r];
-          if (U) v = v * uv[i][j][r];
-          if (beta != 0.0) v = v + vb * cv[i][j][r];
+          if (U) v = v * uv[j][r];
+          if (beta != 0.0) v = v + vb * cv[j][r];
           if (zero00 && gm == 0 && gn == 0) v = (V)0;
           C[(size_t)gm * ldc + gn] = (TC)v;
         }
       }
+  }
 }
 
 // Tile shapes of the launch (BM x BN); the accumulation order over K is the same for all
 // of them, so the choice never changes a result bit.
 enum { TILE_64x64, TILE_32x32, TILE_64x32, TILE_32x64, NTILES };
 
-template <int BM, int BN, class V, class TB, class TC>
+template <int BM, int BN, int BK, class V, class TB, class TC>
 void launch_tile(hipStream_t s, bool timed, int M, int N, int K, const V* A, int lda, long sA, const TB* B, int ldb,
                  long sB, TC* C, int ldc, long sC, int batch, double alpha, double beta, const V* U, int ldu, long sU,
                  const BatchMod& bm) {
   dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, batch);
   if (timed) {
-    RBC_LAUNCH((gemm_mfma_kernel<BM, BN, 32, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C,
+    RBC_LAUNCH((gemm_mfma_kernel<BM, BN, BK, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb, sB, C,
                ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u, static_cast<const V*>(bm.a2),
                static_cast<const TB*>(bm.b2), bm.split, bm.zero00);
   } else {
-    hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, 32, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb,
+    hipLaunchKernelGGL((gemm_mfma_kernel<BM, BN, BK, V, TB, TC>), grid, dim3(256), 0, s, M, N, K, A, lda, sA, B, ldb,
                        sB, C, ldc, sC, alpha, beta, U, ldu, sU, bm.a, bm.b, bm.u, static_cast<const V*>(bm.a2),
                        static_cast<const TB*>(bm.b2), bm.split, bm.zero00);
   }
@@ -202,14 +199,14 @@ template <class V, class TB, class TC>
 void launch_variant(int v, hipStream_t s, bool timed, int M, int N, int K, const V* A, int lda, long sA, const TB* B,
                     int ldb, long sB, TC* C, int ldc, long sC, int batch, double alpha, double beta, const V* U,
                     int ldu, long sU, const BatchMod& bm) {
-#define RBC_TILE(BM, BN)                                                                                       \
-  launch_tile<BM, BN, V, TB, TC>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, ldu, \
-                                 sU, bm)
+#define RBC_TILE(BM, BN, BK)                                                                                       \
+  launch_tile<BM, BN, BK, V, TB, TC>(s, timed, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, alpha, beta, U, \
+                                     ldu, sU, bm)
   switch (v) {
-    case TILE_64x64: RBC_TILE(64, 64); break;
-    case TILE_32x32: RBC_TILE(32, 32); break;
-    case TILE_64x32: RBC_TILE(64, 32); break;
-    default: RBC_TILE(32, 64); break;
+    case TILE_64x64: RBC_TILE(64, 64, 32); break;
+    case TILE_32x32: RBC_TILE(32, 32, 32); break;
+    case TILE_64x32: RBC_TILE(64, 32, 32); break;
+    default: RBC_TILE(32, 64, 32); break;
   }
 #undef RBC_TILE
 }
