@@ -1621,8 +1621,8 @@ class Navier {
   }
   // the one-launch y-stage of the FP32 convective terms: DCT rows and the Dirichlet stencil
   // for the three fields (the periodic Rayleigh-Benard bases).  The FP64 step keeps its
-  // three launches: the one-launch kernel holds twice the registers in FP64 and runs two
-  // blocks per CU, which measured slower.
+  // three launches: in FP64 the one-launch kernel holds 168 VGPRs and 69 KB of LDS, two
+  // blocks per CU, and at 1024 x 1025 takes 69.6 us against 68.2 for the three launches.
   bool conv3_ok() const {
     if (!use_conv3 || !cheb_y_supported(ny) || by_u.k != CD || by_u.n != ny || by_u.m != ny - 2 ||
         !dirichlet_taps(by_u))
